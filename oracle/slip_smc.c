@@ -1,0 +1,688 @@
+/*
+ * Oracle part 2: inner slip SMC.  Test infrastructure only (see oracle.h).
+ *
+ * Reference files followed (smc_slip.f90 unless noted):
+ *   8-54      RNG correction, Box-Muller, cdf_norm / pdf_norm
+ *   56-104    slip_calc_likelihood (Gram form)
+ *   106-137   slip_calc_prior       139-182  slip_calc_grad
+ *   184-287   slip_gen_init_particles (sequential Gibbs chain + Newton)
+ *   289-387   slip_calc_mean_std_vector / slip_find_next_gamma
+ *   465-589   normalize (real(16)), mean/cov-diag (real(16)), resampling (real(16) u)
+ *   591-917   slip_hmc_sampling / slip_leapfrog
+ *   919-1214  slip_hmc_tuning
+ *   1347-1443 slip_calc_gsd_gsg      1445-1663 slip_smc_exec
+ *   smc_fault.f90:66-292 fault_calc_likelihood
+ * The Gibbs-sweep mover (mover=1) has no reference counterpart: it is the
+ * truncated-MVN Gibbs mover of the benchmark configuration, specified in
+ * orc_gibbs_move below and restated identically on the device.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <quadmath.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+typedef __float128 q128;
+
+enum { SITE_GIBBS_INIT = 1, SITE_RESAMPLE = 2, SITE_TUNE_DTAU = 3, SITE_TUNE_MOM = 4,
+       SITE_TUNE_MH = 5, SITE_TUNE_RESAMPLE = 6, SITE_HMC_MOM = 7, SITE_HMC_MH = 8,
+       SITE_GIBBS_MOVE = 9 };
+
+static double draw_uniform(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1,
+                           int which) {
+    uint64_t w[2];
+    orc_draw_u64x2(key, site, stage, c2, c1, 0, w);
+    return orc_uniform_raw(w[which]);
+}
+static double draw_bm(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1) {
+    uint64_t w[2];
+    orc_draw_u64x2(key, site, stage, c2, c1, 0, w);
+    return orc_box_muller_sin(orc_clamp_uniform(orc_uniform_raw(w[0])),
+                              orc_clamp_uniform(orc_uniform_raw(w[1])));
+}
+static void draw_quad(uint64_t key, uint32_t site, uint32_t stage, double* hi, double* lo) {
+    uint64_t w[2];
+    orc_draw_u64x2(key, site, stage, 0, 0, 0, w);
+    *hi = (double)(w[0] >> 11) * 0x1.0p-53;
+    *lo = (double)(w[1] >> 11) * 0x1.0p-106;
+}
+
+/* ---------------- Gram quantities, smc_slip.f90:1347-1443 ---------------- */
+void orc_slip_calc_gsd_gsg(const double* gmat, const double* dvec, int nobs, int ndim, int npath,
+                           const int* nsar_index, int nsar_total, const double* sar_sinv,
+                           double sigma_sar2, double sigma_gnss2, const double* obs_sigma,
+                           double* gsdvec, double* gsgmat, double* dsd) {
+    double* sd = (double*)calloc(nobs, sizeof(double));
+    double* gts = (double*)calloc((size_t)ndim * nobs, sizeof(double));
+    size_t off = 0;
+    for (int ip = 0; ip < npath; ++ip) {
+        int n = nsar_index[ip + 1] - nsar_index[ip];
+        int r0 = nsar_index[ip] - 1;
+        const double* A = sar_sinv + off;
+        /* dgemv('n'): y = alpha*A*x, column-oriented accumulation */
+        for (int j = 0; j < n; ++j) {
+            double t = (1.0 / sigma_sar2) * dvec[r0 + j];
+            for (int i = 0; i < n; ++i) sd[r0 + i] += t * A[(size_t)j * n + i];
+        }
+        /* gts(:, r0+c) = (1/s2) * G_path^T * Sinv(:, c) */
+        for (int c = 0; c < n; ++c)
+            for (int d = 0; d < ndim; ++d) {
+                double s = 0.0;
+                for (int k = 0; k < n; ++k) s += gmat[(size_t)d * nobs + r0 + k] * A[(size_t)c * n + k];
+                gts[(size_t)(r0 + c) * ndim + d] = (1.0 / sigma_sar2) * s;
+            }
+        off += (size_t)n * n;
+    }
+    for (int io = nsar_total; io < nobs; ++io)
+        sd[io] = dvec[io] / (sigma_gnss2 * (obs_sigma[io] * obs_sigma[io]));
+    for (int d = 0; d < ndim; ++d) {
+        double s = 0.0;
+        for (int io = 0; io < nobs; ++io) s = s + gmat[(size_t)d * nobs + io] * sd[io];
+        gsdvec[d] = s;
+    }
+    double t = 0.0;
+    for (int io = 0; io < nobs; ++io) t = t + dvec[io] * sd[io];
+    *dsd = t;
+    for (int io = nsar_total; io < nobs; ++io)
+        for (int d = 0; d < ndim; ++d)
+            gts[(size_t)io * ndim + d] = gmat[(size_t)d * nobs + io] / ((obs_sigma[io] * obs_sigma[io]) * sigma_gnss2);
+    for (int j = 0; j < ndim; ++j)
+        for (int i = 0; i < ndim; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < nobs; ++k) s += gts[(size_t)k * ndim + i] * gmat[(size_t)j * nobs + k];
+            gsgmat[(size_t)j * ndim + i] = s;
+        }
+    free(gts); free(sd);
+}
+
+/* ---------------- likelihood / prior / gradient ---------------- */
+double orc_slip_calc_likelihood(const double* s, int ndim, const double* H, const double* b,
+                                double dsd, int nsar_total, int ngnss, double lss2, double lsg2) {
+    double v = 0.0;
+    for (int i = 0; i < ndim; ++i)
+        for (int j = 0; j < i; ++j) v = v + s[i] * H[(size_t)i * ndim + j] * s[j];
+    v = v * 2.0;
+    for (int i = 0; i < ndim; ++i) v = v + s[i] * H[(size_t)i * ndim + i] * s[i];
+    double bs = 0.0;
+    for (int i = 0; i < ndim; ++i) bs = bs + b[i] * s[i];
+    return (v - 2.0 * bs + dsd) / 2.0 + nsar_total * lss2 / 2.0 + 3.0 * ngnss * lsg2 / 2.0;
+}
+
+double orc_slip_calc_prior(const double* s, const double* alpha2_full, const int* li,
+                           const double* lv, int nnode, double* lsvec) {
+    int n = 2 * nnode;
+    for (int i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < 9; ++j) t = t + lv[9 * i + j] * s[li[9 * i + j] - 1];
+        lsvec[i] = t;
+    }
+    double p = 0.0;
+    for (int i = 0; i < n; ++i) p = p + (lsvec[i] * lsvec[i]) / (2.0 * alpha2_full[i]);
+    return p;
+}
+
+void orc_slip_calc_grad(double* grad, const double* s, int ndim, int nnode, int ndof,
+                        const int* li, const double* lv, const int* lti, const double* ltv,
+                        double gamma, const double* alpha2_full, const double* b,
+                        const double* H, double* lsvec) {
+    double* g1 = (double*)calloc(ndim, sizeof(double));
+    for (int j = 0; j < ndim; ++j) {
+        double t = s[j];
+        for (int i = 0; i < ndim; ++i) g1[i] += t * H[(size_t)j * ndim + i];
+    }
+    for (int i = 0; i < ndim; ++i) { g1[i] = g1[i] - b[i]; g1[i] = g1[i] * gamma; }
+    for (int n = 0; n < 2 * nnode; ++n) {
+        double t = 0.0;
+        for (int j = 0; j < 9; ++j) t = t + lv[9 * n + j] * s[li[9 * n + j] - 1];
+        lsvec[n] = t / alpha2_full[n];
+    }
+    for (int d = 0; d < 2 * ndof; ++d) {
+        double t = 0.0;
+        for (int j = 0; j < 9; ++j) t = t + ltv[9 * d + j] * lsvec[lti[9 * d + j] - 1];
+        grad[d] = g1[d] + t;
+    }
+    free(g1);
+}
+
+double orc_cdf_norm(double x, double mu, double sigma2) {
+    return (1.0 + erf((x - mu) / sqrt(2.0 * sigma2))) / 2.0;
+}
+double orc_pdf_norm(double x, double mu, double sigma2) {
+    double pi = 2.0 * asin(1.0);
+    return exp(-((x - mu) * (x - mu)) / (2.0 * sigma2)) / sqrt(2.0 * pi * sigma2);
+}
+
+/* ---------------- prior draw: one sequential Gibbs chain, smc_slip.f90:209-264 ---------------- */
+void orc_slip_gibbs_init_chain(double* particles, int P, int ndim, const double* ll,
+                               const double* alpha2_full, double max_slip, const double* bm) {
+    double* cur = (double*)calloc(ndim, sizeof(double));
+    for (int k = 0; k < P; ++k) {
+        for (int i = 0; i < ndim; ++i) {
+            double s2 = 1.0 / (ll[(size_t)i * ndim + i] / alpha2_full[i]);
+            double mu = 0.0;
+            for (int j = 0; j < ndim; ++j) {
+                if (j == i) continue;
+                double lij = ll[(size_t)j * ndim + i];
+                if (lij == 0.0) continue; /* exact: subtracting a signed zero leaves mu unchanged */
+                mu = mu - lij / alpha2_full[i] * cur[j] * s2;
+            }
+            double v1 = bm[(size_t)k * ndim + i];
+            double x = mu + v1 * sqrt(s2);
+            double fx = orc_cdf_norm(x, mu, s2);
+            double f1 = orc_cdf_norm(max_slip, mu, s2);
+            double f0 = orc_cdf_norm(0.0, mu, s2);
+            double y = mu, err = 1.0, yp = y + err;
+            int cnt = 0;
+            double target = (f1 - f0) * fx + f0;
+            while (err > 1e-8 && cnt < 100) {
+                y = y - (orc_cdf_norm(y, mu, s2) - target) / orc_pdf_norm(y, mu, s2);
+                err = fabs(y - yp);
+                yp = y;
+                cnt++;
+            }
+            cur[i] = y;
+        }
+        memcpy(particles + (size_t)k * ndim, cur, sizeof(double) * ndim);
+    }
+    free(cur);
+}
+
+/* ---------------- tempering: smc_slip.f90:289-387 / smc_fault.f90:372-485 ---------------- */
+double orc_find_next_gamma(double gamma_prev, const double* L, int P, double* w,
+                           double* neglog_evidence, double cv_threshold, double tol, int cap) {
+    double lmin = L[0];
+    for (int i = 0; i < P; ++i) if (L[i] < lmin) lmin = L[i];
+    double lower = gamma_prev, upper = 1.0, err = 1.0, gamma = 1.0;
+    int cnt = 0;
+    while (err > tol) {
+        gamma = (lower + upper) / 2.0;
+        double dg = gamma - gamma_prev;
+        for (int i = 0; i < P; ++i) w[i] = exp(-dg * (L[i] - lmin));
+        double mean = 0.0;
+        for (int i = 0; i < P; ++i) mean = mean + w[i];
+        mean = mean / P;
+        double sd = 0.0;
+        for (int i = 0; i < P; ++i) sd = sd + (w[i] - mean) * (w[i] - mean);
+        sd = sqrt(sd / P);
+        double cv = sd / mean;
+        if (cv > cv_threshold) upper = gamma; else lower = gamma;
+        err = fabs(cv - cv_threshold);
+        if (fabs(gamma - 1.0) < tol) break;
+        if (cnt > cap) break;
+        cnt++;
+    }
+    double ev = 0.0, dg = gamma - gamma_prev;
+    for (int i = 0; i < P; ++i) ev = ev + exp(-dg * (L[i] - lmin));
+    ev = ev / P;
+    *neglog_evidence = -log(ev) + dg * lmin;
+    return gamma;
+}
+
+void orc_slip_normalize_weights(double* w, int P) {
+    q128 s = 0;
+    for (int i = 0; i < P; ++i) s = s + (q128)w[i];
+    for (int i = 0; i < P; ++i) w[i] = (double)((q128)w[i] / s);
+}
+
+void orc_slip_calc_mean_particles(const double* x, const double* w, double* mean, int P, int nd) {
+    q128* m = (q128*)calloc(nd, sizeof(q128));
+    for (int i = 0; i < P; ++i)
+        for (int d = 0; d < nd; ++d) m[d] = m[d] + (q128)(w[i] * x[(size_t)i * nd + d]);
+    for (int d = 0; d < nd; ++d) mean[d] = (double)m[d];
+    free(m);
+}
+
+void orc_slip_calc_cov_particles(const double* x, const double* w, const double* mean,
+                                 double* cov_diag, int P, int nd) {
+    q128* c = (q128*)calloc(nd, sizeof(q128));
+    for (int i = 0; i < P; ++i)
+        for (int d = 0; d < nd; ++d) {
+            double di = x[(size_t)i * nd + d] - mean[d];
+            c[d] = c[d] + (q128)(w[i] * di * di);
+        }
+    for (int d = 0; d < nd; ++d) {
+        cov_diag[d] = (double)c[d];
+        if (fabs(cov_diag[d]) < 1e-3) cov_diag[d] = 1e-3;
+    }
+    free(c);
+}
+
+/* systematic residual resampling with a real(16) running u, smc_slip.f90:572-589 */
+void orc_resample_quad(int P, const double* w, double u_hi, double u_lo, int* assigned) {
+    q128 u = (q128)u_hi + (q128)u_lo;
+    if (u < (q128)1e-5) u = (q128)1e-5;
+    q128 dn = (q128)P;
+    u = u / dn;
+    for (int i = 0; i < P; ++i) {
+        q128 t = ((q128)w[i] - u) * dn;
+        int a = (int)floorq(t) + 1;
+        assigned[i] = a;
+        u = u + (q128)a / dn - (q128)w[i];
+    }
+}
+
+/* ---------------- standard normal quantile: Wichura (1988) AS 241 PPND16 ---------------- */
+double orc_ndtri(double p) {
+    double q = p - 0.5, r, z;
+    if (fabs(q) <= 0.425) {
+        r = 0.180625 - q * q;
+        z = q * (((((((2509.0809287301226727 * r + 33430.575583588128105) * r + 67265.770927008700853) * r +
+                     45921.953931549871457) * r + 13731.693765509461125) * r + 1971.5909503065514427) * r +
+                  133.14166789178437745) * r + 3.387132872796366608) /
+            (((((((5226.495278852545925 * r + 28729.085735721942674) * r + 39307.89580009271061) * r +
+                 21213.794301586595867) * r + 5394.1960214247511077) * r + 687.1870074920579083) * r +
+              42.313330701600911252) * r + 1.0);
+        return z;
+    }
+    r = q < 0 ? p : 1.0 - p;
+    r = sqrt(-log(r));
+    if (r <= 5.0) {
+        r -= 1.6;
+        z = (((((((7.7454501427834140764e-4 * r + 0.0227238449892691845833) * r + 0.24178072517745061177) * r +
+                 1.27045825245236838258) * r + 3.64784832476320460504) * r + 5.7694972214606914055) * r +
+              4.6303378461565452959) * r + 1.42343711074968357734) /
+            (((((((1.05075007164441684324e-9 * r + 5.475938084995344946e-4) * r + 0.0151986665636164571966) * r +
+                 0.14810397642748007459) * r + 0.68976733498510000455) * r + 1.6763848301838038494) * r +
+              2.05319162663775882187) * r + 1.0);
+    } else {
+        r -= 5.0;
+        z = (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r + 0.0012426609473880784386) * r +
+                 0.026532189526576123093) * r + 0.29656057182850489123) * r + 1.7848265399172913358) * r +
+              5.4637849111641143699) * r + 6.6579046435011037772) /
+            (((((((2.04426310338993978564e-15 * r + 1.4215117583164458887e-7) * r + 1.8463183175100546818e-5) * r +
+                 7.868691311456132591e-4) * r + 0.0148753612908506148525) * r + 0.13692988092273580531) * r +
+              0.59983220655588793769) * r + 1.0);
+    }
+    return q < 0 ? -z : z;
+}
+
+/* standard normal truncated to [a, b], by inversion with tail-aware probabilities */
+static double tn_std(double a, double b, double u) {
+    if (a > 0.0) {
+        if (a > 35.0) {
+            double z = a - log1p(-u * (-expm1(-a * (b - a)))) / a;
+            return fmin(z, b);
+        }
+        double qa = 0.5 * erfc(a * M_SQRT1_2), qb = 0.5 * erfc(b * M_SQRT1_2);
+        double p = qa - u * (qa - qb);
+        double z = -orc_ndtri(p);
+        return fmin(fmax(z, a), b);
+    }
+    if (b < 0.0) return -tn_std(-b, -a, u);
+    double pa = 0.5 * erfc(-a * M_SQRT1_2), pb = 0.5 * erfc(-b * M_SQRT1_2);
+    double p = pa + u * (pb - pa), z;
+    if (p <= 0.5) {
+        z = orc_ndtri(p);
+    } else {
+        double qa = 0.5 * erfc(a * M_SQRT1_2), qb = 0.5 * erfc(b * M_SQRT1_2);
+        double qq = qb + (1.0 - u) * (qa - qb);
+        z = -orc_ndtri(qq);
+    }
+    return fmin(fmax(z, a), b);
+}
+
+double orc_trunc_normal(double mu, double sigma, double lo, double hi, double u) {
+    double a = (lo - mu) / sigma, b = (hi - mu) / sigma;
+    double x = mu + sigma * tn_std(a, b, u);
+    return fmin(fmax(x, lo), hi);
+}
+
+static double open_uniform(uint64_t w) { return ((double)(w >> 11) + 0.5) * 0x1.0p-53; }
+
+/* ---------------- per-instance state ---------------- */
+typedef struct {
+    const orc_problem* pb;
+    int ndim, nnode, ndof, P;
+    double *H, *b, dsd, *ll, *alpha2, lss2, lsg2;
+    int *li, *lti;
+    double *lv, *ltv;
+} inst_t;
+
+static double inst_likelihood(const inst_t* I, const double* s) {
+    return orc_slip_calc_likelihood(s, I->ndim, I->H, I->b, I->dsd, I->pb->nsar_total,
+                                    I->pb->ngnss, I->lss2, I->lsg2);
+}
+static double inst_prior(const inst_t* I, const double* s, double* lsvec) {
+    return orc_slip_calc_prior(s, I->alpha2, I->li, I->lv, I->nnode, lsvec);
+}
+
+/* ---------------- HMC: smc_slip.f90:732-917 ---------------- */
+static void leapfrog(const inst_t* I, double gamma, double post_cur, double* p, const double* cov,
+                     double* cand, const double* cur, double dtau, int ntau, double* ham_cur,
+                     double* like_c, double* prior_c, double* post_c, double* ham_c, double* grad,
+                     double* lsvec) {
+    int nd = I->ndim;
+    double smax = I->pb->max_slip;
+    double h = post_cur;
+    for (int i = 0; i < nd; ++i) h = h + p[i] * cov[i] * p[i] / 2.0;
+    *ham_cur = h;
+    orc_slip_calc_grad(grad, cand, nd, I->nnode, I->ndof, I->li, I->lv, I->lti, I->ltv, gamma,
+                       I->alpha2, I->b, I->H, lsvec);
+    for (int i = 0; i < nd; ++i) p[i] = p[i] - grad[i] * 5e-1 * dtau;
+    for (int it = 0; it <= ntau; ++it) {
+        for (int i = 0; i < nd; ++i) {
+            cand[i] = (it < ntau) ? cand[i] + cov[i] * p[i] * dtau : cand[i] + cov[i] * p[i] * 5e-1 * dtau;
+            if (fabs(cand[i]) > 1e3) {
+                memcpy(cand, cur, sizeof(double) * nd);
+                *ham_c = 1e20;
+                return;
+            }
+            while (cand[i] < 0.0 || cand[i] > smax) {
+                if (cand[i] > smax) { cand[i] = 2.0 * smax - cand[i]; p[i] = -p[i]; }
+                if (cand[i] < 0.0) { cand[i] = -cand[i]; p[i] = -p[i]; }
+            }
+        }
+        if (it == ntau) break;
+        orc_slip_calc_grad(grad, cand, nd, I->nnode, I->ndof, I->li, I->lv, I->lti, I->ltv, gamma,
+                           I->alpha2, I->b, I->H, lsvec);
+        for (int i = 0; i < nd; ++i) p[i] = p[i] - grad[i] * dtau;
+    }
+    *like_c = inst_likelihood(I, cand);
+    *prior_c = inst_prior(I, cand, lsvec);
+    *post_c = gamma * *like_c + *prior_c;
+    h = *post_c;
+    for (int i = 0; i < nd; ++i) h = h + p[i] * cov[i] * p[i] / 2.0;
+    *ham_c = h;
+}
+
+typedef struct {
+    double log_dtau_upper, log_dtau_lower;
+    int ntau_upper, ntau_lower;
+    double* dtau_ls;
+    int* ntau_ls;
+} hmc_tune_t;
+
+static int hmc_tuning(const inst_t* I, uint64_t key, int stage, double gamma, const double* X,
+                      const double* L, const double* PR, const double* cov, const int* assigned,
+                      const int* id_start, hmc_tune_t* T) {
+    int P = I->P, nd = I->ndim, tf = 5, cnt = 0;
+    for (int i = 0; i < P; i += tf)
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
+            cnt++;
+            double u1 = orc_clamp_uniform(draw_uniform(key, SITE_TUNE_DTAU, stage, j, 0, 0));
+            double u2 = orc_clamp_uniform(draw_uniform(key, SITE_TUNE_DTAU, stage, j, 0, 1));
+            T->ntau_ls[j] = (int)(T->ntau_lower + u1 * (T->ntau_upper - T->ntau_lower));
+            double ld = T->log_dtau_lower + u2 * (T->log_dtau_upper - T->log_dtau_lower);
+            T->dtau_ls[j] = exp(ld);
+        }
+    if (cnt == 0) return 0;
+    double* score = (double*)malloc(sizeof(double) * P);
+    for (int j = 0; j < P; ++j) score[j] = -1.0;
+    #pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < P; i += tf) {
+        double* cur = (double*)malloc(sizeof(double) * nd);
+        double* cand = (double*)malloc(sizeof(double) * nd);
+        double* p = (double*)malloc(sizeof(double) * nd);
+        double* grad = (double*)malloc(sizeof(double) * nd);
+        double* lsv = (double*)malloc(sizeof(double) * 2 * I->nnode);
+        memcpy(cur, X + (size_t)i * nd, sizeof(double) * nd);
+        double lc = L[i], pc = PR[i], postc = gamma * lc + pc;
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
+            double dtau = T->dtau_ls[j];
+            memcpy(cand, cur, sizeof(double) * nd);
+            for (int d = 0; d < nd; ++d) p[d] = draw_bm(key, SITE_TUNE_MOM, stage, j, d) / sqrt(cov[d]);
+            double hc, lcand = 0, pcand = 0, postcand = 0, hcand;
+            leapfrog(I, gamma, postc, p, cov, cand, cur, dtau, 50, &hc, &lcand, &pcand, &postcand,
+                     &hcand, grad, lsv);
+            double dham = hc - hcand, sc = 0.0;
+            for (int d = 0; d < nd; ++d) sc = sc + ((cand[d] - cur[d]) * (cand[d] - cur[d])) / cov[d];
+            sc = sc * exp(fmin(0.0, dham));
+            score[j] = sc;
+            double mh = orc_clamp_uniform(draw_uniform(key, SITE_TUNE_MH, stage, j, 0, 0));
+            if (hc - hcand > log(mh)) {
+                memcpy(cur, cand, sizeof(double) * nd);
+                lc = lcand; pc = pcand; postc = postcand;
+            }
+        }
+        free(cur); free(cand); free(p); free(grad); free(lsv);
+    }
+    double ss = 0.0;
+    for (int i = 0; i < P; i += tf)
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) ss = ss + score[j];
+    if (ss < 1e-5) {
+        ss = 0.0;
+        for (int i = 0; i < P; i += tf)
+            for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) { score[j] = 1.0; ss = ss + 1.0; }
+    }
+    double uh, ul;
+    draw_quad(key, SITE_TUNE_RESAMPLE, stage, &uh, &ul);
+    q128 u = (q128)uh + (q128)ul;
+    if (u < (q128)1e-5) u = (q128)1e-5;
+    q128 dn = (q128)P;
+    u = u / dn;
+    double* dnew = (double*)malloc(sizeof(double) * P);
+    int* nnew = (int*)malloc(sizeof(int) * P);
+    int c2 = 0;
+    for (int i = 0; i < P; i += tf)
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
+            double wj = score[j] / ss;
+            int a = (int)floorq(((q128)wj - u) * dn) + 1;
+            u = u + (q128)a / dn - (q128)wj;
+            for (int k = 0; k < a && c2 < P; ++k) { dnew[c2] = T->dtau_ls[j]; nnew[c2] = T->ntau_ls[j]; c2++; }
+        }
+    int ok = (c2 == P);
+    for (int j = 0; j < c2; ++j) { T->dtau_ls[j] = dnew[j]; T->ntau_ls[j] = nnew[j]; }
+    q128 m = 0;
+    for (int j = 0; j < P; ++j) m = m + (q128)T->dtau_ls[j];
+    m = m / dn;
+    T->log_dtau_upper = (double)(logq(m) + (q128)log(5.0));
+    T->log_dtau_lower = (double)(logq(m) - (q128)log(5.0));
+    free(dnew); free(nnew); free(score);
+    return ok ? 0 : -1;
+}
+
+/* ---------------- HMC sampling over all parents, smc_slip.f90:591-730 ---------------- */
+static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma, const double* X,
+                         const double* L, const double* PR, double* Xn, double* Ln, double* PRn,
+                         const double* cov, const int* assigned, const int* id_start,
+                         const double* dtau_ls) {
+    int P = I->P, nd = I->ndim;
+    #pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < P; ++i) {
+        if (assigned[i] == 0) continue;
+        double* cur = (double*)malloc(sizeof(double) * nd);
+        double* cand = (double*)malloc(sizeof(double) * nd);
+        double* p = (double*)malloc(sizeof(double) * nd);
+        double* grad = (double*)malloc(sizeof(double) * nd);
+        double* lsv = (double*)malloc(sizeof(double) * 2 * I->nnode);
+        memcpy(cur, X + (size_t)i * nd, sizeof(double) * nd);
+        double lc = L[i], pc = PR[i], postc = gamma * lc + pc;
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
+            memcpy(cand, cur, sizeof(double) * nd);
+            for (int d = 0; d < nd; ++d) p[d] = draw_bm(key, SITE_HMC_MOM, stage, j, d) / sqrt(cov[d]);
+            double hc, lcand = 0, pcand = 0, postcand = 0, hcand;
+            leapfrog(I, gamma, postc, p, cov, cand, cur, dtau_ls[j], 50, &hc, &lcand, &pcand, &postcand,
+                     &hcand, grad, lsv);
+            double mh = orc_clamp_uniform(draw_uniform(key, SITE_HMC_MH, stage, j, 0, 0));
+            if ((hc - hcand) > log(mh)) {
+                memcpy(cur, cand, sizeof(double) * nd);
+                lc = lcand; pc = pcand; postc = postcand;
+            }
+            memcpy(Xn + (size_t)j * nd, cur, sizeof(double) * nd);
+            Ln[j] = lc; PRn[j] = pc;
+        }
+        free(cur); free(cand); free(p); free(grad); free(lsv);
+    }
+}
+
+/* ---------------- Gibbs-sweep mover (benchmark configuration; no reference counterpart) --------
+ * Target at stage gamma: exp(-gamma*likelihood(s) - prior_quadratic(s)) on [0, max_slip]^ndim,
+ * with precision Q = gamma*H + LL/alpha^2 and linear term gamma*b.  Each child j starts from
+ * its parent and runs `sweeps` full systematic-scan sweeps (i = 1..ndim); coordinate i is
+ * redrawn from N(mu_i, 1/Q_ii) truncated to [0, max_slip] with
+ *   mu_i = s_i - ((Q s)_i - gamma*b_i) / Q_ii
+ * using the open uniform u = ((w>>11) + 1/2) 2^-53 of draw (site 9, stage, c2=j, c1=i, c0=sweep).
+ */
+static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, int sweeps,
+                       const double* X, double* Xn, double* Ln, double* PRn, const int* assigned,
+                       const int* id_start) {
+    int P = I->P, nd = I->ndim;
+    double smax = I->pb->max_slip;
+    double a2 = I->alpha2[0];
+    double* Q = (double*)malloc(sizeof(double) * (size_t)nd * nd);
+    double* hb = (double*)malloc(sizeof(double) * nd);
+    for (size_t k = 0; k < (size_t)nd * nd; ++k) Q[k] = gamma * I->H[k] + I->ll[k] / a2;
+    for (int i = 0; i < nd; ++i) hb[i] = gamma * I->b[i];
+    #pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < P; ++i) {
+        double* lsv = (double*)malloc(sizeof(double) * 2 * I->nnode);
+        for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
+            double* s = Xn + (size_t)j * nd;
+            memcpy(s, X + (size_t)i * nd, sizeof(double) * nd);
+            for (int sw = 0; sw < sweeps; ++sw)
+                for (int c = 0; c < nd; ++c) {
+                    double qs = 0.0;
+                    for (int k = 0; k < nd; ++k) qs += Q[(size_t)k * nd + c] * s[k];
+                    double prec = Q[(size_t)c * nd + c];
+                    double mu = s[c] - (qs - hb[c]) / prec;
+                    uint64_t w[2];
+                    orc_draw_u64x2(key, SITE_GIBBS_MOVE, stage, j, c, sw, w);
+                    s[c] = orc_trunc_normal(mu, 1.0 / sqrt(prec), 0.0, smax, open_uniform(w[0]));
+                }
+            Ln[j] = inst_likelihood(I, s);
+            PRn[j] = inst_prior(I, s, lsv);
+        }
+        free(lsv);
+    }
+    free(Q); free(hb);
+}
+
+/* ---------------- instance setup shared by slip_smc_exec and fault_calc_likelihood ------------- */
+static void inst_build(inst_t* I, const orc_problem* pb, const double* theta, const double* gmat) {
+    memset(I, 0, sizeof(*I));
+    I->pb = pb;
+    int npatch;
+    orc_mesh_sizes(&pb->fault, &I->nnode, &I->ndof, &npatch, NULL);
+    I->ndim = 2 * I->ndof;
+    I->P = pb->nparticle_slip;
+    int nnode = I->nnode, ndof = I->ndof, nd = I->ndim;
+    int* cny = (int*)malloc(sizeof(int) * 4 * npatch);
+    double* coor = (double*)malloc(sizeof(double) * 2 * nnode);
+    int* n2e = (int*)malloc(sizeof(int) * 4 * nnode);
+    int* n2es = (int*)malloc(sizeof(int) * nnode);
+    int* id_dof = (int*)malloc(sizeof(int) * ndof);
+    orc_discretize_fault(&pb->fault, theta, cny, coor, n2e, n2es, id_dof);
+    double* luni = (double*)malloc(sizeof(double) * (size_t)nnode * nnode);
+    double* lmat = (double*)malloc(sizeof(double) * (size_t)2 * nnode * 2 * ndof);
+    orc_gen_laplacian(&pb->fault, theta, nnode, id_dof, ndof, luni, lmat);
+    I->li = (int*)malloc(sizeof(int) * 9 * 2 * nnode);
+    I->lv = (double*)malloc(sizeof(double) * 9 * 2 * nnode);
+    I->lti = (int*)malloc(sizeof(int) * 9 * 2 * ndof);
+    I->ltv = (double*)malloc(sizeof(double) * 9 * 2 * ndof);
+    orc_gen_sparse_lmat(lmat, nnode, ndof, I->li, I->lv, I->lti, I->ltv);
+    I->ll = (double*)malloc(sizeof(double) * (size_t)nd * nd);
+    orc_calc_ll(lmat, nnode, ndof, I->ll);
+    double la2;
+    orc_get_sigma(theta, pb->fault.fix_xbend, pb->fault.nplane, &I->lss2, &I->lsg2, &la2);
+    I->alpha2 = (double*)malloc(sizeof(double) * 2 * nnode);
+    for (int k = 0; k < 2 * nnode; ++k) I->alpha2[k] = exp(la2);
+    I->H = (double*)malloc(sizeof(double) * (size_t)nd * nd);
+    I->b = (double*)malloc(sizeof(double) * nd);
+    orc_slip_calc_gsd_gsg(gmat, pb->dvec, pb->nobs, nd, pb->npath, pb->nsar_index, pb->nsar_total,
+                          pb->sar_sinv, exp(I->lss2), exp(I->lsg2), pb->obs_sigma, I->b, I->H, &I->dsd);
+    free(luni); free(lmat); free(id_dof); free(n2es); free(n2e); free(coor); free(cny);
+}
+
+static void inst_free(inst_t* I) {
+    free(I->li); free(I->lv); free(I->lti); free(I->ltv); free(I->ll); free(I->alpha2);
+    free(I->H); free(I->b);
+}
+
+/* ---------------- slip_smc_exec, smc_slip.f90:1445-1663 ---------------- */
+static void smc_run(const inst_t* I, uint64_t key, double* X, double* L, double* PR, double* mean,
+                    orc_slip_result* res) {
+    int P = I->P, nd = I->ndim;
+    double* bm = (double*)malloc(sizeof(double) * (size_t)P * nd);
+    #pragma omp parallel for
+    for (int k = 0; k < P; ++k)
+        for (int i = 0; i < nd; ++i) bm[(size_t)k * nd + i] = draw_bm(key, SITE_GIBBS_INIT, 0, k, i);
+    orc_slip_gibbs_init_chain(X, P, nd, I->ll, I->alpha2, I->pb->max_slip, bm);
+    free(bm);
+    #pragma omp parallel for
+    for (int k = 0; k < P; ++k) {
+        double* lsv = (double*)malloc(sizeof(double) * 2 * I->nnode);
+        L[k] = inst_likelihood(I, X + (size_t)k * nd);
+        PR[k] = inst_prior(I, X + (size_t)k * nd, lsv);
+        free(lsv);
+    }
+    double* w = (double*)malloc(sizeof(double) * P);
+    double* cov = (double*)malloc(sizeof(double) * nd);
+    int* assigned = (int*)malloc(sizeof(int) * P);
+    int* id_start = (int*)malloc(sizeof(int) * P);
+    double* Xn = (double*)malloc(sizeof(double) * (size_t)P * nd);
+    double* Ln = (double*)malloc(sizeof(double) * P);
+    double* PRn = (double*)malloc(sizeof(double) * P);
+    hmc_tune_t T;
+    T.log_dtau_upper = 0.0; T.log_dtau_lower = -1e1; T.ntau_lower = 1; T.ntau_upper = 100;
+    T.dtau_ls = (double*)malloc(sizeof(double) * P);
+    T.ntau_ls = (int*)malloc(sizeof(int) * P);
+    for (int k = 0; k < P; ++k) { T.dtau_ls[k] = exp(-5.0); T.ntau_ls[k] = 50; }
+    int iter = 1;
+    double gamma = 0.0, neglog = 0.0;
+    while (1.0 - gamma > 10e-8) {
+        double ev;
+        gamma = orc_find_next_gamma(gamma, L, P, w, &ev, 2.0, 1e-10, 100000);
+        neglog = neglog + ev;
+        if (res && res->gamma_trace && iter <= 201) res->gamma_trace[iter - 1] = gamma;
+        if (res && res->evid_trace && iter <= 201) res->evid_trace[iter - 1] = ev;
+        if (iter > 200) { neglog = 1e10; break; }
+        orc_slip_normalize_weights(w, P);
+        orc_slip_calc_mean_particles(X, w, mean, P, nd);
+        orc_slip_calc_cov_particles(X, w, mean, cov, P, nd);
+        double uh, ul;
+        draw_quad(key, SITE_RESAMPLE, iter, &uh, &ul);
+        orc_resample_quad(P, w, uh, ul, assigned);
+        id_start[0] = 0;
+        for (int k = 0; k < P - 1; ++k) id_start[k + 1] = id_start[k] + assigned[k];
+        if (I->pb->mover == 1) {
+            gibbs_move(I, key, iter, gamma, I->pb->gibbs_sweeps, X, Xn, Ln, PRn, assigned, id_start);
+        } else {
+            hmc_tuning(I, key, iter, gamma, X, L, PR, cov, assigned, id_start, &T);
+            hmc_sampling(I, key, iter, gamma, X, L, PR, Xn, Ln, PRn, cov, assigned, id_start, T.dtau_ls);
+        }
+        memcpy(X, Xn, sizeof(double) * (size_t)P * nd);
+        memcpy(L, Ln, sizeof(double) * P);
+        memcpy(PR, PRn, sizeof(double) * P);
+        iter++;
+    }
+    if (res) { res->nstage = iter - 1; res->neglog = neglog; }
+    free(w); free(cov); free(assigned); free(id_start); free(Xn); free(Ln); free(PRn);
+    free(T.dtau_ls); free(T.ntau_ls);
+}
+
+void orc_slip_smc_exec(const orc_problem* pb, const double* theta, const double* gmat,
+                       uint64_t key, double* particles, double* likelihood, double* prior,
+                       double* mean, orc_slip_result* res) {
+    inst_t I;
+    inst_build(&I, pb, theta, gmat);
+    smc_run(&I, key, particles, likelihood, prior, mean, res);
+    inst_free(&I);
+}
+
+/* fault_calc_likelihood, smc_fault.f90:66-292 */
+double orc_fault_calc_likelihood(const orc_problem* pb, const double* theta, uint64_t key,
+                                 orc_slip_result* res) {
+    int nnode, ndof, npatch;
+    orc_mesh_sizes(&pb->fault, &nnode, &ndof, &npatch, NULL);
+    int nd = 2 * ndof, P = pb->nparticle_slip;
+    double* g = (double*)malloc(sizeof(double) * (size_t)pb->nobs * nd);
+    orc_calc_greens_func(&pb->fault, theta, pb->nobs, pb->nsar_total, pb->ngnss, pb->obs_points,
+                         pb->obs_unitvec, g);
+    double* X = (double*)malloc(sizeof(double) * (size_t)P * nd);
+    double* L = (double*)malloc(sizeof(double) * P);
+    double* PR = (double*)malloc(sizeof(double) * P);
+    double* mean = (double*)malloc(sizeof(double) * nd);
+    orc_slip_result r = {0};
+    if (res) r = *res;
+    orc_slip_smc_exec(pb, theta, g, key, X, L, PR, mean, &r);
+    if (res) *res = r;
+    free(g); free(X); free(L); free(PR); free(mean);
+    return r.neglog;
+}
+
+void orc_fault_calc_likelihood_batch(const orc_problem* pb, const double* theta, int ndim_fault,
+                                     int B, const uint64_t* keys, double* out) {
+    for (int b = 0; b < B; ++b)
+        out[b] = orc_fault_calc_likelihood(pb, theta + (size_t)b * ndim_fault, keys[b], NULL);
+}
