@@ -1,0 +1,136 @@
+/*
+ * libsmcf: MI355X implementation of the nested SMC fault-inversion hot path.
+ *
+ * C ABI (plain pointers and sizes).  Arrays follow the reference's Fortran
+ * conventions: column-major, 1-based index VALUES inside integer arrays, FP64.
+ * Every entry point returns SMCF_OK (0) or an error code; smcf_last_error()
+ * gives the message.  The reference has no error codes; its numeric sentinels
+ * are reproduced in the outputs (-log Z = 1e10 after more than 200 inner stages,
+ * smc_slip.f90:1557-1560).  All host buffers are caller-allocated, as in the
+ * reference (main.f90:349-390); device state lives in the context/problem.
+ *
+ * A context owns one HIP stream on one device and is not thread-safe; use one
+ * context per host thread.
+ */
+#ifndef SMCF_H
+#define SMCF_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMCF_OK 0
+#define SMCF_EINVAL 1
+#define SMCF_EHIP 2
+#define SMCF_ENOMEM 3
+#define SMCF_EUNSUPPORTED 4
+#define SMCF_ESTATE 5
+
+/* inner movers */
+#define SMCF_MOVER_HMC 0   /* reference HMC: ntau 50 + ESJD dtau tuning (smc_slip.f90:591-1214) */
+#define SMCF_MOVER_GIBBS 1 /* truncated-MVN Gibbs sweeps (benchmark configuration) */
+
+typedef struct smcf_ctx smcf_ctx;
+typedef struct smcf_problem smcf_problem;
+
+const char* smcf_last_error(void);
+int smcf_ctx_create(int device, smcf_ctx** out);
+int smcf_ctx_destroy(smcf_ctx* ctx);
+/* kernel timing with HIP events on the context's stream (off by default) */
+int smcf_set_profiling(smcf_ctx* ctx, int on);
+/* cumulative device time of kernels named `name` since profiling was enabled */
+int smcf_kernel_stats(smcf_ctx* ctx, const char* name, long* count, double* total_ms);
+
+/*
+ * Observation + fault-mesh + sampler settings.  Mirrors what main.f90 reads and
+ * broadcasts (main.f90:107-307) and the fixed workspace sizes of main.f90:349-390.
+ * Rows 1..nsar_total are InSAR, the rest GNSS triplets (init.f90:24-80).
+ */
+typedef struct {
+  int nplane;
+  const int* nxi_ls;          /* nplane */
+  const int* neta_ls;         /* nplane */
+  double xmin, xmax, zmin;
+  int fix_xbend;
+  const double* xbend;        /* nplane-1, may be NULL when nplane == 1 */
+  int nobs, nsar_total, ngnss, npath;
+  const int* nsar_index;      /* npath+1, 1-based */
+  const double* obs_points;   /* 2 x nobs */
+  const double* obs_unitvec;  /* 3 x nobs */
+  const double* obs_sigma;    /* nobs */
+  const double* dvec;         /* nobs */
+  const double* sar_sinv;     /* inverse SAR covariance blocks; only nsar_total == 0 is supported */
+  double max_slip;
+  int nparticle_slip;
+  int mover;                  /* SMCF_MOVER_* */
+  int gibbs_sweeps;
+} smcf_problem_desc;
+
+/* Upload a problem; device buffers are sized for up to max_batch geometries in flight. */
+int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* desc, int max_batch, smcf_problem** out);
+int smcf_problem_destroy(smcf_problem* pb);
+int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_slip, int* ndim_fault,
+                      int* nobs);
+
+/* DC3D0 for n points (replaces DC3Dfortran.f90:2-118).  in: 10 x n
+ * (alpha, x, y, z, depth, dip, pot1..pot4); u: 12 x n; iret: n (left at its input
+ * value when depth < 0, as the reference does). */
+int smcf_dc3d0(smcf_ctx* ctx, int n, const double* in, double* u, int* iret);
+
+/* Green's matrices for B geometries (replaces calc_greens_func, gfunc.f90:242-348).
+ * theta: ndim_fault x B; gmat: nobs x ndim_slip x B. */
+int smcf_calc_greens_func(smcf_problem* pb, int B, const double* theta, double* gmat);
+
+/* gen_laplacian + gen_sparse_lmat + calc_ll for one theta (init.f90:295-541).
+ * Any output pointer may be NULL.  lmat_index/val: 9 x 2nnode; ltmat: 9 x 2ndof;
+ * llmat: ndim x ndim. */
+int smcf_laplacian(smcf_problem* pb, const double* theta, int* lmat_index, double* lmat_val,
+                   int* ltmat_index, double* ltmat_val, double* llmat);
+
+/* Gram quantities for B geometries (replaces slip_calc_gsd_gsg, smc_slip.f90:1347-1443).
+ * gsgmat: ndim x ndim x B; gsdvec: ndim x B; dsd: B.  NULL outputs are skipped. */
+int smcf_slip_calc_gsd_gsg(smcf_problem* pb, int B, const double* theta, double* gsgmat, double* gsdvec,
+                           double* dsd);
+
+/* Full inner SMC for one theta (replaces slip_smc_exec, smc_slip.f90:1445-1663, called
+ * from fault_calc_likelihood, smc_fault.f90:66-292).  Outputs may be NULL.
+ * particles: ndim x P (column-major); gamma_trace: >= 202 entries. */
+int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, double* particles,
+                       double* likelihood, double* prior, double* mean, int* nstage, double* neglog,
+                       double* gamma_trace);
+
+/* Batched fault_calc_likelihood (smc_fault.f90:66-292): -log Z for B thetas, each with its own
+ * RNG key.  This is what work_eval_init_particles / work_mcmc_sampling waves call. */
+int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys,
+                                     double* neglog, int* nstage);
+
+/* Inner-SMC components on the device (for parity checks and reuse). */
+/* slip_find_next_gamma / fault_find_next_gamma (smc_slip.f90:314-387, smc_fault.f90:392-485) */
+int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gamma_prev, const double* likelihood, double cv,
+                         double tol, int cap, double* gamma, double* weights, double* neglog_evidence);
+/* slip_normalize_weights + slip_resample_particles (smc_slip.f90:465-483, 572-589);
+ * u_hi + u_lo is the unclamped real(16) uniform. */
+int smcf_normalize_resample(smcf_ctx* ctx, int P, const double* weights, double u_hi, double u_lo,
+                            double* weights_norm, int* assigned_num);
+/* truncated-normal inversion of the Gibbs mover (no reference counterpart) */
+int smcf_trunc_normal(smcf_ctx* ctx, int n, const double* mu, const double* sigma, double lo, double hi,
+                      const double* u, double* out);
+/* raw Philox4x32-10 blocks: ctr 4 x n, key 2 x n, out 4 x n */
+int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+
+/* Outer SMC (replaces fault_smc_exec, smc_fault.f90:785-1075, with numprocs = 1 semantics).
+ * range: 2 x ndim_fault (range.dat layout); particles: ndim_fault x N out; likelihood: N out;
+ * gamma_trace: >= max_stages entries (may be NULL).  Inner evaluations run as device batches of
+ * at most max_batch (<= the problem's max_batch) geometries. */
+int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
+                        double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace);
+/* RNG key of the inner SMC of one outer evaluation: (seed, outer stage, role 0 parent / 1 child, slot) */
+uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint32_t slot);
+
+/* sort.cpp:3-7 - 1-based index sorted by descending arr(index) (std::sort, same tie order). */
+void qsort_(int* arr, int* index, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -297,19 +297,39 @@ double orc_ndtri(double p) {
     return q < 0 ? -z : z;
 }
 
+/* log Q(x) (standard normal survival) and Mills ratio Q/phi for x >= 35 from the
+ * asymptotic series of the Mills ratio, R(x) = (1/x)(1 - x^-2 + 3x^-4 - 15x^-6 + ...) */
+static double mills_series(double x) {
+    double r = 1.0 / (x * x);
+    return 1.0 + r * (-1.0 + r * (3.0 + r * (-15.0 + r * (105.0 + r * (-945.0 + r * (10395.0 + r * -135135.0))))));
+}
+static double logq_far(double x) {
+    return -0.5 * x * x - log(x) - 0.91893853320467274178 + log(mills_series(x));
+}
+
+/* upper-tail truncation beyond 35 sigma: solve log Q(z) = log Q(a) + log(1 - u (1 - Q(b)/Q(a)))
+ * by Newton (d/dz log Q = -1/R) from the exponential-tail start */
+static double tn_far(double a, double b, double u) {
+    double la = logq_far(a), lb = logq_far(b);
+    double T = la + log1p(-u * (-expm1(lb - la)));
+    double z = a - log1p(-u * (-expm1(-a * (b - a)))) / a;
+    for (int it = 0; it < 4; ++it) {
+        double g = logq_far(z) - T;
+        z = z + g * (mills_series(z) / z);
+    }
+    return fmin(fmax(z, a), b);
+}
+
 /* standard normal truncated to [a, b], by inversion with tail-aware probabilities */
 static double tn_std(double a, double b, double u) {
     if (a > 0.0) {
-        if (a > 35.0) {
-            double z = a - log1p(-u * (-expm1(-a * (b - a)))) / a;
-            return fmin(z, b);
-        }
+        if (a > 35.0) return tn_far(a, b, u);
         double qa = 0.5 * erfc(a * M_SQRT1_2), qb = 0.5 * erfc(b * M_SQRT1_2);
         double p = qa - u * (qa - qb);
         double z = -orc_ndtri(p);
         return fmin(fmax(z, a), b);
     }
-    if (b < 0.0) return -tn_std(-b, -a, u);
+    if (b < 0.0) return -tn_std(-b, -a, 1.0 - u);  /* mirror: quantile u of z is 1-u of -z */
     double pa = 0.5 * erfc(-a * M_SQRT1_2), pb = 0.5 * erfc(-b * M_SQRT1_2);
     double p = pa + u * (pb - pa), z;
     if (p <= 0.5) {
@@ -508,10 +528,11 @@ static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma,
 
 /* ---------------- Gibbs-sweep mover (benchmark configuration; no reference counterpart) --------
  * Target at stage gamma: exp(-gamma*likelihood(s) - prior_quadratic(s)) on [0, max_slip]^ndim,
- * with precision Q = gamma*H + LL/alpha^2 and linear term gamma*b.  Each child j starts from
- * its parent and runs `sweeps` full systematic-scan sweeps (i = 1..ndim); coordinate i is
- * redrawn from N(mu_i, 1/Q_ii) truncated to [0, max_slip] with
- *   mu_i = s_i - ((Q s)_i - gamma*b_i) / Q_ii
+ * with precision Q = gamma*H + LL/alpha^2 (formed entrywise as gamma*H_ki + LL_ki/alpha^2) and
+ * linear term gamma*b.  Each child j starts from its parent and runs `sweeps` full
+ * systematic-scan sweeps (i = 1..ndim); coordinate i is redrawn from N(mu_i, 1/Q_ii)
+ * truncated to [0, max_slip] with
+ *   mu_i = s_i - (sum_k Q_ki s_k - gamma*b_i) / Q_ii       (column i of Q)
  * using the open uniform u = ((w>>11) + 1/2) 2^-53 of draw (site 9, stage, c2=j, c1=i, c0=sweep).
  */
 static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, int sweeps,
@@ -532,8 +553,14 @@ static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, i
             memcpy(s, X + (size_t)i * nd, sizeof(double) * nd);
             for (int sw = 0; sw < sweeps; ++sw)
                 for (int c = 0; c < nd; ++c) {
-                    double qs = 0.0;
-                    for (int k = 0; k < nd; ++k) qs += Q[(size_t)k * nd + c] * s[k];
+                    /* sum_k Q(k, c) s_k with 8 partial sums (the mover fixes no order) */
+                    const double* qc = Q + (size_t)c * nd;
+                    double ps[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    int k = 0;
+                    for (; k + 8 <= nd; k += 8)
+                        for (int r = 0; r < 8; ++r) ps[r] += qc[k + r] * s[k + r];
+                    for (; k < nd; ++k) ps[k & 7] += qc[k] * s[k];
+                    double qs = ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
                     double prec = Q[(size_t)c * nd + c];
                     double mu = s[c] - (qs - hb[c]) / prec;
                     uint64_t w[2];

@@ -1,0 +1,318 @@
+"""MI355X implementation of the nested SMC fault inversion hot path (libsmcf).
+
+Python host mirror of the reference's Fortran procedures over the C ABI of
+libsmcf.so (include/smcf.h).  Names follow the reference:
+
+    Problem.calc_greens_func          gfunc.f90:242-348
+    Problem.laplacian                 init.f90:295-541 (gen_laplacian, gen_sparse_lmat, calc_ll)
+    Problem.slip_calc_gsd_gsg         smc_slip.f90:1347-1443
+    Problem.slip_smc_exec             smc_slip.f90:1445-1663
+    Problem.fault_calc_likelihood     smc_fault.f90:66-292 (batched over thetas)
+    Context.dc3d0                     DC3Dfortran.f90:2-118
+    Context.find_next_gamma           smc_slip.f90:314-387 / smc_fault.f90:392-485
+    Context.normalize_resample        smc_slip.f90:465-483, 572-589
+    qsort                             sort.cpp:3-7
+
+Arrays use the reference layout (Fortran order, 1-based index values).  There is no
+CPU fallback: a missing library or device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsmcf.so")
+_LIB = None
+
+MOVER_HMC = 0
+MOVER_GIBBS = 1
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U64 = C.POINTER(C.c_uint64)
+_U32 = C.POINTER(C.c_uint32)
+
+
+class SmcfError(RuntimeError):
+    pass
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [("nplane", C.c_int), ("nxi_ls", _I), ("neta_ls", _I), ("xmin", C.c_double),
+                ("xmax", C.c_double), ("zmin", C.c_double), ("fix_xbend", C.c_int), ("xbend", _D),
+                ("nobs", C.c_int), ("nsar_total", C.c_int), ("ngnss", C.c_int), ("npath", C.c_int),
+                ("nsar_index", _I), ("obs_points", _D), ("obs_unitvec", _D), ("obs_sigma", _D),
+                ("dvec", _D), ("sar_sinv", _D), ("max_slip", C.c_double), ("nparticle_slip", C.c_int),
+                ("mover", C.c_int), ("gibbs_sweeps", C.c_int)]
+
+
+EXPORTS = [
+    "smcf_last_error", "smcf_ctx_create", "smcf_ctx_destroy", "smcf_set_profiling", "smcf_kernel_stats",
+    "smcf_problem_create", "smcf_problem_destroy", "smcf_problem_dims", "smcf_dc3d0",
+    "smcf_calc_greens_func", "smcf_laplacian", "smcf_slip_calc_gsd_gsg", "smcf_slip_smc_exec",
+    "smcf_fault_calc_likelihood_batch", "smcf_find_next_gamma", "smcf_normalize_resample",
+    "smcf_trunc_normal", "smcf_philox", "smcf_fault_smc_exec", "smcf_inner_key", "qsort_",
+]
+
+
+def lib():
+    """Load libsmcf.so (built in-tree by __graft_entry__.build() / make)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise SmcfError("libsmcf.so is not built: run `make -C smc_inversion_fort_amd`")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _LIB = L
+    return _LIB
+
+
+def _declare(L):
+    L.smcf_last_error.restype = C.c_char_p
+    L.smcf_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    L.smcf_ctx_destroy.argtypes = [C.c_void_p]
+    L.smcf_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    L.smcf_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_double)]
+    L.smcf_problem_create.argtypes = [C.c_void_p, C.POINTER(ProblemDesc), C.c_int, C.POINTER(C.c_void_p)]
+    L.smcf_problem_destroy.argtypes = [C.c_void_p]
+    L.smcf_problem_dims.argtypes = [C.c_void_p, _I, _I, _I, _I, _I]
+    L.smcf_dc3d0.argtypes = [C.c_void_p, C.c_int, _D, _D, _I]
+    L.smcf_calc_greens_func.argtypes = [C.c_void_p, C.c_int, _D, _D]
+    L.smcf_laplacian.argtypes = [C.c_void_p, _D, _I, _D, _I, _D, _D]
+    L.smcf_slip_calc_gsd_gsg.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D]
+    L.smcf_slip_smc_exec.argtypes = [C.c_void_p, _D, C.c_uint64, _D, _D, _D, _D, _I, _D, _D]
+    L.smcf_fault_calc_likelihood_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _I]
+    L.smcf_find_next_gamma.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, C.c_double, C.c_double, C.c_int,
+                                       _D, _D, _D]
+    L.smcf_normalize_resample.argtypes = [C.c_void_p, C.c_int, _D, C.c_double, C.c_double, _D, _I]
+    L.smcf_trunc_normal.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
+    L.smcf_philox.argtypes = [C.c_void_p, C.c_int, _U32, _U32, _U32]
+    L.smcf_fault_smc_exec.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _I, _D, _D]
+    L.smcf_inner_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.smcf_inner_key.restype = C.c_uint64
+    L.qsort_.argtypes = [_I, _I, _I]
+
+
+def _check(rc):
+    if rc != 0:
+        raise SmcfError("libsmcf error %d: %s" % (rc, lib().smcf_last_error().decode()))
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def inner_key(seed, outer_stage, role, slot):
+    """RNG key of the inner SMC of one outer evaluation."""
+    return lib().smcf_inner_key(seed, outer_stage, role, slot)
+
+
+def qsort(arr, index):
+    """sort.cpp:3-7: sort the 1-based `index` in place by descending arr(index)."""
+    arr = _i32(arr)
+    index = np.ascontiguousarray(index, dtype=np.int32)
+    n = C.c_int(index.shape[0])
+    lib().qsort_(_ip(arr), _ip(index), C.byref(n))
+    return index
+
+
+class Context:
+    """One device and one HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().smcf_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().smcf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on=True):
+        _check(lib().smcf_set_profiling(self.h, int(on)))
+
+    def kernel_stats(self, name):
+        n = C.c_long()
+        t = C.c_double()
+        _check(lib().smcf_kernel_stats(self.h, name.encode(), C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def dc3d0(self, inputs):
+        """inputs: (n, 10) rows of (alpha, x, y, z, depth, dip, pot1..pot4) -> (u (n,12), iret (n,))"""
+        a = np.ascontiguousarray(inputs, dtype=np.float64)
+        n = a.shape[0]
+        u = np.zeros((n, 12))
+        iret = np.full(n, -99, dtype=np.int32)
+        _check(lib().smcf_dc3d0(self.h, n, _dp(a), _dp(u), _ip(iret)))
+        return u, iret
+
+    def find_next_gamma(self, gamma_prev, likelihood, cv=2.0, tol=1e-10, cap=100000):
+        L = np.ascontiguousarray(likelihood, dtype=np.float64)
+        w = np.zeros_like(L)
+        g = C.c_double()
+        ev = C.c_double()
+        _check(lib().smcf_find_next_gamma(self.h, L.shape[0], gamma_prev, _dp(L), cv, tol, cap, C.byref(g),
+                                          _dp(w), C.byref(ev)))
+        return g.value, w, ev.value
+
+    def normalize_resample(self, weights, u_hi, u_lo):
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        wn = np.zeros_like(w)
+        a = np.zeros(w.shape[0], dtype=np.int32)
+        _check(lib().smcf_normalize_resample(self.h, w.shape[0], _dp(w), u_hi, u_lo, _dp(wn), _ip(a)))
+        return wn, a
+
+    def trunc_normal(self, mu, sigma, lo, hi, u):
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros_like(mu)
+        _check(lib().smcf_trunc_normal(self.h, mu.shape[0], _dp(mu), _dp(sg), lo, hi, _dp(u), _dp(out)))
+        return out
+
+    def philox(self, ctr, key):
+        c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+        k = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
+        out = np.zeros_like(c)
+        _check(lib().smcf_philox(self.h, c.shape[0], c.ctypes.data_as(_U32), k.ctypes.data_as(_U32),
+                                 out.ctypes.data_as(_U32)))
+        return out
+
+
+class Problem:
+    """Observations + fault mesh + inner-SMC settings resident on the device."""
+
+    def __init__(self, ctx, nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip,
+                 mover=MOVER_GIBBS, gibbs_sweeps=5, max_batch=1, fix_xbend=False, xbend=()):
+        self.ctx = ctx
+        self._keep = dict(
+            nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
+            nsar_index=_i32(obs.get("nsar_index", [1, 1 + int(obs.get("nsar_total", 0))])),
+            obs_points=_f64(obs["obs_points"]), obs_unitvec=_f64(obs["obs_unitvec"]),
+            obs_sigma=_f64(obs["obs_sigma"]), dvec=_f64(obs["dvec"]), sar=_f64(np.zeros(1)))
+        k = self._keep
+        nobs = k["dvec"].shape[0]
+        self.desc = ProblemDesc(len(k["nxi"]), _ip(k["nxi"]), _ip(k["neta"]), xmin, xmax, zmin, int(bool(fix_xbend)),
+                                _dp(k["xbend"]), nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
+                                len(k["nsar_index"]) - 1, _ip(k["nsar_index"]), _dp(k["obs_points"]),
+                                _dp(k["obs_unitvec"]), _dp(k["obs_sigma"]), _dp(k["dvec"]), _dp(k["sar"]),
+                                float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps))
+        h = C.c_void_p()
+        _check(lib().smcf_problem_create(ctx.h, C.byref(self.desc), int(max_batch), C.byref(h)))
+        self.h = h
+        dims = [C.c_int() for _ in range(5)]
+        _check(lib().smcf_problem_dims(h, *[C.byref(x) for x in dims]))
+        self.nnode, self.ndof, self.ndim, self.ndim_fault, self.nobs = [x.value for x in dims]
+        self.P = int(nparticle_slip)
+        self.max_batch = int(max_batch)
+
+    def close(self):
+        if self.h:
+            lib().smcf_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _theta(self, theta):
+        th = np.asfortranarray(np.asarray(theta, dtype=np.float64))
+        if th.ndim == 1:
+            th = th.reshape(-1, 1, order="F")
+        if th.shape[0] != self.ndim_fault:
+            raise ValueError("theta must have %d rows" % self.ndim_fault)
+        return th
+
+    def calc_greens_func(self, theta):
+        th = self._theta(theta)
+        B = th.shape[1]
+        g = np.zeros((self.nobs, self.ndim, B), order="F")
+        _check(lib().smcf_calc_greens_func(self.h, B, _dp(th), _dp(g)))
+        return g[:, :, 0] if B == 1 else g
+
+    def laplacian(self, theta):
+        th = self._theta(theta)
+        li = np.zeros((9, 2 * self.nnode), np.int32, order="F")
+        lv = np.zeros((9, 2 * self.nnode), order="F")
+        lti = np.zeros((9, 2 * self.ndof), np.int32, order="F")
+        ltv = np.zeros((9, 2 * self.ndof), order="F")
+        ll = np.zeros((self.ndim, self.ndim), order="F")
+        _check(lib().smcf_laplacian(self.h, _dp(th), _ip(li), _dp(lv), _ip(lti), _dp(ltv), _dp(ll)))
+        return dict(li=li, lv=lv, lti=lti, ltv=ltv, ll=ll)
+
+    def slip_calc_gsd_gsg(self, theta):
+        th = self._theta(theta)
+        B = th.shape[1]
+        H = np.zeros((self.ndim, self.ndim, B), order="F")
+        b = np.zeros((self.ndim, B), order="F")
+        dsd = np.zeros(B)
+        _check(lib().smcf_slip_calc_gsd_gsg(self.h, B, _dp(th), _dp(H), _dp(b), _dp(dsd)))
+        if B == 1:
+            return H[:, :, 0], b[:, 0], float(dsd[0])
+        return H, b, dsd
+
+    def slip_smc_exec(self, theta, key):
+        th = self._theta(theta)
+        X = np.zeros((self.ndim, self.P), order="F")
+        L = np.zeros(self.P)
+        PR = np.zeros(self.P)
+        mean = np.zeros(self.ndim)
+        ns = C.c_int()
+        nl = C.c_double()
+        gt = np.zeros(202)
+        _check(lib().smcf_slip_smc_exec(self.h, _dp(th), int(key), _dp(X), _dp(L), _dp(PR), _dp(mean),
+                                        C.byref(ns), C.byref(nl), _dp(gt)))
+        return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=ns.value, neglog=nl.value,
+                    gamma=gt[:ns.value].copy())
+
+    def fault_smc_exec(self, range_, N, seed, max_stages=100):
+        """Outer SMC over the geometry (smc_fault.f90:785-1075, one rank)."""
+        r = _f64(range_)
+        X = np.zeros((self.ndim_fault, N), order="F")
+        L = np.zeros(N)
+        ns = C.c_int()
+        ev = C.c_double()
+        gt = np.zeros(max_stages + 1)
+        _check(lib().smcf_fault_smc_exec(self.h, _dp(r), int(N), int(seed), int(max_stages), self.max_batch,
+                                         _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
+        return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy())
+
+    def fault_calc_likelihood(self, thetas, keys):
+        """-log Z for each column of `thetas` (ndim_fault x B), RNG key per column."""
+        th = self._theta(thetas)
+        B = th.shape[1]
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros(B)
+        ns = np.zeros(B, dtype=np.int32)
+        for s in range(0, B, self.max_batch):
+            e = min(B, s + self.max_batch)
+            sub = np.asfortranarray(th[:, s:e])
+            o = np.zeros(e - s)
+            n = np.zeros(e - s, dtype=np.int32)
+            _check(lib().smcf_fault_calc_likelihood_batch(self.h, e - s, _dp(sub), k[s:e].ctypes.data_as(_U64),
+                                                          _dp(o), _ip(n)))
+            out[s:e] = o
+            ns[s:e] = n
+        return out, ns

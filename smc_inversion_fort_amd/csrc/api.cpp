@@ -1,0 +1,613 @@
+// C ABI of libsmcf: contexts, problems, the batched inner-SMC driver and component entry
+// points.  The stage loop mirrors slip_smc_exec (smc_slip.f90:1445-1663) for many geometries
+// at once: every kernel of a stage runs over all instances, instances that have reached
+// gamma = 1 (or the 200-stage cap) are masked, and the host stops when all are done.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "smcf_internal.h"
+
+namespace smcf {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace smcf
+
+using namespace smcf;
+
+static constexpr int kMaxInnerParticles = 3584;
+
+struct smcf_problem {
+  Problem p;
+};
+
+#define CHECK_LAUNCH(expr)                                                           \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+      return SMCF_EHIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+namespace {
+
+struct ProfScope {
+  smcf_ctx* ctx;
+  const char* name;
+  hipEvent_t a{}, b{};
+  ProfScope(smcf_ctx* c, const char* n) : ctx(c), name(n) {
+    if (ctx->profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~ProfScope() {
+    if (ctx->profiling) {
+      (void)hipEventRecord(b, ctx->stream);
+      ctx->pending.push_back({name, {a, b}});
+    }
+  }
+};
+
+void resolve_profiles(smcf_ctx* ctx) {
+  for (auto& pe : ctx->pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(pe.second.second);
+    (void)hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+    auto it = std::find_if(ctx->stats.begin(), ctx->stats.end(),
+                           [&](const smcf_ctx::Stat& s) { return s.name == pe.first; });
+    if (it == ctx->stats.end()) ctx->stats.push_back({pe.first, 1, (double)ms});
+    else { it->count++; it->total_ms += ms; }
+  }
+  ctx->pending.clear();
+}
+
+template <class T>
+int dalloc(Problem& P, T** ptr, size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " bytes): " + hipGetErrorString(e));
+    return SMCF_ENOMEM;
+  }
+  P.allocs.push_back(p);
+  *ptr = (T*)p;
+  return SMCF_OK;
+}
+
+template <class T>
+int upload(Problem& P, const T** dst, const std::vector<T>& v) {
+  T* d = nullptr;
+  int rc = dalloc(P, &d, v.size());
+  if (rc) return rc;
+  if (!v.empty()) SMCF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *dst = d;
+  return SMCF_OK;
+}
+
+__global__ void k_state_init(BatchDev bt, int B) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  bt.gamma[b] = 0.0;
+  bt.neglog[b] = 0.0;
+  bt.iter[b] = 1;
+  bt.active[b] = 0;
+  bt.done[b] = 0;
+  bt.par[b] = 0;
+}
+
+int run_batch(Problem& P, int B, const double* theta, const uint64_t* keys, bool want_moments) {
+  smcf_ctx* ctx = P.ctx;
+  hipStream_t s = ctx->stream;
+  MeshDev& m = P.m;
+  BatchDev& bt = P.bt;
+  if (B < 1 || B > bt.cap) {
+    set_error("batch size " + std::to_string(B) + " outside [1, " + std::to_string(bt.cap) + "]");
+    return SMCF_EINVAL;
+  }
+  if (m.nsar > 0) {
+    set_error("InSAR rows (dense per-path covariance) are not supported by the device Gram yet");
+    return SMCF_EUNSUPPORTED;
+  }
+  if (P.mover != SMCF_MOVER_GIBBS) {
+    set_error("only the Gibbs mover is available on the device in this build");
+    return SMCF_EUNSUPPORTED;
+  }
+  SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * m.ndim_fault * B, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_state_init, dim3((B + 63) / 64), dim3(64), 0, s, bt, B);
+  {
+    ProfScope ps(ctx, "geometry_greens");
+    CHECK_LAUNCH(launch_geometry(m, bt, B, s));
+    CHECK_LAUNCH(launch_greens(m, bt, B, s));
+  }
+  CHECK_LAUNCH(launch_laplacian(m, bt, B, s));
+  {
+    ProfScope ps(ctx, "gram");
+    CHECK_LAUNCH(launch_gram(m, bt, B, s));
+  }
+  {
+    ProfScope ps(ctx, "init_chain");
+    CHECK_LAUNCH(launch_init_chain(m, bt, B, P.max_slip, s));
+  }
+  CHECK_LAUNCH(launch_likelihood(m, bt, B, 0, s));
+  std::vector<int> done(B);
+  StageCfg cfg{P.max_slip, bt.P, P.sweeps};
+  for (int stage = 1; stage <= 202; ++stage) {
+    {
+      ProfScope ps(ctx, "gamma_resample");
+      CHECK_LAUNCH(launch_gamma_resample(m, bt, B, s));
+    }
+    if (want_moments) CHECK_LAUNCH(launch_moments(m, bt, B, s));
+    CHECK_LAUNCH(launch_gather(m, bt, B, s));
+    CHECK_LAUNCH(launch_form_q(m, bt, B, s));
+    {
+      ProfScope ps(ctx, "gibbs");
+      CHECK_LAUNCH(launch_gibbs(m, bt, B, cfg, s));
+    }
+    {
+      ProfScope ps(ctx, "likelihood");
+      CHECK_LAUNCH(launch_likelihood(m, bt, B, 1, s));
+    }
+    SMCF_HIP(hipMemcpyAsync(done.data(), bt.done, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipStreamSynchronize(s));
+    if (std::all_of(done.begin(), done.end(), [](int d) { return d != 0; })) break;
+  }
+  SMCF_HIP(hipStreamSynchronize(s));
+  resolve_profiles(ctx);
+  return SMCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* smcf_last_error(void) { return g_err.c_str(); }
+
+int smcf_ctx_create(int device, smcf_ctx** out) {
+  if (!out) return SMCF_EINVAL;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    set_error("no HIP device available");
+    return SMCF_EHIP;
+  }
+  if (device < 0 || device >= n) {
+    set_error("device index out of range");
+    return SMCF_EINVAL;
+  }
+  SMCF_HIP(hipSetDevice(device));
+  auto* c = new smcf_ctx();
+  c->device = device;
+  c->profiling = false;
+  hipError_t es = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (es != hipSuccess) {
+    delete c;
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(es));
+    return SMCF_EHIP;
+  }
+  *out = c;
+  return SMCF_OK;
+}
+
+int smcf_ctx_destroy(smcf_ctx* ctx) {
+  if (!ctx) return SMCF_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  resolve_profiles(ctx);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SMCF_OK;
+}
+
+int smcf_set_profiling(smcf_ctx* ctx, int on) {
+  if (!ctx) return SMCF_EINVAL;
+  ctx->profiling = on != 0;
+  if (on) ctx->stats.clear();
+  return SMCF_OK;
+}
+
+int smcf_kernel_stats(smcf_ctx* ctx, const char* name, long* count, double* total_ms) {
+  if (!ctx || !name) return SMCF_EINVAL;
+  (void)hipStreamSynchronize(ctx->stream);
+  resolve_profiles(ctx);
+  *count = 0;
+  *total_ms = 0.0;
+  for (auto& s : ctx->stats)
+    if (s.name == name) { *count = s.count; *total_ms = s.total_ms; }
+  return SMCF_OK;
+}
+
+int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch, smcf_problem** out) {
+  if (!ctx || !d || !out || max_batch < 1 || d->nplane < 1 || d->nparticle_slip < 1) {
+    set_error("invalid problem description");
+    return SMCF_EINVAL;
+  }
+  if (d->nparticle_slip > kMaxInnerParticles) {
+    set_error("nparticle_slip > " + std::to_string(kMaxInnerParticles) +
+              ": the tempering workgroup keeps all weights in 64 KiB of LDS");
+    return SMCF_EINVAL;
+  }
+  SMCF_HIP(hipSetDevice(ctx->device));
+  auto holder = std::make_unique<smcf_problem>();
+  Problem& P = holder->p;
+  P.ctx = ctx;
+  P.max_slip = d->max_slip;
+  P.P = d->nparticle_slip;
+  P.mover = d->mover;
+  P.sweeps = d->gibbs_sweeps > 0 ? d->gibbs_sweeps : 5;
+  MeshDev& m = P.m;
+  memset(&m, 0, sizeof(m));
+  int np = d->nplane;
+  m.nplane = np;
+  m.fix_xbend = d->fix_xbend;
+  m.xmin = d->xmin; m.xmax = d->xmax; m.zmin = d->zmin;
+  m.ndim_fault = d->fix_xbend ? 2 * np + 4 : 3 * np + 3;
+  // mesh topology (discretize_fault, init.f90:198-293; sizes main.f90:217-238)
+  std::vector<int> pnxi(np), pneta(np), pnode0(np), cny, patch_plane, node_plane, node_k, node_l, id_dof;
+  int nnode = 0, npatch = 0;
+  for (int ip = 0; ip < np; ++ip) {
+    pnxi[ip] = d->nxi_ls[ip];
+    pneta[ip] = d->neta_ls[ip];
+    if (pnxi[ip] < 1 || pneta[ip] < 1) { set_error("nxi/neta must be >= 1"); return SMCF_EINVAL; }
+    pnode0[ip] = nnode;
+    int W = pnxi[ip] + 1;
+    for (int j = 1; j <= pneta[ip] + 1; ++j)
+      for (int i = 1; i <= W; ++i) {
+        node_plane.push_back(ip);
+        node_k.push_back(i);
+        node_l.push_back(j);
+        bool dof = !(j == 1 || (ip == 0 && i == 1) || (ip == np - 1 && i == W));
+        if (dof) id_dof.push_back(nnode + i + W * (j - 1));
+      }
+    for (int j = 1; j <= pneta[ip]; ++j)
+      for (int i = 1; i <= pnxi[ip]; ++i) {
+        int n1 = i + W * (j - 1) + nnode;
+        cny.insert(cny.end(), {n1, n1 + 1, n1 + 1 + W, n1 + W});
+        patch_plane.push_back(ip);
+      }
+    nnode += W * (pneta[ip] + 1);
+    npatch += pnxi[ip] * pneta[ip];
+  }
+  int ndof = (int)id_dof.size();
+  std::vector<int> dof_of_node(nnode, -1), n2e(4 * nnode, -1), n2e_slot(4 * nnode, 0), n2es(nnode, 0);
+  for (int k = 0; k < ndof; ++k) dof_of_node[id_dof[k] - 1] = k;
+  for (int e = 0; e < npatch; ++e)
+    for (int a = 0; a < 4; ++a) {
+      int nd0 = cny[4 * e + a] - 1;
+      int c = n2es[nd0]++;
+      n2e[4 * nd0 + c] = e + 1;
+      n2e_slot[4 * nd0 + c] = a;
+    }
+  m.nnode = nnode; m.ndof = ndof; m.npatch = npatch; m.ndim = 2 * ndof;
+  // observation points: SAR rows, then one point per GNSS triplet (gfunc.f90:212-239)
+  m.nobs = d->nobs; m.nsar = d->nsar_total; m.ngnss = d->ngnss;
+  if (m.nsar + 3 * m.ngnss != m.nobs) { set_error("nobs != nsar_total + 3*ngnss"); return SMCF_EINVAL; }
+  std::vector<double> pts;
+  std::vector<int> row0, nrow;
+  for (int r = 0; r < m.nsar; ++r) {
+    pts.push_back(d->obs_points[2 * r]); pts.push_back(d->obs_points[2 * r + 1]);
+    row0.push_back(r); nrow.push_back(1);
+  }
+  for (int g = 0; g < m.ngnss; ++g) {
+    int r = m.nsar + 3 * g;
+    pts.push_back(d->obs_points[2 * r]); pts.push_back(d->obs_points[2 * r + 1]);
+    row0.push_back(r); nrow.push_back(3);
+  }
+  m.npts = (int)row0.size();
+  std::vector<double> xb(std::max(1, np - 1), 0.0);
+  for (int k = 0; k < np - 1; ++k) xb[k] = d->xbend ? d->xbend[k] : 0.0;
+  std::vector<double> uv(d->obs_unitvec, d->obs_unitvec + 3 * (size_t)m.nobs);
+  std::vector<double> os(d->obs_sigma, d->obs_sigma + m.nobs), dv(d->dvec, d->dvec + m.nobs);
+  int rc = 0;
+  rc |= upload(P, &m.xbend, xb);
+  rc |= upload(P, &m.plane_nxi, pnxi);
+  rc |= upload(P, &m.plane_neta, pneta);
+  rc |= upload(P, &m.plane_node0, pnode0);
+  rc |= upload(P, &m.cny, cny);
+  rc |= upload(P, &m.patch_plane, patch_plane);
+  rc |= upload(P, &m.node_plane, node_plane);
+  rc |= upload(P, &m.node_k, node_k);
+  rc |= upload(P, &m.node_l, node_l);
+  rc |= upload(P, &m.id_dof, id_dof);
+  rc |= upload(P, &m.dof_of_node, dof_of_node);
+  rc |= upload(P, &m.n2e, n2e);
+  rc |= upload(P, &m.n2e_slot, n2e_slot);
+  rc |= upload(P, &m.n2es, n2es);
+  rc |= upload(P, &m.pts, pts);
+  rc |= upload(P, &m.pt_row0, row0);
+  rc |= upload(P, &m.pt_nrow, nrow);
+  rc |= upload(P, &m.unitvec, uv);
+  rc |= upload(P, &m.obs_sigma, os);
+  rc |= upload(P, &m.dvec, dv);
+  if (rc) return rc;
+  // batch buffers
+  BatchDev& bt = P.bt;
+  memset(&bt, 0, sizeof(bt));
+  size_t C = (size_t)max_batch, nd = m.ndim, Pn = P.P;
+  bt.cap = max_batch;
+  bt.P = P.P;
+  rc |= dalloc(P, &bt.theta, m.ndim_fault * C);
+  rc |= dalloc(P, &bt.geo, 16 * np * C);
+  rc |= dalloc(P, &bt.scal, 8 * C);
+  rc |= dalloc(P, &bt.G, (size_t)m.nobs * nd * C);
+  rc |= dalloc(P, &bt.gscr, (size_t)24 * m.npts * npatch * C);
+  rc |= dalloc(P, &bt.luni, (size_t)9 * nnode * C);
+  rc |= dalloc(P, &bt.li, (size_t)9 * 2 * nnode * C);
+  rc |= dalloc(P, &bt.lv, (size_t)9 * 2 * nnode * C);
+  rc |= dalloc(P, &bt.lti, (size_t)9 * 2 * ndof * C);
+  rc |= dalloc(P, &bt.ltv, (size_t)9 * 2 * ndof * C);
+  rc |= dalloc(P, &bt.llc_idx, (size_t)25 * nd * C);
+  rc |= dalloc(P, &bt.llc_val, (size_t)25 * nd * C);
+  rc |= dalloc(P, &bt.H, nd * nd * C);
+  rc |= dalloc(P, &bt.bvec, nd * C);
+  rc |= dalloc(P, &bt.dsd, C);
+  rc |= dalloc(P, &bt.Q, nd * nd * C);
+  rc |= dalloc(P, &bt.S, nd * Pn * C);
+  rc |= dalloc(P, &bt.S2, nd * Pn * C);
+  rc |= dalloc(P, &bt.Y, nd * Pn * C);
+  rc |= dalloc(P, &bt.like, Pn * C);
+  rc |= dalloc(P, &bt.prior, Pn * C);
+  rc |= dalloc(P, &bt.like2, Pn * C);
+  rc |= dalloc(P, &bt.prior2, Pn * C);
+  rc |= dalloc(P, &bt.w, Pn * C);
+  rc |= dalloc(P, &bt.parent, Pn * C);
+  rc |= dalloc(P, &bt.assigned, Pn * C);
+  rc |= dalloc(P, &bt.mean, nd * C);
+  rc |= dalloc(P, &bt.cov, nd * C);
+  rc |= dalloc(P, &bt.key, C);
+  rc |= dalloc(P, &bt.gamma, C);
+  rc |= dalloc(P, &bt.neglog, C);
+  rc |= dalloc(P, &bt.iter, C);
+  rc |= dalloc(P, &bt.active, C);
+  rc |= dalloc(P, &bt.done, C);
+  rc |= dalloc(P, &bt.par, C);
+  rc |= dalloc(P, &bt.gtrace, 202 * C);
+  if (rc) {
+    for (void* p : P.allocs) (void)hipFree(p);
+    return rc;
+  }
+  SMCF_HIP(hipMemset(bt.parent, 0, sizeof(int) * Pn * C));
+  SMCF_HIP(hipMemset(bt.gtrace, 0, sizeof(double) * 202 * C));
+  *out = holder.release();
+  return SMCF_OK;
+}
+
+int smcf_problem_destroy(smcf_problem* pb) {
+  if (!pb) return SMCF_OK;
+  (void)hipSetDevice(pb->p.ctx->device);
+  (void)hipStreamSynchronize(pb->p.ctx->stream);
+  for (void* p : pb->p.allocs) (void)hipFree(p);
+  delete pb;
+  return SMCF_OK;
+}
+
+int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_slip, int* ndim_fault, int* nobs) {
+  if (!pb) return SMCF_EINVAL;
+  if (nnode) *nnode = pb->p.m.nnode;
+  if (ndof) *ndof = pb->p.m.ndof;
+  if (ndim_slip) *ndim_slip = pb->p.m.ndim;
+  if (ndim_fault) *ndim_fault = pb->p.m.ndim_fault;
+  if (nobs) *nobs = pb->p.m.nobs;
+  return SMCF_OK;
+}
+
+int smcf_dc3d0(smcf_ctx* ctx, int n, const double* in, double* u, int* iret) {
+  if (!ctx || n < 0 || (n > 0 && (!in || !u || !iret))) return SMCF_EINVAL;
+  if (n == 0) return SMCF_OK;
+  hipStream_t s = ctx->stream;
+  double *din = nullptr, *du = nullptr;
+  int* dir = nullptr;
+  SMCF_HIP(hipMalloc(&din, sizeof(double) * 10 * n));
+  SMCF_HIP(hipMalloc(&du, sizeof(double) * 12 * n));
+  SMCF_HIP(hipMalloc(&dir, sizeof(int) * n));
+  SMCF_HIP(hipMemcpyAsync(din, in, sizeof(double) * 10 * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dir, iret, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_dc3d0_batch(n, din, du, dir, s));
+  SMCF_HIP(hipMemcpyAsync(u, du, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(iret, dir, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  (void)(void)hipFree(din); (void)hipFree(du); (void)hipFree(dir);
+  return SMCF_OK;
+}
+
+int smcf_calc_greens_func(smcf_problem* pb, int B, const double* theta, double* gmat) {
+  if (!pb || !theta || !gmat) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  if (B < 1 || B > P.bt.cap) { set_error("batch size outside capacity"); return SMCF_EINVAL; }
+  hipStream_t s = P.ctx->stream;
+  SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * P.m.ndim_fault * B, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_geometry(P.m, P.bt, B, s));
+  CHECK_LAUNCH(launch_greens(P.m, P.bt, B, s));
+  SMCF_HIP(hipMemcpyAsync(gmat, P.bt.G, sizeof(double) * P.m.nobs * P.m.ndim * B, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_laplacian(smcf_problem* pb, const double* theta, int* li, double* lv, int* lti, double* ltv, double* ll) {
+  if (!pb || !theta) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  MeshDev& m = P.m;
+  hipStream_t s = P.ctx->stream;
+  SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * m.ndim_fault, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_geometry(m, P.bt, 1, s));
+  CHECK_LAUNCH(launch_laplacian(m, P.bt, 1, s));
+  if (li) SMCF_HIP(hipMemcpyAsync(li, P.bt.li, sizeof(int) * 18 * m.nnode, hipMemcpyDeviceToHost, s));
+  if (lv) SMCF_HIP(hipMemcpyAsync(lv, P.bt.lv, sizeof(double) * 18 * m.nnode, hipMemcpyDeviceToHost, s));
+  if (lti) SMCF_HIP(hipMemcpyAsync(lti, P.bt.lti, sizeof(int) * 18 * m.ndof, hipMemcpyDeviceToHost, s));
+  if (ltv) SMCF_HIP(hipMemcpyAsync(ltv, P.bt.ltv, sizeof(double) * 18 * m.ndof, hipMemcpyDeviceToHost, s));
+  std::vector<int> ix;
+  std::vector<double> vx;
+  if (ll) {
+    ix.resize(25 * (size_t)m.ndim);
+    vx.resize(25 * (size_t)m.ndim);
+    SMCF_HIP(hipMemcpyAsync(ix.data(), P.bt.llc_idx, sizeof(int) * ix.size(), hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(vx.data(), P.bt.llc_val, sizeof(double) * vx.size(), hipMemcpyDeviceToHost, s));
+  }
+  SMCF_HIP(hipStreamSynchronize(s));
+  if (ll) {
+    size_t nd = m.ndim;
+    std::fill(ll, ll + nd * nd, 0.0);
+    for (size_t i = 0; i < nd; ++i)
+      for (int w = 0; w < 25; ++w) {
+        int r = ix[i * 25 + w];
+        if (r < 0) break;
+        ll[i * nd + r] = vx[i * 25 + w];
+      }
+  }
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_gsd_gsg(smcf_problem* pb, int B, const double* theta, double* gsgmat, double* gsdvec, double* dsd) {
+  if (!pb || !theta) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  MeshDev& m = P.m;
+  if (B < 1 || B > P.bt.cap) { set_error("batch size outside capacity"); return SMCF_EINVAL; }
+  if (m.nsar > 0) { set_error("InSAR Gram not supported on the device yet"); return SMCF_EUNSUPPORTED; }
+  hipStream_t s = P.ctx->stream;
+  SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * m.ndim_fault * B, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_geometry(m, P.bt, B, s));
+  CHECK_LAUNCH(launch_greens(m, P.bt, B, s));
+  CHECK_LAUNCH(launch_gram(m, P.bt, B, s));
+  size_t nd = m.ndim;
+  if (gsgmat) SMCF_HIP(hipMemcpyAsync(gsgmat, P.bt.H, sizeof(double) * nd * nd * B, hipMemcpyDeviceToHost, s));
+  if (gsdvec) SMCF_HIP(hipMemcpyAsync(gsdvec, P.bt.bvec, sizeof(double) * nd * B, hipMemcpyDeviceToHost, s));
+  if (dsd) SMCF_HIP(hipMemcpyAsync(dsd, P.bt.dsd, sizeof(double) * B, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, double* particles, double* likelihood,
+                       double* prior, double* mean, int* nstage, double* neglog, double* gamma_trace) {
+  if (!pb || !theta) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  int rc = run_batch(P, 1, theta, &key, mean != nullptr);
+  if (rc) return rc;
+  hipStream_t s = P.ctx->stream;
+  MeshDev& m = P.m;
+  BatchDev& bt = P.bt;
+  size_t nd = m.ndim, Pn = bt.P;
+  int par = 0, it = 0;
+  SMCF_HIP(hipMemcpy(&par, bt.par, sizeof(int), hipMemcpyDeviceToHost));
+  SMCF_HIP(hipMemcpy(&it, bt.iter, sizeof(int), hipMemcpyDeviceToHost));
+  if (particles) {
+    std::vector<double> dm(nd * Pn);
+    SMCF_HIP(hipMemcpy(dm.data(), par ? bt.S2 : bt.S, sizeof(double) * nd * Pn, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < Pn; ++p)
+      for (size_t i = 0; i < nd; ++i) particles[p * nd + i] = dm[i * Pn + p];
+  }
+  if (likelihood)
+    SMCF_HIP(hipMemcpy(likelihood, par ? bt.like2 : bt.like, sizeof(double) * Pn, hipMemcpyDeviceToHost));
+  if (prior) SMCF_HIP(hipMemcpy(prior, par ? bt.prior2 : bt.prior, sizeof(double) * Pn, hipMemcpyDeviceToHost));
+  if (mean) SMCF_HIP(hipMemcpy(mean, bt.mean, sizeof(double) * nd, hipMemcpyDeviceToHost));
+  if (neglog) SMCF_HIP(hipMemcpy(neglog, bt.neglog, sizeof(double), hipMemcpyDeviceToHost));
+  if (gamma_trace) SMCF_HIP(hipMemcpy(gamma_trace, bt.gtrace, sizeof(double) * 202, hipMemcpyDeviceToHost));
+  if (nstage) *nstage = std::min(it - 1, 201);
+  (void)s;
+  return SMCF_OK;
+}
+
+int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys,
+                                     double* neglog, int* nstage) {
+  if (!pb || !theta || !keys || !neglog) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  int rc = run_batch(P, B, theta, keys, false);
+  if (rc) return rc;
+  SMCF_HIP(hipMemcpy(neglog, P.bt.neglog, sizeof(double) * B, hipMemcpyDeviceToHost));
+  if (nstage) {
+    std::vector<int> it(B);
+    SMCF_HIP(hipMemcpy(it.data(), P.bt.iter, sizeof(int) * B, hipMemcpyDeviceToHost));
+    for (int b = 0; b < B; ++b) nstage[b] = std::min(it[b] - 1, 201);
+  }
+  return SMCF_OK;
+}
+
+int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, double cv, double tol, int cap,
+                         double* gamma, double* w, double* nev) {
+  if (!ctx || P < 1 || !L) return SMCF_EINVAL;
+  if (P > kMaxInnerParticles) { set_error("P exceeds the tempering workgroup's LDS"); return SMCF_EINVAL; }
+  hipStream_t s = ctx->stream;
+  double *dL, *dw, *dout;
+  SMCF_HIP(hipMalloc(&dL, sizeof(double) * P));
+  SMCF_HIP(hipMalloc(&dw, sizeof(double) * P));
+  SMCF_HIP(hipMalloc(&dout, sizeof(double) * 3));
+  SMCF_HIP(hipMemcpyAsync(dL, L, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_find_next_gamma(P, gprev, dL, cv, tol, cap, dout, dw, s));
+  double out3[3];
+  SMCF_HIP(hipMemcpyAsync(out3, dout, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+  if (w) SMCF_HIP(hipMemcpyAsync(w, dw, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  if (gamma) *gamma = out3[0];
+  if (nev) *nev = out3[1];
+  (void)(void)hipFree(dL); (void)hipFree(dw); (void)hipFree(dout);
+  return SMCF_OK;
+}
+
+int smcf_normalize_resample(smcf_ctx* ctx, int P, const double* w, double uh, double ul, double* wn, int* assigned) {
+  if (!ctx || P < 1 || !w) return SMCF_EINVAL;
+  if (P > kMaxInnerParticles) { set_error("P exceeds the tempering workgroup's LDS"); return SMCF_EINVAL; }
+  hipStream_t s = ctx->stream;
+  double *dw, *dwn;
+  int* da;
+  SMCF_HIP(hipMalloc(&dw, sizeof(double) * P));
+  SMCF_HIP(hipMalloc(&dwn, sizeof(double) * P));
+  SMCF_HIP(hipMalloc(&da, sizeof(int) * 2 * P));
+  SMCF_HIP(hipMemcpyAsync(dw, w, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_normalize_resample(P, dw, uh, ul, dwn, da, s));
+  if (wn) SMCF_HIP(hipMemcpyAsync(wn, dwn, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  if (assigned) SMCF_HIP(hipMemcpyAsync(assigned, da, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  (void)(void)hipFree(dw); (void)hipFree(dwn); (void)hipFree(da);
+  return SMCF_OK;
+}
+
+int smcf_trunc_normal(smcf_ctx* ctx, int n, const double* mu, const double* sg, double lo, double hi, const double* u,
+                      double* out) {
+  if (!ctx || n < 0 || (n > 0 && (!mu || !sg || !u || !out))) return SMCF_EINVAL;
+  if (n == 0) return SMCF_OK;
+  hipStream_t s = ctx->stream;
+  double *dm, *ds, *du, *dout;
+  SMCF_HIP(hipMalloc(&dm, sizeof(double) * n));
+  SMCF_HIP(hipMalloc(&ds, sizeof(double) * n));
+  SMCF_HIP(hipMalloc(&du, sizeof(double) * n));
+  SMCF_HIP(hipMalloc(&dout, sizeof(double) * n));
+  SMCF_HIP(hipMemcpyAsync(dm, mu, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(ds, sg, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(du, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_trunc_normal(n, dm, ds, lo, hi, du, dout, s));
+  SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  (void)(void)hipFree(dm); (void)hipFree(ds); (void)hipFree(du); (void)hipFree(dout);
+  return SMCF_OK;
+}
+
+int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  if (!ctx || n < 0 || (n > 0 && (!ctr || !key || !out))) return SMCF_EINVAL;
+  if (n == 0) return SMCF_OK;
+  hipStream_t s = ctx->stream;
+  uint32_t *dc, *dk, *dout;
+  SMCF_HIP(hipMalloc(&dc, sizeof(uint32_t) * 4 * n));
+  SMCF_HIP(hipMalloc(&dk, sizeof(uint32_t) * 2 * n));
+  SMCF_HIP(hipMalloc(&dout, sizeof(uint32_t) * 4 * n));
+  SMCF_HIP(hipMemcpyAsync(dc, ctr, sizeof(uint32_t) * 4 * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dk, key, sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_philox(n, dc, dk, dout, s));
+  SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  (void)(void)hipFree(dc); (void)hipFree(dk); (void)hipFree(dout);
+  return SMCF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+// Dense FP64 linear algebra of the inner SMC, batched over geometries.
+//
+//  k_scal      theta -> alpha^2, sigma^2 and the likelihood constant (init.f90:175-196,
+//              smc_fault.f90:148-173, smc_slip.f90:84-87)
+//  k_gram      H = G^T diag(1/(sigma_i^2 sigma_gnss^2)) G          (smc_slip.f90:1401-1409)
+//  k_gram_vec  b = G^T sd, dsd = d^T sd                              (smc_slip.f90:1371-1388)
+//  k_hs        Y = H S over all particles (S dimension-major)        (smc_slip.f90:72-87)
+//  k_like      Gram-form likelihood and ELL prior per particle      (smc_slip.f90:56-137)
+//  k_form_q    Q = gamma H + LL / alpha^2 for the Gibbs mover
+// The two GEMMs use 64x64 LDS tiles with a 4x4 register block per thread.
+#include "smcf_device.h"
+#include "smcf_internal.h"
+
+namespace smcf {
+
+__global__ void k_scal(MeshDev m, const double* __restrict__ theta, double* __restrict__ scal, int B) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const double* th = theta + (size_t)b * m.ndim_fault;
+  double la2, lss2, lsg2;
+  if (m.fix_xbend) {
+    la2 = th[2 * m.nplane + 1]; lss2 = th[2 * m.nplane + 2]; lsg2 = th[2 * m.nplane + 3];
+  } else {
+    la2 = th[3 * m.nplane]; lss2 = th[3 * m.nplane + 1]; lsg2 = th[3 * m.nplane + 2];
+  }
+  double* s = scal + 8 * (size_t)b;
+  s[0] = exp(la2);
+  s[1] = lss2;
+  s[2] = lsg2;
+  s[3] = m.nsar * lss2 / 2.0;
+  s[4] = 3.0 * m.ngnss * lsg2 / 2.0;
+  s[5] = exp(lsg2);
+  s[6] = exp(lss2);
+  s[7] = 0.0;
+}
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+// H(i, j) = sum_k (G(k, i) / c_k) G(k, j), c_k = obs_sigma_k^2 * sigma_gnss^2 (GNSS rows)
+__global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restrict__ Gall,
+                                              const double* __restrict__ scal, double* __restrict__ Hall) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  int b = blockIdx.z;
+  int i0 = blockIdx.x * TM, j0 = blockIdx.y * TN;
+  int nobs = m.nobs, nd = m.ndim;
+  const double* G = Gall + (size_t)b * nobs * nd;
+  double sg2 = scal[8 * (size_t)b + 5];
+  int t = threadIdx.x, ty = t / 16, tx = t % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int k0 = 0; k0 < nobs; k0 += TK) {
+#pragma unroll
+    for (int rep = 0; rep < 4; ++rep) {
+      int kk = t % 16, ii = t / 16 + rep * 16;
+      int k = k0 + kk;
+      int i = i0 + ii, j = j0 + ii;
+      double av = 0.0, bv = 0.0;
+      if (k < nobs) {
+        double sk = m.obs_sigma[k];
+        double ck = (sk * sk) * sg2;
+        if (i < nd) av = G[(size_t)i * nobs + k] / ck;
+        if (j < nd) bv = G[(size_t)j * nobs + k];
+      }
+      As[kk][ii] = av;
+      Bs[kk][ii] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      double a4[4], b4[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b4[c] = Bs[kk][tx * 4 + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] += a4[a] * b4[c];
+    }
+    __syncthreads();
+  }
+  double* H = Hall + (size_t)b * nd * nd;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int i = i0 + ty * 4 + a, j = j0 + tx * 4 + c;
+      if (i < nd && j < nd) H[(size_t)j * nd + i] = acc[a][c];
+    }
+}
+
+__global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const double* __restrict__ scal,
+                           double* __restrict__ bvec, double* __restrict__ dsd, int B) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  int nobs = m.nobs, nd = m.ndim;
+  double sg2 = scal[8 * (size_t)b + 5];
+  if (i < nd) {
+    const double* g = Gall + ((size_t)b * nd + i) * nobs;
+    double s = 0.0;
+    for (int k = 0; k < nobs; ++k) {
+      double sk = m.obs_sigma[k];
+      double sdk = m.dvec[k] / (sg2 * (sk * sk));
+      s = s + g[k] * sdk;
+    }
+    bvec[(size_t)b * nd + i] = s;
+  } else if (i == nd) {
+    double s = 0.0;
+    for (int k = 0; k < nobs; ++k) {
+      double sk = m.obs_sigma[k];
+      double sdk = m.dvec[k] / (sg2 * (sk * sk));
+      s = s + m.dvec[k] * sdk;
+    }
+    dsd[b] = s;
+  }
+}
+
+hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B);
+  dim3 g((m.ndim + TM - 1) / TM, (m.ndim + TN - 1) / TN, B);
+  hipLaunchKernelGGL(k_gram, g, dim3(256), 0, s, m, bt.G, bt.scal, bt.H);
+  hipLaunchKernelGGL(k_gram_vec, dim3((m.ndim + 1 + 127) / 128, B), dim3(128), 0, s, m, bt.G, bt.scal, bt.bvec,
+                     bt.dsd, B);
+  return hipGetLastError();
+}
+
+// Y(i, p) = sum_j H(i, j) S(j, p); S and Y dimension-major (row i = dimension, P contiguous)
+__global__ void __launch_bounds__(256) k_hs(int nd, int P, const double* __restrict__ Hall,
+                                            const double* __restrict__ S0, const double* __restrict__ S1,
+                                            const int* __restrict__ par, const int* __restrict__ active,
+                                            double* __restrict__ Yall, int use_active) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  int b = blockIdx.z;
+  if (use_active && !active[b]) return;
+  int i0 = blockIdx.y * TM, p0 = blockIdx.x * TN;
+  const double* H = Hall + (size_t)b * nd * nd;
+  const double* S = (par[b] ? S1 : S0) + (size_t)b * nd * P;
+  double* Y = Yall + (size_t)b * nd * P;
+  int t = threadIdx.x, ty = t / 16, tx = t % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int k0 = 0; k0 < nd; k0 += TK) {
+#pragma unroll
+    for (int rep = 0; rep < 4; ++rep) {
+      int ii = t % 64, kk = t / 64 + rep * 4;
+      int i = i0 + ii, k = k0 + kk, p = p0 + ii;
+      As[kk][ii] = (i < nd && k < nd) ? H[(size_t)k * nd + i] : 0.0;
+      Bs[kk][ii] = (p < P && k < nd) ? S[(size_t)k * P + p] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      double a4[4], b4[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b4[c] = Bs[kk][tx * 4 + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] += a4[a] * b4[c];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int i = i0 + ty * 4 + a, p = p0 + tx * 4 + c;
+      if (i < nd && p < P) Y[(size_t)i * P + p] = acc[a][c];
+    }
+}
+
+// likelihood (Gram form) and prior of every particle of the current buffer
+__global__ void k_like(MeshDev m, int P, const double* __restrict__ S0, const double* __restrict__ S1,
+                       const int* __restrict__ par, const int* __restrict__ active, const double* __restrict__ Yall,
+                       const double* __restrict__ bvec, const double* __restrict__ dsd,
+                       const double* __restrict__ scal, const int* __restrict__ liall,
+                       const double* __restrict__ lvall, double* __restrict__ L0, double* __restrict__ L1,
+                       double* __restrict__ R0, double* __restrict__ R1, int use_active) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (use_active && !active[b]) return;
+  if (p >= P) return;
+  int nd = m.ndim;
+  int pb = par[b];
+  const double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
+  const double* Y = Yall + (size_t)b * nd * P;
+  const double* bv = bvec + (size_t)b * nd;
+  double v = 0.0, bs = 0.0;
+  for (int i = 0; i < nd; ++i) {
+    double s = S[(size_t)i * P + p];
+    v = v + s * Y[(size_t)i * P + p];
+    bs = bs + bv[i] * s;
+  }
+  const double* sc = scal + 8 * (size_t)b;
+  double like = (v - 2.0 * bs + dsd[b]) / 2.0 + sc[3] + sc[4];
+  const int* li = liall + (size_t)b * 2 * m.nnode * 9;
+  const double* lv = lvall + (size_t)b * 2 * m.nnode * 9;
+  double two_a2 = 2.0 * sc[0];
+  double pr = 0.0;
+  for (int r = 0; r < 2 * m.nnode; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) t = t + lv[9 * r + q] * S[(size_t)(li[9 * r + q] - 1) * P + p];
+    pr = pr + (t * t) / two_a2;
+  }
+  (pb ? L1 : L0)[(size_t)b * P + p] = like;
+  (pb ? R1 : R0)[(size_t)b * P + p] = pr;
+}
+
+hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s) {
+  int P = bt.P;
+  dim3 g((P + TN - 1) / TN, (m.ndim + TM - 1) / TM, B);
+  hipLaunchKernelGGL(k_hs, g, dim3(256), 0, s, m.ndim, P, bt.H, bt.S, bt.S2, bt.par, bt.active, bt.Y, use_active);
+  hipLaunchKernelGGL(k_like, dim3((P + 127) / 128, B), dim3(128), 0, s, m, P, bt.S, bt.S2, bt.par, bt.active, bt.Y,
+                     bt.bvec, bt.dsd, bt.scal, bt.li, bt.lv, bt.like, bt.like2, bt.prior, bt.prior2, use_active);
+  return hipGetLastError();
+}
+
+// Q(r, i) = gamma H(r, i) (+ LL(r, i) / alpha^2 on the sparse pattern, second kernel)
+__global__ void k_form_q(int nd, const double* __restrict__ Hall, const double* __restrict__ gamma,
+                         const int* __restrict__ active, double* __restrict__ Qall) {
+  int b = blockIdx.y;
+  if (!active[b]) return;
+  size_t n = (size_t)nd * nd;
+  double g = gamma[b];
+  const double* H = Hall + (size_t)b * n;
+  double* Q = Qall + (size_t)b * n;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    Q[k] = g * H[k];
+}
+
+__global__ void k_form_q_sparse(int nd, const double* __restrict__ Hall, const double* __restrict__ gamma,
+                                const int* __restrict__ active, const double* __restrict__ scal,
+                                const int* __restrict__ llidx, const double* __restrict__ llval,
+                                double* __restrict__ Qall) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!active[b] || i >= nd) return;
+  double g = gamma[b], a2 = scal[8 * (size_t)b];
+  const double* H = Hall + (size_t)b * nd * nd;
+  double* Q = Qall + (size_t)b * nd * nd;
+  const int* ix = llidx + ((size_t)b * nd + i) * 25;
+  const double* vx = llval + ((size_t)b * nd + i) * 25;
+  for (int w = 0; w < 25; ++w) {
+    int r = ix[w];
+    if (r < 0) break;
+    size_t k = (size_t)i * nd + r;
+    Q[k] = g * H[k] + vx[w] / a2;
+  }
+}
+
+hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  int nd = m.ndim;
+  int blocks = (int)(((size_t)nd * nd + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_form_q, dim3(blocks, B), dim3(256), 0, s, nd, bt.H, bt.gamma, bt.active, bt.Q);
+  hipLaunchKernelGGL(k_form_q_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, bt.H, bt.gamma, bt.active,
+                     bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
+  return hipGetLastError();
+}
+
+}  // namespace smcf

@@ -1,0 +1,440 @@
+// Inner slip SMC kernels, batched over geometries (one instance = one outer particle).
+//
+//  k_init_chain      prior draw: one sequential Gibbs chain per instance, Newton-solved
+//                    truncated normals (smc_slip.f90:184-264); one lane per instance
+//  k_gamma_resample  one workgroup per instance: bisection on gamma for CV = 2, evidence,
+//                    real(16)-grade normalisation and systematic residual resampling in
+//                    double-double (smc_slip.f90:314-387, 465-483, 572-589)
+//  k_moments         weighted mean / variance diagonal in double-double (smc_slip.f90:485-570)
+//  k_gather          children <- parents (index map from the resampling)
+//  k_gibbs           truncated-MVN Gibbs sweeps on the tempered posterior (benchmark mover)
+#include "smcf_device.h"
+#include "smcf_internal.h"
+
+namespace smcf {
+
+// ---------------------------------------------------------------------------
+__global__ void k_init_chain(MeshDev m, int P, int B, double max_slip, const uint64_t* __restrict__ keys,
+                             const double* __restrict__ scal, const int* __restrict__ llidx,
+                             const double* __restrict__ llval, double* __restrict__ Sall) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  int nd = m.ndim;
+  uint64_t key = keys[b];
+  double a2 = scal[8 * (size_t)b];
+  const int* IX = llidx + (size_t)b * nd * 25;
+  const double* VX = llval + (size_t)b * nd * 25;
+  double* S = Sall + (size_t)b * nd * P;
+  for (int k = 0; k < P; ++k) {
+    for (int i = 0; i < nd; ++i) {
+      const int* ix = IX + (size_t)i * 25;
+      const double* vx = VX + (size_t)i * 25;
+      double lii = 0.0, mu = 0.0;
+      for (int w = 0; w < 25; ++w) {
+        if (ix[w] == i) { lii = vx[w]; break; }
+      }
+      double s2 = 1.0 / (lii / a2);
+      for (int w = 0; w < 25; ++w) {
+        int j = ix[w];
+        if (j < 0) break;
+        if (j == i) continue;
+        double cj = j < i ? S[(size_t)j * P + k] : (k > 0 ? S[(size_t)j * P + k - 1] : 0.0);
+        mu = mu - vx[w] / a2 * cj * s2;
+      }
+      double v1 = draw_normal(key, kSiteGibbsInit, 0, (uint32_t)k, (uint32_t)i);
+      double x = mu + v1 * sqrt(s2);
+      double fx = cdf_norm(x, mu, s2);
+      double f1 = cdf_norm(max_slip, mu, s2);
+      double f0 = cdf_norm(0.0, mu, s2);
+      double target = (f1 - f0) * fx + f0;
+      double y = mu, err = 1.0, yp = y + err;
+      int cnt = 0;
+      while (err > 1e-8 && cnt < 100) {
+        y = y - (cdf_norm(y, mu, s2) - target) / pdf_norm(y, mu, s2);
+        err = fabs(y - yp);
+        yp = y;
+        cnt++;
+      }
+      S[(size_t)i * P + k] = y;
+    }
+  }
+}
+
+hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_chain, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.P, B, max_slip, bt.key, bt.scal,
+                     bt.llc_idx, bt.llc_val, bt.S);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tempering step for one instance.  Layout: 256 threads; thread t owns the contiguous
+// particle range [t*C, (t+1)*C) with C = ceil(P/256), so per-thread partial sums and
+// prefix sums run in ascending particle order.
+// ---------------------------------------------------------------------------
+constexpr int GT = 256;
+
+struct GammaOut {
+  double gamma, neglog_ev, wsum_hi, wsum_lo;
+};
+
+// bisection on gamma (returns gamma, leaves w[] = exp(-dg (L - Lmin)) of the final gamma)
+SMCF_DEV double gamma_search(const double* L, double* w, int P, double gprev, double cv_t, double tol, int cap,
+                             double* red, double* neglog_ev) {
+  int t = threadIdx.x;
+  int C = (P + GT - 1) / GT, i0 = t * C, i1 = min(P, i0 + C);
+  double lmin_t = i0 < P ? L[i0] : L[0];
+  for (int i = i0; i < i1; ++i) lmin_t = fmin(lmin_t, L[i]);
+  double lmin = block_min<GT>(lmin_t, red);
+  double lower = gprev, upper = 1.0, err = 1.0, gamma = 1.0, mean = 1.0;
+  int cnt = 0;
+  while (err > tol) {
+    gamma = (lower + upper) / 2.0;
+    double dg = gamma - gprev;
+    double s = 0.0;
+    for (int i = i0; i < i1; ++i) {
+      double v = exp(-dg * (L[i] - lmin));
+      w[i] = v;
+      s = s + v;
+    }
+    mean = block_sum<GT>(s, red) / P;
+    double q = 0.0;
+    for (int i = i0; i < i1; ++i) q = q + (w[i] - mean) * (w[i] - mean);
+    double sd = sqrt(block_sum<GT>(q, red) / P);
+    double cv = sd / mean;
+    if (cv > cv_t) upper = gamma; else lower = gamma;
+    err = fabs(cv - cv_t);
+    if (fabs(gamma - 1.0) < tol) break;
+    if (cnt > cap) break;
+    cnt++;
+  }
+  double dg = gamma - gprev;
+  *neglog_ev = -log(mean) + dg * lmin;
+  return gamma;
+}
+
+// w <- w / sum(w) (double-double sum), prefix sums, resampling counts and parent map
+SMCF_DEV void normalize_resample(double* w, int P, dd u, int* assigned, int* parent, double* red, dd* scan) {
+  int t = threadIdx.x;
+  int C = (P + GT - 1) / GT, i0 = t * C, i1 = min(P, i0 + C);
+  dd s = {0.0, 0.0};
+  for (int i = i0; i < i1; ++i) s = dd_add_d(s, w[i]);
+  scan[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    dd run = {0.0, 0.0};
+    for (int k = 0; k < GT; ++k) {
+      dd v = scan[k];
+      scan[k] = run;  // exclusive prefix of the thread totals
+      run = dd_add(run, v);
+    }
+    scan[GT] = run;
+  }
+  __syncthreads();
+  dd tot = scan[GT];
+  for (int i = i0; i < i1; ++i) w[i] = d_div_dd(w[i], tot);
+  __syncthreads();
+  // prefix sums of the normalised weights
+  dd c = {0.0, 0.0};
+  for (int i = i0; i < i1; ++i) c = dd_add_d(c, w[i]);
+  scan[t] = c;
+  __syncthreads();
+  if (t == 0) {
+    dd run = {0.0, 0.0};
+    for (int k = 0; k < GT; ++k) {
+      dd v = scan[k];
+      scan[k] = run;
+      run = dd_add(run, v);
+    }
+  }
+  __syncthreads();
+  if (dd_lt_d(u, 1e-5)) u = dd{1e-5, 0.0};
+  dd u0 = dd_div_d(u, (double)P);
+  dd cum = scan[t];
+  int Aprev;
+  {
+    // A(i0 - 1)
+    if (i0 == 0) Aprev = 0;
+    else {
+      dd x = dd_mul_d(dd_add(cum, dd_neg(u0)), (double)P);
+      Aprev = (int)dd_floor(x) + 1;
+    }
+  }
+  Aprev = max(0, min(P, Aprev));
+  for (int i = i0; i < i1; ++i) {
+    cum = dd_add_d(cum, w[i]);
+    dd x = dd_mul_d(dd_add(cum, dd_neg(u0)), (double)P);
+    int A = (int)dd_floor(x) + 1;
+    A = max(Aprev, min(P, A));
+    assigned[i] = A - Aprev;
+    for (int j = Aprev; j < A; ++j) parent[j] = i;
+    Aprev = A;
+  }
+  // counts that fall short of P (only possible through rounding at the very end of the
+  // cumulative sum) leave the last children on the last particle
+  if (i1 == P && i0 < P)
+    for (int j = Aprev; j < P; ++j) parent[j] = P - 1;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(GT) k_gamma_resample(int P, const double* __restrict__ L0,
+                                                       const double* __restrict__ L1, BatchDev bt) {
+  extern __shared__ double smem[];
+  double* Ls = smem;            // P
+  double* ws = Ls + P;          // P
+  double* red = ws + P;         // GT
+  dd* scan = (dd*)(red + GT);   // GT + 1
+  int b = blockIdx.x, t = threadIdx.x;
+  if (bt.done[b]) {
+    if (t == 0) bt.active[b] = 0;
+    return;
+  }
+  // every thread reads the stage state before the first barrier; thread 0 updates it
+  // only after the barriers inside gamma_search
+  int pb = bt.par[b];
+  int it = bt.iter[b];
+  double gprev = bt.gamma[b];
+  const double* L = (pb ? L1 : L0) + (size_t)b * P;
+  for (int i = t; i < P; i += GT) Ls[i] = L[i];
+  __syncthreads();
+  double nev;
+  double gamma = gamma_search(Ls, ws, P, gprev, 2.0, 1e-10, 100000, red, &nev);
+  if (t == 0) {
+    double nl = bt.neglog[b] + nev;
+    if (it <= 201) bt.gtrace[(size_t)b * 202 + it - 1] = gamma;
+    bt.gamma[b] = gamma;
+    bt.iter[b] = it + 1;
+    if (it > 200) {
+      nl = 1e10;
+      bt.done[b] = 1;
+      bt.active[b] = 0;
+    } else {
+      bt.active[b] = 1;
+      bt.par[b] = 1 - pb;
+      if (1.0 - gamma <= 10e-8) bt.done[b] = 1;
+    }
+    bt.neglog[b] = nl;
+  }
+  if (it > 200) return;
+  uint64_t w0, w1;
+  philox_draw(bt.key[b], kSiteResample, (uint32_t)it, 0, 0, 0, w0, w1);
+  dd u = two_sum((double)(w0 >> 11) * 0x1.0p-53, (double)(w1 >> 11) * 0x1.0p-106);
+  normalize_resample(ws, P, u, bt.assigned + (size_t)b * P, bt.parent + (size_t)b * P, red, scan);
+  for (int i = t; i < P; i += GT) bt.w[(size_t)b * P + i] = ws[i];
+}
+
+hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  (void)m;
+  size_t sh = sizeof(double) * (2 * (size_t)bt.P + GT) + sizeof(dd) * (GT + 1);
+  hipLaunchKernelGGL(k_gamma_resample, dim3(B), dim3(GT), sh, s, bt.P, bt.like, bt.like2, bt);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// weighted mean and variance diagonal of the pre-resampling particles (source buffer)
+__global__ void k_moments(int nd, int P, BatchDev bt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!bt.active[b] || i >= nd) return;
+  const double* S = (bt.par[b] ? bt.S : bt.S2) + (size_t)b * nd * P;  // source = previous buffer
+  const double* w = bt.w + (size_t)b * P;
+  dd m = {0.0, 0.0};
+  for (int p = 0; p < P; ++p) m = dd_add_d(m, w[p] * S[(size_t)i * P + p]);
+  double mean = m.hi + m.lo;
+  dd c = {0.0, 0.0};
+  for (int p = 0; p < P; ++p) {
+    double di = S[(size_t)i * P + p] - mean;
+    c = dd_add_d(c, w[p] * di * di);
+  }
+  double cv = c.hi + c.lo;
+  if (fabs(cv) < 1e-3) cv = 1e-3;
+  bt.mean[(size_t)b * nd + i] = mean;
+  bt.cov[(size_t)b * nd + i] = cv;
+}
+
+hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments, dim3((m.ndim + 63) / 64, B), dim3(64), 0, s, m.ndim, bt.P, bt);
+  return hipGetLastError();
+}
+
+// children <- parents into the (new) current buffer; likelihood / prior follow
+__global__ void k_gather(int nd, int P, BatchDev bt) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.z;
+  if (!bt.active[b] || j >= P) return;
+  int cur = bt.par[b];
+  const double* Ss = (cur ? bt.S : bt.S2) + (size_t)b * nd * P;
+  double* Sd = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  int pa = bt.parent[(size_t)b * P + j];
+  for (int i = blockIdx.y; i < nd; i += gridDim.y) Sd[(size_t)i * P + j] = Ss[(size_t)i * P + pa];
+  if (blockIdx.y == 0) {
+    const double* Ls = (cur ? bt.like : bt.like2) + (size_t)b * P;
+    double* Ld = (cur ? bt.like2 : bt.like) + (size_t)b * P;
+    const double* Rs = (cur ? bt.prior : bt.prior2) + (size_t)b * P;
+    double* Rd = (cur ? bt.prior2 : bt.prior) + (size_t)b * P;
+    Ld[j] = Ls[pa];
+    Rd[j] = Rs[pa];
+  }
+}
+
+hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  int gy = m.ndim < 64 ? m.ndim : 64;
+  hipLaunchKernelGGL(k_gather, dim3((bt.P + 255) / 256, gy, B), dim3(256), 0, s, m.ndim, bt.P, bt);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gibbs sweeps.  One thread per particle, 256 particles per workgroup.  Coordinates are
+// processed in blocks of KB: the workgroup stages Q(:, i0:i0+KB) in LDS, every thread forms
+// q_k = sum_j Q(j, i0+k) s_j for its particle, then samples the KB coordinates in order,
+// adding the in-block corrections Q(i0+m, i0+k) * delta_m of the coordinates already redrawn.
+// ---------------------------------------------------------------------------
+template <int KB>
+__global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double max_slip, BatchDev bt) {
+  extern __shared__ double Qs[];  // nd x KB, [j*KB + k]
+  int b = blockIdx.y;
+  if (!bt.active[b]) return;
+  int t = threadIdx.x;
+  int p = blockIdx.x * 256 + t;
+  bool valid = p < P;
+  int cur = bt.par[b];
+  double* S = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  const double* Q = bt.Q + (size_t)b * nd * nd;
+  const double* bv = bt.bvec + (size_t)b * nd;
+  double gamma = bt.gamma[b];
+  uint64_t key = bt.key[b];
+  uint32_t stage = (uint32_t)(bt.iter[b] - 1);
+  int pc = valid ? p : 0;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int i0 = 0; i0 < nd; i0 += KB) {
+      int kb = min(KB, nd - i0);
+      __syncthreads();
+      for (int idx = t; idx < nd * KB; idx += 256) {
+        int k = idx / nd, j = idx - k * nd;
+        Qs[j * KB + k] = k < kb ? Q[(size_t)(i0 + k) * nd + j] : 0.0;
+      }
+      __syncthreads();
+      double q[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) q[k] = 0.0;
+      for (int j = 0; j < nd; ++j) {
+        double sj = S[(size_t)j * P + pc];
+        const double* qr = Qs + j * KB;
+#pragma unroll
+        for (int k = 0; k < KB; ++k) q[k] = q[k] + qr[k] * sj;
+      }
+      double delta[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        delta[k] = 0.0;
+        if (k < kb) {
+          int i = i0 + k;
+          double qi = q[k];
+#pragma unroll
+          for (int mm = 0; mm < k; ++mm) qi = qi + Qs[(i0 + mm) * KB + k] * delta[mm];
+          double prec = Qs[i * KB + k];
+          double so = S[(size_t)i * P + pc];
+          double mu = so - (qi - gamma * bv[i]) / prec;
+          uint64_t w0, w1;
+          philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)i, (uint32_t)sw, w0, w1);
+          double sn = trunc_normal(mu, 1.0 / sqrt(prec), 0.0, max_slip, u01_open(w0));
+          delta[k] = sn - so;
+          if (valid) S[(size_t)i * P + p] = sn;
+        }
+      }
+    }
+  }
+}
+
+hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
+  int nd = m.ndim;
+  dim3 g((bt.P + 255) / 256, B);
+  // staged Q block: nd * KB doubles of LDS, kept within 64 KiB
+  if (nd <= 512) {
+    size_t sh = sizeof(double) * nd * 16;
+    hipLaunchKernelGGL(k_gibbs<16>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
+  } else if (nd <= 1024) {
+    size_t sh = sizeof(double) * nd * 8;
+    hipLaunchKernelGGL(k_gibbs<8>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
+  } else {
+    size_t sh = sizeof(double) * nd * 4;
+    hipLaunchKernelGGL(k_gibbs<4>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// standalone component kernels (single problem, for parity checks)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(GT) k_find_next_gamma(int P, double gprev, const double* __restrict__ L,
+                                                        double cv, double tol, int cap, double* __restrict__ out3,
+                                                        double* __restrict__ w) {
+  extern __shared__ double smem[];
+  double* Ls = smem;
+  double* ws = Ls + P;
+  double* red = ws + P;
+  for (int i = threadIdx.x; i < P; i += GT) Ls[i] = L[i];
+  __syncthreads();
+  double nev;
+  double g = gamma_search(Ls, ws, P, gprev, cv, tol, cap, red, &nev);
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += GT) w[i] = ws[i];
+  if (threadIdx.x == 0) { out3[0] = g; out3[1] = nev; }
+}
+
+hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,
+                                  double* out3, double* w, hipStream_t s) {
+  size_t sh = sizeof(double) * (2 * (size_t)P + GT);
+  hipLaunchKernelGGL(k_find_next_gamma, dim3(1), dim3(GT), sh, s, P, gprev, L, cv, tol, cap, out3, w);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(GT) k_normalize_resample(int P, const double* __restrict__ w, double uh, double ul,
+                                                           double* __restrict__ wn, int* __restrict__ assigned,
+                                                           int* __restrict__ parent) {
+  extern __shared__ double smem[];
+  double* ws = smem;
+  double* red = ws + P;
+  dd* scan = (dd*)(red + GT);
+  for (int i = threadIdx.x; i < P; i += GT) ws[i] = w[i];
+  __syncthreads();
+  normalize_resample(ws, P, two_sum(uh, ul), assigned, parent, red, scan);
+  for (int i = threadIdx.x; i < P; i += GT) wn[i] = ws[i];
+}
+
+hipError_t launch_normalize_resample(int P, const double* w, double uh, double ul, double* wn, int* assigned,
+                                     hipStream_t s) {
+  size_t sh = sizeof(double) * ((size_t)P + GT) + sizeof(dd) * (GT + 1);
+  int* parent = assigned + P;
+  hipLaunchKernelGGL(k_normalize_resample, dim3(1), dim3(GT), sh, s, P, w, uh, ul, wn, assigned, parent);
+  return hipGetLastError();
+}
+
+__global__ void k_trunc_normal(int n, const double* mu, const double* sg, double lo, double hi, const double* u,
+                               double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = trunc_normal(mu[i], sg[i], lo, hi, u[i]);
+}
+hipError_t launch_trunc_normal(int n, const double* mu, const double* sig, double lo, double hi, const double* u,
+                               double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_trunc_normal, dim3((n + 255) / 256), dim3(256), 0, s, n, mu, sig, lo, hi, u, out);
+  return hipGetLastError();
+}
+
+__global__ void k_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* c = ctr + 4 * i;
+  uint64_t k = ((uint64_t)key[2 * i + 1] << 32) | key[2 * i];
+  uint64_t w0, w1;
+  philox_draw(k, c[3] >> 24, c[3] & 0xFFFFFFu, c[2], c[1], c[0], w0, w1);
+  out[4 * i] = (uint32_t)w0;
+  out[4 * i + 1] = (uint32_t)(w0 >> 32);
+  out[4 * i + 2] = (uint32_t)w1;
+  out[4 * i + 3] = (uint32_t)(w1 >> 32);
+}
+hipError_t launch_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_philox, dim3((n + 255) / 256), dim3(256), 0, s, n, ctr, key, out);
+  return hipGetLastError();
+}
+
+}  // namespace smcf

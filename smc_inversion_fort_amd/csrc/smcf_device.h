@@ -1,0 +1,251 @@
+// Device helpers shared by the kernels of libsmcf: keyed counter-based RNG,
+// double-double arithmetic and the truncated-normal inversion of the Gibbs mover.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SMCF_DEV __device__ __forceinline__
+
+namespace smcf {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  A draw is addressed by
+//   key     = 64-bit instance key
+//   counter = (c0, c1, c2, site << 24 | stage)
+// and yields two 64-bit words.  Every random number of the inner and outer SMC is
+// one such draw, so results do not depend on launch order or thread count.
+// ---------------------------------------------------------------------------
+enum Site : uint32_t {
+  kSiteGibbsInit = 1,
+  kSiteResample = 2,
+  kSiteTuneDtau = 3,
+  kSiteTuneMom = 4,
+  kSiteTuneMH = 5,
+  kSiteTuneResample = 6,
+  kSiteHmcMom = 7,
+  kSiteHmcMH = 8,
+  kSiteGibbsMove = 9,
+  kSiteOuterInit = 16,
+  kSiteOuterResample = 17,
+  kSiteOuterProp = 18,
+  kSiteOuterMH = 19,
+};
+
+SMCF_DEV void philox_draw(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1,
+                          uint32_t c0, uint64_t& w0, uint64_t& w1) {
+  uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = (site << 24) | (stage & 0xFFFFFFu);
+  uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
+    uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
+    uint32_t y0 = hi1 ^ x1 ^ k0, y2 = hi0 ^ x3 ^ k1;
+    x0 = y0; x1 = lo1; x2 = y2; x3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  w0 = ((uint64_t)x1 << 32) | x0;
+  w1 = ((uint64_t)x3 << 32) | x2;
+}
+
+SMCF_DEV double u01(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+// strictly inside (0, 1): used by the inversion sampler
+SMCF_DEV double u01_open(uint64_t w) { return ((double)(w >> 11) + 0.5) * 0x1.0p-53; }
+// reference clamp of the intrinsic uniform at 1e-5 (smc_slip.f90:8-15)
+SMCF_DEV double clamp_u(double u) { return u < 1e-5 ? 1e-5 : u; }
+// Box-Muller, sine branch (smc_slip.f90:26-38)
+SMCF_DEV double box_muller_sin(double r, double s) {
+  const double pi = 3.141592653589793;  // 2*asin(1) in double
+  return sqrt(-2.0 * log(r)) * sin(2.0 * pi * s);
+}
+SMCF_DEV double draw_normal(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1) {
+  uint64_t w0, w1;
+  philox_draw(key, site, stage, c2, c1, 0, w0, w1);
+  return box_muller_sin(clamp_u(u01(w0)), clamp_u(u01(w1)));
+}
+
+// ---------------------------------------------------------------------------
+// double-double (Dekker / Knuth error-free transformations)
+// ---------------------------------------------------------------------------
+struct dd {
+  double hi, lo;
+};
+SMCF_DEV dd two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+SMCF_DEV dd quick_two_sum(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+SMCF_DEV dd dd_add(dd a, dd b) {
+  dd s = two_sum(a.hi, b.hi);
+  dd t = two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = quick_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+SMCF_DEV dd dd_add_d(dd a, double b) {
+  dd s = two_sum(a.hi, b);
+  s.lo += a.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+SMCF_DEV dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+SMCF_DEV dd dd_mul_d(dd a, double b) {
+  double p = a.hi * b;
+  double e = fma(a.hi, b, -p);
+  e = fma(a.lo, b, e);
+  return quick_two_sum(p, e);
+}
+SMCF_DEV dd dd_div_d(dd a, double b) {
+  double q1 = a.hi / b;
+  dd r = dd_add(a, dd_neg(dd_mul_d(dd{q1, 0.0}, b)));
+  double q2 = r.hi / b;
+  return quick_two_sum(q1, q2);
+}
+// round(x / s) for double x and double-double s
+SMCF_DEV double d_div_dd(double x, dd s) {
+  double q1 = x / s.hi;
+  // r = x - q1*s exactly enough
+  double p = q1 * s.hi;
+  double e = fma(q1, s.hi, -p);
+  double r = ((x - p) - e) - q1 * s.lo;
+  return q1 + r / s.hi;
+}
+SMCF_DEV double dd_floor(dd a) {
+  double f = floor(a.hi);
+  if (f == a.hi && a.lo < 0.0) f -= 1.0;
+  return f;
+}
+SMCF_DEV bool dd_lt_d(dd a, double b) { return a.hi < b || (a.hi == b && a.lo < 0.0); }
+
+// ---------------------------------------------------------------------------
+// Standard normal quantile, Wichura (1988) AS 241 PPND16 (relative error ~1e-16)
+// ---------------------------------------------------------------------------
+SMCF_DEV double ndtri(double p) {
+  double q = p - 0.5, r, z;
+  if (fabs(q) <= 0.425) {
+    r = 0.180625 - q * q;
+    return q * (((((((2509.0809287301226727 * r + 33430.575583588128105) * r + 67265.770927008700853) * r +
+                    45921.953931549871457) * r + 13731.693765509461125) * r + 1971.5909503065514427) * r +
+                 133.14166789178437745) * r + 3.387132872796366608) /
+           (((((((5226.495278852545925 * r + 28729.085735721942674) * r + 39307.89580009271061) * r +
+                21213.794301586595867) * r + 5394.1960214247511077) * r + 687.1870074920579083) * r +
+             42.313330701600911252) * r + 1.0);
+  }
+  r = q < 0 ? p : 1.0 - p;
+  r = sqrt(-log(r));
+  if (r <= 5.0) {
+    r -= 1.6;
+    z = (((((((7.7454501427834140764e-4 * r + 0.0227238449892691845833) * r + 0.24178072517745061177) * r +
+             1.27045825245236838258) * r + 3.64784832476320460504) * r + 5.7694972214606914055) * r +
+          4.6303378461565452959) * r + 1.42343711074968357734) /
+        (((((((1.05075007164441684324e-9 * r + 5.475938084995344946e-4) * r + 0.0151986665636164571966) * r +
+             0.14810397642748007459) * r + 0.68976733498510000455) * r + 1.6763848301838038494) * r +
+          2.05319162663775882187) * r + 1.0);
+  } else {
+    r -= 5.0;
+    z = (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r + 0.0012426609473880784386) * r +
+             0.026532189526576123093) * r + 0.29656057182850489123) * r + 1.7848265399172913358) * r +
+          5.4637849111641143699) * r + 6.6579046435011037772) /
+        (((((((2.04426310338993978564e-15 * r + 1.4215117583164458887e-7) * r + 1.8463183175100546818e-5) * r +
+             7.868691311456132591e-4) * r + 0.0148753612908506148525) * r + 0.13692988092273580531) * r +
+          0.59983220655588793769) * r + 1.0);
+  }
+  return q < 0 ? -z : z;
+}
+
+// Standard normal truncated to [a, b] by inversion; the upper tail works with
+// complementary probabilities so that far-tail truncation keeps full precision.
+// Mills ratio series R(x) x = 1 - x^-2 + 3x^-4 - ... and log Q(x), valid for x >= 35
+SMCF_DEV double mills_series(double x) {
+  double r = 1.0 / (x * x);
+  return 1.0 + r * (-1.0 + r * (3.0 + r * (-15.0 + r * (105.0 + r * (-945.0 + r * (10395.0 + r * -135135.0))))));
+}
+SMCF_DEV double logq_far(double x) { return -0.5 * x * x - log(x) - 0.91893853320467274178 + log(mills_series(x)); }
+// beyond 35 sigma: Newton on log Q(z) = log Q(a) + log(1 - u (1 - Q(b)/Q(a)))
+SMCF_DEV double tn_far(double a, double b, double u) {
+  double la = logq_far(a), lb = logq_far(b);
+  double T = la + log1p(-u * (-expm1(lb - la)));
+  double z = a - log1p(-u * (-expm1(-a * (b - a)))) / a;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    double g = logq_far(z) - T;
+    z = z + g * (mills_series(z) / z);
+  }
+  return fmin(fmax(z, a), b);
+}
+SMCF_DEV double tn_upper(double a, double b, double u) {  // requires a > 0
+  if (a > 35.0) return tn_far(a, b, u);
+  const double r2 = 0.70710678118654752440;
+  double qa = 0.5 * erfc(a * r2), qb = 0.5 * erfc(b * r2);
+  double p = qa - u * (qa - qb);
+  double z = -ndtri(p);
+  return fmin(fmax(z, a), b);
+}
+SMCF_DEV double tn_std(double a, double b, double u) {
+  const double r2 = 0.70710678118654752440;
+  if (a > 0.0) return tn_upper(a, b, u);
+  if (b < 0.0) return -tn_upper(-b, -a, 1.0 - u);  // mirror: quantile u of z is 1-u of -z
+  double pa = 0.5 * erfc(-a * r2), pb = 0.5 * erfc(-b * r2);
+  double p = pa + u * (pb - pa), z;
+  if (p <= 0.5) {
+    z = ndtri(p);
+  } else {
+    double qa = 0.5 * erfc(a * r2), qb = 0.5 * erfc(b * r2);
+    double qq = qb + (1.0 - u) * (qa - qb);
+    z = -ndtri(qq);
+  }
+  return fmin(fmax(z, a), b);
+}
+SMCF_DEV double trunc_normal(double mu, double sigma, double lo, double hi, double u) {
+  double a = (lo - mu) / sigma, b = (hi - mu) / sigma;
+  double x = mu + sigma * tn_std(a, b, u);
+  return fmin(fmax(x, lo), hi);
+}
+
+// cdf_norm / pdf_norm of smc_slip.f90:40-54
+SMCF_DEV double cdf_norm(double x, double mu, double sigma2) {
+  return (1.0 + erf((x - mu) / sqrt(2.0 * sigma2))) / 2.0;
+}
+SMCF_DEV double pdf_norm(double x, double mu, double sigma2) {
+  const double pi = 3.141592653589793;
+  return exp(-((x - mu) * (x - mu)) / (2.0 * sigma2)) / sqrt(2.0 * pi * sigma2);
+}
+
+// ---------------------------------------------------------------------------
+// block reductions (deterministic: fixed tree for a fixed block size)
+// ---------------------------------------------------------------------------
+template <int NT>
+SMCF_DEV double block_sum(double v, double* red) {
+  int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = red[t] + red[t + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+template <int NT>
+SMCF_DEV double block_min(double v, double* red) {
+  int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmin(red[t], red[t + s]);
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace smcf

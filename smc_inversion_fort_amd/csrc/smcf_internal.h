@@ -1,0 +1,145 @@
+// Internal structures of libsmcf (host side + kernel launch interfaces).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/smcf.h"
+
+namespace smcf {
+
+void set_error(const std::string& msg);
+
+#define SMCF_HIP(call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      ::smcf::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                   \
+      return SMCF_EHIP;                                                                       \
+    }                                                                                         \
+  } while (0)
+
+// Mesh/observation description resident on the device (read-only, shared by a batch).
+struct MeshDev {
+  int nplane, nnode, ndof, npatch, ndim, ndim_fault;
+  int npts, nobs, nsar, ngnss;
+  int fix_xbend;
+  double xmin, xmax, zmin;
+  const double* xbend;      // nplane-1 (>=1 allocated)
+  const int* plane_nxi;     // nplane
+  const int* plane_neta;    // nplane
+  const int* plane_node0;   // nplane: first node (0-based)
+  const int* cny;           // 4 x npatch, 1-based node ids
+  const int* patch_plane;   // npatch
+  const int* node_plane;    // nnode
+  const int* node_k;        // nnode: 1-based column index inside its plane
+  const int* node_l;        // nnode: 1-based row index inside its plane
+  const int* id_dof;        // ndof, 1-based node ids
+  const int* dof_of_node;   // nnode, 0-based dof index or -1
+  const int* n2e;           // 4 x nnode, 1-based patch ids (node_to_elem_val)
+  const int* n2e_slot;      // 4 x nnode, position of the node inside that patch (0..3)
+  const int* n2es;          // nnode
+  const double* pts;        // 2 x npts (SAR points then GNSS stations)
+  const int* pt_row0;       // npts: first observation row (0-based)
+  const int* pt_nrow;       // npts: 1 (SAR) or 3 (GNSS)
+  const double* unitvec;    // 3 x nobs
+  const double* obs_sigma;  // nobs
+  const double* dvec;       // nobs
+};
+
+// Per-batch device state.  Index b is the instance (geometry particle) slot.
+struct BatchDev {
+  int cap;                  // instance capacity
+  int P;                    // inner particles
+  double* theta;            // ndim_fault x cap
+  double* geo;              // 16 x nplane x cap
+  double* scal;             // 8 x cap: alpha2, log_sigma_sar2, log_sigma_gnss2, like_const, sigma_gnss2
+  double* G;                // nobs x ndim x cap (column-major per instance)
+  double* gscr;             // 24 x npts x npatch x cap (Green's phase-1 partial responses)
+  double* luni;             // 9 x nnode x cap (stencil slots)
+  int* li;                  // 9 x 2nnode x cap (ELL of L, 1-based columns)
+  double* lv;
+  int* lti;                 // 9 x 2ndof x cap (ELL of L^T, 1-based rows)
+  double* ltv;
+  int* llc_idx;             // 25 x ndim x cap (sparse columns of L^T L, 0-based rows, -1 pad)
+  double* llc_val;
+  double* H;                // ndim x ndim x cap
+  double* bvec;             // ndim x cap
+  double* dsd;              // cap
+  double* Q;                // ndim x ndim x cap (Gibbs precision)
+  double* S;                // ndim x P x cap, dimension-major (S[i*P + p])
+  double* S2;
+  double* Y;                // ndim x P x cap (H S)
+  double* like;             // P x cap
+  double* prior;
+  double* like2;
+  double* prior2;
+  double* w;                // P x cap
+  int* parent;              // P x cap
+  int* assigned;            // P x cap
+  double* mean;             // ndim x cap
+  double* cov;              // ndim x cap
+  // stage state
+  uint64_t* key;            // cap
+  double* gamma;            // cap
+  double* neglog;           // cap
+  int* iter;                // cap
+  int* active;              // cap: runs the mover this stage
+  int* done;                // cap
+  int* par;                 // cap: which particle buffer (S or S2) is current
+  double* gtrace;           // 202 x cap
+};
+
+struct Problem {
+  smcf_ctx* ctx;
+  MeshDev m;
+  BatchDev bt;
+  double max_slip;
+  int P, mover, sweeps;
+  std::vector<void*> allocs;
+};
+
+struct StageCfg {
+  double max_slip;
+  int P, sweeps;
+};
+
+// kernels (launchers return hipError_t of the launch)
+hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s);
+hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s);
+hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
+hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
+
+hipError_t launch_dc3d0_batch(int n, const double* in, double* u, int* iret, hipStream_t s);
+hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,
+                                  double* out3, double* w, hipStream_t s);
+hipError_t launch_normalize_resample(int P, const double* w, double uh, double ul, double* wn,
+                                     int* assigned, hipStream_t s);
+hipError_t launch_trunc_normal(int n, const double* mu, const double* sig, double lo, double hi,
+                               const double* u, double* out, hipStream_t s);
+hipError_t launch_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out, hipStream_t s);
+
+}  // namespace smcf
+
+struct smcf_ctx {
+  int device;
+  hipStream_t stream;
+  bool profiling;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  struct Stat {
+    std::string name;
+    long count;
+    double total_ms;
+  };
+  std::vector<Stat> stats;
+};

@@ -1,0 +1,120 @@
+"""Seeded synthetic inversion problems (observation sets, truth, priors).
+
+The reference ships no data, so every workload is generated here from a seed:
+GNSS 3-component stations uniform in a square box centred on the fault-trace
+midpoint, a surface-breaking rectangular fault (strike 30 deg, length 40 km,
+dip 45 deg, width 20 km), a Gaussian slip bump (peak 3 m) on the mesh nodes,
+Gaussian noise.  Units follow the reference: km, m of slip, cm of displacement
+(gfunc.f90:97-99).  Data are d = G(theta_true) s_true + noise, with G supplied
+by the caller (device or any other implementation of calc_greens_func).
+"""
+import math
+
+import numpy as np
+
+# configuration sizes (patch grid, stations, noise sigma [cm], outer N, inner P, mover)
+CONFIGS = {
+    "c1": dict(nxi=8, neta=8, nsta=100, sigma=0.5, N=1000, P=1000, mover="hmc", sweeps=5),
+    "c2": dict(nxi=16, neta=16, nsta=400, sigma=0.5, N=10000, P=2000, mover="gibbs", sweeps=5),
+    "c3": dict(nxi=32, neta=16, nsta=1000, sigma=0.3, N=20000, P=2000, mover="hmc10", sweeps=5),
+}
+
+STRIKE_DEG = 30.0
+LENGTH_KM = 40.0
+DIP_DEG = 45.0
+WIDTH_KM = 20.0
+MAX_SLIP = 10.0
+PEAK_SLIP = 3.0
+BOX_KM = 100.0
+# hyper-parameter prior ranges fixed by this project (the reference reads them from range.dat)
+LOG_ALPHA2_RANGE = (-6.0, 0.0)
+LOG_SIGMA_SAR2_RANGE = (-1.0, 1.0)
+LOG_SIGMA_GNSS2_RANGE = (-1.0, 1.0)
+LOG_ALPHA2_TRUE = -3.0
+
+
+def fault_frame():
+    """xmin, xmax, zmin and the true (ymin, ymax, dip) for the surface-breaking fault."""
+    s = math.radians(STRIKE_DEG)
+    dx, dy = LENGTH_KM * math.sin(s), LENGTH_KM * math.cos(s)
+    xmin, xmax = -dx / 2, dx / 2
+    ymin, ymax = -dy / 2, dy / 2
+    zmin = -WIDTH_KM * math.sin(math.radians(DIP_DEG))
+    return xmin, xmax, zmin, (ymin, ymax, DIP_DEG)
+
+
+def theta_true():
+    _, _, _, (ymin, ymax, dip) = fault_frame()
+    return np.array([ymin, ymax, dip, LOG_ALPHA2_TRUE, 0.0, 0.0])
+
+
+def prior_range(frac=0.2):
+    """2 x 6 range (range.dat layout): +-frac of the true geometry, fixed hyper-parameter ranges."""
+    _, _, _, (ymin, ymax, dip) = fault_frame()
+    r = np.zeros((2, 6))
+    for k, v in enumerate((ymin, ymax, dip)):
+        lo, hi = v * (1 - frac), v * (1 + frac)
+        r[:, k] = (min(lo, hi), max(lo, hi))
+    r[:, 3] = LOG_ALPHA2_RANGE
+    r[:, 4] = LOG_SIGMA_SAR2_RANGE
+    r[:, 5] = LOG_SIGMA_GNSS2_RANGE
+    return r
+
+
+def stations(nsta, rng):
+    xy = rng.uniform(-BOX_KM / 2, BOX_KM / 2, size=(2, nsta))
+    nobs = 3 * nsta
+    pts = np.zeros((2, nobs), order="F")
+    uv = np.zeros((3, nobs), order="F")
+    for i in range(nsta):
+        for c in range(3):
+            pts[:, 3 * i + c] = xy[:, i]
+            uv[c, 3 * i + c] = 1.0
+    return pts, uv
+
+
+def truth_slip(nxi, neta, theta):
+    """Gaussian bump on the DOF nodes (both slip components >= 0), DOF order of discretize_fault."""
+    xmin, xmax, zmin, _ = fault_frame()
+    ymin, ymax, dip = theta[0], theta[1], theta[2]
+    lxi = math.hypot(xmax - xmin, ymax - ymin)
+    leta = -zmin / math.sin(math.radians(dip))
+    dxi, deta = lxi / nxi, leta / neta
+    vals = []
+    for j in range(1, neta + 2):
+        for i in range(1, nxi + 2):
+            if j == 1 or i == 1 or i == nxi + 1:
+                continue
+            xi = (i - 1) * dxi - lxi / 2
+            eta = (j - 1) * deta - leta / 2
+            a = PEAK_SLIP * math.exp(-(xi ** 2) / (2 * (lxi / 5) ** 2) - (eta ** 2) / (2 * (leta / 4) ** 2))
+            vals += [a, a]
+    return np.array(vals)
+
+
+def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, neta=None):
+    """Build the observation dict of config `cfg_name` (sizes overridable for small tests).
+
+    greens(theta, obs_points, obs_unitvec, nxi, neta) -> G (nobs x ndim) at the true geometry.
+    """
+    cfg = dict(CONFIGS[cfg_name])
+    nsta = nsta or cfg["nsta"]
+    sigma = sigma if sigma is not None else cfg["sigma"]
+    nxi = nxi or cfg["nxi"]
+    neta = neta or cfg["neta"]
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0x5A17]))
+    pts, uv = stations(nsta, rng)
+    th = theta_true()
+    G = greens(th, pts, uv, nxi, neta)
+    s = truth_slip(nxi, neta, th)
+    d = G @ s + sigma * rng.standard_normal(G.shape[0])
+    return dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.full(G.shape[0], sigma), dvec=d,
+                nsar_total=0, ngnss=nsta, nsar_index=np.array([1, 1], np.int32), nxi=nxi, neta=neta,
+                slip_true=s)
+
+
+def sample_thetas(B, seed, frac=0.2):
+    """B geometry particles drawn uniformly from the prior box (fault_sample_init_particles)."""
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0x7E7A]))
+    r = prior_range(frac)
+    return np.asfortranarray(r[0][:, None] + rng.uniform(size=(6, B)) * (r[1] - r[0])[:, None])

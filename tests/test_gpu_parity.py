@@ -1,0 +1,263 @@
+"""Parity of the HIP library (through its C ABI) with the golden vectors of the reference's
+own Fortran and with the CPU oracle.  Needs an MI355X."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import smc_inversion_fort_amd as S
+from smc_inversion_fort_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def test_philox_known_answers(ctx):
+    say("case: philox KAT")
+    ctr = np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], np.uint32)
+    key = np.array([[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]], np.uint32)
+    out = ctx.philox(ctr, key)
+    assert out[0].tolist() == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert out[1].tolist() == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert out[2].tolist() == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+    rng = np.random.default_rng(0)
+    c = rng.integers(0, 2 ** 32, size=(500, 4), dtype=np.uint64).astype(np.uint32)
+    k = rng.integers(0, 2 ** 32, size=(500, 2), dtype=np.uint64).astype(np.uint32)
+    d = ctx.philox(c, k)
+    for i in range(0, 500, 37):
+        assert d[i].tolist() == O.philox(c[i].tolist(), k[i].tolist())
+
+
+def test_dc3d0_golden(ctx, golden):
+    say("case: dc3d0 golden")
+    g = golden("dc3d0")
+    u, iret = ctx.dc3d0(g["inputs"])
+    np.testing.assert_array_equal(iret, g["iret"])
+    for k in range(len(u)):
+        scale = max(np.abs(g["u"][k]).max(), 1e-300)
+        assert np.abs(u[k] - g["u"][k]).max() <= 1e-12 * scale, k
+
+
+FAULTS = {
+    "p1_3x2": (1, [3], [2], 0, -10.0, 10.0, -14.142, [], [-17.3, 17.1, 45.0, -1.0, 0.2, 0.1]),
+    "p1_4x3": (1, [4], [3], 0, -9.0, 11.0, -12.0, [], [-15.0, 19.5, 38.0, 0.5, 0.0, -0.3]),
+    "p1_8x8": (1, [8], [8], 0, -10.0, 10.0, -14.142, [], [-17.32, 17.32, 45.0, -2.0, 0.0, 0.0]),
+    "p2_free": (2, [3, 2], [2, 2], 0, -10.0, 12.0, -10.0, [], [-14.0, 16.0, 1.0, 2.0, 50.0, 70.0, 0.1, 0.2, 0.3]),
+    "p2_fixed": (2, [2, 3], [3, 2], 1, -8.0, 9.0, -11.0, [0.5], [-12.0, 13.0, 1.5, 55.0, 42.0, 0.0, 0.0, 0.0]),
+}
+
+
+def golden_problem(ctx, golden, name, P=64):
+    nplane, nxi, neta, fix, xmin, xmax, zmin, xbend, theta = FAULTS[name]
+    g = golden("greens")
+    nsi = g[name + "__nsar_index"]
+    nobs = g[name + "__obs_points"].shape[1]
+    obs = dict(obs_points=g[name + "__obs_points"], obs_unitvec=g[name + "__obs_unitvec"],
+               obs_sigma=np.ones(nobs), dvec=np.zeros(nobs), nsar_total=int(nsi[-1] - 1),
+               ngnss=int(g[name + "__ngnss"]), nsar_index=nsi)
+    pb = S.Problem(ctx, nxi, neta, xmin, xmax, zmin, obs, 10.0, P, fix_xbend=fix, xbend=xbend)
+    return pb, theta
+
+
+@pytest.mark.parametrize("name", list(FAULTS))
+def test_greens_golden(ctx, golden, name):
+    say("case: greens golden", name)
+    pb, theta = golden_problem(ctx, golden, name)
+    G = pb.calc_greens_func(theta)
+    Gr = golden("greens")[name + "__gmat"]
+    colmax = np.abs(Gr).max(axis=0)
+    err = np.abs(G - Gr) / colmax[None, :]
+    assert err.max() <= 1e-10, err.max()
+    # normwise per column
+    nrm = np.linalg.norm(G - Gr, axis=0) / np.linalg.norm(Gr, axis=0)
+    assert nrm.max() <= 1e-10
+    pb.close()
+
+
+@pytest.mark.parametrize("name", list(FAULTS))
+def test_laplacian_golden(ctx, golden, name):
+    say("case: laplacian golden", name)
+    pb, theta = golden_problem(ctx, golden, name)
+    lap = pb.laplacian(theta)
+    g = golden("mesh")
+    for k in ("li", "lv", "lti", "ltv"):
+        np.testing.assert_array_equal(lap[k], g[name + "__" + k], err_msg=k)
+    lm = g[name + "__lmat"]
+    f = O.Fault(*FAULTS[name][1:3], *FAULTS[name][4:7], fix_xbend=FAULTS[name][3], xbend=FAULTS[name][7])
+    np.testing.assert_array_equal(lap["ll"], f.laplacian(theta)["ll"])
+    pb.close()
+
+
+def small_obs(seed=3, nsta=12, nxi=4, neta=3, sigma=0.5):
+    def greens(th, pts, uv, nx, ne):
+        xmin, xmax, zmin, _ = syn.fault_frame()
+        return O.Fault([nx], [ne], xmin, xmax, zmin).greens(th, pts, uv, 0, pts.shape[1] // 3)
+    return syn.make_observations("c2", seed, greens, nsta=nsta, sigma=sigma, nxi=nxi, neta=neta)
+
+
+def oracle_problem(obs, P, sweeps=3):
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    f = O.Fault([obs["nxi"]], [obs["neta"]], xmin, xmax, zmin)
+    return O.ProblemHolder(f, obs, syn.MAX_SLIP, P, mover=1, gibbs_sweeps=sweeps)
+
+
+def device_problem(ctx, obs, P, sweeps=3, max_batch=4):
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    return S.Problem(ctx, [obs["nxi"]], [obs["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, P,
+                     mover=S.MOVER_GIBBS, gibbs_sweeps=sweeps, max_batch=max_batch)
+
+
+def test_gram_vs_oracle(ctx):
+    say("case: gram vs oracle")
+    obs = small_obs()
+    pb = device_problem(ctx, obs, 64)
+    th = syn.sample_thetas(3, 11)
+    H, b, dsd = pb.slip_calc_gsd_gsg(th)
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    for k in range(3):
+        G = O.Fault([4], [3], xmin, xmax, zmin).greens(th[:, k], obs["obs_points"], obs["obs_unitvec"], 0,
+                                                        obs["ngnss"])
+        Hr, br, dr = O.gsd_gsg(G, obs["dvec"], obs["obs_sigma"], math.exp(th[4, k]), math.exp(th[5, k]))
+        np.testing.assert_allclose(H[:, :, k], Hr, rtol=1e-11, atol=1e-11 * np.abs(Hr).max())
+        np.testing.assert_allclose(b[:, k], br, rtol=1e-11, atol=1e-11 * np.abs(br).max())
+        assert dsd[k] == pytest.approx(dr, rel=1e-12)
+    pb.close()
+
+
+def test_find_next_gamma_golden_and_oracle(ctx, golden):
+    say("case: gamma golden")
+    g = golden("gamma")
+    i = 0
+    while "c%d__L" % i in g:
+        gam, w, ev = ctx.find_next_gamma(float(g["c%d__gprev" % i]), g["c%d__L" % i])
+        assert gam == pytest.approx(float(g["c%d__gamma" % i]), abs=1e-12)
+        np.testing.assert_allclose(w, g["c%d__w" % i], rtol=1e-13, atol=1e-300)
+        assert ev == pytest.approx(float(g["c%d__neglog_ev" % i]), rel=1e-12, abs=1e-12)
+        i += 1
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        L = rng.uniform(0, 10 ** rng.uniform(1, 7), size=int(rng.integers(2, 3000)))
+        gp = float(rng.choice([0.0, rng.uniform(0, 0.9)]))
+        gd, wd, evd = ctx.find_next_gamma(gp, L)
+        go, wo, evo = O.find_next_gamma(gp, L)
+        assert gd == pytest.approx(go, abs=1e-12)
+        assert evd == pytest.approx(evo, rel=1e-11, abs=1e-11)
+        # outer settings: CV 0.2, tol 1e-8, cap 1000
+        gd, _, _ = ctx.find_next_gamma(gp, L, cv=0.2, tol=1e-8, cap=1000)
+        go, _, _ = O.find_next_gamma(gp, L, cv=0.2, tol=1e-8, cap=1000)
+        assert gd == pytest.approx(go, abs=1e-10)
+
+
+def test_normalize_resample_bit_exact(ctx, golden):
+    say("case: resample golden")
+    g = golden("resample")
+    i = 0
+    while "c%d__w" % i in g:
+        u = g["c%d__u" % i]
+        wn, a = ctx.normalize_resample(g["c%d__w" % i], u[0], u[1])
+        np.testing.assert_array_equal(a, g["c%d__assigned" % i])
+        i += 1
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        P = int(rng.integers(1, 3000))
+        w = rng.exponential(size=P) ** rng.uniform(1, 6) * (rng.uniform(size=P) > rng.uniform(0, 0.9))
+        if w.sum() == 0:
+            w[-1] = 1.0
+        uh = float(rng.uniform())
+        ul = float(rng.uniform()) * 2.0 ** -53
+        wn_d, a_d = ctx.normalize_resample(w, uh, ul)
+        wn_o = O.normalize_quad(w)
+        np.testing.assert_array_equal(wn_d, wn_o)
+        np.testing.assert_array_equal(a_d, O.resample_quad(wn_o, uh, ul))
+        assert a_d.sum() == P
+
+
+def test_trunc_normal_vs_oracle(ctx):
+    say("case: trunc normal")
+    rng = np.random.default_rng(5)
+    n = 20000
+    mu = rng.normal(0, 20, n) * np.exp(rng.uniform(-3, 1, n))
+    sg = np.exp(rng.uniform(-6, 2, n))
+    u = (rng.integers(0, 2 ** 53, n).astype(np.float64) + 0.5) * 2.0 ** -53
+    x = ctx.trunc_normal(mu, sg, 0.0, 10.0, u)
+    xo = np.array([O.trunc_normal(mu[i], sg[i], 0.0, 10.0, u[i]) for i in range(n)])
+    assert np.all((x >= 0) & (x <= 10))
+    np.testing.assert_allclose(x, xo, rtol=1e-10, atol=1e-12)
+
+
+def test_slip_smc_exec_vs_oracle(ctx):
+    say("case: slip_smc_exec small vs oracle")
+    obs = small_obs()
+    P = 256
+    pb = device_problem(ctx, obs, P)
+    po = oracle_problem(obs, P)
+    theta = syn.theta_true()
+    key = O.inner_key(7, 1, 1, 3)
+    d = pb.slip_smc_exec(theta, key)
+    G = pb.calc_greens_func(theta)
+    o = po.slip_smc(theta, G, key)
+    say("stages", d["nstage"], o["nstage"], "neglog", d["neglog"], o["neglog"])
+    assert d["nstage"] == o["nstage"]
+    np.testing.assert_allclose(d["gamma"], o["gamma"], rtol=0, atol=1e-9)
+    assert d["neglog"] == pytest.approx(o["neglog"], rel=1e-8)
+    np.testing.assert_allclose(d["particles"], o["particles"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(d["likelihood"], o["likelihood"], rtol=1e-8)
+    np.testing.assert_allclose(d["prior"], o["prior"], rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(d["mean"], o["mean"], rtol=1e-8, atol=1e-10)
+    pb.close()
+
+
+def test_fault_likelihood_batch_vs_oracle_and_alone(ctx):
+    say("case: fault likelihood batch")
+    obs = small_obs(seed=8)
+    P = 128
+    pb = device_problem(ctx, obs, P, max_batch=4)
+    po = oracle_problem(obs, P)
+    th = syn.sample_thetas(4, 21)
+    keys = [O.inner_key(99, 0, 0, i) for i in range(4)]
+    nl, ns = pb.fault_calc_likelihood(th, keys)
+    for k in range(4):
+        ref, nst = po.fault_likelihood(th[:, k], keys[k])
+        say("theta", k, nl[k], ref, ns[k], nst)
+        assert ns[k] == nst
+        assert nl[k] == pytest.approx(ref, rel=1e-8)
+    # a result is the same alone and inside a batch
+    alone, _ = pb.fault_calc_likelihood(th[:, 2:3], keys[2:3])
+    assert alone[0] == nl[2]
+    pb.close()
+
+
+def test_c2_size_greens_and_gram_vs_oracle(ctx):
+    """Full benchmark size (16x16 patches, 400 stations): G within 1e-10 of the oracle."""
+    say("case: c2-size greens/gram")
+    rng = np.random.default_rng(12)
+    pts, uv = syn.stations(400, rng)
+    nobs = pts.shape[1]
+    obs = dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.full(nobs, 0.5), dvec=rng.normal(size=nobs),
+               nsar_total=0, ngnss=400)
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    pb = S.Problem(ctx, [16], [16], xmin, xmax, zmin, obs, 10.0, 64)
+    th = syn.sample_thetas(1, 3)[:, 0]
+    G = pb.calc_greens_func(th)
+    Gr = O.Fault([16], [16], xmin, xmax, zmin).greens(th, pts, uv, 0, 400)
+    colmax = np.abs(Gr).max(axis=0)
+    assert (np.abs(G - Gr) / colmax).max() <= 1e-10
+    H, b, dsd = pb.slip_calc_gsd_gsg(th)
+    Hr, br, dr = O.gsd_gsg(Gr, obs["dvec"], obs["obs_sigma"], math.exp(th[4]), math.exp(th[5]))
+    assert np.abs(H - Hr).max() <= 1e-10 * np.abs(Hr).max()
+    assert dsd == pytest.approx(dr, rel=1e-12)
+    pb.close()
