@@ -26,6 +26,7 @@ struct smcf_problem {
   Problem p;
 };
 
+
 #define CHECK_LAUNCH(expr)                                                           \
   do {                                                                               \
     hipError_t e_ = (expr);                                                          \
@@ -119,13 +120,11 @@ int run_batch(Problem& P, int B, const double* theta, const uint64_t* keys, bool
     set_error("InSAR rows (dense per-path covariance) are not supported by the device Gram yet");
     return SMCF_EUNSUPPORTED;
   }
-  if (P.mover != SMCF_MOVER_GIBBS) {
-    set_error("only the Gibbs mover is available on the device in this build");
-    return SMCF_EUNSUPPORTED;
-  }
+  bool hmc = P.mover == SMCF_MOVER_HMC;
   SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * m.ndim_fault * B, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_state_init, dim3((B + 63) / 64), dim3(64), 0, s, bt, B);
+  if (hmc) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, B, s));
   {
     ProfScope ps(ctx, "geometry_greens");
     CHECK_LAUNCH(launch_geometry(m, bt, B, s));
@@ -141,23 +140,28 @@ int run_batch(Problem& P, int B, const double* theta, const uint64_t* keys, bool
     CHECK_LAUNCH(launch_init_chain(m, bt, B, P.max_slip, s));
   }
   CHECK_LAUNCH(launch_likelihood(m, bt, B, 0, s));
-  std::vector<int> done(B);
+  std::vector<int> done(B), amax(2 * (size_t)B);
   StageCfg cfg{P.max_slip, bt.P, P.sweeps};
   for (int stage = 1; stage <= 202; ++stage) {
     {
       ProfScope ps(ctx, "gamma_resample");
       CHECK_LAUNCH(launch_gamma_resample(m, bt, B, s));
     }
-    if (want_moments) CHECK_LAUNCH(launch_moments(m, bt, B, s));
-    CHECK_LAUNCH(launch_gather(m, bt, B, s));
-    CHECK_LAUNCH(launch_form_q(m, bt, B, s));
-    {
-      ProfScope ps(ctx, "gibbs");
-      CHECK_LAUNCH(launch_gibbs(m, bt, B, cfg, s));
-    }
-    {
-      ProfScope ps(ctx, "likelihood");
-      CHECK_LAUNCH(launch_likelihood(m, bt, B, 1, s));
+    if (want_moments || hmc) CHECK_LAUNCH(launch_moments(m, bt, B, s));
+    if (hmc) {
+      ProfScope ps(ctx, "hmc");
+      CHECK_LAUNCH(launch_hmc_stage(m, bt, P.hm, B, P.max_slip, s, amax.data()));
+    } else {
+      CHECK_LAUNCH(launch_gather(m, bt, B, s));
+      CHECK_LAUNCH(launch_form_q(m, bt, B, s));
+      {
+        ProfScope ps(ctx, "gibbs");
+        CHECK_LAUNCH(launch_gibbs(m, bt, B, cfg, s));
+      }
+      {
+        ProfScope ps(ctx, "likelihood");
+        CHECK_LAUNCH(launch_likelihood(m, bt, B, 1, s));
+      }
     }
     SMCF_HIP(hipMemcpyAsync(done.data(), bt.done, sizeof(int) * B, hipMemcpyDeviceToHost, s));
     SMCF_HIP(hipStreamSynchronize(s));
@@ -352,7 +356,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.H, nd * nd * C);
   rc |= dalloc(P, &bt.bvec, nd * C);
   rc |= dalloc(P, &bt.dsd, C);
-  rc |= dalloc(P, &bt.Q, nd * nd * C);
+  rc |= dalloc(P, &bt.Q, (nd + 16) * nd * C);  // blocked Gibbs layout, padded to whole blocks
   rc |= dalloc(P, &bt.S, nd * Pn * C);
   rc |= dalloc(P, &bt.S2, nd * Pn * C);
   rc |= dalloc(P, &bt.Y, nd * Pn * C);
@@ -373,6 +377,38 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.done, C);
   rc |= dalloc(P, &bt.par, C);
   rc |= dalloc(P, &bt.gtrace, 202 * C);
+  if (P.mover == SMCF_MOVER_HMC) {
+    HmcDev& h = P.hm;
+    size_t nn2 = 2 * (size_t)nnode;
+    rc |= dalloc(P, &h.Xc, nd * Pn * C);
+    rc |= dalloc(P, &h.Xt, nd * Pn * C);
+    rc |= dalloc(P, &h.Pm, nd * Pn * C);
+    rc |= dalloc(P, &h.LS, nn2 * Pn * C);
+    rc |= dalloc(P, &h.Lc, Pn * C);
+    rc |= dalloc(P, &h.Rc, Pn * C);
+    rc |= dalloc(P, &h.postc, Pn * C);
+    rc |= dalloc(P, &h.hcur, Pn * C);
+    rc |= dalloc(P, &h.dtau, Pn * C);
+    rc |= dalloc(P, &h.dtmp, Pn * C);
+    rc |= dalloc(P, &h.score, Pn * C);
+    rc |= dalloc(P, &h.ldlo, C);
+    rc |= dalloc(P, &h.ldhi, C);
+    rc |= dalloc(P, &h.wptr, (Pn + 1) * C);
+    rc |= dalloc(P, &h.tptr, (Pn + 1) * C);
+    rc |= dalloc(P, &h.wcols, Pn * C);
+    rc |= dalloc(P, &h.tcols, Pn * C);
+    rc |= dalloc(P, &h.wamax, C);
+    rc |= dalloc(P, &h.tamax, C);
+    rc |= dalloc(P, &h.ncol, C);
+    rc |= dalloc(P, &h.cpar, Pn * C);
+    rc |= dalloc(P, &h.cchild, Pn * C);
+    rc |= dalloc(P, &h.dead, Pn * C);
+    rc |= dalloc(P, &h.idstart, Pn * C);
+    rc |= dalloc(P, &h.flag, C);
+  } else if (P.mover != SMCF_MOVER_GIBBS) {
+    rc = SMCF_EINVAL;
+    set_error("unknown mover");
+  }
   if (rc) {
     for (void* p : P.allocs) (void)hipFree(p);
     return rc;

@@ -8,6 +8,8 @@
 //  k_like      Gram-form likelihood and ELL prior per particle      (smc_slip.f90:56-137)
 //  k_form_q    Q = gamma H + LL / alpha^2 for the Gibbs mover
 // The two GEMMs use 64x64 LDS tiles with a 4x4 register block per thread.
+#include <algorithm>
+
 #include "smcf_device.h"
 #include "smcf_internal.h"
 
@@ -226,46 +228,56 @@ hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_acti
   return hipGetLastError();
 }
 
-// Q(r, i) = gamma H(r, i) (+ LL(r, i) / alpha^2 on the sparse pattern, second kernel)
-__global__ void k_form_q(int nd, const double* __restrict__ Hall, const double* __restrict__ gamma,
-                         const int* __restrict__ active, double* __restrict__ Qall) {
+// Gibbs precision Q = gamma H + LL / alpha^2, written in the blocked layout of the Gibbs
+// kernel: Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k) (zero past the last column).
+int gibbs_kb(int nd);
+
+__global__ void k_form_qb(int nd, int KB, const double* __restrict__ Hall, const double* __restrict__ gamma,
+                          const int* __restrict__ active, double* __restrict__ Qball) {
   int b = blockIdx.y;
   if (!active[b]) return;
-  size_t n = (size_t)nd * nd;
+  int nblk = (nd + KB - 1) / KB;
+  size_t n = (size_t)nblk * nd * KB;
   double g = gamma[b];
-  const double* H = Hall + (size_t)b * n;
-  double* Q = Qall + (size_t)b * n;
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-    Q[k] = g * H[k];
+  const double* H = Hall + (size_t)b * nd * nd;
+  double* Qb = Qball + (size_t)b * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int k = (int)(e % KB);
+    size_t rest = e / KB;
+    int j = (int)(rest % nd), blk = (int)(rest / nd);
+    int i = blk * KB + k;
+    Qb[e] = i < nd ? g * H[(size_t)i * nd + j] : 0.0;
+  }
 }
 
-__global__ void k_form_q_sparse(int nd, const double* __restrict__ Hall, const double* __restrict__ gamma,
-                                const int* __restrict__ active, const double* __restrict__ scal,
-                                const int* __restrict__ llidx, const double* __restrict__ llval,
-                                double* __restrict__ Qall) {
+__global__ void k_form_qb_sparse(int nd, int KB, const double* __restrict__ Hall, const double* __restrict__ gamma,
+                                 const int* __restrict__ active, const double* __restrict__ scal,
+                                 const int* __restrict__ llidx, const double* __restrict__ llval,
+                                 double* __restrict__ Qball) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!active[b] || i >= nd) return;
+  int nblk = (nd + KB - 1) / KB;
   double g = gamma[b], a2 = scal[8 * (size_t)b];
   const double* H = Hall + (size_t)b * nd * nd;
-  double* Q = Qall + (size_t)b * nd * nd;
+  double* Qb = Qball + (size_t)b * nblk * nd * KB;
   const int* ix = llidx + ((size_t)b * nd + i) * 25;
   const double* vx = llval + ((size_t)b * nd + i) * 25;
+  int blk = i / KB, k = i % KB;
   for (int w = 0; w < 25; ++w) {
     int r = ix[w];
     if (r < 0) break;
-    size_t k = (size_t)i * nd + r;
-    Q[k] = g * H[k] + vx[w] / a2;
+    Qb[((size_t)blk * nd + r) * KB + k] = g * H[(size_t)i * nd + r] + vx[w] / a2;
   }
 }
 
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int nd = m.ndim;
-  int blocks = (int)(((size_t)nd * nd + 255) / 256);
-  if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_form_q, dim3(blocks, B), dim3(256), 0, s, nd, bt.H, bt.gamma, bt.active, bt.Q);
-  hipLaunchKernelGGL(k_form_q_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, bt.H, bt.gamma, bt.active,
-                     bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
+  int nd = m.ndim, KB = gibbs_kb(nd);
+  size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_form_qb, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.H, bt.gamma, bt.active, bt.Q);
+  hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.H, bt.gamma,
+                     bt.active, bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
   return hipGetLastError();
 }
 

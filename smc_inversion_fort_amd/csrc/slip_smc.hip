@@ -288,37 +288,60 @@ hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
 // q_k = sum_j Q(j, i0+k) s_j for its particle, then samples the KB coordinates in order,
 // adding the in-block corrections Q(i0+m, i0+k) * delta_m of the coordinates already redrawn.
 // ---------------------------------------------------------------------------
-template <int KB>
-__global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double max_slip, BatchDev bt) {
-  extern __shared__ double Qs[];  // nd x KB, [j*KB + k]
+// one coordinate redraw: Philox uniform + truncated-normal inversion (kept out of line so the
+// unrolled block loop does not replicate it)
+__device__ __noinline__ double gibbs_draw(double mu, double prec, double max_slip, uint64_t key, uint32_t stage,
+                                          uint32_t p, uint32_t i, uint32_t sw) {
+  uint64_t w0, w1;
+  philox_draw(key, kSiteGibbsMove, stage, p, i, sw, w0, w1);
+  return trunc_normal(mu, 1.0 / sqrt(prec), 0.0, max_slip, u01_open(w0));
+}
+
+// Q is stored per stage in blocks of KB columns, row-major inside the block:
+//   Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k)   (zero for columns past nd)
+// so the KB values a wavefront needs at row j are contiguous and uniform across lanes.
+template <int KB, int JU>
+__global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double max_slip,
+                                               const double* __restrict__ Qball, double* __restrict__ S0,
+                                               double* __restrict__ S1, const double* __restrict__ bvall,
+                                               const double* __restrict__ gam, const uint64_t* __restrict__ keys,
+                                               const int* __restrict__ iters, const int* __restrict__ active,
+                                               const int* __restrict__ par) {
   int b = blockIdx.y;
-  if (!bt.active[b]) return;
-  int t = threadIdx.x;
-  int p = blockIdx.x * 256 + t;
+  if (!active[b]) return;
+  int p = blockIdx.x * 256 + threadIdx.x;
   bool valid = p < P;
-  int cur = bt.par[b];
-  double* S = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
-  const double* Q = bt.Q + (size_t)b * nd * nd;
-  const double* bv = bt.bvec + (size_t)b * nd;
-  double gamma = bt.gamma[b];
-  uint64_t key = bt.key[b];
-  uint32_t stage = (uint32_t)(bt.iter[b] - 1);
   int pc = valid ? p : 0;
+  double* S = (par[b] ? S1 : S0) + (size_t)b * nd * P;
+  const double* Qb = Qball + (size_t)b * nd * ((nd + KB - 1) / KB) * KB;
+  const double* bv = bvall + (size_t)b * nd;
+  double gamma = gam[b];
+  uint64_t key = keys[b];
+  uint32_t stage = (uint32_t)(iters[b] - 1);
+  int nblk = (nd + KB - 1) / KB;
   for (int sw = 0; sw < sweeps; ++sw) {
-    for (int i0 = 0; i0 < nd; i0 += KB) {
+    for (int blk = 0; blk < nblk; ++blk) {
+      int i0 = blk * KB;
       int kb = min(KB, nd - i0);
-      __syncthreads();
-      for (int idx = t; idx < nd * KB; idx += 256) {
-        int k = idx / nd, j = idx - k * nd;
-        Qs[j * KB + k] = k < kb ? Q[(size_t)(i0 + k) * nd + j] : 0.0;
-      }
-      __syncthreads();
+      const double* Qr = Qb + (size_t)blk * nd * KB;
       double q[KB];
 #pragma unroll
       for (int k = 0; k < KB; ++k) q[k] = 0.0;
-      for (int j = 0; j < nd; ++j) {
+      int j = 0;
+      for (; j + JU <= nd; j += JU) {
+        double sv[JU];
+#pragma unroll
+        for (int u = 0; u < JU; ++u) sv[u] = S[(size_t)(j + u) * P + pc];
+#pragma unroll
+        for (int u = 0; u < JU; ++u) {
+          const double* qr = Qr + (size_t)(j + u) * KB;
+#pragma unroll
+          for (int k = 0; k < KB; ++k) q[k] = q[k] + qr[k] * sv[u];
+        }
+      }
+      for (; j < nd; ++j) {
         double sj = S[(size_t)j * P + pc];
-        const double* qr = Qs + j * KB;
+        const double* qr = Qr + (size_t)j * KB;
 #pragma unroll
         for (int k = 0; k < KB; ++k) q[k] = q[k] + qr[k] * sj;
       }
@@ -330,13 +353,11 @@ __global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double
           int i = i0 + k;
           double qi = q[k];
 #pragma unroll
-          for (int mm = 0; mm < k; ++mm) qi = qi + Qs[(i0 + mm) * KB + k] * delta[mm];
-          double prec = Qs[i * KB + k];
+          for (int mm = 0; mm < k; ++mm) qi = qi + Qr[(size_t)(i0 + mm) * KB + k] * delta[mm];
+          double prec = Qr[(size_t)i * KB + k];
           double so = S[(size_t)i * P + pc];
           double mu = so - (qi - gamma * bv[i]) / prec;
-          uint64_t w0, w1;
-          philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)i, (uint32_t)sw, w0, w1);
-          double sn = trunc_normal(mu, 1.0 / sqrt(prec), 0.0, max_slip, u01_open(w0));
+          double sn = gibbs_draw(mu, prec, max_slip, key, stage, (uint32_t)pc, (uint32_t)i, (uint32_t)sw);
           delta[k] = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
         }
@@ -345,20 +366,17 @@ __global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double
   }
 }
 
+int gibbs_kb(int nd) { return nd <= 1024 ? 16 : 8; }
+
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
   int nd = m.ndim;
   dim3 g((bt.P + 255) / 256, B);
-  // staged Q block: nd * KB doubles of LDS, kept within 64 KiB
-  if (nd <= 512) {
-    size_t sh = sizeof(double) * nd * 16;
-    hipLaunchKernelGGL(k_gibbs<16>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
-  } else if (nd <= 1024) {
-    size_t sh = sizeof(double) * nd * 8;
-    hipLaunchKernelGGL(k_gibbs<8>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
-  } else {
-    size_t sh = sizeof(double) * nd * 4;
-    hipLaunchKernelGGL(k_gibbs<4>, g, dim3(256), sh, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt);
-  }
+  if (gibbs_kb(nd) == 16)
+    hipLaunchKernelGGL((k_gibbs<16, 4>), g, dim3(256), 0, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
+                       bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
+  else
+    hipLaunchKernelGGL((k_gibbs<8, 4>), g, dim3(256), 0, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
+                       bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
   return hipGetLastError();
 }
 

@@ -92,10 +92,21 @@ struct BatchDev {
   double* gtrace;           // 202 x cap
 };
 
+// state of the HMC mover (per instance: P columns / parents / children)
+struct HmcDev {
+  double *Xc, *Lc, *Rc, *postc;  // chain state per parent
+  double *Xt, *Pm, *LS, *hcur;   // trajectory per wave column (candidate, momentum, L x / alpha^2, H_cur)
+  double *dtau, *dtmp, *score;   // per child
+  double *ldlo, *ldhi;           // per instance log-dtau range
+  int *wptr, *wcols, *tptr, *tcols, *wamax, *tamax;  // wave lists (sampling / tuning)
+  int *ncol, *cpar, *cchild, *dead, *idstart, *flag;
+};
+
 struct Problem {
   smcf_ctx* ctx;
   MeshDev m;
   BatchDev bt;
+  HmcDev hm;
   double max_slip;
   int P, mover, sweeps;
   std::vector<void*> allocs;
@@ -119,6 +130,10 @@ hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
 hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
+
+hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s);
+hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, hipStream_t s,
+                            int* host_amax);
 
 hipError_t launch_dc3d0_batch(int n, const double* in, double* u, int* iret, hipStream_t s);
 hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,

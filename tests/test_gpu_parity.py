@@ -261,3 +261,23 @@ def test_c2_size_greens_and_gram_vs_oracle(ctx):
     assert np.abs(H - Hr).max() <= 1e-10 * np.abs(Hr).max()
     assert dsd == pytest.approx(dr, rel=1e-12)
     pb.close()
+
+
+def test_slip_smc_exec_hmc_vs_oracle(ctx):
+    """Reference mover: ESJD-tuned HMC with ntau = 50 (smc_slip.f90:591-1214)."""
+    say("case: slip_smc_exec HMC vs oracle")
+    obs = small_obs(seed=5)
+    P = 64
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    pb = S.Problem(ctx, [obs["nxi"]], [obs["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, P, mover=S.MOVER_HMC)
+    po = O.ProblemHolder(O.Fault([obs["nxi"]], [obs["neta"]], xmin, xmax, zmin), obs, syn.MAX_SLIP, P, mover=0)
+    theta = syn.theta_true()
+    key = O.inner_key(3, 2, 1, 0)
+    d = pb.slip_smc_exec(theta, key)
+    o = po.slip_smc(theta, pb.calc_greens_func(theta), key)
+    say("stages", d["nstage"], o["nstage"], "neglog", d["neglog"], o["neglog"])
+    assert d["nstage"] == o["nstage"]
+    np.testing.assert_allclose(d["gamma"], o["gamma"], rtol=0, atol=1e-8)
+    assert d["neglog"] == pytest.approx(o["neglog"], rel=1e-8)
+    np.testing.assert_allclose(d["particles"], o["particles"], rtol=0, atol=1e-6)
+    pb.close()
