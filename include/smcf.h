@@ -124,6 +124,15 @@ int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, 
  * at most max_batch (<= the problem's max_batch) geometries. */
 int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
                         double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace);
+/* Outer SMC with the Gaussian proposal q = N(qmean, qcov) (the gsmc branch of fault_smc_exec,
+ * smc_fault.f90:41-52, 411-481, 708-775; q from fault_calc_neglog_q, smc_fault.f90:9-27).
+ * qcov_chol is the lower Cholesky factor of qcov (main.f90:332 factors it before the call), qcov_inv its
+ * inverse, both ndim x ndim column-major; log_qcov_det = log det qcov.  Initial particles are
+ * qmean + chol z; weights temper L - q; the MH test is gamma (L_cur - L_cand) + (1 - gamma)(q_cur - q_cand). */
+int smcf_fault_smc_exec_gsmc(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages,
+                             int max_batch, const double* qmean, const double* qcov_chol, const double* qcov_inv,
+                             double log_qcov_det, double* particles, double* likelihood, int* nstage,
+                             double* evidence, double* gamma_trace);
 /* RNG key of the inner SMC of one outer evaluation: (seed, outer stage, role 0 parent / 1 child, slot) */
 uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint32_t slot);
 

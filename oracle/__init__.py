@@ -54,6 +54,10 @@ class FaultResult(C.Structure):
     _fields_ = [("nstage", C.c_int), ("evidence", C.c_double), ("gamma_trace", D)]
 
 
+class Gsmc(C.Structure):
+    _fields_ = [("qmean", D), ("qcov_chol", D), ("qcov_inv", D), ("log_det", C.c_double)]
+
+
 def _declare(L):
     L.orc_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.orc_draw_u64x2.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -100,6 +104,10 @@ def _declare(L):
     L.orc_fault_reorder.argtypes = [I, D, D, C.c_int, C.c_int, I]
     L.orc_fault_smc_exec.argtypes = [C.POINTER(Problem), D, C.c_int, C.c_uint64, D, D, C.c_int,
                                      C.POINTER(FaultResult)]
+    L.orc_fault_smc_exec_gsmc.argtypes = [C.POINTER(Problem), D, C.c_int, C.c_uint64, D, D, C.c_int,
+                                          C.POINTER(FaultResult), C.POINTER(Gsmc)]
+    L.orc_fault_calc_neglog_q.argtypes = [D, D, D, C.c_double, C.c_int]
+    L.orc_fault_calc_neglog_q.restype = C.c_double
 
 
 def dp(a):
@@ -245,14 +253,21 @@ class ProblemHolder:
                                               k.ctypes.data_as(C.POINTER(C.c_uint64)), dp(out))
         return out
 
-    def fault_smc(self, rng, N, seed, max_stages=50):
+    def fault_smc(self, rng, N, seed, max_stages=50, gsmc=None):
+        """Outer SMC; gsmc = (qmean, qcov_chol, qcov_inv, log_qcov_det) selects the gsmc branch."""
         nd = self.fault.ndim_fault
         X = np.zeros((nd, N), order="F")
         L = np.zeros(N)
         gt = np.zeros(max_stages + 1)
         res = FaultResult(0, 0.0, dp(gt))
-        lib().orc_fault_smc_exec(C.byref(self.pb), dp(f64(rng)), N, seed, dp(X), dp(L), max_stages,
-                                 C.byref(res))
+        if gsmc is None:
+            lib().orc_fault_smc_exec(C.byref(self.pb), dp(f64(rng)), N, seed, dp(X), dp(L), max_stages,
+                                     C.byref(res))
+        else:
+            keep = [f64(gsmc[0]), f64(gsmc[1]), f64(gsmc[2])]
+            g = Gsmc(dp(keep[0]), dp(keep[1]), dp(keep[2]), float(gsmc[3]))
+            lib().orc_fault_smc_exec_gsmc(C.byref(self.pb), dp(f64(rng)), N, seed, dp(X), dp(L), max_stages,
+                                          C.byref(res), C.byref(g))
         return dict(particles=X, likelihood=L, nstage=res.nstage, evidence=res.evidence,
                     gamma=gt[:res.nstage].copy())
 
@@ -362,3 +377,9 @@ def fault_mean_cov(X, w):
     lib().orc_fault_calc_mean_particles(dp(X), dp(f64(w)), dp(m), N, nd)
     lib().orc_fault_calc_cov_particles(dp(X), dp(f64(w)), dp(m), dp(cov), N, nd, C.byref(info))
     return m, cov, info.value
+
+
+def fault_neglog_q(x, qmean, qcov_inv, log_det):
+    """fault_calc_neglog_q (smc_fault.f90:9-27)."""
+    x = f64(x)
+    return lib().orc_fault_calc_neglog_q(dp(x), dp(f64(qmean)), dp(f64(qcov_inv)), float(log_det), x.shape[0])

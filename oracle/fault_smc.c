@@ -95,18 +95,49 @@ static void trmv_lower(int n, const double* A, double* x) {
     }
 }
 
+/* fault_calc_neglog_q, smc_fault.f90:9-27 */
+double orc_fault_calc_neglog_q(const double* x, const double* qmean, const double* qinv, double log_det, int nd) {
+    double pi = 2.0 * asin(1.0), tmp = 0.0, delta[64];
+    for (int i = 0; i < nd; ++i) delta[i] = x[i] - qmean[i];
+    for (int j = 0; j < nd; ++j)
+        for (int i = 0; i < nd; ++i) tmp = tmp + delta[i] * qinv[j * nd + i] * delta[j];
+    return tmp / 2.0 + log(2.0 * pi) * nd / 2.0 + log_det / 2.0;
+}
+
 void orc_fault_smc_exec(const orc_problem* pb, const double* range, int N, uint64_t seed,
                         double* X, double* L, int max_stages, orc_fault_result* res) {
+    orc_fault_smc_exec_gsmc(pb, range, N, seed, X, L, max_stages, res, NULL);
+}
+
+void orc_fault_smc_exec_gsmc(const orc_problem* pb, const double* range, int N, uint64_t seed,
+                             double* X, double* L, int max_stages, orc_fault_result* res, const orc_gsmc* g) {
     int nd = pb->fault.fix_xbend ? 2 * pb->fault.nplane + 4 : 3 * pb->fault.nplane + 3;
     uint64_t okey = orc_splitmix64(seed ^ 0x5EEDF00Dull);
-    for (int i = 0; i < N; ++i)
-        for (int d = 0; d < nd; ++d) {
-            double r = orc_clamp_uniform(draw_u(okey, SITE_OUTER_INIT, 0, i, d, 0));
-            X[(size_t)i * nd + d] = range[2 * d] + r * (range[2 * d + 1] - range[2 * d]);
+    double* Q = (double*)malloc(sizeof(double) * N);
+    double* Qn = (double*)malloc(sizeof(double) * N);
+    for (int i = 0; i < N; ++i) {
+        if (g) {  /* qmean + chol(qcov) z, smc_fault.f90:41-52 */
+            double z[64];
+            for (int d = 0; d < nd; ++d) {
+                uint64_t w[2];
+                orc_draw_u64x2(okey, SITE_OUTER_INIT, 0, i, d, 0, w);
+                z[d] = orc_box_muller_sin(orc_clamp_uniform(orc_uniform_raw(w[0])),
+                                          orc_clamp_uniform(orc_uniform_raw(w[1])));
+            }
+            trmv_lower(nd, g->qcov_chol, z);
+            for (int d = 0; d < nd; ++d) X[(size_t)i * nd + d] = z[d] + g->qmean[d];
+        } else {
+            for (int d = 0; d < nd; ++d) {
+                double r = orc_clamp_uniform(draw_u(okey, SITE_OUTER_INIT, 0, i, d, 0));
+                X[(size_t)i * nd + d] = range[2 * d] + r * (range[2 * d + 1] - range[2 * d]);
+            }
         }
+    }
     #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < N; ++i)
         L[i] = orc_fault_calc_likelihood(pb, X + (size_t)i * nd, orc_inner_key(seed, 0, 0, i), NULL);
+    for (int i = 0; i < N; ++i)
+        Q[i] = g ? orc_fault_calc_neglog_q(X + (size_t)i * nd, g->qmean, g->qcov_inv, g->log_det, nd) : 0.0;
     double* w = (double*)malloc(sizeof(double) * N);
     double* mean = (double*)malloc(sizeof(double) * nd);
     double* cov = (double*)malloc(sizeof(double) * nd * nd);
@@ -120,7 +151,12 @@ void orc_fault_smc_exec(const orc_problem* pb, const double* range, int N, uint6
     int iter = 1;
     while (1.0 - gamma > 1e-8 && iter <= max_stages) {
         double gprev = gamma, ev;
-        gamma = orc_find_next_gamma(gprev, L, N, w, &ev, 0.2, 1e-8, 1000);
+        if (g) {  /* tempering of L - q (smc_fault.f90:411-416, 433-438, 479-481) */
+            for (int i = 0; i < N; ++i) Ln[i] = L[i] - Q[i];
+            gamma = orc_find_next_gamma(gprev, Ln, N, w, &ev, 0.2, 1e-8, 1000);
+        } else {
+            gamma = orc_find_next_gamma(gprev, L, N, w, &ev, 0.2, 1e-8, 1000);
+        }
         evidence = evidence + ev;
         if (res && res->gamma_trace) res->gamma_trace[iter - 1] = gamma;
         orc_fault_normalize_weights(w, N);
@@ -132,6 +168,8 @@ void orc_fault_smc_exec(const orc_problem* pb, const double* range, int N, uint6
         orc_resample_quad(N, w, (double)(wq[0] >> 11) * 0x1.0p-53, (double)(wq[1] >> 11) * 0x1.0p-106,
                           assigned);
         orc_fault_reorder(assigned, X, L, N, nd, sidx);
+        for (int i = 0; i < N; ++i) Qn[i] = Q[i];
+        for (int i = 0; i < N; ++i) Q[i] = Qn[sidx[i] - 1];
         id_start[0] = 0;
         for (int i = 0; i < N - 1; ++i) id_start[i + 1] = id_start[i] + assigned[i];
         /* wave 0: parent re-evaluations (only parents with children matter) */
@@ -143,7 +181,7 @@ void orc_fault_smc_exec(const orc_problem* pb, const double* range, int N, uint6
         for (int i = 0; i < N; ++i) {
             double cur[64], cand[64], z[64];
             memcpy(cur, X + (size_t)i * nd, sizeof(double) * nd);
-            double lc = Lcur[i];
+            double lc = Lcur[i], qc = Q[i];
             for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) {
                 for (int d = 0; d < nd; ++d) {
                     uint64_t wz[2];
@@ -161,19 +199,26 @@ void orc_fault_smc_exec(const orc_problem* pb, const double* range, int N, uint6
                     }
                 }
                 double lcand = orc_fault_calc_likelihood(pb, cand, orc_inner_key(seed, iter, 1, j), NULL);
+                double qcand = g ? orc_fault_calc_neglog_q(cand, g->qmean, g->qcov_inv, g->log_det, nd) : 0.0;
                 double mh = orc_clamp_uniform(draw_u(okey, SITE_OUTER_MH, iter, j, 0, 0));
-                if (gamma * (lc - lcand) > log(mh)) {
+                int acc = g ? (gamma * (lc - lcand) + (1.0 - gamma) * (qc - qcand) > log(mh))
+                            : (gamma * (lc - lcand) > log(mh));
+                if (acc) {
                     memcpy(cur, cand, sizeof(double) * nd);
                     lc = lcand;
+                    qc = qcand;
                 }
                 Ln[j] = lc;
+                Qn[j] = qc;
                 memcpy(Xn + (size_t)j * nd, cur, sizeof(double) * nd);
             }
         }
         memcpy(X, Xn, sizeof(double) * (size_t)N * nd);
         memcpy(L, Ln, sizeof(double) * N);
+        memcpy(Q, Qn, sizeof(double) * N);
         iter++;
     }
+    free(Q); free(Qn);
     if (res) { res->nstage = iter - 1; res->evidence = evidence; }
     free(w); free(mean); free(cov); free(assigned); free(sidx); free(id_start); free(Xn); free(Ln); free(Lcur);
 }

@@ -149,6 +149,15 @@ typedef struct {
     double evidence;       /* fault_evidence (negative log) */
     double* gamma_trace;   /* optional */
 } orc_fault_result;
+typedef struct {
+    const double* qmean;      /* nd */
+    const double* qcov_chol;  /* nd x nd, lower Cholesky factor of qcov (main.f90:332) */
+    const double* qcov_inv;   /* nd x nd */
+    double log_det;
+} orc_gsmc;
+double orc_fault_calc_neglog_q(const double* x, const double* qmean, const double* qinv, double log_det, int nd);
+void orc_fault_smc_exec_gsmc(const orc_problem* pb, const double* range, int N, uint64_t seed, double* particles,
+                             double* likelihood, int max_stages, orc_fault_result* res, const orc_gsmc* g);
 void orc_fault_smc_exec(const orc_problem* pb, const double* range /* 2 x nd */, int N, uint64_t seed,
                         double* particles /* nd x N out */, double* likelihood /* N out */,
                         int max_stages, orc_fault_result* res);

@@ -52,7 +52,7 @@ EXPORTS = [
     "smcf_problem_create", "smcf_problem_destroy", "smcf_problem_dims", "smcf_dc3d0",
     "smcf_calc_greens_func", "smcf_laplacian", "smcf_slip_calc_gsd_gsg", "smcf_slip_smc_exec",
     "smcf_fault_calc_likelihood_batch", "smcf_find_next_gamma", "smcf_normalize_resample",
-    "smcf_trunc_normal", "smcf_philox", "smcf_fault_smc_exec", "smcf_inner_key", "qsort_",
+    "smcf_trunc_normal", "smcf_philox", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
 ]
 
 
@@ -89,6 +89,8 @@ def _declare(L):
     L.smcf_trunc_normal.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
     L.smcf_philox.argtypes = [C.c_void_p, C.c_int, _U32, _U32, _U32]
     L.smcf_fault_smc_exec.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _I, _D, _D]
+    L.smcf_fault_smc_exec_gsmc.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _D,
+                                           C.c_double, _D, _D, _I, _D, _D]
     L.smcf_inner_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
     L.smcf_inner_key.restype = C.c_uint64
     L.qsort_.argtypes = [_I, _I, _I]
@@ -287,16 +289,29 @@ class Problem:
         return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=ns.value, neglog=nl.value,
                     gamma=gt[:ns.value].copy())
 
-    def fault_smc_exec(self, range_, N, seed, max_stages=100):
-        """Outer SMC over the geometry (smc_fault.f90:785-1075, one rank)."""
+    def fault_smc_exec(self, range_, N, seed, max_stages=100, gsmc=None):
+        """Outer SMC over the geometry (smc_fault.f90:785-1075, one rank).
+
+        gsmc = (qmean, qcov_chol, qcov_inv, log_qcov_det) runs the gsmc branch: initial particles from
+        N(qmean, qcov), tempering of L - q, the mixed MH ratio (qcov_chol is the lower Cholesky factor,
+        as main.f90:332 passes it)."""
         r = _f64(range_)
         X = np.zeros((self.ndim_fault, N), order="F")
         L = np.zeros(N)
         ns = C.c_int()
         ev = C.c_double()
         gt = np.zeros(max_stages + 1)
-        _check(lib().smcf_fault_smc_exec(self.h, _dp(r), int(N), int(seed), int(max_stages), self.max_batch,
-                                         _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
+        if gsmc is None:
+            _check(lib().smcf_fault_smc_exec(self.h, _dp(r), int(N), int(seed), int(max_stages), self.max_batch,
+                                             _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
+        else:
+            qm, qc, qi = (_f64(a) for a in gsmc[:3])
+            nd = self.ndim_fault
+            if qm.size != nd or qc.size != nd * nd or qi.size != nd * nd:
+                raise SmcfError("gsmc: qmean / qcov shapes do not match ndim_fault")
+            _check(lib().smcf_fault_smc_exec_gsmc(self.h, _dp(r), int(N), int(seed), int(max_stages),
+                                                  self.max_batch, _dp(qm), _dp(qc), _dp(qi), float(gsmc[3]),
+                                                  _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
         return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy())
 
     def fault_calc_likelihood(self, thetas, keys):

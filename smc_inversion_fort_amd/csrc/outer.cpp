@@ -108,23 +108,59 @@ uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint
   return splitmix64(a ^ splitmix64(b));
 }
 
-int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages,
-                        int max_batch, double* particles, double* likelihood, int* nstage, double* evidence,
-                        double* gamma_trace) {
+}  // extern "C"
+
+namespace {
+
+struct Gsmc {
+  const double* qmean;      // nd
+  const double* qcov_chol;  // nd x nd column-major, lower Cholesky factor of qcov
+  const double* qcov_inv;   // nd x nd column-major
+  double log_det;
+};
+
+// fault_calc_neglog_q (smc_fault.f90:9-27), same summation order
+double neglog_q(const double* x, const Gsmc& g, int nd) {
+  const double pi = 3.141592653589793;
+  double delta[64], tmp = 0.0;
+  for (int i = 0; i < nd; ++i) delta[i] = x[i] - g.qmean[i];
+  for (int j = 0; j < nd; ++j)
+    for (int i = 0; i < nd; ++i) tmp = tmp + delta[i] * g.qcov_inv[j * nd + i] * delta[j];
+  return tmp / 2.0 + std::log(2.0 * pi) * nd / 2.0 + g.log_det / 2.0;
+}
+
+int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
+              double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace,
+              const Gsmc* g) {
   int nnode, ndof, nds, nd, nobs;
   int rc = smcf_problem_dims(pb, &nnode, &ndof, &nds, &nd, &nobs);
   if (rc) return rc;
   if (!range || N < 1 || max_batch < 1 || !particles || !likelihood || nd > 64) return SMCF_EINVAL;
   uint64_t okey = splitmix64(seed ^ 0x5EEDF00Dull);
   std::vector<double> X((size_t)nd * N), L(N), w(N), Xn((size_t)nd * N), Ln(N), Lcur(N), cov(nd * nd), mean(nd);
+  std::vector<double> Q(N, 0.0), Qn(N, 0.0), R(N);
   std::vector<int> assigned(N), sidx(N), id_start(N);
-  for (int i = 0; i < N; ++i)
-    for (int d = 0; d < nd; ++d) {
-      uint64_t ww[2];
-      Philox::draw(okey, kOuterInit, 0, i, d, 0, ww);
-      double r = clampu(u01(ww[0]));
-      X[(size_t)i * nd + d] = range[2 * d] + r * (range[2 * d + 1] - range[2 * d]);
+  std::vector<double> chol;
+  if (g) chol.assign(g->qcov_chol, g->qcov_chol + (size_t)nd * nd);
+  for (int i = 0; i < N; ++i) {
+    if (g) {  // fault_sample_init_particles, gsmc branch: qmean + chol(qcov) z
+      double z[64];
+      for (int d = 0; d < nd; ++d) {
+        uint64_t ww[2];
+        Philox::draw(okey, kOuterInit, 0, i, d, 0, ww);
+        z[d] = box_muller(ww[0], ww[1]);
+      }
+      trmv_lower(nd, chol, z);
+      for (int d = 0; d < nd; ++d) X[(size_t)i * nd + d] = z[d] + g->qmean[d];
+    } else {
+      for (int d = 0; d < nd; ++d) {
+        uint64_t ww[2];
+        Philox::draw(okey, kOuterInit, 0, i, d, 0, ww);
+        double r = clampu(u01(ww[0]));
+        X[(size_t)i * nd + d] = range[2 * d] + r * (range[2 * d + 1] - range[2 * d]);
+      }
     }
+  }
   // batched evaluation helper
   auto eval = [&](const std::vector<double>& th, const std::vector<uint64_t>& keys, std::vector<double>& out) -> int {
     int B = (int)keys.size();
@@ -144,11 +180,18 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
     if ((rc = eval(X, keys, out))) return rc;
     L = out;
   }
+  if (g)
+    for (int i = 0; i < N; ++i) Q[i] = neglog_q(&X[(size_t)i * nd], *g, nd);
   double gamma = 0.0, ev_total = 0.0;
   int iter = 1;
   while (1.0 - gamma > 1e-8 && iter <= max_stages) {
     double gprev = gamma, ev;
-    gamma = find_next_gamma(gprev, L, w, &ev);
+    if (g) {  // tempering of L - q
+      for (int i = 0; i < N; ++i) R[i] = L[i] - Q[i];
+      gamma = find_next_gamma(gprev, R, w, &ev);
+    } else {
+      gamma = find_next_gamma(gprev, L, w, &ev);
+    }
     ev_total = ev_total + ev;
     if (gamma_trace) gamma_trace[iter - 1] = gamma;
     // fault_normalize_weights / fault_calc_mean_particles / fault_calc_cov_particles
@@ -196,7 +239,7 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
     // fault_reorder_to_send (numprocs = 1): descending by assigned_num through qsort_
     {
       std::vector<int> ta = assigned;
-      std::vector<double> tx = X, tl = L;
+      std::vector<double> tx = X, tl = L, tq = Q;
       for (int i = 0; i < N; ++i) sidx[i] = i + 1;
       int n = N;
       qsort_(assigned.data(), sidx.data(), &n);
@@ -205,6 +248,7 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
         assigned[i] = ta[o];
         std::memcpy(&X[(size_t)i * nd], &tx[(size_t)o * nd], sizeof(double) * nd);
         L[i] = tl[o];
+        Q[i] = tq[o];
       }
     }
     id_start[0] = 0;
@@ -224,7 +268,7 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
     std::vector<double> out;
     if ((rc = eval(th, keys, out))) return rc;
     for (size_t q = 0; q < who.size(); ++q) Lcur[who[q]] = out[q];
-    std::vector<double> cur = X;
+    std::vector<double> cur = X, qcur = Q;
     int amax = *std::max_element(assigned.begin(), assigned.end());
     for (int k = 0; k < amax; ++k) {
       th.clear(); keys.clear(); who.clear();
@@ -259,16 +303,27 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
         uint64_t wm[2];
         Philox::draw(okey, kOuterMH, iter, j, 0, 0, wm);
         double mh = clampu(u01(wm[0]));
-        if (gamma * (Lcur[i] - out[q]) > std::log(mh)) {
+        bool acc;
+        double qc = 0.0;
+        if (g) {
+          qc = neglog_q(&th[q * nd], *g, nd);
+          acc = gamma * (Lcur[i] - out[q]) + (1.0 - gamma) * (qcur[i] - qc) > std::log(mh);
+        } else {
+          acc = gamma * (Lcur[i] - out[q]) > std::log(mh);
+        }
+        if (acc) {
           std::memcpy(&cur[(size_t)i * nd], &th[q * nd], sizeof(double) * nd);
           Lcur[i] = out[q];
+          qcur[i] = qc;
         }
         Ln[j] = Lcur[i];
+        Qn[j] = qcur[i];
         std::memcpy(&Xn[(size_t)j * nd], &cur[(size_t)i * nd], sizeof(double) * nd);
       }
     }
     X = Xn;
     L = Ln;
+    Q = Qn;
     iter++;
   }
   std::memcpy(particles, X.data(), sizeof(double) * X.size());
@@ -276,6 +331,27 @@ int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t s
   if (nstage) *nstage = iter - 1;
   if (evidence) *evidence = ev_total;
   return SMCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages,
+                        int max_batch, double* particles, double* likelihood, int* nstage, double* evidence,
+                        double* gamma_trace) {
+  return fault_smc(pb, range, N, seed, max_stages, max_batch, particles, likelihood, nstage, evidence,
+                   gamma_trace, nullptr);
+}
+
+int smcf_fault_smc_exec_gsmc(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages,
+                             int max_batch, const double* qmean, const double* qcov_chol, const double* qcov_inv,
+                             double log_qcov_det, double* particles, double* likelihood, int* nstage,
+                             double* evidence, double* gamma_trace) {
+  if (!qmean || !qcov_chol || !qcov_inv) return SMCF_EINVAL;
+  Gsmc g{qmean, qcov_chol, qcov_inv, log_qcov_det};
+  return fault_smc(pb, range, N, seed, max_stages, max_batch, particles, likelihood, nstage, evidence,
+                   gamma_trace, &g);
 }
 
 }  // extern "C"

@@ -281,3 +281,51 @@ def test_slip_smc_exec_hmc_vs_oracle(ctx):
     assert d["neglog"] == pytest.approx(o["neglog"], rel=1e-8)
     np.testing.assert_allclose(d["particles"], o["particles"], rtol=0, atol=1e-6)
     pb.close()
+
+
+def _outer_pair(ctx, P=32, N=16):
+    xmin, xmax, zmin, _ = syn.fault_frame()
+
+    def greens(th, pts, uv, nx, ne):
+        return O.Fault([nx], [ne], xmin, xmax, zmin).greens(th, pts, uv, 0, pts.shape[1] // 3)
+
+    obs = syn.make_observations("c2", 3, greens, nsta=8, sigma=0.5, nxi=3, neta=2)
+    pb = S.Problem(ctx, [3], [2], xmin, xmax, zmin, obs, syn.MAX_SLIP, P, mover=S.MOVER_GIBBS, gibbs_sweeps=2,
+                   max_batch=N)
+    po = O.ProblemHolder(O.Fault([3], [2], xmin, xmax, zmin), obs, syn.MAX_SLIP, P, mover=1, gibbs_sweeps=2)
+    return pb, po
+
+
+def _check_outer(d, o):
+    say("outer stages", d["nstage"], o["nstage"], "evidence", d["evidence"], o["evidence"])
+    assert d["nstage"] == o["nstage"]
+    np.testing.assert_allclose(d["gamma"], o["gamma"], rtol=0, atol=1e-8)
+    assert d["evidence"] == pytest.approx(o["evidence"], rel=1e-8, abs=1e-8)
+    np.testing.assert_allclose(d["particles"], o["particles"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(d["likelihood"], o["likelihood"], rtol=1e-8)
+
+
+def test_fault_smc_exec_vs_oracle(ctx):
+    """Outer SMC (smc_fault.f90:785-1075): device waves + host stage logic vs the oracle."""
+    say("case: fault_smc_exec N=16 P=32 3 stages vs oracle")
+    pb, po = _outer_pair(ctx)
+    rng = syn.prior_range()
+    d = pb.fault_smc_exec(rng, 16, 5, max_stages=3)
+    o = po.fault_smc(rng, 16, 5, max_stages=3)
+    _check_outer(d, o)
+    pb.close()
+
+
+def test_fault_smc_exec_gsmc_vs_oracle(ctx):
+    """gsmc branch: Gaussian initial draws, tempering of L - q, mixed MH ratio."""
+    say("case: fault_smc_exec gsmc N=16 P=32 3 stages vs oracle")
+    pb, po = _outer_pair(ctx)
+    rng = syn.prior_range()
+    A = np.random.default_rng(4).normal(size=(6, 6)) * 0.3
+    qcov = A @ A.T + np.diag([1.0, 1.0, 4.0, 1.0, 0.3, 0.3])
+    g = (syn.theta_true(), np.asfortranarray(np.linalg.cholesky(qcov)), np.asfortranarray(np.linalg.inv(qcov)),
+         np.linalg.slogdet(qcov)[1])
+    d = pb.fault_smc_exec(rng, 16, 5, max_stages=3, gsmc=g)
+    o = po.fault_smc(rng, 16, 5, max_stages=3, gsmc=g)
+    _check_outer(d, o)
+    pb.close()
