@@ -106,6 +106,8 @@ def _declare(L):
                                      C.POINTER(FaultResult)]
     L.orc_fault_smc_exec_gsmc.argtypes = [C.POINTER(Problem), D, C.c_int, C.c_uint64, D, D, C.c_int,
                                           C.POINTER(FaultResult), C.POINTER(Gsmc)]
+    L.orc_inst_grad.argtypes = [C.POINTER(Problem), D, D, C.c_double, D, D]
+    L.orc_hmc_leapfrog.argtypes = [C.POINTER(Problem), D, D, C.c_double, D, D, D, C.c_double, C.c_int, D, D, D, D]
     L.orc_fault_calc_neglog_q.argtypes = [D, D, D, C.c_double, C.c_int]
     L.orc_fault_calc_neglog_q.restype = C.c_double
 
@@ -236,6 +238,24 @@ class ProblemHolder:
                                 dp(mean), C.byref(res))
         return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=res.nstage, neglog=res.neglog,
                     gamma=gt[:res.nstage].copy(), evid=et[:res.nstage].copy())
+
+    def grad(self, theta, gmat, gamma, s):
+        """Tempered gradient of gamma * likelihood + prior on this geometry (slip_calc_grad)."""
+        s = f64(s)
+        g = np.zeros(s.shape[0])
+        lib().orc_inst_grad(C.byref(self.pb), dp(f64(theta)), dp(f64(gmat)), gamma, dp(s), dp(g))
+        return g
+
+    def leapfrog(self, theta, gmat, gamma, cur, p0, cov, dtau, ntau):
+        """One slip_leapfrog trajectory; returns (cand, p, ham_cur, ham_cand)."""
+        cur = f64(cur)
+        cand = np.zeros_like(cur)
+        p = np.zeros_like(cur)
+        hc = C.c_double()
+        hn = C.c_double()
+        lib().orc_hmc_leapfrog(C.byref(self.pb), dp(f64(theta)), dp(f64(gmat)), gamma, dp(cur), dp(f64(p0)),
+                               dp(f64(cov)), dtau, ntau, dp(cand), dp(p), C.byref(hc), C.byref(hn))
+        return cand, p, hc.value, hn.value
 
     def fault_likelihood(self, theta, key):
         gt = np.zeros(202)

@@ -149,6 +149,11 @@ typedef struct {
     double evidence;       /* fault_evidence (negative log) */
     double* gamma_trace;   /* optional */
 } orc_fault_result;
+void orc_inst_grad(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
+                   const double* s, double* grad);
+void orc_hmc_leapfrog(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
+                      const double* cur, const double* p0, const double* cov, double dtau, int ntau,
+                      double* cand, double* p, double* ham_cur, double* ham_cand);
 typedef struct {
     const double* qmean;      /* nd */
     const double* qcov_chol;  /* nd x nd, lower Cholesky factor of qcov (main.f90:332) */

@@ -687,6 +687,35 @@ void orc_slip_smc_exec(const orc_problem* pb, const double* theta, const double*
     inst_free(&I);
 }
 
+/* Test hooks on one instance (theta, gmat): the tempered gradient and one leapfrog trajectory
+ * (slip_leapfrog, smc_slip.f90:732-917) from (cur, p0) with mass diag(cov). */
+void orc_inst_grad(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
+                   const double* s, double* grad) {
+    inst_t I;
+    inst_build(&I, pb, theta, gmat);
+    double* lsv = (double*)malloc(sizeof(double) * 2 * I.nnode);
+    orc_slip_calc_grad(grad, s, I.ndim, I.nnode, I.ndof, I.li, I.lv, I.lti, I.ltv, gamma, I.alpha2, I.b, I.H, lsv);
+    free(lsv);
+    inst_free(&I);
+}
+
+void orc_hmc_leapfrog(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
+                      const double* cur, const double* p0, const double* cov, double dtau, int ntau,
+                      double* cand, double* p, double* ham_cur, double* ham_cand) {
+    inst_t I;
+    inst_build(&I, pb, theta, gmat);
+    int nd = I.ndim;
+    double* grad = (double*)malloc(sizeof(double) * nd);
+    double* lsv = (double*)malloc(sizeof(double) * 2 * I.nnode);
+    double post = gamma * inst_likelihood(&I, cur) + inst_prior(&I, cur, lsv);
+    double lc, pc, postc;
+    memcpy(cand, cur, sizeof(double) * nd);
+    memcpy(p, p0, sizeof(double) * nd);
+    leapfrog(&I, gamma, post, p, cov, cand, cur, dtau, ntau, ham_cur, &lc, &pc, &postc, ham_cand, grad, lsv);
+    free(grad); free(lsv);
+    inst_free(&I);
+}
+
 /* fault_calc_likelihood, smc_fault.f90:66-292 */
 double orc_fault_calc_likelihood(const orc_problem* pb, const double* theta, uint64_t key,
                                  orc_slip_result* res) {
