@@ -71,16 +71,19 @@ struct dd {
   double hi, lo;
 };
 SMCF_DEV dd two_sum(double a, double b) {
+#pragma clang fp contract(off)
   double s = a + b;
   double bb = s - a;
   double e = (a - (s - bb)) + (b - bb);
   return {s, e};
 }
 SMCF_DEV dd quick_two_sum(double a, double b) {
+#pragma clang fp contract(off)
   double s = a + b;
   return {s, b - (s - a)};
 }
 SMCF_DEV dd dd_add(dd a, dd b) {
+#pragma clang fp contract(off)
   dd s = two_sum(a.hi, b.hi);
   dd t = two_sum(a.lo, b.lo);
   s.lo += t.hi;
@@ -89,30 +92,33 @@ SMCF_DEV dd dd_add(dd a, dd b) {
   return quick_two_sum(s.hi, s.lo);
 }
 SMCF_DEV dd dd_add_d(dd a, double b) {
+#pragma clang fp contract(off)
   dd s = two_sum(a.hi, b);
   s.lo += a.lo;
   return quick_two_sum(s.hi, s.lo);
 }
 SMCF_DEV dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
 SMCF_DEV dd dd_mul_d(dd a, double b) {
+#pragma clang fp contract(off)
   double p = a.hi * b;
   double e = fma(a.hi, b, -p);
   e = fma(a.lo, b, e);
   return quick_two_sum(p, e);
 }
 SMCF_DEV dd dd_div_d(dd a, double b) {
+#pragma clang fp contract(off)
   double q1 = a.hi / b;
   dd r = dd_add(a, dd_neg(dd_mul_d(dd{q1, 0.0}, b)));
   double q2 = r.hi / b;
   return quick_two_sum(q1, q2);
 }
-// round(x / s) for double x and double-double s
+// round(x / s) for double x and double-double s.  The residual x - q1 * s.hi is exact in one FMA
+// (q1 = fl(x / s.hi)); it is written with explicit FMAs so that contraction cannot change it.
 SMCF_DEV double d_div_dd(double x, dd s) {
+#pragma clang fp contract(off)
   double q1 = x / s.hi;
-  // r = x - q1*s exactly enough
-  double p = q1 * s.hi;
-  double e = fma(q1, s.hi, -p);
-  double r = ((x - p) - e) - q1 * s.lo;
+  double r = fma(-q1, s.hi, x);
+  r = fma(-q1, s.lo, r);
   return q1 + r / s.hi;
 }
 SMCF_DEV double dd_floor(dd a) {
