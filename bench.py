@@ -1,10 +1,11 @@
 """Benchmark: outer-particle evaluations per second of the nested SMC at 16x16 patches.
 
-A step is one batched fault_calc_likelihood (smc_fault.f90:66-292) over B geometry
-particles drawn from the prior: per geometry the Green's matrix, the Gram matrices and a
-full inner slip SMC (P = 2000 particles, truncated-MVN Gibbs, 5 sweeps per stage) tempered
-to gamma = 1.  Observations stay resident on the device; each step uploads only the B
-thetas and reads back B values of -log Z.
+A step is one wave of fault_calc_likelihood (smc_fault.f90:66-292) evaluations, as the outer
+SMC issues them: E geometry particles drawn from the prior, streamed through S device slots
+(a slot whose inner SMC has finished takes the next geometry).  Per geometry: the Green's
+matrix, the Gram matrices and a full inner slip SMC (P = 2000 particles, truncated-MVN Gibbs,
+5 sweeps per stage) tempered to gamma = 1.  Observations stay resident on the device; each
+step uploads only the thetas and reads back one -log Z per geometry.
 
 Prints one JSON line with `roofline` (Gibbs-sweep kernel, FP64, timed with HIP events on
 the library's stream) and `cpu_baseline` (the C oracle on this host's cores, a bounded
@@ -30,7 +31,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--evals", type=int, default=1024, help="geometry evaluations per step (one wave)")
+    ap.add_argument("--slots", type=int, default=256, help="device slots (instances in flight)")
     ap.add_argument("--cpu-evals", type=int, default=1, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -70,16 +72,17 @@ def main():
         p.close()
         return g
     obs = syn.make_observations("c2", a.seed, greens)
-    B = a.batch
+    B = a.evals
     pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
-                   mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=B)
+                   mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots)
     ndim, P, sweeps = pb.ndim, cfg["P"], cfg["sweeps"]
 
     def keys_for(step):
         base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(step + 1)
         return (np.arange(B, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
 
-    print("bench: c2 16x16 patches ndim=%d nobs=%d P=%d batch=%d" % (ndim, pb.nobs, P, B), flush=True)
+    print("bench: c2 16x16 patches ndim=%d nobs=%d P=%d evals/step=%d slots=%d" % (ndim, pb.nobs, P, B, a.slots),
+          flush=True)
     for w in range(a.warmup):
         t0 = time.time()
         nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(B, 1000 + w), keys_for(1000 + w))
@@ -127,11 +130,13 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded GNSS stations, Gaussian slip bump, 5 mm noise)",
            "config": {"workload": "c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, "
-                                  "truncated-MVN Gibbs x5 per stage, FP64, batch of %d geometry particles" % B,
-                      "batch": B, "ndim_slip": ndim, "nobs": pb.nobs, "inner_particles": P,
+                                  "truncated-MVN Gibbs x5 per stage, FP64; waves of %d geometry particles "
+                                  "streamed through %d device slots" % (B, a.slots),
+                      "evals_per_step": B, "slots": a.slots, "ndim_slip": ndim, "nobs": pb.nobs,
+                      "inner_particles": P,
                       "mean_inner_stages": round(float(stages.mean()), 2),
-                      "p50_batch_latency_s": round(float(np.percentile(lat, 50)), 3),
-                      "p99_batch_latency_s": round(float(np.percentile(lat, 99)), 3)},
+                      "p50_wave_latency_s": round(float(np.percentile(lat, 50)), 3),
+                      "p99_wave_latency_s": round(float(np.percentile(lat, 99)), 3)},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     pb.close()

@@ -100,7 +100,9 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
                        double* gamma_trace);
 
 /* Batched fault_calc_likelihood (smc_fault.f90:66-292): -log Z for B thetas, each with its own
- * RNG key.  This is what work_eval_init_particles / work_mcmc_sampling waves call. */
+ * RNG key.  This is what work_eval_init_particles / work_mcmc_sampling waves call.  Any B >= 1:
+ * the geometries stream through the problem's max_batch device slots, a finished inner SMC's slot
+ * taking the next geometry; results do not depend on B, the slot count or the order. */
 int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys,
                                      double* neglog, int* nstage);
 
@@ -120,8 +122,9 @@ int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, 
 
 /* Outer SMC (replaces fault_smc_exec, smc_fault.f90:785-1075, with numprocs = 1 semantics).
  * range: 2 x ndim_fault (range.dat layout); particles: ndim_fault x N out; likelihood: N out;
- * gamma_trace: >= max_stages entries (may be NULL).  Inner evaluations run as device batches of
- * at most max_batch (<= the problem's max_batch) geometries. */
+ * gamma_trace: >= max_stages entries (may be NULL).  Inner evaluations of one wave are issued as
+ * smcf_fault_calc_likelihood_batch calls of at most max_batch geometries (any value >= 1; each call
+ * streams through the problem's slots, so a whole wave per call keeps the device fullest). */
 int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
                         double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace);
 /* Outer SMC with the Gaussian proposal q = N(qmean, qcov) (the gsmc branch of fault_smc_exec,

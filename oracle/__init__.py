@@ -91,6 +91,8 @@ def _declare(L):
     L.orc_ndtri.argtypes = [C.c_double]
     L.orc_ndtri.restype = C.c_double
     L.orc_trunc_normal.argtypes = [C.c_double] * 5
+    L.orc_trunc_normal_prec.argtypes = [C.c_double] * 5
+    L.orc_trunc_normal_prec.restype = C.c_double
     L.orc_trunc_normal.restype = C.c_double
     L.orc_slip_smc_exec.argtypes = [C.POINTER(Problem), D, D, C.c_uint64, D, D, D, D, C.POINTER(SlipResult)]
     L.orc_fault_calc_likelihood.argtypes = [C.POINTER(Problem), D, C.c_uint64, C.POINTER(SlipResult)]
@@ -373,6 +375,11 @@ def ndtri(p):
 
 def trunc_normal(mu, sigma, lo, hi, u):
     return lib().orc_trunc_normal(mu, sigma, lo, hi, u)
+
+
+def trunc_normal_prec(mu, s, lo, hi, u):
+    """The Gibbs mover's draw: s = sqrt(precision)."""
+    return lib().orc_trunc_normal_prec(mu, s, lo, hi, u)
 
 
 def qsort_desc(arr):

@@ -100,6 +100,7 @@ void orc_slip_calc_cov_particles(const double* particles, const double* w, const
 /* u_hi + u_lo is the (unclamped) quad uniform; clamping is done inside */
 void orc_resample_quad(int P, const double* w, double u_hi, double u_lo, int* assigned);
 double orc_ndtri(double p);
+double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u);
 double orc_trunc_normal(double mu, double sigma, double lo, double hi, double u);
 
 typedef struct {

@@ -320,31 +320,37 @@ static double tn_far(double a, double b, double u) {
     return fmin(fmax(z, a), b);
 }
 
-/* standard normal truncated to [a, b], by inversion with tail-aware probabilities */
+/* standard normal truncated to [a, b], by inversion with tail-aware probabilities.
+ * One erfc per bound gives t = Q(|x|), from which both Phi(x) and Q(x) follow without
+ * cancellation; a bound is skipped when its tail is below 1e-20 of the interval's mass
+ * (x <= -10 with x^2 > y^2 + 100 for the lower bound, mirrored for the upper).  The target
+ * p = Phi(a) + u m and its complement q = Q(b) + (1 - u) m (m the mass) are both formed
+ * directly, and the smaller one is inverted: z = ndtri(p) or z = -ndtri(q).  Beyond 35 sigma
+ * (Q underflows near 38) the Mills-ratio Newton solve takes over. */
 static double tn_std(double a, double b, double u) {
-    if (a > 0.0) {
-        if (a > 35.0) return tn_far(a, b, u);
-        double qa = 0.5 * erfc(a * M_SQRT1_2), qb = 0.5 * erfc(b * M_SQRT1_2);
-        double p = qa - u * (qa - qb);
-        double z = -orc_ndtri(p);
-        return fmin(fmax(z, a), b);
-    }
-    if (b < 0.0) return -tn_std(-b, -a, 1.0 - u);  /* mirror: quantile u of z is 1-u of -z */
-    double pa = 0.5 * erfc(-a * M_SQRT1_2), pb = 0.5 * erfc(-b * M_SQRT1_2);
-    double p = pa + u * (pb - pa), z;
-    if (p <= 0.5) {
-        z = orc_ndtri(p);
-    } else {
-        double qa = 0.5 * erfc(a * M_SQRT1_2), qb = 0.5 * erfc(b * M_SQRT1_2);
-        double qq = qb + (1.0 - u) * (qa - qb);
-        z = -orc_ndtri(qq);
-    }
+    if (a > 35.0) return tn_far(a, b, u);
+    if (b < -35.0) return -tn_far(-b, -a, 1.0 - u);  /* mirror: quantile u of z is 1-u of -z */
+    double ta = (a <= -10.0 && a * a > b * b + 100.0) ? 0.0 : 0.5 * erfc(fabs(a) * M_SQRT1_2);
+    double tb = (b >= 10.0 && b * b > a * a + 100.0) ? 0.0 : 0.5 * erfc(fabs(b) * M_SQRT1_2);
+    double pa = a < 0.0 ? ta : 1.0 - ta, qa = a < 0.0 ? 1.0 - ta : ta;
+    double pb = b < 0.0 ? tb : 1.0 - tb, qb = b < 0.0 ? 1.0 - tb : tb;
+    double m = a >= 0.0 ? qa - qb : pb - pa;
+    double p = pa + u * m, q = qb + (1.0 - u) * m;
+    double z = p <= q ? orc_ndtri(p) : -orc_ndtri(q);
     return fmin(fmax(z, a), b);
 }
 
 double orc_trunc_normal(double mu, double sigma, double lo, double hi, double u) {
     double a = (lo - mu) / sigma, b = (hi - mu) / sigma;
     double x = mu + sigma * tn_std(a, b, u);
+    return fmin(fmax(x, lo), hi);
+}
+
+/* the same draw parameterised by the square root of the precision, as the Gibbs mover uses it:
+ * a = (lo - mu) s, b = (hi - mu) s, x = mu + z / s */
+double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u) {
+    double a = (lo - mu) * s, b = (hi - mu) * s;
+    double x = mu + tn_std(a, b, u) / s;
     return fmin(fmax(x, lo), hi);
 }
 
@@ -533,7 +539,9 @@ static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma,
  * systematic-scan sweeps (i = 1..ndim); coordinate i is redrawn from N(mu_i, 1/Q_ii)
  * truncated to [0, max_slip] with
  *   mu_i = s_i - (sum_k Q_ki s_k - gamma*b_i) / Q_ii       (column i of Q)
- * using the open uniform u = ((w>>11) + 1/2) 2^-53 of draw (site 9, stage, c2=j, c1=i, c0=sweep).
+ * using the open uniform u = ((w>>11) + 1/2) 2^-53 of word (i & 1) of draw
+ * (site 9, stage, c2=j, c1=i>>1, c0=sweep): coordinates 2m and 2m+1 share one Philox block.
+ * The truncated normal is parameterised by s = sqrt(Q_ii): a = -mu s, b = (max_slip - mu) s.
  */
 static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, int sweeps,
                        const double* X, double* Xn, double* Ln, double* PRn, const int* assigned,
@@ -564,8 +572,8 @@ static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, i
                     double prec = Q[(size_t)c * nd + c];
                     double mu = s[c] - (qs - hb[c]) / prec;
                     uint64_t w[2];
-                    orc_draw_u64x2(key, SITE_GIBBS_MOVE, stage, j, c, sw, w);
-                    s[c] = orc_trunc_normal(mu, 1.0 / sqrt(prec), 0.0, smax, open_uniform(w[0]));
+                    orc_draw_u64x2(key, SITE_GIBBS_MOVE, stage, j, c >> 1, sw, w);
+                    s[c] = orc_trunc_normal_prec(mu, sqrt(prec), 0.0, smax, open_uniform(w[c & 1]));
                 }
             Ln[j] = inst_likelihood(I, s);
             PRn[j] = inst_prior(I, s, lsv);

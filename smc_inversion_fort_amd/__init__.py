@@ -289,20 +289,21 @@ class Problem:
         return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=ns.value, neglog=nl.value,
                     gamma=gt[:ns.value].copy())
 
-    def fault_smc_exec(self, range_, N, seed, max_stages=100, gsmc=None):
+    def fault_smc_exec(self, range_, N, seed, max_stages=100, gsmc=None, call_batch=None):
         """Outer SMC over the geometry (smc_fault.f90:785-1075, one rank).
 
         gsmc = (qmean, qcov_chol, qcov_inv, log_qcov_det) runs the gsmc branch: initial particles from
         N(qmean, qcov), tempering of L - q, the mixed MH ratio (qcov_chol is the lower Cholesky factor,
-        as main.f90:332 passes it)."""
+        as main.f90:332 passes it).  call_batch: geometries per library call (default: a whole wave)."""
         r = _f64(range_)
+        cb = int(call_batch or N)
         X = np.zeros((self.ndim_fault, N), order="F")
         L = np.zeros(N)
         ns = C.c_int()
         ev = C.c_double()
         gt = np.zeros(max_stages + 1)
         if gsmc is None:
-            _check(lib().smcf_fault_smc_exec(self.h, _dp(r), int(N), int(seed), int(max_stages), self.max_batch,
+            _check(lib().smcf_fault_smc_exec(self.h, _dp(r), int(N), int(seed), int(max_stages), cb,
                                              _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
         else:
             qm, qc, qi = (_f64(a) for a in gsmc[:3])
@@ -310,7 +311,7 @@ class Problem:
             if qm.size != nd or qc.size != nd * nd or qi.size != nd * nd:
                 raise SmcfError("gsmc: qmean / qcov shapes do not match ndim_fault")
             _check(lib().smcf_fault_smc_exec_gsmc(self.h, _dp(r), int(N), int(seed), int(max_stages),
-                                                  self.max_batch, _dp(qm), _dp(qc), _dp(qi), float(gsmc[3]),
+                                                  cb, _dp(qm), _dp(qc), _dp(qi), float(gsmc[3]),
                                                   _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
         return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy())
 
@@ -321,13 +322,7 @@ class Problem:
         k = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.zeros(B)
         ns = np.zeros(B, dtype=np.int32)
-        for s in range(0, B, self.max_batch):
-            e = min(B, s + self.max_batch)
-            sub = np.asfortranarray(th[:, s:e])
-            o = np.zeros(e - s)
-            n = np.zeros(e - s, dtype=np.int32)
-            _check(lib().smcf_fault_calc_likelihood_batch(self.h, e - s, _dp(sub), k[s:e].ctypes.data_as(_U64),
-                                                          _dp(o), _ip(n)))
-            out[s:e] = o
-            ns[s:e] = n
+        # the library streams the B geometries through the problem's max_batch device slots
+        _check(lib().smcf_fault_calc_likelihood_batch(self.h, B, _dp(np.asfortranarray(th)),
+                                                      k.ctypes.data_as(_U64), _dp(out), _ip(ns)))
         return out, ns

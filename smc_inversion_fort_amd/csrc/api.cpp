@@ -96,9 +96,10 @@ int upload(Problem& P, const T** dst, const std::vector<T>& v) {
   return SMCF_OK;
 }
 
-__global__ void k_state_init(BatchDev bt, int B) {
+__global__ void k_state_init(BatchDev bt, int B, const int* __restrict__ sel) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  if (sel) b = sel[b];
   bt.gamma[b] = 0.0;
   bt.neglog[b] = 0.0;
   bt.iter[b] = 1;
@@ -107,13 +108,47 @@ __global__ void k_state_init(BatchDev bt, int B) {
   bt.par[b] = 0;
 }
 
-int run_batch(Problem& P, int B, const double* theta, const uint64_t* keys, bool want_moments) {
+// Set up n instances: slots 0 .. n-1 (sel == nullptr) or the slots listed in bt.sel.  Thetas and keys
+// must already be in their slots.
+int setup_slots(Problem& P, int n, const int* sel) {
   smcf_ctx* ctx = P.ctx;
   hipStream_t s = ctx->stream;
   MeshDev& m = P.m;
   BatchDev& bt = P.bt;
-  if (B < 1 || B > bt.cap) {
-    set_error("batch size " + std::to_string(B) + " outside [1, " + std::to_string(bt.cap) + "]");
+  hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel);
+  if (P.mover == SMCF_MOVER_HMC) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, n, s, sel));
+  {
+    ProfScope ps(ctx, "geometry_greens");
+    CHECK_LAUNCH(launch_geometry(m, bt, n, s, sel));
+    CHECK_LAUNCH(launch_greens(m, bt, n, s, sel));
+  }
+  CHECK_LAUNCH(launch_laplacian(m, bt, n, s, sel));
+  {
+    ProfScope ps(ctx, "gram");
+    CHECK_LAUNCH(launch_gram(m, bt, n, s, sel));
+  }
+  {
+    ProfScope ps(ctx, "init_chain");
+    CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
+  }
+  CHECK_LAUNCH(launch_likelihood(m, bt, n, 0, s, sel));
+  return SMCF_OK;
+}
+
+// Inner SMC for N geometries streamed through the problem's slots.  The first min(N, cap)
+// geometries start together; after every stage the slots whose instance has finished are
+// harvested and refilled with the next geometries, so the device stays full while instances with
+// few stages leave early.  Every instance keeps its own stage counter and RNG key, so a result
+// does not depend on which slot or batch it ran in.  With N <= cap the slot b holds geometry b
+// at the end (its particles stay readable).
+int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
+               int* nstage) {
+  smcf_ctx* ctx = P.ctx;
+  hipStream_t s = ctx->stream;
+  MeshDev& m = P.m;
+  BatchDev& bt = P.bt;
+  if (N < 1) {
+    set_error("no geometries");
     return SMCF_EINVAL;
   }
   if (m.nsar > 0) {
@@ -121,51 +156,80 @@ int run_batch(Problem& P, int B, const double* theta, const uint64_t* keys, bool
     return SMCF_EUNSUPPORTED;
   }
   bool hmc = P.mover == SMCF_MOVER_HMC;
-  SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * m.ndim_fault * B, hipMemcpyHostToDevice, s));
-  SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * B, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_state_init, dim3((B + 63) / 64), dim3(64), 0, s, bt, B);
-  if (hmc) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, B, s));
-  {
-    ProfScope ps(ctx, "geometry_greens");
-    CHECK_LAUNCH(launch_geometry(m, bt, B, s));
-    CHECK_LAUNCH(launch_greens(m, bt, B, s));
-  }
-  CHECK_LAUNCH(launch_laplacian(m, bt, B, s));
-  {
-    ProfScope ps(ctx, "gram");
-    CHECK_LAUNCH(launch_gram(m, bt, B, s));
-  }
-  {
-    ProfScope ps(ctx, "init_chain");
-    CHECK_LAUNCH(launch_init_chain(m, bt, B, P.max_slip, s));
-  }
-  CHECK_LAUNCH(launch_likelihood(m, bt, B, 0, s));
-  std::vector<int> done(B), amax(2 * (size_t)B);
+  const int nf = m.ndim_fault;
+  const int W = std::min(N, bt.cap);  // slots in use
+  SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * nf * W, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * W, hipMemcpyHostToDevice, s));
+  int rc = setup_slots(P, W, nullptr);
+  if (rc) return rc;
+  std::vector<int> job(W), done(W), iter(W);
+  std::vector<double> nl(W);
+  for (int b = 0; b < W; ++b) job[b] = b;
+  int next = W, busy = W;
+  // refill once this many slots are free (or when nothing else is running)
+  const int admit_min = std::max(1, W / 16);
+  std::vector<int> freed;
+  std::vector<double> th_stage((size_t)nf * W);
+  std::vector<uint64_t> key_stage(W);
   StageCfg cfg{P.max_slip, bt.P, P.sweeps};
-  for (int stage = 1; stage <= 202; ++stage) {
+  std::vector<int> amax(2 * (size_t)W);
+  while (busy > 0) {
     {
       ProfScope ps(ctx, "gamma_resample");
-      CHECK_LAUNCH(launch_gamma_resample(m, bt, B, s));
+      CHECK_LAUNCH(launch_gamma_resample(m, bt, W, s));
     }
-    if (want_moments || hmc) CHECK_LAUNCH(launch_moments(m, bt, B, s));
+    if (want_moments || hmc) CHECK_LAUNCH(launch_moments(m, bt, W, s));
     if (hmc) {
       ProfScope ps(ctx, "hmc");
-      CHECK_LAUNCH(launch_hmc_stage(m, bt, P.hm, B, P.max_slip, s, amax.data()));
+      CHECK_LAUNCH(launch_hmc_stage(m, bt, P.hm, W, P.max_slip, s, amax.data()));
     } else {
-      CHECK_LAUNCH(launch_gather(m, bt, B, s));
-      CHECK_LAUNCH(launch_form_q(m, bt, B, s));
+      CHECK_LAUNCH(launch_gather(m, bt, W, s));
+      CHECK_LAUNCH(launch_form_q(m, bt, W, s));
       {
         ProfScope ps(ctx, "gibbs");
-        CHECK_LAUNCH(launch_gibbs(m, bt, B, cfg, s));
+        CHECK_LAUNCH(launch_gibbs(m, bt, W, cfg, s));
       }
       {
         ProfScope ps(ctx, "likelihood");
-        CHECK_LAUNCH(launch_likelihood(m, bt, B, 1, s));
+        CHECK_LAUNCH(launch_likelihood(m, bt, W, 1, s));
       }
     }
-    SMCF_HIP(hipMemcpyAsync(done.data(), bt.done, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(iter.data(), bt.iter, sizeof(int) * W, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(nl.data(), bt.neglog, sizeof(double) * W, hipMemcpyDeviceToHost, s));
     SMCF_HIP(hipStreamSynchronize(s));
-    if (std::all_of(done.begin(), done.end(), [](int d) { return d != 0; })) break;
+    for (int b = 0; b < W; ++b)
+      if (job[b] >= 0 && done[b]) {
+        if (neglog) neglog[job[b]] = nl[b];
+        if (nstage) nstage[job[b]] = std::min(iter[b] - 1, 201);
+        job[b] = -1;
+        --busy;
+        freed.push_back(b);
+      }
+    if (next < N && !freed.empty() && ((int)freed.size() >= admit_min || busy == 0)) {
+      int n = std::min((int)freed.size(), N - next);
+      std::vector<int> slots(freed.begin(), freed.begin() + n);
+      freed.erase(freed.begin(), freed.begin() + n);
+      for (int q = 0; q < n; ++q) {
+        int b = slots[q], g = next + q;
+        std::copy(theta + (size_t)g * nf, theta + (size_t)(g + 1) * nf, th_stage.begin() + (size_t)q * nf);
+        key_stage[q] = keys[g];
+        job[b] = g;
+      }
+      // scatter the new thetas and keys into their slots
+      for (int q = 0; q < n; ++q) {
+        SMCF_HIP(hipMemcpyAsync(bt.theta + (size_t)slots[q] * nf, th_stage.data() + (size_t)q * nf,
+                                sizeof(double) * nf, hipMemcpyHostToDevice, s));
+        SMCF_HIP(hipMemcpyAsync(bt.key + slots[q], &key_stage[q], sizeof(uint64_t), hipMemcpyHostToDevice, s));
+      }
+      SMCF_HIP(hipMemcpyAsync(bt.sel, slots.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+      rc = setup_slots(P, n, bt.sel);
+      if (rc) return rc;
+      // the host staging buffers must outlive the asynchronous copies
+      SMCF_HIP(hipStreamSynchronize(s));
+      next += n;
+      busy += n;
+    }
   }
   SMCF_HIP(hipStreamSynchronize(s));
   resolve_profiles(ctx);
@@ -294,6 +358,24 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
       n2e_slot[4 * nd0 + c] = a;
     }
   m.nnode = nnode; m.ndof = ndof; m.npatch = npatch; m.ndim = 2 * ndof;
+  {
+    // LL couples a DOF only with DOFs of the 5 x 5 node window on its own plane (one slip component);
+    // the prior chain's pipeline lag must exceed the widest forward reach in DOF order
+    int band = 0;
+    for (int k = 0; k < ndof; ++k) {
+      int ji = id_dof[k] - 1, ip = node_plane[ji], W = pnxi[ip] + 1;
+      for (int wl = -2; wl <= 2; ++wl)
+        for (int wk = -2; wk <= 2; ++wk) {
+          int kk = node_k[ji] + wk, ll = node_l[ji] + wl;
+          if (kk < 1 || kk > W || ll < 1 || ll > pneta[ip] + 1) continue;
+          int dr = dof_of_node[pnode0[ip] + (kk - 1) + (ll - 1) * W];
+          if (dr >= 0) band = std::max(band, dr - k);
+        }
+    }
+    m.chain_lag = std::max(band + 1, (ndof + kChainLanes - 2) / (kChainLanes - 1));
+    m.chain_w = 1;
+    while (m.chain_w < m.chain_lag + 1) m.chain_w *= 2;
+  }
   // observation points: SAR rows, then one point per GNSS triplet (gfunc.f90:212-239)
   m.nobs = d->nobs; m.nsar = d->nsar_total; m.ngnss = d->ngnss;
   if (m.nsar + 3 * m.ngnss != m.nobs) { set_error("nobs != nsar_total + 3*ngnss"); return SMCF_EINVAL; }
@@ -356,7 +438,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.H, nd * nd * C);
   rc |= dalloc(P, &bt.bvec, nd * C);
   rc |= dalloc(P, &bt.dsd, C);
-  rc |= dalloc(P, &bt.Q, (nd + 16) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  rc |= dalloc(P, &bt.Q, (nd + 32) * nd * C);  // blocked Gibbs layout, padded to whole blocks
   rc |= dalloc(P, &bt.S, nd * Pn * C);
   rc |= dalloc(P, &bt.S2, nd * Pn * C);
   rc |= dalloc(P, &bt.Y, nd * Pn * C);
@@ -375,6 +457,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.iter, C);
   rc |= dalloc(P, &bt.active, C);
   rc |= dalloc(P, &bt.done, C);
+  rc |= dalloc(P, &bt.sel, C);
   rc |= dalloc(P, &bt.par, C);
   rc |= dalloc(P, &bt.gtrace, 202 * C);
   if (P.mover == SMCF_MOVER_HMC) {
@@ -528,7 +611,7 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
   if (!pb || !theta) return SMCF_EINVAL;
   Problem& P = pb->p;
   SMCF_HIP(hipSetDevice(P.ctx->device));
-  int rc = run_batch(P, 1, theta, &key, mean != nullptr);
+  int rc = run_stream(P, 1, theta, &key, mean != nullptr, nullptr, nullptr);
   if (rc) return rc;
   hipStream_t s = P.ctx->stream;
   MeshDev& m = P.m;
@@ -559,15 +642,7 @@ int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* thet
   if (!pb || !theta || !keys || !neglog) return SMCF_EINVAL;
   Problem& P = pb->p;
   SMCF_HIP(hipSetDevice(P.ctx->device));
-  int rc = run_batch(P, B, theta, keys, false);
-  if (rc) return rc;
-  SMCF_HIP(hipMemcpy(neglog, P.bt.neglog, sizeof(double) * B, hipMemcpyDeviceToHost));
-  if (nstage) {
-    std::vector<int> it(B);
-    SMCF_HIP(hipMemcpy(it.data(), P.bt.iter, sizeof(int) * B, hipMemcpyDeviceToHost));
-    for (int b = 0; b < B; ++b) nstage[b] = std::min(it[b] - 1, 201);
-  }
-  return SMCF_OK;
+  return run_stream(P, B, theta, keys, false, neglog, nstage);
 }
 
 int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, double cv, double tol, int cap,

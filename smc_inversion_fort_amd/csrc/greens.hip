@@ -16,10 +16,12 @@ namespace smcf {
 
 static constexpr double kPI2 = 6.283185307179586;
 
-__global__ void k_geometry(MeshDev m, const double* __restrict__ theta, double* __restrict__ geo, int B) {
+__global__ void k_geometry(MeshDev m, const double* __restrict__ theta, double* __restrict__ geo, int B,
+                           const int* __restrict__ sel) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B * m.nplane) return;
   int b = t / m.nplane, ip = t % m.nplane + 1;  // 1-based plane
+  if (sel) b = sel[b];
   const double* th = theta + (size_t)b * m.ndim_fault;
   const double pi = 3.141592653589793;  // 4*atan(1)
   int np = m.nplane;
@@ -108,10 +110,10 @@ SMCF_DEV void ub0_disp_z0(double x, double y, double d, double sd, double cd, do
 }
 
 __global__ void __launch_bounds__(128) k_greens_p1(MeshDev m, const double* __restrict__ geo,
-                                                   double* __restrict__ gscr, int B) {
+                                                   double* __restrict__ gscr, int B, const int* __restrict__ sel) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;  // observation point
   int e = blockIdx.y;                              // patch (0-based)
-  int b = blockIdx.z;
+  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
   if (k >= m.npts) return;
   int ip = m.patch_plane[e];
   const double* g = geo + ((size_t)b * m.nplane + ip) * 16;
@@ -194,10 +196,10 @@ __global__ void __launch_bounds__(128) k_greens_p1(MeshDev m, const double* __re
 }
 
 __global__ void __launch_bounds__(128) k_greens_p2(MeshDev m, const double* __restrict__ gscr,
-                                                   double* __restrict__ G, int B) {
+                                                   double* __restrict__ G, int B, const int* __restrict__ sel) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int d = blockIdx.y;
-  int b = blockIdx.z;
+  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
   if (k >= m.npts) return;
   int node = m.id_dof[d] - 1;
   double resp[6] = {0, 0, 0, 0, 0, 0};
@@ -224,17 +226,17 @@ __global__ void __launch_bounds__(128) k_greens_p2(MeshDev m, const double* __re
   }
 }
 
-hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
   int n = B * m.nplane;
-  hipLaunchKernelGGL(k_geometry, dim3((n + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.geo, B);
+  hipLaunchKernelGGL(k_geometry, dim3((n + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.geo, B, sel);
   return hipGetLastError();
 }
 
-hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
   dim3 g1((m.npts + 127) / 128, m.npatch, B);
-  hipLaunchKernelGGL(k_greens_p1, g1, dim3(128), 0, s, m, bt.geo, bt.gscr, B);
+  hipLaunchKernelGGL(k_greens_p1, g1, dim3(128), 0, s, m, bt.geo, bt.gscr, B, sel);
   dim3 g2((m.npts + 127) / 128, m.ndof, B);
-  hipLaunchKernelGGL(k_greens_p2, g2, dim3(128), 0, s, m, bt.gscr, bt.G, B);
+  hipLaunchKernelGGL(k_greens_p2, g2, dim3(128), 0, s, m, bt.gscr, bt.G, B, sel);
   return hipGetLastError();
 }
 

@@ -376,17 +376,18 @@ __global__ void k_tune_resample(int P, BatchDev bt, HmcDev h, int B) {
   h.ldlo[b] = dd_add_d(lm, -l5).hi;
 }
 
-__global__ void k_hmc_tune_init(HmcDev h, int B, int P) {
+__global__ void k_hmc_tune_init(HmcDev h, int B, int P, const int* __restrict__ sel) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  if (sel) b = sel[b];
   h.ldhi[b] = 0.0;
   h.ldlo[b] = -1e1;
   h.flag[b] = 0;
   for (int q = 0; q < P; ++q) h.dtau[(size_t)b * P + q] = exp(-5.0);
 }
 
-hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s) {
-  hipLaunchKernelGGL(k_hmc_tune_init, dim3((B + 63) / 64), dim3(64), 0, s, h, B, bt.P);
+hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel) {
+  hipLaunchKernelGGL(k_hmc_tune_init, dim3((B + 63) / 64), dim3(64), 0, s, h, B, bt.P, sel);
   return hipGetLastError();
 }
 

@@ -16,9 +16,10 @@ namespace smcf {
 
 SMCF_DEV int slot_of(int dk, int dl) { return (dk + 1) + 3 * (dl + 1); }
 
-__global__ void k_luni(MeshDev m, const double* __restrict__ geo, double* __restrict__ luni, int B) {
+__global__ void k_luni(MeshDev m, const double* __restrict__ geo, double* __restrict__ luni, int B,
+                       const int* __restrict__ sel) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   if (n >= m.nnode) return;
   int ip = m.node_plane[n];
   const double* g = geo + ((size_t)b * m.nplane + ip) * 16;
@@ -66,9 +67,9 @@ SMCF_DEV int slot_node(const MeshDev& m, int n, int dk, int dl) {
 }
 
 __global__ void k_lell(MeshDev m, const double* __restrict__ luni, int* __restrict__ li, double* __restrict__ lv,
-                       int* __restrict__ lti, double* __restrict__ ltv, int B) {
+                       int* __restrict__ lti, double* __restrict__ ltv, int B, const int* __restrict__ sel) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   const double* lu = luni + (size_t)b * m.nnode * 9;
   if (r < 2 * m.nnode) {
     // row r = 2*n + c of L: columns 2*dof(j) + c for DOF nodes j in ascending order
@@ -113,11 +114,11 @@ __global__ void k_lell(MeshDev m, const double* __restrict__ luni, int* __restri
 // Column i = 2*d + c of L^T L: LL(r, i) = sum_n luni(n, j_r) luni(n, j_i) over nodes n in
 // ascending order (all nodes, same component), kept in a 5x5 window around node j_i.
 __global__ void k_llcols(MeshDev m, const double* __restrict__ luni, int* __restrict__ idx,
-                         double* __restrict__ val, int B) {
+                         double* __restrict__ val, int B, const int* __restrict__ sel) {
   // separate multiply and add, as in the reference's dense product
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   if (i >= m.ndim) return;
   const double* lu = luni + (size_t)b * m.nnode * 9;
   int d = i >> 1, c = i & 1;
@@ -161,13 +162,13 @@ __global__ void k_llcols(MeshDev m, const double* __restrict__ luni, int* __rest
   for (; cnt < 25; ++cnt) { oi[cnt] = -1; ov[cnt] = 0.0; }
 }
 
-hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  hipLaunchKernelGGL(k_luni, dim3((m.nnode + 127) / 128, B), dim3(128), 0, s, m, bt.geo, bt.luni, B);
+hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+  hipLaunchKernelGGL(k_luni, dim3((m.nnode + 127) / 128, B), dim3(128), 0, s, m, bt.geo, bt.luni, B, sel);
   int rows = 2 * m.nnode + 2 * m.ndof;
   hipLaunchKernelGGL(k_lell, dim3((rows + 127) / 128, B), dim3(128), 0, s, m, bt.luni, bt.li, bt.lv, bt.lti,
-                     bt.ltv, B);
+                     bt.ltv, B, sel);
   hipLaunchKernelGGL(k_llcols, dim3((m.ndim + 127) / 128, B), dim3(128), 0, s, m, bt.luni, bt.llc_idx,
-                     bt.llc_val, B);
+                     bt.llc_val, B, sel);
   return hipGetLastError();
 }
 

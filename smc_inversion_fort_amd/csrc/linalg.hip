@@ -15,9 +15,11 @@
 
 namespace smcf {
 
-__global__ void k_scal(MeshDev m, const double* __restrict__ theta, double* __restrict__ scal, int B) {
+__global__ void k_scal(MeshDev m, const double* __restrict__ theta, double* __restrict__ scal, int B,
+                       const int* __restrict__ sel) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  if (sel) b = sel[b];
   const double* th = theta + (size_t)b * m.ndim_fault;
   double la2, lss2, lsg2;
   if (m.fix_xbend) {
@@ -40,10 +42,11 @@ constexpr int TM = 64, TN = 64, TK = 16;
 
 // H(i, j) = sum_k (G(k, i) / c_k) G(k, j), c_k = obs_sigma_k^2 * sigma_gnss^2 (GNSS rows)
 __global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restrict__ Gall,
-                                              const double* __restrict__ scal, double* __restrict__ Hall) {
+                                              const double* __restrict__ scal, double* __restrict__ Hall,
+                                              const int* __restrict__ sel) {
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
-  int b = blockIdx.z;
+  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
   int i0 = blockIdx.x * TM, j0 = blockIdx.y * TN;
   int nobs = m.nobs, nd = m.ndim;
   const double* G = Gall + (size_t)b * nobs * nd;
@@ -96,9 +99,9 @@ __global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restric
 }
 
 __global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const double* __restrict__ scal,
-                           double* __restrict__ bvec, double* __restrict__ dsd, int B) {
+                           double* __restrict__ bvec, double* __restrict__ dsd, int B, const int* __restrict__ sel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   int nobs = m.nobs, nd = m.ndim;
   double sg2 = scal[8 * (size_t)b + 5];
   if (i < nd) {
@@ -121,12 +124,12 @@ __global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const dou
   }
 }
 
-hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B);
+hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+  hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B, sel);
   dim3 g((m.ndim + TM - 1) / TM, (m.ndim + TN - 1) / TN, B);
-  hipLaunchKernelGGL(k_gram, g, dim3(256), 0, s, m, bt.G, bt.scal, bt.H);
+  hipLaunchKernelGGL(k_gram, g, dim3(256), 0, s, m, bt.G, bt.scal, bt.H, sel);
   hipLaunchKernelGGL(k_gram_vec, dim3((m.ndim + 1 + 127) / 128, B), dim3(128), 0, s, m, bt.G, bt.scal, bt.bvec,
-                     bt.dsd, B);
+                     bt.dsd, B, sel);
   return hipGetLastError();
 }
 
@@ -134,10 +137,10 @@ hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
 __global__ void __launch_bounds__(256) k_hs(int nd, int P, const double* __restrict__ Hall,
                                             const double* __restrict__ S0, const double* __restrict__ S1,
                                             const int* __restrict__ par, const int* __restrict__ active,
-                                            double* __restrict__ Yall, int use_active) {
+                                            double* __restrict__ Yall, int use_active, const int* __restrict__ sel) {
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
-  int b = blockIdx.z;
+  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
   if (use_active && !active[b]) return;
   int i0 = blockIdx.y * TM, p0 = blockIdx.x * TN;
   const double* H = Hall + (size_t)b * nd * nd;
@@ -187,9 +190,10 @@ __global__ void k_like(MeshDev m, int P, const double* __restrict__ S0, const do
                        const double* __restrict__ bvec, const double* __restrict__ dsd,
                        const double* __restrict__ scal, const int* __restrict__ liall,
                        const double* __restrict__ lvall, double* __restrict__ L0, double* __restrict__ L1,
-                       double* __restrict__ R0, double* __restrict__ R1, int use_active) {
+                       double* __restrict__ R0, double* __restrict__ R1, int use_active,
+                       const int* __restrict__ sel) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   if (use_active && !active[b]) return;
   if (p >= P) return;
   int nd = m.ndim;
@@ -219,12 +223,13 @@ __global__ void k_like(MeshDev m, int P, const double* __restrict__ S0, const do
   (pb ? R1 : R0)[(size_t)b * P + p] = pr;
 }
 
-hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s) {
+hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel) {
   int P = bt.P;
   dim3 g((P + TN - 1) / TN, (m.ndim + TM - 1) / TM, B);
-  hipLaunchKernelGGL(k_hs, g, dim3(256), 0, s, m.ndim, P, bt.H, bt.S, bt.S2, bt.par, bt.active, bt.Y, use_active);
+  hipLaunchKernelGGL(k_hs, g, dim3(256), 0, s, m.ndim, P, bt.H, bt.S, bt.S2, bt.par, bt.active, bt.Y, use_active,
+                     sel);
   hipLaunchKernelGGL(k_like, dim3((P + 127) / 128, B), dim3(128), 0, s, m, P, bt.S, bt.S2, bt.par, bt.active, bt.Y,
-                     bt.bvec, bt.dsd, bt.scal, bt.li, bt.lv, bt.like, bt.like2, bt.prior, bt.prior2, use_active);
+                     bt.bvec, bt.dsd, bt.scal, bt.li, bt.lv, bt.like, bt.like2, bt.prior, bt.prior2, use_active, sel);
   return hipGetLastError();
 }
 

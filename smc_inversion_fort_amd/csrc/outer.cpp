@@ -161,7 +161,7 @@ int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int m
       }
     }
   }
-  // batched evaluation helper
+  // batched evaluation helper: calls of at most max_batch geometries, each streamed through the slots
   auto eval = [&](const std::vector<double>& th, const std::vector<uint64_t>& keys, std::vector<double>& out) -> int {
     int B = (int)keys.size();
     out.resize(B);

@@ -14,19 +14,44 @@
 namespace smcf {
 
 // ---------------------------------------------------------------------------
-__global__ void k_init_chain(MeshDev m, int P, int B, double max_slip, const uint64_t* __restrict__ keys,
-                             const double* __restrict__ scal, const int* __restrict__ llidx,
-                             const double* __restrict__ llval, double* __restrict__ Sall) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  int nd = m.ndim;
+// Prior draw: one sequential Gibbs chain over all P particles, one sweep per particle, started at
+// s = 0, with Newton on the normal CDF for the truncated draw (slip_gen_init_particles,
+// smc_slip.f90:184-287).
+//
+// The chain is pipelined without changing a single operation.  LL couples strike slip (even
+// dimensions) only with strike slip and dip slip only with dip slip, so each component is its
+// own chain.  Within a component, coordinate d of particle k reads particle k's coordinates
+// d' < d and particle k-1's coordinates d < d' <= d + band.  Lane l (of the 32 lanes of a
+// component) runs particles k = l, l + 32, ...; particle k starts at round k * lag with
+// lag > band, so in every round particle k-1 is exactly lag coordinates ahead and has written
+// everything particle k reads.  Each lane keeps its particle's last W coordinates in an LDS
+// ring (W >= lag + 1); 31 * lag >= ndof guarantees that a lane's ring is not reused while the
+// next lane still reads it, and that particle 0 reads zeros from the ring of lane 31.
+__global__ void __launch_bounds__(64) k_init_chain(MeshDev m, int P, double max_slip,
+                                                   const uint64_t* __restrict__ keys,
+                                                   const double* __restrict__ scal, const int* __restrict__ llidx,
+                                                   const double* __restrict__ llval, double* __restrict__ Sall,
+                                                   const int* __restrict__ sel) {
+  extern __shared__ double ring[];
+  int b = sel ? sel[blockIdx.x] : (int)blockIdx.x;
+  int lane = threadIdx.x, c = lane / kChainLanes, l = lane % kChainLanes;
+  int nd = m.ndim, ndof = nd / 2, lag = m.chain_lag, W = m.chain_w;
   uint64_t key = keys[b];
   double a2 = scal[8 * (size_t)b];
   const int* IX = llidx + (size_t)b * nd * 25;
   const double* VX = llval + (size_t)b * nd * 25;
   double* S = Sall + (size_t)b * nd * P;
-  for (int k = 0; k < P; ++k) {
-    for (int i = 0; i < nd; ++i) {
+  double* mine = ring + (size_t)lane * W;
+  const double* prev = ring + (size_t)(c * kChainLanes + (l + kChainLanes - 1) % kChainLanes) * W;
+  for (int w = 0; w < W; ++w) mine[w] = 0.0;
+  __syncthreads();
+  long rounds = (long)(P - 1) * lag + ndof;
+  for (long t = 0; t < rounds; ++t) {
+    long kmax = t / lag;
+    long k = kmax - (((kmax - l) % kChainLanes) + kChainLanes) % kChainLanes;
+    long dl = t - k * lag;
+    if (k >= 0 && k < P && dl < ndof) {
+      int d = (int)dl, i = 2 * d + c;
       const int* ix = IX + (size_t)i * 25;
       const double* vx = VX + (size_t)i * 25;
       double lii = 0.0, mu = 0.0;
@@ -38,7 +63,8 @@ __global__ void k_init_chain(MeshDev m, int P, int B, double max_slip, const uin
         int j = ix[w];
         if (j < 0) break;
         if (j == i) continue;
-        double cj = j < i ? S[(size_t)j * P + k] : (k > 0 ? S[(size_t)j * P + k - 1] : 0.0);
+        int dj = j >> 1;  // same component as i
+        double cj = j < i ? mine[dj & (W - 1)] : prev[dj & (W - 1)];
         mu = mu - vx[w] / a2 * cj * s2;
       }
       double v1 = draw_normal(key, kSiteGibbsInit, 0, (uint32_t)k, (uint32_t)i);
@@ -55,14 +81,18 @@ __global__ void k_init_chain(MeshDev m, int P, int B, double max_slip, const uin
         yp = y;
         cnt++;
       }
+      mine[d & (W - 1)] = y;
       S[(size_t)i * P + k] = y;
     }
+    __syncthreads();
   }
 }
 
-hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s) {
-  hipLaunchKernelGGL(k_init_chain, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.P, B, max_slip, bt.key, bt.scal,
-                     bt.llc_idx, bt.llc_val, bt.S);
+hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s, const int* sel) {
+  size_t sh = sizeof(double) * 2 * kChainLanes * (size_t)m.chain_w;
+  if (sh > 64 * 1024) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k_init_chain, dim3(B), dim3(2 * kChainLanes), sh, s, m, bt.P, max_slip, bt.key, bt.scal,
+                     bt.llc_idx, bt.llc_val, bt.S, sel);
   return hipGetLastError();
 }
 
@@ -290,28 +320,47 @@ hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // one coordinate redraw: Philox uniform + truncated-normal inversion (kept out of line so the
 // unrolled block loop does not replicate it)
-__device__ __noinline__ double gibbs_draw(double mu, double prec, double max_slip, uint64_t key, uint32_t stage,
-                                          uint32_t p, uint32_t i, uint32_t sw) {
-  uint64_t w0, w1;
-  philox_draw(key, kSiteGibbsMove, stage, p, i, sw, w0, w1);
-  return trunc_normal(mu, 1.0 / sqrt(prec), 0.0, max_slip, u01_open(w0));
+__device__ __noinline__ double gibbs_draw(double mu, double prec, double max_slip, double u) {
+  return trunc_normal_prec(mu, sqrt(prec), 0.0, max_slip, u);
 }
 
-// Q is stored per stage in blocks of KB columns, row-major inside the block:
-//   Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k)   (zero for columns past nd)
-// so the KB values a wavefront needs at row j are contiguous and uniform across lanes.
-template <int KB, int JU>
-__global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double max_slip,
+// Q is stored per stage in blocks of KB = 32 columns, row-major inside the block:
+//   Qb[(blk*nd + j)*32 + k] = Q(j, blk*32 + k)   (zero for columns past nd)
+//
+// One wavefront owns 64 particles; a workgroup of 4 wavefronts owns 256 particles of one instance.
+// Per block of 32 coordinates (two 16-coordinate tiles m) and sweep:
+//  * matrix phase: q(i0 + 16m + r, p) = sum_j Q(j, i0 + 16m + r) s_j(p) for the wave's 64 particles
+//    (four 16-particle tiles n) as 8 accumulators of v_mfma_f64_16x16x4f64 over j in steps of 4.
+//    Operand fragments are plain coalesced loads: A(r, k) = Q(j0 + k, col) at lane r + 16k,
+//    B(k, c) = s_{j0+k}(p0 + c) at lane c + 16k; lane l holds D(4t + l/16, l%16) in register t.
+//  * draw phase, per tile m: the accumulator tile is transposed through 8 KB of wave-private LDS
+//    so that lane l reads q of its own particle, then the 16 coordinates are redrawn in order with
+//    the in-block corrections Q(i0 + mm, i0 + k) delta_mm of the coordinates already redrawn.
+// While one wavefront draws (VALU) another can run its matrix phase on the matrix cores.
+//
+// Workgroup -> (instance, particle chunk) is XCD-aware: workgroup w runs on XCD w % 8, and all
+// chunks of one instance get the same w % 8, so an instance's Q blocks are fetched into one L2.
+constexpr int kXcd = 8;
+constexpr int kGibbsKB = 32;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
                                                const double* __restrict__ Qball, double* __restrict__ S0,
                                                double* __restrict__ S1, const double* __restrict__ bvall,
                                                const double* __restrict__ gam, const uint64_t* __restrict__ keys,
                                                const int* __restrict__ iters, const int* __restrict__ active,
                                                const int* __restrict__ par) {
-  int b = blockIdx.y;
-  if (!active[b]) return;
-  int p = blockIdx.x * 256 + threadIdx.x;
+  constexpr int KB = kGibbsKB;
+  __shared__ double T[4][16][65];
+  int chunks = (P + 255) / 256;
+  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
+  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
+  if (b >= B || !active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int pw0 = chunk * 256 + wave * 64;
+  int p = pw0 + lane;
   bool valid = p < P;
-  int pc = valid ? p : 0;
+  int pc = valid ? p : P - 1;
   double* S = (par[b] ? S1 : S0) + (size_t)b * nd * P;
   const double* Qb = Qball + (size_t)b * nd * ((nd + KB - 1) / KB) * KB;
   const double* bv = bvall + (size_t)b * nd;
@@ -319,64 +368,113 @@ __global__ void __launch_bounds__(256) k_gibbs(int nd, int P, int sweeps, double
   uint64_t key = keys[b];
   uint32_t stage = (uint32_t)(iters[b] - 1);
   int nblk = (nd + KB - 1) / KB;
+  int fi = lane % 16, fk = lane / 16;
+  // fragment columns: particles of the four 16-particle tiles (clamped into range; unused if invalid)
+  int pcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) pcol[n] = min(pw0 + 16 * n + fi, P - 1);
+  int nd4 = nd & ~3;
+  double (*Tw)[65] = T[wave];
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int blk = 0; blk < nblk; ++blk) {
       int i0 = blk * KB;
-      int kb = min(KB, nd - i0);
       const double* Qr = Qb + (size_t)blk * nd * KB;
-      double q[KB];
+      dbl4 acc[2][4];
 #pragma unroll
-      for (int k = 0; k < KB; ++k) q[k] = 0.0;
-      int j = 0;
-      for (; j + JU <= nd; j += JU) {
-        double sv[JU];
+      for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int u = 0; u < JU; ++u) sv[u] = S[(size_t)(j + u) * P + pc];
+        for (int n = 0; n < 4; ++n) acc[m][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+      const double* qa = Qr + (size_t)fk * KB + fi;
+      const double* sb = S + (size_t)fk * P;
+#pragma unroll 2
+      for (int j0 = 0; j0 < nd4; j0 += 4) {
+        double a0 = qa[(size_t)j0 * KB], a1 = qa[(size_t)j0 * KB + 16];
+        double bb[4];
 #pragma unroll
-        for (int u = 0; u < JU; ++u) {
-          const double* qr = Qr + (size_t)(j + u) * KB;
+        for (int n = 0; n < 4; ++n) bb[n] = sb[(size_t)j0 * P + pcol[n]];
 #pragma unroll
-          for (int k = 0; k < KB; ++k) q[k] = q[k] + qr[k] * sv[u];
+        for (int n = 0; n < 4; ++n) {
+          acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb[n], acc[0][n], 0, 0, 0);
+          acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb[n], acc[1][n], 0, 0, 0);
         }
       }
-      for (; j < nd; ++j) {
-        double sj = S[(size_t)j * P + pc];
-        const double* qr = Qr + (size_t)j * KB;
+      if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
+        bool in = nd4 + fk < nd;
+        double a0 = in ? qa[(size_t)nd4 * KB] : 0.0, a1 = in ? qa[(size_t)nd4 * KB + 16] : 0.0;
 #pragma unroll
-        for (int k = 0; k < KB; ++k) q[k] = q[k] + qr[k] * sj;
+        for (int n = 0; n < 4; ++n) {
+          double bn = in ? sb[(size_t)nd4 * P + pcol[n]] : 0.0;
+          acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bn, acc[0][n], 0, 0, 0);
+          acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bn, acc[1][n], 0, 0, 0);
+        }
       }
-      double delta[KB];
 #pragma unroll
-      for (int k = 0; k < KB; ++k) {
-        delta[k] = 0.0;
-        if (k < kb) {
+      for (int m = 0; m < 2; ++m) {
+        if (m == 1) {
+          // corrections of tile 1 by the redrawn tile 0: acc1(r, p) += sum_k Q(i0 + k, i0 + 16 + r) delta_k(p),
+          // with delta_k(p) left in Tw[k][p] by the draws of tile 0
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            int kq = 4 * s4 + fk;
+            double a = (i0 + kq < nd) ? Qr[(size_t)(i0 + kq) * KB + 16 + fi] : 0.0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n)
+              acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Tw[kq][16 * n + fi], acc[1][n], 0, 0, 0);
+          }
+          __builtin_amdgcn_wave_barrier();  // every lane has read tile 0's deltas before they are overwritten
+        }
+        // transpose tile m through LDS: Tw[row][particle]
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) Tw[4 * t + fk][16 * n + fi] = acc[m][n][t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t wodd = 0;
+        for (int kk = 0; kk < 16; ++kk) {
+          int k = 16 * m + kk;
           int i = i0 + k;
-          double qi = q[k];
-#pragma unroll
-          for (int mm = 0; mm < k; ++mm) qi = qi + Qr[(size_t)(i0 + mm) * KB + k] * delta[mm];
+          if (i >= nd) {
+            Tw[kk][lane] = 0.0;
+            continue;
+          }
+          // coordinates 2m' and 2m'+1 share one Philox block (word i & 1); i0 and the tile offset are even
+          double u;
+          if ((kk & 1) == 0) {
+            uint64_t w0;
+            philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, w0, wodd);
+            u = u01_open(w0);
+          } else {
+            u = u01_open(wodd);
+          }
+          // Tw[mm][lane] holds delta of coordinate i0 + 16m + mm once it is redrawn
+          double qi = Tw[kk][lane];
+          for (int mm = 0; mm < kk; ++mm) qi = fma(Qr[(size_t)(i0 + 16 * m + mm) * KB + k], Tw[mm][lane], qi);
           double prec = Qr[(size_t)i * KB + k];
           double so = S[(size_t)i * P + pc];
           double mu = so - (qi - gamma * bv[i]) / prec;
-          double sn = gibbs_draw(mu, prec, max_slip, key, stage, (uint32_t)pc, (uint32_t)i, (uint32_t)sw);
-          delta[k] = sn - so;
+          double sn = gibbs_draw(mu, prec, max_slip, u);
+          Tw[kk][lane] = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
   }
 }
 
-int gibbs_kb(int nd) { return nd <= 1024 ? 16 : 8; }
+int gibbs_kb(int) { return kGibbsKB; }
 
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
   int nd = m.ndim;
-  dim3 g((bt.P + 255) / 256, B);
-  if (gibbs_kb(nd) == 16)
-    hipLaunchKernelGGL((k_gibbs<16, 4>), g, dim3(256), 0, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
-                       bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
-  else
-    hipLaunchKernelGGL((k_gibbs<8, 4>), g, dim3(256), 0, s, nd, bt.P, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
-                       bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
+  int chunks = (bt.P + 255) / 256;
+  int groups = (B + kXcd - 1) / kXcd;
+  dim3 g(groups * kXcd * chunks);
+  hipLaunchKernelGGL(k_gibbs, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
+                     bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
   return hipGetLastError();
 }
 

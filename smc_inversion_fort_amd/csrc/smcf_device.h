@@ -184,32 +184,33 @@ SMCF_DEV double tn_far(double a, double b, double u) {
   }
   return fmin(fmax(z, a), b);
 }
-SMCF_DEV double tn_upper(double a, double b, double u) {  // requires a > 0
-  if (a > 35.0) return tn_far(a, b, u);
-  const double r2 = 0.70710678118654752440;
-  double qa = 0.5 * erfc(a * r2), qb = 0.5 * erfc(b * r2);
-  double p = qa - u * (qa - qb);
-  double z = -ndtri(p);
-  return fmin(fmax(z, a), b);
-}
+// Standard normal truncated to [a, b] by inversion.  One erfc per bound gives t = Q(|x|), from
+// which Phi(x) and Q(x) follow without cancellation; a bound whose tail is below 1e-20 of the
+// interval's mass is not evaluated.  p = Phi(a) + u m and q = Q(b) + (1 - u) m are both formed
+// directly and the smaller is inverted, so every lane runs the same straight-line path except for
+// the Mills-ratio solve beyond 35 sigma.
 SMCF_DEV double tn_std(double a, double b, double u) {
+  if (a > 35.0) return tn_far(a, b, u);
+  if (b < -35.0) return -tn_far(-b, -a, 1.0 - u);  // mirror: quantile u of z is 1-u of -z
   const double r2 = 0.70710678118654752440;
-  if (a > 0.0) return tn_upper(a, b, u);
-  if (b < 0.0) return -tn_upper(-b, -a, 1.0 - u);  // mirror: quantile u of z is 1-u of -z
-  double pa = 0.5 * erfc(-a * r2), pb = 0.5 * erfc(-b * r2);
-  double p = pa + u * (pb - pa), z;
-  if (p <= 0.5) {
-    z = ndtri(p);
-  } else {
-    double qa = 0.5 * erfc(a * r2), qb = 0.5 * erfc(b * r2);
-    double qq = qb + (1.0 - u) * (qa - qb);
-    z = -ndtri(qq);
-  }
+  double ta = (a <= -10.0 && a * a > b * b + 100.0) ? 0.0 : 0.5 * erfc(fabs(a) * r2);
+  double tb = (b >= 10.0 && b * b > a * a + 100.0) ? 0.0 : 0.5 * erfc(fabs(b) * r2);
+  double pa = a < 0.0 ? ta : 1.0 - ta, qa = a < 0.0 ? 1.0 - ta : ta;
+  double pb = b < 0.0 ? tb : 1.0 - tb, qb = b < 0.0 ? 1.0 - tb : tb;
+  double m = a >= 0.0 ? qa - qb : pb - pa;
+  double p = pa + u * m, q = qb + (1.0 - u) * m;
+  double z = p <= q ? ndtri(p) : -ndtri(q);
   return fmin(fmax(z, a), b);
 }
 SMCF_DEV double trunc_normal(double mu, double sigma, double lo, double hi, double u) {
   double a = (lo - mu) / sigma, b = (hi - mu) / sigma;
   double x = mu + sigma * tn_std(a, b, u);
+  return fmin(fmax(x, lo), hi);
+}
+// the Gibbs mover's parameterisation: s = sqrt(precision)
+SMCF_DEV double trunc_normal_prec(double mu, double s, double lo, double hi, double u) {
+  double a = (lo - mu) * s, b = (hi - mu) * s;
+  double x = mu + tn_std(a, b, u) / s;
   return fmin(fmax(x, lo), hi);
 }
 

@@ -10,6 +10,9 @@
 
 namespace smcf {
 
+// lanes per slip component in the pipelined prior chain (one 64-lane wave per instance)
+constexpr int kChainLanes = 32;
+
 void set_error(const std::string& msg);
 
 #define SMCF_HIP(call)                                                                        \
@@ -26,6 +29,8 @@ struct MeshDev {
   int nplane, nnode, ndof, npatch, ndim, ndim_fault;
   int npts, nobs, nsar, ngnss;
   int fix_xbend;
+  int chain_lag;            // pipelined prior chain: rounds between consecutive particles (> LL band per component)
+  int chain_w;              // per-lane LDS ring length (power of two >= chain_lag + 1)
   double xmin, xmax, zmin;
   const double* xbend;      // nplane-1 (>=1 allocated)
   const int* plane_nxi;     // nplane
@@ -89,6 +94,7 @@ struct BatchDev {
   int* active;              // cap: runs the mover this stage
   int* done;                // cap
   int* par;                 // cap: which particle buffer (S or S2) is current
+  int* sel;                 // cap: slots being admitted (indices for the set-up kernels)
   double* gtrace;           // 202 x cap
 };
 
@@ -118,12 +124,15 @@ struct StageCfg {
 };
 
 // kernels (launchers return hipError_t of the launch)
-hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s);
-hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s);
+// sel (optional): the B instance slots to work on, instead of slots 0 .. B-1
+hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s,
+                             const int* sel = nullptr);
+hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s,
+                             const int* sel = nullptr);
 hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
@@ -131,7 +140,7 @@ hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
 hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 
-hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s);
+hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, hipStream_t s,
                             int* host_amax);
 

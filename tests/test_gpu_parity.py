@@ -348,3 +348,21 @@ def test_fault_smc_exec_gsmc_vs_oracle(ctx):
     o = po.fault_smc(rng, 16, 5, max_stages=3, gsmc=g)
     _check_outer(d, o)
     pb.close()
+
+
+def test_streamed_slots_match_one_wide_batch(ctx):
+    """Geometries streamed through 3 slots (a finished instance's slot takes the next geometry)
+    give bit-identical -log Z and stage counts to one batch holding all 8 at once."""
+    say("case: streamed slots vs wide batch")
+    obs = small_obs(seed=8)
+    th = syn.sample_thetas(8, 31)
+    keys = [O.inner_key(5, 0, 0, i) for i in range(8)]
+    narrow = device_problem(ctx, obs, 128, max_batch=3)
+    wide = device_problem(ctx, obs, 128, max_batch=8)
+    a, na = narrow.fault_calc_likelihood(th, keys)
+    b, nb = wide.fault_calc_likelihood(th, keys)
+    say("stages", na.tolist(), "neglog", a.tolist())
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(na, nb)
+    narrow.close()
+    wide.close()

@@ -168,3 +168,32 @@ def test_dc3d0_surface_identity_and_linearity():
     f1, _ = O.dc3d0(2 / 3, 100.0, 0.0, 0.0, 5.0, 45.0, 0.0, 1.0, 0.0, 0.0)
     f2, _ = O.dc3d0(2 / 3, 200.0, 0.0, 0.0, 10.0, 45.0, 0.0, 1.0, 0.0, 0.0)
     assert np.abs(f2[:3]).max() == pytest.approx(np.abs(f1[:3]).max() / 4, rel=1e-12)
+
+
+def _tn_fraction(a, b, z):
+    """(Phi(z) - Phi(a)) / (Phi(b) - Phi(a)) evaluated stably in every regime."""
+    if a >= 0:  # upper tail, in log Q
+        la = sp.log_ndtr(-a)
+        return -math.expm1(sp.log_ndtr(-z) - la) / -math.expm1(sp.log_ndtr(-b) - la)
+    if b <= 0:  # lower tail, in log Phi
+        lb = sp.log_ndtr(b)
+        ea, ez = math.exp(sp.log_ndtr(a) - lb), math.exp(sp.log_ndtr(z) - lb)
+        return (ez - ea) / (1.0 - ea)
+    return (sp.ndtr(z) - sp.ndtr(a)) / (sp.ndtr(b) - sp.ndtr(a))
+
+
+@pytest.mark.parametrize("a,b", [(-12.0, 0.5), (-10.5, 3.0), (-9.5, 9.5), (0.5, 12.0), (9.0, 11.0), (12.0, 12.5),
+                                 (-12.5, -12.0), (-11.0, -9.0), (-3.0, 40.0), (20.0, 60.0), (-60.0, -36.0)])
+def test_trunc_normal_prec_regimes(a, b):
+    """The Gibbs draw (one erfc per bound, skipped below 1e-20 of the mass) inverts the CDF
+    in every regime, including the skip thresholds and the Mills-ratio tails."""
+    for u in (1e-15, 1e-6, 0.01, 0.3, 0.5, 0.7, 0.99, 1 - 1e-6, 1 - 1e-15):
+        s = 2.0
+        mu = 1.0
+        lo, hi = mu + a / s, mu + b / s
+        x = O.trunc_normal_prec(mu, s, lo, hi, u)
+        assert lo <= x <= hi
+        z = (x - mu) * s
+        assert _tn_fraction(a, b, z) == pytest.approx(u, rel=1e-6, abs=1e-12), (a, b, u, z)
+    xs = [O.trunc_normal_prec(0.0, 1.0, a, b, u) for u in (0.1, 0.5, 0.9)]
+    assert xs[0] <= xs[1] <= xs[2]
