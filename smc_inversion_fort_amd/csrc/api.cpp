@@ -41,17 +41,18 @@ namespace {
 struct ProfScope {
   smcf_ctx* ctx;
   const char* name;
+  hipStream_t st;
   hipEvent_t a{}, b{};
-  ProfScope(smcf_ctx* c, const char* n) : ctx(c), name(n) {
+  ProfScope(smcf_ctx* c, const char* n, hipStream_t s = nullptr) : ctx(c), name(n), st(s ? s : c->stream) {
     if (ctx->profiling) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, ctx->stream);
+      (void)hipEventRecord(a, st);
     }
   }
   ~ProfScope() {
     if (ctx->profiling) {
-      (void)hipEventRecord(b, ctx->stream);
+      (void)hipEventRecord(b, st);
       ctx->pending.push_back({name, {a, b}});
     }
   }
@@ -96,7 +97,8 @@ int upload(Problem& P, const T** dst, const std::vector<T>& v) {
   return SMCF_OK;
 }
 
-__global__ void k_state_init(BatchDev bt, int B, const int* __restrict__ sel) {
+// hold = 1 keeps the slot marked done (skipped by the stage kernels) until k_activate releases it
+__global__ void k_state_init(BatchDev bt, int B, const int* __restrict__ sel, int hold) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   if (sel) b = sel[b];
@@ -104,31 +106,36 @@ __global__ void k_state_init(BatchDev bt, int B, const int* __restrict__ sel) {
   bt.neglog[b] = 0.0;
   bt.iter[b] = 1;
   bt.active[b] = 0;
-  bt.done[b] = 0;
+  bt.done[b] = hold;
   bt.par[b] = 0;
 }
 
-// Set up n instances: slots 0 .. n-1 (sel == nullptr) or the slots listed in bt.sel.  Thetas and keys
-// must already be in their slots.
-int setup_slots(Problem& P, int n, const int* sel) {
+__global__ void k_activate(BatchDev bt, int B, const int* __restrict__ sel) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  bt.done[sel[b]] = 0;
+}
+
+// Set up n instances on stream s: slots 0 .. n-1 (sel == nullptr) or the slots listed in sel.
+// Thetas and keys must already be in their slots.  hold = 1 leaves them marked done.
+int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
   smcf_ctx* ctx = P.ctx;
-  hipStream_t s = ctx->stream;
   MeshDev& m = P.m;
   BatchDev& bt = P.bt;
-  hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel);
+  hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel, hold);
   if (P.mover == SMCF_MOVER_HMC) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, n, s, sel));
   {
-    ProfScope ps(ctx, "geometry_greens");
+    ProfScope ps(ctx, "geometry_greens", s);
     CHECK_LAUNCH(launch_geometry(m, bt, n, s, sel));
     CHECK_LAUNCH(launch_greens(m, bt, n, s, sel));
   }
   CHECK_LAUNCH(launch_laplacian(m, bt, n, s, sel));
   {
-    ProfScope ps(ctx, "gram");
+    ProfScope ps(ctx, "gram", s);
     CHECK_LAUNCH(launch_gram(m, bt, n, s, sel));
   }
   {
-    ProfScope ps(ctx, "init_chain");
+    ProfScope ps(ctx, "init_chain", s);
     CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
   }
   CHECK_LAUNCH(launch_likelihood(m, bt, n, 0, s, sel));
@@ -136,11 +143,12 @@ int setup_slots(Problem& P, int n, const int* sel) {
 }
 
 // Inner SMC for N geometries streamed through the problem's slots.  The first min(N, cap)
-// geometries start together; after every stage the slots whose instance has finished are
-// harvested and refilled with the next geometries, so the device stays full while instances with
-// few stages leave early.  Every instance keeps its own stage counter and RNG key, so a result
-// does not depend on which slot or batch it ran in.  With N <= cap the slot b holds geometry b
-// at the end (its particles stay readable).
+// geometries start together.  After every stage the slots whose instance has finished are
+// harvested; once enough are free, the next geometries are set up in them on the set-up stream
+// (overlapping the stage loop, with the slots held as done so that the stage kernels skip them)
+// and released into the stage loop at the first stage boundary after their set-up completed.
+// Every instance keeps its own stage counter and RNG key, so a result does not depend on which
+// slot, batch or moment it ran in.  With N <= cap, slot b holds geometry b at the end.
 int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
                int* nstage) {
   smcf_ctx* ctx = P.ctx;
@@ -160,77 +168,114 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
   const int W = std::min(N, bt.cap);  // slots in use
   SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * nf * W, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * W, hipMemcpyHostToDevice, s));
-  int rc = setup_slots(P, W, nullptr);
+  int rc = setup_slots(P, W, nullptr, s, 0);
   if (rc) return rc;
   std::vector<int> job(W), done(W), iter(W);
   std::vector<double> nl(W);
   for (int b = 0; b < W; ++b) job[b] = b;
   int next = W, busy = W;
-  // refill once this many slots are free (or when nothing else is running)
-  const int admit_min = std::max(1, W / 16);
+  const int admit_min = std::max(1, W / 16);  // refill once this many slots are free
   std::vector<int> freed;
-  std::vector<double> th_stage((size_t)nf * W);
-  std::vector<uint64_t> key_stage(W);
+  struct Admission {
+    bool live = false;
+    int half = 0;
+    std::vector<int> slots, jobs;
+    std::vector<double> th;
+    std::vector<uint64_t> key;
+    hipEvent_t ready{};
+  } adm;
+  SMCF_HIP(hipEventCreateWithFlags(&adm.ready, hipEventDisableTiming));
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(ctx->setup_stream);
+    (void)hipEventDestroy(adm.ready);
+    return code;
+  };
   StageCfg cfg{P.max_slip, bt.P, P.sweeps};
   std::vector<int> amax(2 * (size_t)W);
-  while (busy > 0) {
-    {
-      ProfScope ps(ctx, "gamma_resample");
-      CHECK_LAUNCH(launch_gamma_resample(m, bt, W, s));
-    }
-    if (want_moments || hmc) CHECK_LAUNCH(launch_moments(m, bt, W, s));
-    if (hmc) {
-      ProfScope ps(ctx, "hmc");
-      CHECK_LAUNCH(launch_hmc_stage(m, bt, P.hm, W, P.max_slip, s, amax.data()));
-    } else {
-      CHECK_LAUNCH(launch_gather(m, bt, W, s));
-      CHECK_LAUNCH(launch_form_q(m, bt, W, s));
+  while (busy > 0 || adm.live) {
+    if (busy > 0) {
       {
-        ProfScope ps(ctx, "gibbs");
-        CHECK_LAUNCH(launch_gibbs(m, bt, W, cfg, s));
+        ProfScope ps(ctx, "gamma_resample");
+        if (launch_gamma_resample(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
       }
-      {
-        ProfScope ps(ctx, "likelihood");
-        CHECK_LAUNCH(launch_likelihood(m, bt, W, 1, s));
+      if ((want_moments || hmc) && launch_moments(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
+      if (hmc) {
+        ProfScope ps(ctx, "hmc");
+        if (launch_hmc_stage(m, bt, P.hm, W, P.max_slip, s, amax.data()) != hipSuccess) return fail(SMCF_EHIP);
+      } else {
+        if (launch_gather(m, bt, W, s) != hipSuccess || launch_form_q(m, bt, W, s) != hipSuccess)
+          return fail(SMCF_EHIP);
+        {
+          ProfScope ps(ctx, "gibbs");
+          if (launch_gibbs(m, bt, W, cfg, s) != hipSuccess) return fail(SMCF_EHIP);
+        }
+        {
+          ProfScope ps(ctx, "likelihood");
+          if (launch_likelihood(m, bt, W, 1, s) != hipSuccess) return fail(SMCF_EHIP);
+        }
+      }
+      if (hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(iter.data(), bt.iter, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipMemcpyAsync(nl.data(), bt.neglog, sizeof(double) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return fail(SMCF_EHIP);
+      for (int b = 0; b < W; ++b)
+        if (job[b] >= 0 && done[b]) {
+          if (neglog) neglog[job[b]] = nl[b];
+          if (nstage) nstage[job[b]] = std::min(iter[b] - 1, 201);
+          job[b] = -1;
+          --busy;
+          freed.push_back(b);
+        }
+    }
+    // release a completed admission at this stage boundary
+    if (adm.live) {
+      hipError_t q = busy > 0 ? hipEventQuery(adm.ready) : hipEventSynchronize(adm.ready);
+      if (q == hipSuccess) {
+        int n = (int)adm.slots.size();
+        hipLaunchKernelGGL(k_activate, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, bt.sel + adm.half * bt.cap);
+        if (hipGetLastError() != hipSuccess) return fail(SMCF_EHIP);
+        for (int q2 = 0; q2 < n; ++q2) job[adm.slots[q2]] = adm.jobs[q2];
+        busy += n;
+        adm.live = false;
+      } else if (q != hipErrorNotReady) {
+        return fail(SMCF_EHIP);
       }
     }
-    SMCF_HIP(hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s));
-    SMCF_HIP(hipMemcpyAsync(iter.data(), bt.iter, sizeof(int) * W, hipMemcpyDeviceToHost, s));
-    SMCF_HIP(hipMemcpyAsync(nl.data(), bt.neglog, sizeof(double) * W, hipMemcpyDeviceToHost, s));
-    SMCF_HIP(hipStreamSynchronize(s));
-    for (int b = 0; b < W; ++b)
-      if (job[b] >= 0 && done[b]) {
-        if (neglog) neglog[job[b]] = nl[b];
-        if (nstage) nstage[job[b]] = std::min(iter[b] - 1, 201);
-        job[b] = -1;
-        --busy;
-        freed.push_back(b);
-      }
-    if (next < N && !freed.empty() && ((int)freed.size() >= admit_min || busy == 0)) {
+    // start the next admission
+    if (!adm.live && next < N && !freed.empty() && ((int)freed.size() >= admit_min || busy == 0)) {
       int n = std::min((int)freed.size(), N - next);
-      std::vector<int> slots(freed.begin(), freed.begin() + n);
+      adm.half = 1 - adm.half;
+      adm.slots.assign(freed.begin(), freed.begin() + n);
       freed.erase(freed.begin(), freed.begin() + n);
+      adm.jobs.resize(n);
+      adm.th.resize((size_t)nf * n);
+      adm.key.resize(n);
       for (int q = 0; q < n; ++q) {
-        int b = slots[q], g = next + q;
-        std::copy(theta + (size_t)g * nf, theta + (size_t)(g + 1) * nf, th_stage.begin() + (size_t)q * nf);
-        key_stage[q] = keys[g];
-        job[b] = g;
+        adm.jobs[q] = next + q;
+        std::copy(theta + (size_t)(next + q) * nf, theta + (size_t)(next + q + 1) * nf,
+                  adm.th.begin() + (size_t)q * nf);
+        adm.key[q] = keys[next + q];
       }
-      // scatter the new thetas and keys into their slots
+      hipStream_t s2 = ctx->setup_stream;
       for (int q = 0; q < n; ++q) {
-        SMCF_HIP(hipMemcpyAsync(bt.theta + (size_t)slots[q] * nf, th_stage.data() + (size_t)q * nf,
-                                sizeof(double) * nf, hipMemcpyHostToDevice, s));
-        SMCF_HIP(hipMemcpyAsync(bt.key + slots[q], &key_stage[q], sizeof(uint64_t), hipMemcpyHostToDevice, s));
+        if (hipMemcpyAsync(bt.theta + (size_t)adm.slots[q] * nf, adm.th.data() + (size_t)q * nf, sizeof(double) * nf,
+                           hipMemcpyHostToDevice, s2) != hipSuccess ||
+            hipMemcpyAsync(bt.key + adm.slots[q], &adm.key[q], sizeof(uint64_t), hipMemcpyHostToDevice, s2) !=
+                hipSuccess)
+          return fail(SMCF_EHIP);
       }
-      SMCF_HIP(hipMemcpyAsync(bt.sel, slots.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-      rc = setup_slots(P, n, bt.sel);
-      if (rc) return rc;
-      // the host staging buffers must outlive the asynchronous copies
-      SMCF_HIP(hipStreamSynchronize(s));
+      int* sel = bt.sel + adm.half * bt.cap;
+      if (hipMemcpyAsync(sel, adm.slots.data(), sizeof(int) * n, hipMemcpyHostToDevice, s2) != hipSuccess)
+        return fail(SMCF_EHIP);
+      rc = setup_slots(P, n, sel, s2, 1);
+      if (rc) return fail(rc);
+      if (hipEventRecord(adm.ready, s2) != hipSuccess) return fail(SMCF_EHIP);
       next += n;
-      busy += n;
+      adm.live = true;
     }
   }
+  (void)hipEventDestroy(adm.ready);
   SMCF_HIP(hipStreamSynchronize(s));
   resolve_profiles(ctx);
   return SMCF_OK;
@@ -259,6 +304,10 @@ int smcf_ctx_create(int device, smcf_ctx** out) {
   c->device = device;
   c->profiling = false;
   hipError_t es = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (es == hipSuccess) {
+    es = hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking);
+    if (es != hipSuccess) (void)hipStreamDestroy(c->stream);
+  }
   if (es != hipSuccess) {
     delete c;
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(es));
@@ -272,7 +321,9 @@ int smcf_ctx_destroy(smcf_ctx* ctx) {
   if (!ctx) return SMCF_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->setup_stream);
   resolve_profiles(ctx);
+  (void)hipStreamDestroy(ctx->setup_stream);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SMCF_OK;
@@ -457,7 +508,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.iter, C);
   rc |= dalloc(P, &bt.active, C);
   rc |= dalloc(P, &bt.done, C);
-  rc |= dalloc(P, &bt.sel, C);
+  rc |= dalloc(P, &bt.sel, 2 * C);
   rc |= dalloc(P, &bt.par, C);
   rc |= dalloc(P, &bt.gtrace, 202 * C);
   if (P.mover == SMCF_MOVER_HMC) {

@@ -448,15 +448,20 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
           } else {
             u = u01_open(wodd);
           }
-          // Tw[mm][lane] holds delta of coordinate i0 + 16m + mm once it is redrawn
+          // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
+          const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
           double qi = Tw[kk][lane];
-          for (int mm = 0; mm < kk; ++mm) qi = fma(Qr[(size_t)(i0 + 16 * m + mm) * KB + k], Tw[mm][lane], qi);
-          double prec = Qr[(size_t)i * KB + k];
+          double prec = qrow[kk];
           double so = S[(size_t)i * P + pc];
           double mu = so - (qi - gamma * bv[i]) / prec;
           double sn = gibbs_draw(mu, prec, max_slip, u);
-          Tw[kk][lane] = sn - so;
+          double dlt = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
+          // right-looking correction of the later coordinates of the tile, Tw[kk'] += Q(i, i0+16m+kk') delta;
+          // the other entries get a zero coefficient, and fma(0, x, t) = t leaves them unchanged
+#pragma unroll
+          for (int k2 = 0; k2 < 16; ++k2) Tw[k2][lane] = fma(k2 > kk ? qrow[k2] : 0.0, dlt, Tw[k2][lane]);
+          Tw[kk][lane] = dlt;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

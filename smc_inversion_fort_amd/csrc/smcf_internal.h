@@ -94,7 +94,7 @@ struct BatchDev {
   int* active;              // cap: runs the mover this stage
   int* done;                // cap
   int* par;                 // cap: which particle buffer (S or S2) is current
-  int* sel;                 // cap: slots being admitted (indices for the set-up kernels)
+  int* sel;                 // 2 x cap: slots being admitted (two halves, used alternately)
   double* gtrace;           // 202 x cap
 };
 
@@ -158,6 +158,7 @@ hipError_t launch_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32
 struct smcf_ctx {
   int device;
   hipStream_t stream;
+  hipStream_t setup_stream;  // set-up of streamed-in instances, overlapping the stage loop
   bool profiling;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   struct Stat {
