@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--cpu-evals", type=int, default=1, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
     return ap.parse_args()
 
 
@@ -57,6 +60,16 @@ def main():
         sys.exit(2)
     import smc_inversion_fort_amd as S
     from smc_inversion_fort_amd import synthetic as syn
+
+    if a.copy_probe:
+        import torch
+        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        x.fill_(1.0)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+        print("copy probe: filled and copied 1 GiB", flush=True)
 
     cfg = syn.CONFIGS["c2"]
     xmin, xmax, zmin, _ = syn.fault_frame()

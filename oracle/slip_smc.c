@@ -347,10 +347,10 @@ double orc_trunc_normal(double mu, double sigma, double lo, double hi, double u)
 }
 
 /* the same draw parameterised by the square root of the precision, as the Gibbs mover uses it:
- * a = (lo - mu) s, b = (hi - mu) s, x = mu + z / s */
+ * a = (lo - mu) s, b = (hi - mu) s, x = mu + z (1 / s) */
 double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u) {
     double a = (lo - mu) * s, b = (hi - mu) * s;
-    double x = mu + tn_std(a, b, u) / s;
+    double x = mu + tn_std(a, b, u) * (1.0 / s);
     return fmin(fmax(x, lo), hi);
 }
 
@@ -538,7 +538,7 @@ static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma,
  * linear term gamma*b.  Each child j starts from its parent and runs `sweeps` full
  * systematic-scan sweeps (i = 1..ndim); coordinate i is redrawn from N(mu_i, 1/Q_ii)
  * truncated to [0, max_slip] with
- *   mu_i = s_i - (sum_k Q_ki s_k - gamma*b_i) / Q_ii       (column i of Q)
+ *   mu_i = s_i - (sum_k Q_ki s_k - gamma*b_i) * (1 / Q_ii)       (column i of Q)
  * using the open uniform u = ((w>>11) + 1/2) 2^-53 of word (i & 1) of draw
  * (site 9, stage, c2=j, c1=i>>1, c0=sweep): coordinates 2m and 2m+1 share one Philox block.
  * The truncated normal is parameterised by s = sqrt(Q_ii): a = -mu s, b = (max_slip - mu) s.
@@ -553,6 +553,13 @@ static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, i
     double* hb = (double*)malloc(sizeof(double) * nd);
     for (size_t k = 0; k < (size_t)nd * nd; ++k) Q[k] = gamma * I->H[k] + I->ll[k] / a2;
     for (int i = 0; i < nd; ++i) hb[i] = gamma * I->b[i];
+    /* per-coordinate constants of the stage: 1 / Q_ii and sqrt(Q_ii) */
+    double* ip = (double*)malloc(sizeof(double) * nd);
+    double* sq = (double*)malloc(sizeof(double) * nd);
+    for (int i = 0; i < nd; ++i) {
+        ip[i] = 1.0 / Q[(size_t)i * nd + i];
+        sq[i] = sqrt(Q[(size_t)i * nd + i]);
+    }
     #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < P; ++i) {
         double* lsv = (double*)malloc(sizeof(double) * 2 * I->nnode);
@@ -569,18 +576,17 @@ static void gibbs_move(const inst_t* I, uint64_t key, int stage, double gamma, i
                         for (int r = 0; r < 8; ++r) ps[r] += qc[k + r] * s[k + r];
                     for (; k < nd; ++k) ps[k & 7] += qc[k] * s[k];
                     double qs = ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
-                    double prec = Q[(size_t)c * nd + c];
-                    double mu = s[c] - (qs - hb[c]) / prec;
+                    double mu = s[c] - (qs - hb[c]) * ip[c];
                     uint64_t w[2];
                     orc_draw_u64x2(key, SITE_GIBBS_MOVE, stage, j, c >> 1, sw, w);
-                    s[c] = orc_trunc_normal_prec(mu, sqrt(prec), 0.0, smax, open_uniform(w[c & 1]));
+                    s[c] = orc_trunc_normal_prec(mu, sq[c], 0.0, smax, open_uniform(w[c & 1]));
                 }
             Ln[j] = inst_likelihood(I, s);
             PRn[j] = inst_prior(I, s, lsv);
         }
         free(lsv);
     }
-    free(Q); free(hb);
+    free(Q); free(hb); free(ip); free(sq);
 }
 
 /* ---------------- instance setup shared by slip_smc_exec and fault_calc_likelihood ------------- */

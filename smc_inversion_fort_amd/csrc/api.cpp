@@ -138,7 +138,8 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
     ProfScope ps(ctx, "init_chain", s);
     CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
   }
-  CHECK_LAUNCH(launch_likelihood(m, bt, n, 0, s, sel));
+  CHECK_LAUNCH(launch_like(m, bt, n, 0, s, sel));
+  CHECK_LAUNCH(launch_prior(m, bt, n, s, sel));
   return SMCF_OK;
 }
 
@@ -211,7 +212,8 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
         }
         {
           ProfScope ps(ctx, "likelihood");
-          if (launch_likelihood(m, bt, W, 1, s) != hipSuccess) return fail(SMCF_EHIP);
+          // the Gibbs mover reads no prior values; they are formed once for the final particles
+          if (launch_like(m, bt, W, 1, s) != hipSuccess) return fail(SMCF_EHIP);
         }
       }
       if (hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -490,6 +492,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.bvec, nd * C);
   rc |= dalloc(P, &bt.dsd, C);
   rc |= dalloc(P, &bt.Q, (nd + 32) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  rc |= dalloc(P, &bt.qdiag, 4 * nd * C);
   rc |= dalloc(P, &bt.S, nd * Pn * C);
   rc |= dalloc(P, &bt.S2, nd * Pn * C);
   rc |= dalloc(P, &bt.Y, nd * Pn * C);
@@ -667,6 +670,10 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
   hipStream_t s = P.ctx->stream;
   MeshDev& m = P.m;
   BatchDev& bt = P.bt;
+  if (prior && P.mover == SMCF_MOVER_GIBBS) {  // the Gibbs stage loop leaves the prior of the final particles to here
+    CHECK_LAUNCH(launch_prior(m, bt, 1, s));
+    SMCF_HIP(hipStreamSynchronize(s));
+  }
   size_t nd = m.ndim, Pn = bt.P;
   int par = 0, it = 0;
   SMCF_HIP(hipMemcpy(&par, bt.par, sizeof(int), hipMemcpyDeviceToHost));

@@ -4,8 +4,7 @@
 //              smc_fault.f90:148-173, smc_slip.f90:84-87)
 //  k_gram      H = G^T diag(1/(sigma_i^2 sigma_gnss^2)) G          (smc_slip.f90:1401-1409)
 //  k_gram_vec  b = G^T sd, dsd = d^T sd                              (smc_slip.f90:1371-1388)
-//  k_hs        Y = H S over all particles (S dimension-major)        (smc_slip.f90:72-87)
-//  k_like      Gram-form likelihood and ELL prior per particle      (smc_slip.f90:56-137)
+//  k_prior     ELL prior per particle                                (smc_slip.f90:106-137)
 //  k_form_q    Q = gamma H + LL / alpha^2 for the Gibbs mover
 // The two GEMMs use 64x64 LDS tiles with a 4x4 register block per thread.
 #include <algorithm>
@@ -133,82 +132,19 @@ hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, con
   return hipGetLastError();
 }
 
-// Y(i, p) = sum_j H(i, j) S(j, p); S and Y dimension-major (row i = dimension, P contiguous)
-__global__ void __launch_bounds__(256) k_hs(int nd, int P, const double* __restrict__ Hall,
-                                            const double* __restrict__ S0, const double* __restrict__ S1,
-                                            const int* __restrict__ par, const int* __restrict__ active,
-                                            double* __restrict__ Yall, int use_active, const int* __restrict__ sel) {
-  __shared__ double As[TK][TM + 1];
-  __shared__ double Bs[TK][TN + 1];
-  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
-  if (use_active && !active[b]) return;
-  int i0 = blockIdx.y * TM, p0 = blockIdx.x * TN;
-  const double* H = Hall + (size_t)b * nd * nd;
-  const double* S = (par[b] ? S1 : S0) + (size_t)b * nd * P;
-  double* Y = Yall + (size_t)b * nd * P;
-  int t = threadIdx.x, ty = t / 16, tx = t % 16;
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int k0 = 0; k0 < nd; k0 += TK) {
-#pragma unroll
-    for (int rep = 0; rep < 4; ++rep) {
-      int ii = t % 64, kk = t / 64 + rep * 4;
-      int i = i0 + ii, k = k0 + kk, p = p0 + ii;
-      As[kk][ii] = (i < nd && k < nd) ? H[(size_t)k * nd + i] : 0.0;
-      Bs[kk][ii] = (p < P && k < nd) ? S[(size_t)k * P + p] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; ++kk) {
-      double a4[4], b4[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b4[c] = Bs[kk][tx * 4 + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] += a4[a] * b4[c];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      int i = i0 + ty * 4 + a, p = p0 + tx * 4 + c;
-      if (i < nd && p < P) Y[(size_t)i * P + p] = acc[a][c];
-    }
-}
-
-// likelihood (Gram form) and prior of every particle of the current buffer
-__global__ void k_like(MeshDev m, int P, const double* __restrict__ S0, const double* __restrict__ S1,
-                       const int* __restrict__ par, const int* __restrict__ active, const double* __restrict__ Yall,
-                       const double* __restrict__ bvec, const double* __restrict__ dsd,
-                       const double* __restrict__ scal, const int* __restrict__ liall,
-                       const double* __restrict__ lvall, double* __restrict__ L0, double* __restrict__ L1,
-                       double* __restrict__ R0, double* __restrict__ R1, int use_active,
-                       const int* __restrict__ sel) {
+// prior of the current particles (slip_calc_prior, smc_slip.f90:106-137): sum over the rows of L s
+// (ELL, 9 entries per row) of (L s)^2 / (2 alpha^2)
+__global__ void k_prior(MeshDev m, int P, const double* __restrict__ S0, const double* __restrict__ S1,
+                        const int* __restrict__ par, const double* __restrict__ scal, const int* __restrict__ liall,
+                        const double* __restrict__ lvall, double* __restrict__ R0, double* __restrict__ R1,
+                        const int* __restrict__ sel) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
-  if (use_active && !active[b]) return;
   if (p >= P) return;
   int nd = m.ndim;
   int pb = par[b];
   const double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
-  const double* Y = Yall + (size_t)b * nd * P;
-  const double* bv = bvec + (size_t)b * nd;
-  double v = 0.0, bs = 0.0;
-  for (int i = 0; i < nd; ++i) {
-    double s = S[(size_t)i * P + p];
-    v = v + s * Y[(size_t)i * P + p];
-    bs = bs + bv[i] * s;
-  }
   const double* sc = scal + 8 * (size_t)b;
-  double like = (v - 2.0 * bs + dsd[b]) / 2.0 + sc[3] + sc[4];
   const int* li = liall + (size_t)b * 2 * m.nnode * 9;
   const double* lv = lvall + (size_t)b * 2 * m.nnode * 9;
   double two_a2 = 2.0 * sc[0];
@@ -219,17 +155,13 @@ __global__ void k_like(MeshDev m, int P, const double* __restrict__ S0, const do
     for (int q = 0; q < 9; ++q) t = t + lv[9 * r + q] * S[(size_t)(li[9 * r + q] - 1) * P + p];
     pr = pr + (t * t) / two_a2;
   }
-  (pb ? L1 : L0)[(size_t)b * P + p] = like;
   (pb ? R1 : R0)[(size_t)b * P + p] = pr;
 }
 
-hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel) {
+hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
   int P = bt.P;
-  dim3 g((P + TN - 1) / TN, (m.ndim + TM - 1) / TM, B);
-  hipLaunchKernelGGL(k_hs, g, dim3(256), 0, s, m.ndim, P, bt.H, bt.S, bt.S2, bt.par, bt.active, bt.Y, use_active,
-                     sel);
-  hipLaunchKernelGGL(k_like, dim3((P + 127) / 128, B), dim3(128), 0, s, m, P, bt.S, bt.S2, bt.par, bt.active, bt.Y,
-                     bt.bvec, bt.dsd, bt.scal, bt.li, bt.lv, bt.like, bt.like2, bt.prior, bt.prior2, use_active, sel);
+  hipLaunchKernelGGL(k_prior, dim3((P + 127) / 128, B), dim3(128), 0, s, m, P, bt.S, bt.S2, bt.par, bt.scal, bt.li,
+                     bt.lv, bt.prior, bt.prior2, sel);
   return hipGetLastError();
 }
 
@@ -276,6 +208,24 @@ __global__ void k_form_qb_sparse(int nd, int KB, const double* __restrict__ Hall
   }
 }
 
+// per-coordinate constants of the stage for the Gibbs draws: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+__global__ void k_form_qdiag(int nd, const double* __restrict__ Qball, const double* __restrict__ gamma,
+                             const double* __restrict__ bvec, const int* __restrict__ active,
+                             double* __restrict__ cdall) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!active[b] || i >= nd) return;
+  int nblk = (nd + 31) / 32;
+  const double* Qb = Qball + (size_t)b * nblk * nd * 32;
+  double qii = Qb[((size_t)(i / 32) * nd + i) * 32 + i % 32];
+  double* cd = cdall + (size_t)b * 4 * nd;
+  double sq = sqrt(qii);
+  cd[i] = gamma[b] * bvec[(size_t)b * nd + i];
+  cd[nd + i] = 1.0 / qii;
+  cd[2 * nd + i] = sq;
+  cd[3 * nd + i] = 1.0 / sq;
+}
+
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
   int nd = m.ndim, KB = gibbs_kb(nd);
   size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
@@ -283,6 +233,8 @@ hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
   hipLaunchKernelGGL(k_form_qb, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.H, bt.gamma, bt.active, bt.Q);
   hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.H, bt.gamma,
                      bt.active, bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
+  hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, bt.Q, bt.gamma, bt.bvec,
+                     bt.active, bt.qdiag);
   return hipGetLastError();
 }
 

@@ -313,15 +313,190 @@ hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Gibbs sweeps.  One thread per particle, 256 particles per workgroup.  Coordinates are
-// processed in blocks of KB: the workgroup stages Q(:, i0:i0+KB) in LDS, every thread forms
-// q_k = sum_j Q(j, i0+k) s_j for its particle, then samples the KB coordinates in order,
-// adding the in-block corrections Q(i0+m, i0+k) * delta_m of the coordinates already redrawn.
+// Gibbs sweeps (truncated-MVN Gibbs on the tempered posterior: precision Q = gamma H + LL / alpha^2,
+// conditional mean of coordinate i from q_i = (Q s)_i, box [0, max_slip]).
 // ---------------------------------------------------------------------------
 // one coordinate redraw: Philox uniform + truncated-normal inversion (kept out of line so the
 // unrolled block loop does not replicate it)
-__device__ __noinline__ double gibbs_draw(double mu, double prec, double max_slip, double u) {
-  return trunc_normal_prec(mu, sqrt(prec), 0.0, max_slip, u);
+__device__ __noinline__ double gibbs_draw(double mu, double s, double is, double max_slip, double u) {
+  return trunc_normal_prec(mu, s, is, 0.0, max_slip, u);
+}
+
+// The same draw (bit for bit) for everything except the rare regions -- a bound beyond 35 sigma
+// (Mills-ratio solve) and p < 1.4e-11 (AS 241's far tail) -- where it returns NaN and the caller
+// takes gibbs_draw.  Without those branches the function needs under 32 VGPRs, so a call clobbers
+// only caller-saved registers and saves nothing to scratch.
+__device__ __noinline__ double gibbs_draw_fast(double mu, double s, double is, double max_slip, double u) {
+  double a = (0.0 - mu) * s, b = (max_slip - mu) * s;
+  if (a > 35.0 || b < -35.0) return __builtin_nan("");
+  const double r2 = 0.70710678118654752440;
+  double ta = (a <= -10.0 && a * a > b * b + 100.0) ? 0.0 : 0.5 * erfc(fabs(a) * r2);
+  double tb = (b >= 10.0 && b * b > a * a + 100.0) ? 0.0 : 0.5 * erfc(fabs(b) * r2);
+  double pa = a < 0.0 ? ta : 1.0 - ta, qa = a < 0.0 ? 1.0 - ta : ta;
+  double pb = b < 0.0 ? tb : 1.0 - tb, qb = b < 0.0 ? 1.0 - tb : tb;
+  double m = a >= 0.0 ? qa - qb : pb - pa;
+  double p = pa + u * m, q = qb + (1.0 - u) * m;
+  double pq = fmin(p, q);  // <= 1/2: AS 241's central region or its lower tail
+  double q0 = pq - 0.5;
+  double z = ndtri_central(q0);
+  if (q0 < -0.425) {
+    double r = sqrt(-log(pq));
+    if (r > 5.0) return __builtin_nan("");
+    z = -ndtri_tail(r);
+  }
+  z = p <= q ? z : -z;
+  z = fmin(fmax(z, a), b);
+  double x = mu + z * is;
+  return fmin(fmax(x, 0.0), max_slip);
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+#ifndef SMCF_PANEL_BUFFERS
+#define SMCF_PANEL_BUFFERS 2
+#endif
+constexpr int kPanelBuffers = SMCF_PANEL_BUFFERS;
+
+// One 32-row block of a product Y = A B on the matrix cores (v_mfma_f64_16x16x4f64), 64 columns per
+// wavefront:  acc[m][n] += sum_j A(16m + r, j) B(j, c_n) over j = 0 .. 4 nsteps - 1, where lane
+// (r, k) = (lane % 16, lane / 16) reads A(r, 4st + k) at abase[ao0 + st lda4] (row 16 + r at ao1) and
+// B(4st + k, c_n) at bbase[bo[n] + st ldb4].  Bases are wave-uniform and offsets 32-bit, so the loads
+// take the scalar-base + 32-bit-offset form.  The loop is software-pipelined over NB register buffers:
+// the operands of step st + NB are requested as soon as the products of step st are issued.  The main
+// loop runs whole groups of NB steps and has no branches, so the wait before each group's products
+// covers only that buffer's loads; the last nsteps % NB steps follow it.  (The compiler does not unroll
+// a loop holding MFMA instructions when that needs a remainder loop, so a plain loop would wait for
+// its own loads on every step.)
+template <int NB>
+__device__ __forceinline__ void mfma_panel(const double* __restrict__ abase, uint32_t ao0, uint32_t ao1, uint32_t lda4,
+                                           const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
+                                           int nsteps, dbl4 (&acc)[2][4]) {
+  if (nsteps <= 0) return;
+  double xa[NB], xb[NB], y[NB][4];
+  // loads of step st (clamped to the last step: a group's prefetch past the end re-reads it, unused)
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nsteps - 1);
+    xa[k] = abase[ao0 + q * lda4];
+    xb[k] = abase[ao1 + q * lda4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[k], y[k][n], acc[0][n], 0, 0, 0);
+      acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[k], y[k][n], acc[1][n], 0, 0, 0);
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nsteps - nsteps % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nsteps - full) prod(k);
+}
+
+// Likelihood of slip_calc_likelihood (smc_slip.f90:56-104) in Gram form,
+//   L(s) = (s^T H s - 2 b^T s + dsd) / 2 + c,
+// for the 64 particles p0 .. p0+63 of one wavefront.  Y = H S is formed in 32-row blocks on the matrix
+// cores (A(r, k) = H(i0 + r, j0 + k) read column-major, B(k, c) = s_{j0+k}(p0 + c)); the epilogue of each
+// block accumulates s^T (Y - 2 b) over its rows (lane l holds D(4t + l/16, l%16) in register t), and the
+// four row groups of a particle are summed with lane shuffles.  Y never leaves the registers.
+__device__ __forceinline__ void wave_like(int nd, int P, const double* __restrict__ H, const double* __restrict__ S,
+                                          const double* __restrict__ bv, double dsd, double c, int p0, int lane,
+                                          double* __restrict__ Lout) {
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) pcol[n] = min(p0 + 16 * n + fi, P - 1);
+  uint32_t bo[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
+  int nd4 = nd & ~3, nblk = (nd + 31) / 32;
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  const double* sb = S + (size_t)fk * P;
+  const double* ha = H + (size_t)fk * nd;
+  for (int rb = 0; rb < nblk; ++rb) {
+    int i0 = rb * 32;
+    int r0 = i0 + fi, r1 = i0 + 16 + fi;
+    bool v0 = r0 < nd, v1 = r1 < nd;
+    dbl4 acc[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // rows past nd read row nd - 1; the epilogue skips them
+    mfma_panel<kPanelBuffers>(H, (uint32_t)(fk * nd + min(r0, nd - 1)), (uint32_t)(fk * nd + min(r1, nd - 1)),
+                              (uint32_t)(4 * nd), S, bo, (uint32_t)(4 * P), nd4 / 4, acc);
+    if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
+      bool in = nd4 + fk < nd;
+      double a0 = (in && v0) ? ha[(size_t)nd4 * nd + r0] : 0.0, a1 = (in && v1) ? ha[(size_t)nd4 * nd + r1] : 0.0;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        double bn = in ? sb[(size_t)nd4 * P + pcol[n]] : 0.0;
+        acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bn, acc[0][n], 0, 0, 0);
+        acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bn, acc[1][n], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int row = i0 + 16 * m + 4 * t + fk;
+        if (row < nd) {
+          double b2 = 2.0 * bv[row];
+#pragma unroll
+          for (int n = 0; n < 4; ++n) part[n] = fma(acc[m][n][t] - b2, S[(size_t)row * P + pcol[n]], part[n]);
+        }
+      }
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    double v = part[n];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    int p = p0 + 16 * n + fi;
+    if (fk == 0 && p < P) Lout[p] = (v + dsd) / 2.0 + c;
+  }
+}
+
+// one workgroup of 4 wavefronts per 256 particles of an instance; all chunks of an instance on one
+// XCD (workgroup w runs on XCD w % 8), so that its H is read into one L2
+__global__ void __launch_bounds__(256, 4) k_like_mfma(int nd, int P, int B, const double* __restrict__ Hall,
+                                                      const double* __restrict__ bvec, const double* __restrict__ dsd,
+                                                      const double* __restrict__ scal, const double* __restrict__ S0,
+                                                      const double* __restrict__ S1, const int* __restrict__ par,
+                                                      const int* __restrict__ active, int use_active,
+                                                      const int* __restrict__ sel, double* __restrict__ L0,
+                                                      double* __restrict__ L1) {
+  int chunks = (P + 255) / 256;
+  int w = blockIdx.x, xcd = w % 8, r = w / 8;
+  int bi = (r / chunks) * 8 + xcd, chunk = r % chunks;
+  if (bi >= B) return;
+  int b = sel ? sel[bi] : bi;
+  if (use_active && !active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int p0 = chunk * 256 + wave * 64;
+  if (p0 >= P) return;
+  int pb = par[b];
+  const double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
+  const double* sc = scal + 8 * (size_t)b;
+  wave_like(nd, P, Hall + (size_t)b * nd * nd, S, bvec + (size_t)b * nd, dsd[b], sc[3] + sc[4], p0, lane,
+            (pb ? L1 : L0) + (size_t)b * P);
+}
+
+hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel) {
+  int chunks = (bt.P + 255) / 256;
+  int groups = (B + 7) / 8;
+  hipLaunchKernelGGL(k_like_mfma, dim3(groups * 8 * chunks), dim3(256), 0, s, m.ndim, bt.P, B, bt.H, bt.bvec, bt.dsd,
+                     bt.scal, bt.S, bt.S2, bt.par, bt.active, use_active, sel, bt.like, bt.like2);
+  return hipGetLastError();
 }
 
 // Q is stored per stage in blocks of KB = 32 columns, row-major inside the block:
@@ -330,24 +505,24 @@ __device__ __noinline__ double gibbs_draw(double mu, double prec, double max_sli
 // One wavefront owns 64 particles; a workgroup of 4 wavefronts owns 256 particles of one instance.
 // Per block of 32 coordinates (two 16-coordinate tiles m) and sweep:
 //  * matrix phase: q(i0 + 16m + r, p) = sum_j Q(j, i0 + 16m + r) s_j(p) for the wave's 64 particles
-//    (four 16-particle tiles n) as 8 accumulators of v_mfma_f64_16x16x4f64 over j in steps of 4.
-//    Operand fragments are plain coalesced loads: A(r, k) = Q(j0 + k, col) at lane r + 16k,
-//    B(k, c) = s_{j0+k}(p0 + c) at lane c + 16k; lane l holds D(4t + l/16, l%16) in register t.
-//  * draw phase, per tile m: the accumulator tile is transposed through 8 KB of wave-private LDS
-//    so that lane l reads q of its own particle, then the 16 coordinates are redrawn in order with
-//    the in-block corrections Q(i0 + mm, i0 + k) delta_mm of the coordinates already redrawn.
+//    (four 16-particle tiles n) as 8 accumulators of v_mfma_f64_16x16x4f64 over j in steps of 4,
+//    software-pipelined (mfma_panel).  Operand fragments are plain coalesced loads: A(r, k) =
+//    Q(j0 + k, col) at lane r + 16k, B(k, c) = s_{j0+k}(p0 + c) at lane c + 16k; lane l holds
+//    D(4t + l/16, l%16) in register t.
+//  * draw phase, per tile: the tile's accumulators are transposed through 8 KB of wave-private LDS so
+//    that lane l finds q of its own particle; the 16 coordinates are redrawn in order with right-looking
+//    in-tile corrections (VALU), and tile 0's deltas, left in LDS, correct tile 1 by 16 MFMAs.
 // While one wavefront draws (VALU) another can run its matrix phase on the matrix cores.
 //
 // Workgroup -> (instance, particle chunk) is XCD-aware: workgroup w runs on XCD w % 8, and all
 // chunks of one instance get the same w % 8, so an instance's Q blocks are fetched into one L2.
 constexpr int kXcd = 8;
 constexpr int kGibbsKB = 32;
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
                                                const double* __restrict__ Qball, double* __restrict__ S0,
-                                               double* __restrict__ S1, const double* __restrict__ bvall,
-                                               const double* __restrict__ gam, const uint64_t* __restrict__ keys,
+                                               double* __restrict__ S1, const double* __restrict__ cdall,
+                                               const uint64_t* __restrict__ keys,
                                                const int* __restrict__ iters, const int* __restrict__ active,
                                                const int* __restrict__ par) {
   constexpr int KB = kGibbsKB;
@@ -363,8 +538,8 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
   int pc = valid ? p : P - 1;
   double* S = (par[b] ? S1 : S0) + (size_t)b * nd * P;
   const double* Qb = Qball + (size_t)b * nd * ((nd + KB - 1) / KB) * KB;
-  const double* bv = bvall + (size_t)b * nd;
-  double gamma = gam[b];
+  // per-coordinate constants of the stage: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  const double* cd = cdall + (size_t)b * 4 * nd;
   uint64_t key = keys[b];
   uint32_t stage = (uint32_t)(iters[b] - 1);
   int nblk = (nd + KB - 1) / KB;
@@ -373,6 +548,9 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
   int pcol[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) pcol[n] = min(pw0 + 16 * n + fi, P - 1);
+  uint32_t bo[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
   int nd4 = nd & ~3;
   double (*Tw)[65] = T[wave];
   for (int sw = 0; sw < sweeps; ++sw) {
@@ -386,18 +564,8 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
         for (int n = 0; n < 4; ++n) acc[m][n] = dbl4{0.0, 0.0, 0.0, 0.0};
       const double* qa = Qr + (size_t)fk * KB + fi;
       const double* sb = S + (size_t)fk * P;
-#pragma unroll 2
-      for (int j0 = 0; j0 < nd4; j0 += 4) {
-        double a0 = qa[(size_t)j0 * KB], a1 = qa[(size_t)j0 * KB + 16];
-        double bb[4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) bb[n] = sb[(size_t)j0 * P + pcol[n]];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-          acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bb[n], acc[0][n], 0, 0, 0);
-          acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bb[n], acc[1][n], 0, 0, 0);
-        }
-      }
+      mfma_panel<kPanelBuffers>(Qr, (uint32_t)(fk * KB + fi), (uint32_t)(fk * KB + fi + 16), (uint32_t)(4 * KB), S, bo,
+                                (uint32_t)(4 * P), nd4 / 4, acc);
       if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
         bool in = nd4 + fk < nd;
         double a0 = in ? qa[(size_t)nd4 * KB] : 0.0, a1 = in ? qa[(size_t)nd4 * KB + 16] : 0.0;
@@ -433,8 +601,7 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint64_t wodd = 0;
         for (int kk = 0; kk < 16; ++kk) {
-          int k = 16 * m + kk;
-          int i = i0 + k;
+          int i = i0 + 16 * m + kk;
           if (i >= nd) {
             Tw[kk][lane] = 0.0;
             continue;
@@ -451,10 +618,11 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
           // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
           const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
           double qi = Tw[kk][lane];
-          double prec = qrow[kk];
           double so = S[(size_t)i * P + pc];
-          double mu = so - (qi - gamma * bv[i]) / prec;
-          double sn = gibbs_draw(mu, prec, max_slip, u);
+          double mu = so - (qi - cd[i]) * cd[nd + i];
+          // lean draw for the common regions; the full one (tails beyond 10 sigma) when it declines
+          double sn = gibbs_draw_fast(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u);
+          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u);
           double dlt = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
           // right-looking correction of the later coordinates of the tile, Tw[kk'] += Q(i, i0+16m+kk') delta;
@@ -471,6 +639,7 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
   }
 }
 
+
 int gibbs_kb(int) { return kGibbsKB; }
 
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
@@ -479,7 +648,7 @@ hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& c
   int groups = (B + kXcd - 1) / kXcd;
   dim3 g(groups * kXcd * chunks);
   hipLaunchKernelGGL(k_gibbs, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
-                     bt.bvec, bt.gamma, bt.key, bt.iter, bt.active, bt.par);
+                     bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
   return hipGetLastError();
 }
 

@@ -131,37 +131,45 @@ SMCF_DEV bool dd_lt_d(dd a, double b) { return a.hi < b || (a.hi == b && a.lo < 
 // ---------------------------------------------------------------------------
 // Standard normal quantile, Wichura (1988) AS 241 PPND16 (relative error ~1e-16)
 // ---------------------------------------------------------------------------
+// AS 241: central region |q| <= 0.425, tail regions with r = sqrt(-log(min(p, 1 - p))) <= 5 and
+// r > 5 (p < 1.4e-11).  The central rational is evaluated on every lane; the log / sqrt and the
+// tail rational only in waves with a tail lane, and the far tail in its own (rare) branch.
+SMCF_DEV double ndtri_central(double q) {
+  double r = 0.180625 - q * q;
+  return q * (((((((2509.0809287301226727 * r + 33430.575583588128105) * r + 67265.770927008700853) * r +
+                  45921.953931549871457) * r + 13731.693765509461125) * r + 1971.5909503065514427) * r +
+               133.14166789178437745) * r + 3.387132872796366608) /
+         (((((((5226.495278852545925 * r + 28729.085735721942674) * r + 39307.89580009271061) * r +
+              21213.794301586595867) * r + 5394.1960214247511077) * r + 687.1870074920579083) * r +
+           42.313330701600911252) * r + 1.0);
+}
+SMCF_DEV double ndtri_tail(double r) {  // r <= 5
+  r -= 1.6;
+  return (((((((7.7454501427834140764e-4 * r + 0.0227238449892691845833) * r + 0.24178072517745061177) * r +
+              1.27045825245236838258) * r + 3.64784832476320460504) * r + 5.7694972214606914055) * r +
+           4.6303378461565452959) * r + 1.42343711074968357734) /
+         (((((((1.05075007164441684324e-9 * r + 5.475938084995344946e-4) * r + 0.0151986665636164571966) * r +
+              0.14810397642748007459) * r + 0.68976733498510000455) * r + 1.6763848301838038494) * r +
+           2.05319162663775882187) * r + 1.0);
+}
+__device__ __noinline__ double ndtri_far_tail(double r) {  // r > 5
+  r -= 5.0;
+  return (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r + 0.0012426609473880784386) * r +
+              0.026532189526576123093) * r + 0.29656057182850489123) * r + 1.7848265399172913358) * r +
+           5.4637849111641143699) * r + 6.6579046435011037772) /
+         (((((((2.04426310338993978564e-15 * r + 1.4215117583164458887e-7) * r + 1.8463183175100546818e-5) * r +
+              7.868691311456132591e-4) * r + 0.0148753612908506148525) * r + 0.13692988092273580531) * r +
+           0.59983220655588793769) * r + 1.0);
+}
 SMCF_DEV double ndtri(double p) {
-  double q = p - 0.5, r, z;
-  if (fabs(q) <= 0.425) {
-    r = 0.180625 - q * q;
-    return q * (((((((2509.0809287301226727 * r + 33430.575583588128105) * r + 67265.770927008700853) * r +
-                    45921.953931549871457) * r + 13731.693765509461125) * r + 1971.5909503065514427) * r +
-                 133.14166789178437745) * r + 3.387132872796366608) /
-           (((((((5226.495278852545925 * r + 28729.085735721942674) * r + 39307.89580009271061) * r +
-                21213.794301586595867) * r + 5394.1960214247511077) * r + 687.1870074920579083) * r +
-             42.313330701600911252) * r + 1.0);
+  double q = p - 0.5;
+  double z = ndtri_central(q);
+  if (fabs(q) > 0.425) {
+    double r = sqrt(-log(q < 0 ? p : 1.0 - p));
+    double t = r <= 5.0 ? ndtri_tail(r) : ndtri_far_tail(r);
+    z = q < 0 ? -t : t;
   }
-  r = q < 0 ? p : 1.0 - p;
-  r = sqrt(-log(r));
-  if (r <= 5.0) {
-    r -= 1.6;
-    z = (((((((7.7454501427834140764e-4 * r + 0.0227238449892691845833) * r + 0.24178072517745061177) * r +
-             1.27045825245236838258) * r + 3.64784832476320460504) * r + 5.7694972214606914055) * r +
-          4.6303378461565452959) * r + 1.42343711074968357734) /
-        (((((((1.05075007164441684324e-9 * r + 5.475938084995344946e-4) * r + 0.0151986665636164571966) * r +
-             0.14810397642748007459) * r + 0.68976733498510000455) * r + 1.6763848301838038494) * r +
-          2.05319162663775882187) * r + 1.0);
-  } else {
-    r -= 5.0;
-    z = (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r + 0.0012426609473880784386) * r +
-             0.026532189526576123093) * r + 0.29656057182850489123) * r + 1.7848265399172913358) * r +
-          5.4637849111641143699) * r + 6.6579046435011037772) /
-        (((((((2.04426310338993978564e-15 * r + 1.4215117583164458887e-7) * r + 1.8463183175100546818e-5) * r +
-             7.868691311456132591e-4) * r + 0.0148753612908506148525) * r + 0.13692988092273580531) * r +
-          0.59983220655588793769) * r + 1.0);
-  }
-  return q < 0 ? -z : z;
+  return z;
 }
 
 // Standard normal truncated to [a, b] by inversion; the upper tail works with
@@ -173,7 +181,8 @@ SMCF_DEV double mills_series(double x) {
 }
 SMCF_DEV double logq_far(double x) { return -0.5 * x * x - log(x) - 0.91893853320467274178 + log(mills_series(x)); }
 // beyond 35 sigma: Newton on log Q(z) = log Q(a) + log(1 - u (1 - Q(b)/Q(a)))
-SMCF_DEV double tn_far(double a, double b, double u) {
+// (out of line: rarely taken, and it would otherwise dominate the draw's code and registers)
+__device__ __noinline__ double tn_far(double a, double b, double u) {
   double la = logq_far(a), lb = logq_far(b);
   double T = la + log1p(-u * (-expm1(lb - la)));
   double z = a - log1p(-u * (-expm1(-a * (b - a)))) / a;
@@ -199,7 +208,8 @@ SMCF_DEV double tn_std(double a, double b, double u) {
   double pb = b < 0.0 ? tb : 1.0 - tb, qb = b < 0.0 ? 1.0 - tb : tb;
   double m = a >= 0.0 ? qa - qb : pb - pa;
   double p = pa + u * m, q = qb + (1.0 - u) * m;
-  double z = p <= q ? ndtri(p) : -ndtri(q);
+  double z = ndtri(fmin(p, q));
+  z = p <= q ? z : -z;
   return fmin(fmax(z, a), b);
 }
 SMCF_DEV double trunc_normal(double mu, double sigma, double lo, double hi, double u) {
@@ -207,10 +217,10 @@ SMCF_DEV double trunc_normal(double mu, double sigma, double lo, double hi, doub
   double x = mu + sigma * tn_std(a, b, u);
   return fmin(fmax(x, lo), hi);
 }
-// the Gibbs mover's parameterisation: s = sqrt(precision)
-SMCF_DEV double trunc_normal_prec(double mu, double s, double lo, double hi, double u) {
+// the Gibbs mover's parameterisation: s = sqrt(precision), is = 1 / s
+SMCF_DEV double trunc_normal_prec(double mu, double s, double is, double lo, double hi, double u) {
   double a = (lo - mu) * s, b = (hi - mu) * s;
-  double x = mu + tn_std(a, b, u) / s;
+  double x = mu + tn_std(a, b, u) * is;
   return fmin(fmax(x, lo), hi);
 }
 

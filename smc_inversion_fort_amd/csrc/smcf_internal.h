@@ -74,6 +74,7 @@ struct BatchDev {
   double* bvec;             // ndim x cap
   double* dsd;              // cap
   double* Q;                // ndim x ndim x cap (Gibbs precision)
+  double* qdiag;            // 4 x ndim x cap: gamma b_i, 1/Q_ii, sqrt(Q_ii), 1/sqrt(Q_ii) of the stage
   double* S;                // ndim x P x cap, dimension-major (S[i*P + p])
   double* S2;
   double* Y;                // ndim x P x cap (H S)
@@ -131,8 +132,10 @@ hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s
 hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s,
                              const int* sel = nullptr);
-hipError_t launch_likelihood(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s,
-                             const int* sel = nullptr);
+// likelihood of the current particles (matrix cores; Gram form)
+hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel = nullptr);
+// prior of the current particles
+hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
