@@ -30,6 +30,11 @@ extern "C" {
 #define SMCF_MOVER_HMC 0   /* reference HMC: ntau 50 + ESJD dtau tuning (smc_slip.f90:591-1214) */
 #define SMCF_MOVER_GIBBS 1 /* truncated-MVN Gibbs sweeps (benchmark configuration) */
 
+/* arithmetic of the Gibbs mover (no counterpart in the reference, which is FP64 throughout) */
+#define SMCF_PRECISION_FP64 0  /* every operation in FP64 (default; parity with the reference's arithmetic) */
+#define SMCF_PRECISION_MIXED 1 /* Q.S products in FP32 on the matrix cores and single-precision inverse-CDF
+                                  draws; conditional means, corrections, likelihood, weights and evidence FP64 */
+
 typedef struct smcf_ctx smcf_ctx;
 typedef struct smcf_problem smcf_problem;
 
@@ -64,6 +69,7 @@ typedef struct {
   int nparticle_slip;
   int mover;                  /* SMCF_MOVER_* */
   int gibbs_sweeps;
+  int precision;              /* SMCF_PRECISION_* (MIXED needs the Gibbs mover) */
 } smcf_problem_desc;
 
 /* Upload a problem; device buffers are sized for up to max_batch geometries in flight. */
@@ -117,6 +123,10 @@ int smcf_normalize_resample(smcf_ctx* ctx, int P, const double* weights, double 
 /* truncated-normal inversion of the Gibbs mover (no reference counterpart) */
 int smcf_trunc_normal(smcf_ctx* ctx, int n, const double* mu, const double* sigma, double lo, double hi,
                       const double* u, double* out);
+/* single-precision draw of the mixed-precision Gibbs mover (no reference counterpart): sq = sqrt of the
+ * conditional precision, u single-precision uniforms; NaN where the mover falls back to the FP64 draw */
+int smcf_trunc_normal_f32(smcf_ctx* ctx, int n, const double* mu, const double* sq, double lo, double hi,
+                          const float* u, double* out);
 /* raw Philox4x32-10 blocks: ctr 4 x n, key 2 x n, out 4 x n */
 int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 

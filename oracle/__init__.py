@@ -93,6 +93,10 @@ def _declare(L):
     L.orc_trunc_normal.argtypes = [C.c_double] * 5
     L.orc_trunc_normal_prec.argtypes = [C.c_double] * 5
     L.orc_trunc_normal_prec.restype = C.c_double
+    L.orc_trunc_normal_prec_f32.argtypes = [C.c_double] * 4 + [C.c_float]
+    L.orc_trunc_normal_prec_f32.restype = C.c_double
+    L.orc_ndtri7_lower.argtypes = [C.c_float]
+    L.orc_ndtri7_lower.restype = C.c_float
     L.orc_trunc_normal.restype = C.c_double
     L.orc_slip_smc_exec.argtypes = [C.POINTER(Problem), D, D, C.c_uint64, D, D, D, D, C.POINTER(SlipResult)]
     L.orc_fault_calc_likelihood.argtypes = [C.POINTER(Problem), D, C.c_uint64, C.POINTER(SlipResult)]
@@ -380,6 +384,16 @@ def trunc_normal(mu, sigma, lo, hi, u):
 def trunc_normal_prec(mu, s, lo, hi, u):
     """The Gibbs mover's draw: s = sqrt(precision)."""
     return lib().orc_trunc_normal_prec(mu, s, lo, hi, u)
+
+
+def trunc_normal_prec_f32(mu, s, lo, hi, u):
+    """Single-precision draw of the mixed-precision Gibbs mover (NaN where it declines)."""
+    return lib().orc_trunc_normal_prec_f32(mu, s, lo, hi, u)
+
+
+def ndtri7_lower(p):
+    """AS 241 PPND7 in single precision for p in (0, 1/2]."""
+    return lib().orc_ndtri7_lower(p)
 
 
 def qsort_desc(arr):

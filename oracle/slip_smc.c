@@ -354,6 +354,51 @@ double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u
     return fmin(fmax(x, lo), hi);
 }
 
+/* Single-precision draw of the mixed-precision Gibbs mover (no reference counterpart; the device
+ * restates it in trunc_normal_prec_f32).  Inverse CDF: AS 241 PPND7 (Wichura 1988, Appl. Statist.
+ * 37, 477-484), the 7-digit version of orc_ndtri.  p in (0, 1/2]; returns z <= 0 with Phi(z) = p. */
+float orc_ndtri7_lower(float p) {
+    float q = p - 0.5f;
+    if (q >= -0.425f) {
+        float r = 0.180625f - q * q;
+        return q * (((59.109374720f * r + 159.29113202f) * r + 50.434271938f) * r + 3.3871327179f) /
+               (((67.187563600f * r + 78.757757664f) * r + 17.895169469f) * r + 1.0f);
+    }
+    float r = sqrtf(-logf(p)), z;
+    if (r <= 5.0f) {
+        r -= 1.6f;
+        z = (((0.17023821103f * r + 1.3067284816f) * r + 2.7568153900f) * r + 1.4234372777f) /
+            ((0.12021132975f * r + 0.73700164250f) * r + 1.0f);
+    } else {
+        r -= 5.0f;
+        z = (((0.017337203997f * r + 0.42868294337f) * r + 3.0812263860f) * r + 6.6579051150f) /
+            ((0.012258202635f * r + 0.24197894225f) * r + 1.0f);
+    }
+    return -z;
+}
+
+/* The tn_std construction in single precision: a = (lo - mu) s and b = (hi - mu) s formed in double
+ * and rounded; one erfcf per bound; p = Phi(a) + u m, q = Q(b) + (1 - u) m, the smaller inverted.
+ * Returns NaN (the mover then takes orc_trunc_normal_prec) when a bound lies beyond 8 sigma on the
+ * far side of the mean or the interval holds less than 1e-30 of the mass. */
+double orc_trunc_normal_prec_f32(double mu, double s, double lo, double hi, float u) {
+    double a64 = (lo - mu) * s, b64 = (hi - mu) * s;
+    if (a64 > 8.0 || b64 < -8.0) return NAN;
+    float a = (float)fmax(a64, -40.0), b = (float)fmin(b64, 40.0);
+    const float r2 = 0.70710678118654752f;
+    float ta = 0.5f * erfcf(fabsf(a) * r2), tb = 0.5f * erfcf(fabsf(b) * r2);
+    float pa = a < 0.0f ? ta : 1.0f - ta, qa = a < 0.0f ? 1.0f - ta : ta;
+    float pb = b < 0.0f ? tb : 1.0f - tb, qb = b < 0.0f ? 1.0f - tb : tb;
+    float m = a >= 0.0f ? qa - qb : pb - pa;
+    if (!(m >= 1e-30f)) return NAN;
+    float p = pa + u * m, q = qb + (1.0f - u) * m;
+    float z = orc_ndtri7_lower(fminf(p, q));
+    z = p <= q ? z : -z;
+    z = fminf(fmaxf(z, a), b);
+    double x = mu + (double)z * (1.0 / s);
+    return fmin(fmax(x, lo), hi);
+}
+
 static double open_uniform(uint64_t w) { return ((double)(w >> 11) + 0.5) * 0x1.0p-53; }
 
 /* ---------------- per-instance state ---------------- */

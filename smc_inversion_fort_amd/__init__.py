@@ -27,6 +27,8 @@ _LIB = None
 
 MOVER_HMC = 0
 MOVER_GIBBS = 1
+PRECISION_FP64 = 0   # every operation in FP64
+PRECISION_MIXED = 1  # Gibbs Q.S products in FP32 on the matrix cores, single-precision draws; means/weights FP64
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -44,7 +46,7 @@ class ProblemDesc(C.Structure):
                 ("nobs", C.c_int), ("nsar_total", C.c_int), ("ngnss", C.c_int), ("npath", C.c_int),
                 ("nsar_index", _I), ("obs_points", _D), ("obs_unitvec", _D), ("obs_sigma", _D),
                 ("dvec", _D), ("sar_sinv", _D), ("max_slip", C.c_double), ("nparticle_slip", C.c_int),
-                ("mover", C.c_int), ("gibbs_sweeps", C.c_int)]
+                ("mover", C.c_int), ("gibbs_sweeps", C.c_int), ("precision", C.c_int)]
 
 
 EXPORTS = [
@@ -52,7 +54,7 @@ EXPORTS = [
     "smcf_problem_create", "smcf_problem_destroy", "smcf_problem_dims", "smcf_dc3d0",
     "smcf_calc_greens_func", "smcf_laplacian", "smcf_slip_calc_gsd_gsg", "smcf_slip_smc_exec",
     "smcf_fault_calc_likelihood_batch", "smcf_find_next_gamma", "smcf_normalize_resample",
-    "smcf_trunc_normal", "smcf_philox", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
+    "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
 ]
 
 
@@ -87,6 +89,7 @@ def _declare(L):
                                        _D, _D, _D]
     L.smcf_normalize_resample.argtypes = [C.c_void_p, C.c_int, _D, C.c_double, C.c_double, _D, _I]
     L.smcf_trunc_normal.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
+    L.smcf_trunc_normal_f32.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, C.POINTER(C.c_float), _D]
     L.smcf_philox.argtypes = [C.c_void_p, C.c_int, _U32, _U32, _U32]
     L.smcf_fault_smc_exec.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _I, _D, _D]
     L.smcf_fault_smc_exec_gsmc.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _D,
@@ -192,6 +195,16 @@ class Context:
         _check(lib().smcf_trunc_normal(self.h, mu.shape[0], _dp(mu), _dp(sg), lo, hi, _dp(u), _dp(out)))
         return out
 
+    def trunc_normal_f32(self, mu, sq, lo, hi, u):
+        """Single-precision draw of the mixed-precision Gibbs mover; NaN where it declines."""
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        sq = np.ascontiguousarray(sq, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        out = np.zeros_like(mu)
+        _check(lib().smcf_trunc_normal_f32(self.h, mu.shape[0], _dp(mu), _dp(sq), lo, hi,
+                                           u.ctypes.data_as(C.POINTER(C.c_float)), _dp(out)))
+        return out
+
     def philox(self, ctr, key):
         c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
         k = np.ascontiguousarray(key, dtype=np.uint32).reshape(-1, 2)
@@ -205,7 +218,8 @@ class Problem:
     """Observations + fault mesh + inner-SMC settings resident on the device."""
 
     def __init__(self, ctx, nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip,
-                 mover=MOVER_GIBBS, gibbs_sweeps=5, max_batch=1, fix_xbend=False, xbend=()):
+                 mover=MOVER_GIBBS, gibbs_sweeps=5, max_batch=1, fix_xbend=False, xbend=(),
+                 precision=PRECISION_FP64):
         self.ctx = ctx
         self._keep = dict(
             nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
@@ -218,7 +232,8 @@ class Problem:
                                 _dp(k["xbend"]), nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
                                 len(k["nsar_index"]) - 1, _ip(k["nsar_index"]), _dp(k["obs_points"]),
                                 _dp(k["obs_unitvec"]), _dp(k["obs_sigma"]), _dp(k["dvec"]), _dp(k["sar"]),
-                                float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps))
+                                float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps),
+                                int(precision))
         h = C.c_void_p()
         _check(lib().smcf_problem_create(ctx.h, C.byref(self.desc), int(max_batch), C.byref(h)))
         self.h = h

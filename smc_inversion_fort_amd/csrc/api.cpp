@@ -138,6 +138,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
     ProfScope ps(ctx, "init_chain", s);
     CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
   }
+  if (bt.S32) CHECK_LAUNCH(launch_s32(m, bt, n, s, sel));
   CHECK_LAUNCH(launch_like(m, bt, n, 0, s, sel));
   CHECK_LAUNCH(launch_prior(m, bt, n, s, sel));
   return SMCF_OK;
@@ -367,6 +368,15 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   P.P = d->nparticle_slip;
   P.mover = d->mover;
   P.sweeps = d->gibbs_sweeps > 0 ? d->gibbs_sweeps : 5;
+  P.precision = d->precision;
+  if (P.precision != SMCF_PRECISION_FP64 && P.precision != SMCF_PRECISION_MIXED) {
+    set_error("unknown precision " + std::to_string(P.precision));
+    return SMCF_EINVAL;
+  }
+  if (P.precision == SMCF_PRECISION_MIXED && P.mover != SMCF_MOVER_GIBBS) {
+    set_error("mixed precision is implemented for the Gibbs mover only");
+    return SMCF_EUNSUPPORTED;
+  }
   MeshDev& m = P.m;
   memset(&m, 0, sizeof(m));
   int np = d->nplane;
@@ -491,11 +501,20 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.H, nd * nd * C);
   rc |= dalloc(P, &bt.bvec, nd * C);
   rc |= dalloc(P, &bt.dsd, C);
-  rc |= dalloc(P, &bt.Q, (nd + 32) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  bt.kb = gibbs_kb(P.precision);
+  rc |= dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  bt.Q32 = nullptr;
+  bt.S32 = bt.S32b = nullptr;
+  if (P.precision == SMCF_PRECISION_MIXED) {
+    rc |= dalloc(P, &bt.Q32, (nd + bt.kb) * nd * C);
+    rc |= dalloc(P, &bt.S32, nd * Pn * C);
+    rc |= dalloc(P, &bt.S32b, nd * Pn * C);
+  }
   rc |= dalloc(P, &bt.qdiag, 4 * nd * C);
   rc |= dalloc(P, &bt.S, nd * Pn * C);
   rc |= dalloc(P, &bt.S2, nd * Pn * C);
-  rc |= dalloc(P, &bt.Y, nd * Pn * C);
+  bt.Y = nullptr;
+  if (P.mover == SMCF_MOVER_HMC) rc |= dalloc(P, &bt.Y, nd * Pn * C);  // H X of the leapfrog (HMC only)
   rc |= dalloc(P, &bt.like, Pn * C);
   rc |= dalloc(P, &bt.prior, Pn * C);
   rc |= dalloc(P, &bt.like2, Pn * C);
@@ -759,6 +778,27 @@ int smcf_trunc_normal(smcf_ctx* ctx, int n, const double* mu, const double* sg, 
   SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
   (void)(void)hipFree(dm); (void)hipFree(ds); (void)hipFree(du); (void)hipFree(dout);
+  return SMCF_OK;
+}
+
+int smcf_trunc_normal_f32(smcf_ctx* ctx, int n, const double* mu, const double* sq, double lo, double hi,
+                          const float* u, double* out) {
+  if (!ctx || n < 0 || (n > 0 && (!mu || !sq || !u || !out))) return SMCF_EINVAL;
+  if (n == 0) return SMCF_OK;
+  hipStream_t s = ctx->stream;
+  double *dm, *ds, *dout;
+  float* du;
+  SMCF_HIP(hipMalloc(&dm, sizeof(double) * n));
+  SMCF_HIP(hipMalloc(&ds, sizeof(double) * n));
+  SMCF_HIP(hipMalloc(&du, sizeof(float) * n));
+  SMCF_HIP(hipMalloc(&dout, sizeof(double) * n));
+  SMCF_HIP(hipMemcpyAsync(dm, mu, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(ds, sq, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_trunc_normal_f32(n, dm, ds, lo, hi, du, dout, s));
+  SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  (void)hipFree(dm); (void)hipFree(ds); (void)hipFree(du); (void)hipFree(dout);
   return SMCF_OK;
 }
 

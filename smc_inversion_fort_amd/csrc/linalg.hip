@@ -167,7 +167,6 @@ hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, co
 
 // Gibbs precision Q = gamma H + LL / alpha^2, written in the blocked layout of the Gibbs
 // kernel: Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k) (zero past the last column).
-int gibbs_kb(int nd);
 
 __global__ void k_form_qb(int nd, int KB, const double* __restrict__ Hall, const double* __restrict__ gamma,
                           const int* __restrict__ active, double* __restrict__ Qball) {
@@ -209,15 +208,15 @@ __global__ void k_form_qb_sparse(int nd, int KB, const double* __restrict__ Hall
 }
 
 // per-coordinate constants of the stage for the Gibbs draws: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
-__global__ void k_form_qdiag(int nd, const double* __restrict__ Qball, const double* __restrict__ gamma,
+__global__ void k_form_qdiag(int nd, int KB, const double* __restrict__ Qball, const double* __restrict__ gamma,
                              const double* __restrict__ bvec, const int* __restrict__ active,
                              double* __restrict__ cdall) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!active[b] || i >= nd) return;
-  int nblk = (nd + 31) / 32;
-  const double* Qb = Qball + (size_t)b * nblk * nd * 32;
-  double qii = Qb[((size_t)(i / 32) * nd + i) * 32 + i % 32];
+  int nblk = (nd + KB - 1) / KB;
+  const double* Qb = Qball + (size_t)b * nblk * nd * KB;
+  double qii = Qb[((size_t)(i / KB) * nd + i) * KB + i % KB];
   double* cd = cdall + (size_t)b * 4 * nd;
   double sq = sqrt(qii);
   cd[i] = gamma[b] * bvec[(size_t)b * nd + i];
@@ -226,15 +225,43 @@ __global__ void k_form_qdiag(int nd, const double* __restrict__ Qball, const dou
   cd[3 * nd + i] = 1.0 / sq;
 }
 
+// FP32 copy of the blocked precision (mixed precision)
+__global__ void k_q_to_f32(size_t n, const double* __restrict__ Qball, const int* __restrict__ active,
+                           float* __restrict__ Q32all) {
+  int b = blockIdx.y;
+  if (!active[b]) return;
+  const double* Q = Qball + (size_t)b * n;
+  float* Q32 = Q32all + (size_t)b * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    Q32[e] = (float)Q[e];
+}
+
+// FP32 copy of the current particles of the given slots (mixed precision)
+__global__ void k_s_to_f32(int nd, int P, BatchDev bt, const int* __restrict__ sel) {
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
+  int cur = bt.par[b];
+  size_t n = (size_t)nd * P;
+  const double* S = (cur ? bt.S2 : bt.S) + (size_t)b * n;
+  float* S32 = (cur ? bt.S32b : bt.S32) + (size_t)b * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    S32[e] = (float)S[e];
+}
+
+hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+  hipLaunchKernelGGL(k_s_to_f32, dim3(256, B), dim3(256), 0, s, m.ndim, bt.P, bt, sel);
+  return hipGetLastError();
+}
+
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int nd = m.ndim, KB = gibbs_kb(nd);
+  int nd = m.ndim, KB = bt.kb;
   size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
   int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
   hipLaunchKernelGGL(k_form_qb, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.H, bt.gamma, bt.active, bt.Q);
   hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.H, bt.gamma,
                      bt.active, bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
-  hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, bt.Q, bt.gamma, bt.bvec,
+  hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.Q, bt.gamma, bt.bvec,
                      bt.active, bt.qdiag);
+  if (bt.Q32) hipLaunchKernelGGL(k_q_to_f32, dim3(blocks, B), dim3(256), 0, s, n, bt.Q, bt.active, bt.Q32);
   return hipGetLastError();
 }
 

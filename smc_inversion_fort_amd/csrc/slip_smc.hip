@@ -295,7 +295,16 @@ __global__ void k_gather(int nd, int P, BatchDev bt) {
   const double* Ss = (cur ? bt.S : bt.S2) + (size_t)b * nd * P;
   double* Sd = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
   int pa = bt.parent[(size_t)b * P + j];
-  for (int i = blockIdx.y; i < nd; i += gridDim.y) Sd[(size_t)i * P + j] = Ss[(size_t)i * P + pa];
+  if (bt.S32) {  // mixed precision: the FP32 copy of the children as well
+    float* Td = (cur ? bt.S32b : bt.S32) + (size_t)b * nd * P;
+    for (int i = blockIdx.y; i < nd; i += gridDim.y) {
+      double v = Ss[(size_t)i * P + pa];
+      Sd[(size_t)i * P + j] = v;
+      Td[(size_t)i * P + j] = (float)v;
+    }
+  } else {
+    for (int i = blockIdx.y; i < nd; i += gridDim.y) Sd[(size_t)i * P + j] = Ss[(size_t)i * P + pa];
+  }
   if (blockIdx.y == 0) {
     const double* Ls = (cur ? bt.like : bt.like2) + (size_t)b * P;
     double* Ld = (cur ? bt.like2 : bt.like) + (size_t)b * P;
@@ -640,15 +649,233 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
 }
 
 
-int gibbs_kb(int) { return kGibbsKB; }
+// ---------------------------------------------------------------------------
+// Mixed-precision Gibbs sweeps (SMCF_PRECISION_MIXED).  The sweep of k_gibbs, with
+//  * the matrix phase on v_mfma_f32_16x16x4f32 from FP32 copies of Q and S: half the bytes of S per
+//    block and twice the FP64 matrix rate, over blocks of KB = 64 coordinates (four 16-row tiles; an
+//    FP32 accumulator tile takes 4 registers where FP64 takes 8).  FP32 fragments: A(r, k) at lane
+//    r + 16k, B(k, c) at lane c + 16k; lane l holds D(4 (l/16) + t, l % 16) in register t (checked on
+//    the device before first use; the FP64 instruction orders its rows differently);
+//  * q of each tile converted to double in LDS; the conditional mean, the in-tile corrections and the
+//    old and new values in FP64; the draw in single precision (trunc_normal_prec_f32, inlined), with the
+//    FP64 draw where that declines (bounds beyond 8 sigma on the far side of the mean);
+//  * occupancy 2 (256 registers, 66 KB of LDS per workgroup): the tile's current values are loaded
+//    together before its draws into LDS beside q, and the new values stored together after them, so no
+//    draw waits on a load or a store;
+//  * the corrections of the later tiles by a tile's deltas on the FP32 matrix cores;
+//  * every redrawn value stored to S and to its FP32 copy.
+// ---------------------------------------------------------------------------
+constexpr int kMixedMT = 4;  // 16-row tiles per block
+constexpr int kMixedNB = 3;  // register buffers of the matrix-phase pipeline
+
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// mfma_panel in FP32: acc[m][n] += sum_j A(16m + r, j) B(j, c_n), same operand addressing and pipeline
+template <int MT, int NB>
+__device__ __forceinline__ void mfma_panel_f32(const float* __restrict__ abase, const uint32_t (&ao)[MT],
+                                               uint32_t lda4, const float* __restrict__ bbase,
+                                               const uint32_t (&bo)[4], uint32_t ldb4, int nsteps,
+                                               flt4 (&acc)[MT][4]) {
+  if (nsteps <= 0) return;
+  float x[NB][MT], y[NB][4];
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nsteps - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) x[k][m] = abase[ao[m] + q * lda4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[k][m], y[k][n], acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nsteps - nsteps % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nsteps - full) prod(k);
+}
+
+// one coordinate redraw in single precision, inlined (a call would drain every outstanding memory
+// operation first); NaN where trunc_normal_prec_f32 declines
+__device__ __forceinline__ double gibbs_draw_f32(double mu, double s, double is, double max_slip, float u) {
+  return trunc_normal_prec_f32(mu, s, is, 0.0, max_slip, u);
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_gibbs_mixed(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
+                  const float* __restrict__ Q32all, double* __restrict__ S0, double* __restrict__ S1,
+                  float* __restrict__ T0, float* __restrict__ T1, const double* __restrict__ cdall,
+                  const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
+                  const int* __restrict__ par) {
+  constexpr int MT = kMixedMT, KB = 16 * MT;
+  // per wavefront: q of the tile being drawn (rows 0..15) and the current values of its coordinates
+  // (rows 16..31), lane = particle
+  __shared__ double T[4][32][65];
+  int chunks = (P + 255) / 256;
+  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
+  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
+  if (b >= B || !active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int pw0 = chunk * 256 + wave * 64;
+  int p = pw0 + lane;
+  bool valid = p < P;
+  int pc = valid ? p : P - 1;
+  int pb = par[b];
+  double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
+  float* S32 = (pb ? T1 : T0) + (size_t)b * nd * P;
+  size_t qstride = (size_t)nd * ((nd + KB - 1) / KB) * KB;
+  const double* Qb = Qball + (size_t)b * qstride;
+  const float* Qb32 = Q32all + (size_t)b * qstride;
+  // per-coordinate constants of the stage: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  const double* cd = cdall + (size_t)b * 4 * nd;
+  uint64_t key = keys[b];
+  uint32_t stage = (uint32_t)(iters[b] - 1);
+  int nblk = (nd + KB - 1) / KB;
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) pcol[n] = min(pw0 + 16 * n + fi, P - 1);
+  uint32_t bo[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * KB + 16 * m + fi);
+  int nd4 = nd & ~3;
+  double (*Tw)[65] = T[wave];
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int blk = 0; blk < nblk; ++blk) {
+      int i0 = blk * KB;
+      const double* Qr = Qb + (size_t)blk * nd * KB;
+      const float* Qr32 = Qb32 + (size_t)blk * nd * KB;
+      flt4 acc[MT][4];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = flt4{0.f, 0.f, 0.f, 0.f};
+      mfma_panel_f32<MT, kMixedNB>(Qr32, ao, (uint32_t)(4 * KB), S32, bo, (uint32_t)(4 * P), nd4 / 4, acc);
+      if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
+        bool in = nd4 + fk < nd;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          float bn = in ? S32[(size_t)(nd4 + fk) * P + pcol[n]] : 0.f;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            float a = in ? Qr32[(size_t)(nd4 + fk) * KB + 16 * m + fi] : 0.f;
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bn, acc[m][n], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
+        // transpose tile m through LDS in double: Tw[row][particle]
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) Tw[4 * fk + t][16 * n + fi] = (double)acc[m][n][t];
+        // the tile's current values of the lane's particle, requested together before the draws (coalesced,
+        // in the accumulator layout) and kept in Tw[16 + k]; the new values are stored after the tile
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            int row = i0 + 16 * m + 4 * fk + t;
+            Tw[16 + 4 * fk + t][16 * n + fi] = row < nd ? S[(size_t)row * P + pcol[n]] : 0.0;
+          }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t wodd = 0;
+#pragma unroll 1
+        for (int kk = 0; kk < 16; ++kk) {
+          int i = i0 + 16 * m + kk;
+          if (i >= nd) {
+            Tw[kk][lane] = 0.0;
+            continue;
+          }
+          // coordinates 2m' and 2m'+1 share one Philox block (word i & 1); i0 and the tile offset are even
+          uint64_t wc;
+          if ((kk & 1) == 0) {
+            philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wodd);
+          } else {
+            wc = wodd;
+          }
+          // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
+          const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
+          double qi = Tw[kk][lane];
+          double so = Tw[16 + kk][lane];
+          double mu = so - (qi - cd[i]) * cd[nd + i];
+          double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
+          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open(wc));
+          double dlt = sn - so;
+          Tw[16 + kk][lane] = sn;
+#pragma unroll
+          for (int k2 = 0; k2 < 16; ++k2) Tw[k2][lane] = fma(k2 > kk ? qrow[k2] : 0.0, dlt, Tw[k2][lane]);
+          Tw[kk][lane] = dlt;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the new values, coalesced, to S and its FP32 copy
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            int row = i0 + 16 * m + 4 * fk + t, pp = pw0 + 16 * n + fi;
+            if (row < nd && pp < P) {
+              double v = Tw[16 + 4 * fk + t][16 * n + fi];
+              S[(size_t)row * P + pp] = v;
+              S32[(size_t)row * P + pp] = (float)v;
+            }
+          }
+        // corrections of the later tiles by this tile's deltas (in Tw[k][p]), FP32 matrix cores:
+        //   acc[m2](r, p) += sum_k Q(i0 + 16m + k, i0 + 16m2 + r) delta_k(p)
+#pragma unroll
+        for (int m2 = m + 1; m2 < MT; ++m2) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            int kq = 4 * s4 + fk;
+            float a = (i0 + 16 * m + kq < nd) ? Qr32[(size_t)(i0 + 16 * m + kq) * KB + 16 * m2 + fi] : 0.f;
+#pragma unroll
+            for (int n = 0; n < 4; ++n)
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, (float)Tw[kq][16 * n + fi], acc[m2][n], 0, 0, 0);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // every lane has read the deltas before the next tile overwrites them
+      }
+    }
+  }
+}
+
+int gibbs_kb(int precision) { return precision == SMCF_PRECISION_MIXED ? 16 * kMixedMT : kGibbsKB; }
 
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
   int nd = m.ndim;
   int chunks = (bt.P + 255) / 256;
   int groups = (B + kXcd - 1) / kXcd;
   dim3 g(groups * kXcd * chunks);
-  hipLaunchKernelGGL(k_gibbs, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
-                     bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+  if (bt.Q32) {
+    if (bt.kb != 16 * kMixedMT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.Q32, bt.S,
+                       bt.S2, bt.S32, bt.S32b, bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+  } else {
+    if (bt.kb != kGibbsKB) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gibbs, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
+                       bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+  }
   return hipGetLastError();
 }
 
@@ -704,6 +931,18 @@ __global__ void k_trunc_normal(int n, const double* mu, const double* sg, double
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = trunc_normal(mu[i], sg[i], lo, hi, u[i]);
 }
+__global__ void k_trunc_normal_f32(int n, const double* mu, const double* sq, double lo, double hi, const float* u,
+                                   double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = trunc_normal_prec_f32(mu[i], sq[i], 1.0 / sq[i], lo, hi, u[i]);
+}
+
+hipError_t launch_trunc_normal_f32(int n, const double* mu, const double* sq, double lo, double hi, const float* u,
+                                   double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_trunc_normal_f32, dim3((n + 255) / 256), dim3(256), 0, s, n, mu, sq, lo, hi, u, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_trunc_normal(int n, const double* mu, const double* sig, double lo, double hi, const double* u,
                                double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_trunc_normal, dim3((n + 255) / 256), dim3(256), 0, s, n, mu, sig, lo, hi, u, out);

@@ -37,8 +37,10 @@ SMCF_DEV void philox_draw(uint64_t key, uint32_t site, uint32_t stage, uint32_t 
   uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
-    uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
+    // one 32 x 32 -> 64 product per multiplier (a single v_mad_u64_u32 each, rather than mul_lo + mul_hi)
+    uint64_t p0 = (uint64_t)0xD2511F53u * x0, p1 = (uint64_t)0xCD9E8D57u * x2;
+    uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     uint32_t y0 = hi1 ^ x1 ^ k0, y2 = hi0 ^ x3 ^ k1;
     x0 = y0; x1 = lo1; x2 = y2; x3 = lo0;
     k0 += 0x9E3779B9u;
@@ -221,6 +223,54 @@ SMCF_DEV double trunc_normal(double mu, double sigma, double lo, double hi, doub
 SMCF_DEV double trunc_normal_prec(double mu, double s, double is, double lo, double hi, double u) {
   double a = (lo - mu) * s, b = (hi - mu) * s;
   double x = mu + tn_std(a, b, u) * is;
+  return fmin(fmax(x, lo), hi);
+}
+
+// ---------------------------------------------------------------------------
+// Single-precision truncated-normal draw of the mixed-precision Gibbs mover.  The conditional mean
+// and the returned value stay double; the standardised bounds, the two tail masses and the inverse
+// CDF are single precision.  Inverse CDF: AS 241 PPND7 (Wichura 1988, Appl. Statist. 37, 477-484),
+// the 7-digit version of the algorithm used in double precision above.
+SMCF_DEV float ndtri7_lower(float p) {  // p in (0, 1/2]: z <= 0 with Phi(z) = p
+  float q = p - 0.5f;
+  if (q >= -0.425f) {
+    float r = 0.180625f - q * q;
+    return q * (((59.109374720f * r + 159.29113202f) * r + 50.434271938f) * r + 3.3871327179f) /
+           (((67.187563600f * r + 78.757757664f) * r + 17.895169469f) * r + 1.0f);
+  }
+  float r = sqrtf(-logf(p));
+  float z;
+  if (r <= 5.0f) {
+    r -= 1.6f;
+    z = (((0.17023821103f * r + 1.3067284816f) * r + 2.7568153900f) * r + 1.4234372777f) /
+        ((0.12021132975f * r + 0.73700164250f) * r + 1.0f);
+  } else {
+    r -= 5.0f;
+    z = (((0.017337203997f * r + 0.42868294337f) * r + 3.0812263860f) * r + 6.6579051150f) /
+        ((0.012258202635f * r + 0.24197894225f) * r + 1.0f);
+  }
+  return -z;
+}
+// uniform strictly inside (0, 1) from the top 23 bits of a Philox word (exact in single precision)
+SMCF_DEV float u01_open_f32(uint64_t w) { return ((float)(w >> 41) + 0.5f) * 0x1.0p-23f; }
+// Same construction as tn_std: p = Phi(a) + u m and q = Q(b) + (1 - u) m, the smaller inverted.  NaN
+// (the caller then takes the double-precision draw) when a bound lies beyond 8 sigma on the far side of
+// the mean, or when the interval holds less than 1e-30 of the mass.
+SMCF_DEV double trunc_normal_prec_f32(double mu, double s, double is, double lo, double hi, float u) {
+  double a64 = (lo - mu) * s, b64 = (hi - mu) * s;
+  if (a64 > 8.0 || b64 < -8.0) return __builtin_nan("");
+  float a = (float)fmax(a64, -40.0), b = (float)fmin(b64, 40.0);
+  const float r2 = 0.70710678118654752f;
+  float ta = 0.5f * erfcf(fabsf(a) * r2), tb = 0.5f * erfcf(fabsf(b) * r2);
+  float pa = a < 0.0f ? ta : 1.0f - ta, qa = a < 0.0f ? 1.0f - ta : ta;
+  float pb = b < 0.0f ? tb : 1.0f - tb, qb = b < 0.0f ? 1.0f - tb : tb;
+  float m = a >= 0.0f ? qa - qb : pb - pa;
+  if (!(m >= 1e-30f)) return __builtin_nan("");
+  float p = pa + u * m, q = qb + (1.0f - u) * m;
+  float z = ndtri7_lower(fminf(p, q));
+  z = p <= q ? z : -z;
+  z = fminf(fmaxf(z, a), b);
+  double x = mu + (double)z * is;
   return fmin(fmax(x, lo), hi);
 }
 

@@ -73,11 +73,15 @@ struct BatchDev {
   double* H;                // ndim x ndim x cap
   double* bvec;             // ndim x cap
   double* dsd;              // cap
+  int kb;                   // columns per block of the blocked Gibbs precision (gibbs_kb)
   double* Q;                // ndim x ndim x cap (Gibbs precision)
+  float* Q32;               // same layout in FP32 (mixed precision only, else null)
   double* qdiag;            // 4 x ndim x cap: gamma b_i, 1/Q_ii, sqrt(Q_ii), 1/sqrt(Q_ii) of the stage
   double* S;                // ndim x P x cap, dimension-major (S[i*P + p])
   double* S2;
-  double* Y;                // ndim x P x cap (H S)
+  float* S32;               // FP32 copies of S and S2 read by the mixed-precision Gibbs products (else null)
+  float* S32b;
+  double* Y;                // ndim x P x cap (H X of the HMC leapfrog; null for the Gibbs mover)
   double* like;             // P x cap
   double* prior;
   double* like2;
@@ -115,7 +119,7 @@ struct Problem {
   BatchDev bt;
   HmcDev hm;
   double max_slip;
-  int P, mover, sweeps;
+  int P, mover, sweeps, precision;
   std::vector<void*> allocs;
 };
 
@@ -141,6 +145,10 @@ hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
+// columns per block of the blocked Gibbs precision layout (SMCF_PRECISION_*)
+int gibbs_kb(int precision);
+// FP32 copies of the current particles of the given slots (mixed precision)
+hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 
 hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel = nullptr);
@@ -152,6 +160,8 @@ hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double c
                                   double* out3, double* w, hipStream_t s);
 hipError_t launch_normalize_resample(int P, const double* w, double uh, double ul, double* wn,
                                      int* assigned, hipStream_t s);
+hipError_t launch_trunc_normal_f32(int n, const double* mu, const double* sq, double lo, double hi, const float* u,
+                                   double* out, hipStream_t s);
 hipError_t launch_trunc_normal(int n, const double* mu, const double* sig, double lo, double hi,
                                const double* u, double* out, hipStream_t s);
 hipError_t launch_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out, hipStream_t s);

@@ -33,7 +33,7 @@ module smcf_c
         integer(c_int) :: nobs, nsar_total, ngnss, npath
         type(c_ptr) :: nsar_index, obs_points, obs_unitvec, obs_sigma, dvec, sar_sinv
         real(c_double) :: max_slip
-        integer(c_int) :: nparticle_slip, mover, gibbs_sweeps
+        integer(c_int) :: nparticle_slip, mover, gibbs_sweeps, precision
     end type
 
     interface
@@ -92,7 +92,7 @@ module smcf_c
     double precision, save :: g_sum = -1d300
     integer(c_int64_t), save :: g_seed = 0_c_int64_t
     integer(c_int32_t), save :: g_counter = 0
-    integer, save :: g_mover = 1, g_sweeps = 5, g_max_batch = 64
+    integer, save :: g_mover = 1, g_sweeps = 5, g_max_batch = 64, g_precision = 0
     ! arrays kept alive for the C side
     integer(c_int), allocatable, target, save :: t_nxi(:), t_neta(:), t_nsi(:)
     real(c_double), allocatable, target, save :: t_xb(:), t_pts(:), t_uv(:), t_sig(:), t_d(:), t_sar(:)
@@ -109,6 +109,13 @@ contains
         g_mover = mover
         g_sweeps = sweeps
         g_max_batch = max_batch
+        g_sig = -1
+    end subroutine
+
+    ! arithmetic of the Gibbs mover: 0 = FP64 (default), 1 = mixed (FP32 matrix products and draws)
+    subroutine smcf_fortran_precision(precision)
+        integer, intent(in) :: precision
+        g_precision = precision
         g_sig = -1
     end subroutine
 
@@ -183,6 +190,7 @@ contains
         d%nparticle_slip = nparticle_slip
         d%mover = g_mover
         d%gibbs_sweeps = g_sweeps
+        d%precision = g_precision
         call smcf_check(smcf_problem_create(g_ctx, d, int(g_max_batch, c_int), g_pb), "smcf_problem_create")
         g_sig = sig
         g_sum = s

@@ -197,3 +197,39 @@ def test_trunc_normal_prec_regimes(a, b):
         assert _tn_fraction(a, b, z) == pytest.approx(u, rel=1e-6, abs=1e-12), (a, b, u, z)
     xs = [O.trunc_normal_prec(0.0, 1.0, a, b, u) for u in (0.1, 0.5, 0.9)]
     assert xs[0] <= xs[1] <= xs[2]
+
+
+def test_ndtri7_matches_scipy():
+    """AS 241 PPND7 (single precision) against scipy's ndtri over the lower half, central region and both
+    tail branches (r <= 5 and r > 5): relative error below 1e-6 (PPND7 is accurate to about 1e-7; the rest
+    is single-precision evaluation).  A wrong coefficient shows up as an error of 1e-3 or more."""
+    ps = np.concatenate([np.logspace(-37, -1, 400), np.linspace(0.075, 0.5, 200)]).astype(np.float32)
+    for p in ps:
+        z = O.ndtri7_lower(float(p))
+        ref = sp.ndtri(float(p))
+        assert z <= 0.0
+        assert abs(z - ref) <= 1e-6 * max(1.0, abs(ref)), (p, z, ref)
+
+
+@pytest.mark.parametrize("a,b", [(-7.5, 0.5), (-3.0, 3.0), (-0.5, 20.0), (0.5, 7.9), (2.0, 2.5), (-7.9, -7.0),
+                                 (-30.0, 1.0), (6.0, 30.0), (-1e3, 1e3)])
+def test_trunc_normal_prec_f32_inverts_cdf(a, b):
+    """The single-precision Gibbs draw returns the quantile of the FP64 draw (itself pinned against the
+    CDF above) for the same uniform, to single-precision resolution of z, in every regime it accepts
+    (both bounds inside 8 sigma on the far side of the mean)."""
+    mu, s = 1.0, 2.0
+    lo, hi = mu + a / s, mu + b / s
+    for u in (2.0 ** -24, 1e-6, 0.01, 0.3, 0.5, 0.7, 0.99, 1 - 1e-6, 1 - 2.0 ** -24):
+        uf = float(np.float32(u))
+        x = O.trunc_normal_prec_f32(mu, s, lo, hi, uf)
+        assert lo <= x <= hi
+        z = (x - mu) * s
+        z64 = (O.trunc_normal_prec(mu, s, lo, hi, uf) - mu) * s
+        assert abs(z - z64) <= 1e-6 * max(1.0, abs(z64)), (a, b, u, z, z64)
+
+
+def test_trunc_normal_prec_f32_declines_far_tails():
+    """Bounds beyond 8 sigma on the far side of the mean are left to the FP64 draw (NaN)."""
+    assert math.isnan(O.trunc_normal_prec_f32(0.0, 1.0, 8.5, 20.0, 0.5))
+    assert math.isnan(O.trunc_normal_prec_f32(0.0, 1.0, -20.0, -8.5, 0.5))
+    assert not math.isnan(O.trunc_normal_prec_f32(0.0, 1.0, 7.5, 20.0, 0.5))
