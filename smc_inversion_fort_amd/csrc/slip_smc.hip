@@ -411,15 +411,60 @@ __device__ __forceinline__ void mfma_panel(const double* __restrict__ abase, uin
     if (k < nsteps - full) prod(k);
 }
 
+// mfma_panel for MT 16-row tiles (A(16m + r, 4st + k) at abase[ao[m] + st lda4]); same pipeline.
+template <int MT, int NB>
+__device__ __forceinline__ void mfma_panel_t(const double* __restrict__ abase, const uint32_t (&ao)[MT], uint32_t lda4,
+                                           const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
+                                           int nsteps, dbl4 (&acc)[MT][4]) {
+  if (nsteps <= 0) return;
+  double x[NB][MT], y[NB][4];
+  // loads of step st (clamped to the last step: a group's prefetch past the end re-reads it, unused)
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nsteps - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) x[k][m] = abase[ao[m] + q * lda4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k][m], y[k][n], acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nsteps - nsteps % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nsteps - full) prod(k);
+}
+
 // Likelihood of slip_calc_likelihood (smc_slip.f90:56-104) in Gram form,
 //   L(s) = (s^T H s - 2 b^T s + dsd) / 2 + c,
-// for the 64 particles p0 .. p0+63 of one wavefront.  Y = H S is formed in 32-row blocks on the matrix
+// for the 64 particles p0 .. p0+63 of one wavefront.  Y = H S is formed in 64-row blocks on the matrix
 // cores (A(r, k) = H(i0 + r, j0 + k) read column-major, B(k, c) = s_{j0+k}(p0 + c)); the epilogue of each
 // block accumulates s^T (Y - 2 b) over its rows (lane l holds D(4t + l/16, l%16) in register t), and the
 // four row groups of a particle are summed with lane shuffles.  Y never leaves the registers.
+#ifndef SMCF_LIKE_MT
+#define SMCF_LIKE_MT 4
+#define SMCF_LIKE_NB 3
+#define SMCF_LIKE_OCC 2
+#endif
+constexpr int kLikeMT = SMCF_LIKE_MT;  // 16-row tiles per block of H (64 rows: S is streamed nd / 64 times)
+
 __device__ __forceinline__ void wave_like(int nd, int P, const double* __restrict__ H, const double* __restrict__ S,
                                           const double* __restrict__ bv, double dsd, double c, int p0, int lane,
                                           double* __restrict__ Lout) {
+  constexpr int MT = kLikeMT, KB = 16 * MT;
   int fi = lane % 16, fk = lane / 16;
   int pcol[4];
 #pragma unroll
@@ -427,34 +472,35 @@ __device__ __forceinline__ void wave_like(int nd, int P, const double* __restric
   uint32_t bo[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
-  int nd4 = nd & ~3, nblk = (nd + 31) / 32;
+  int nd4 = nd & ~3, nblk = (nd + KB - 1) / KB;
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  const double* sb = S + (size_t)fk * P;
-  const double* ha = H + (size_t)fk * nd;
   for (int rb = 0; rb < nblk; ++rb) {
-    int i0 = rb * 32;
-    int r0 = i0 + fi, r1 = i0 + 16 + fi;
-    bool v0 = r0 < nd, v1 = r1 < nd;
-    dbl4 acc[2][4];
+    int i0 = rb * KB;
+    dbl4 acc[MT][4];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int n = 0; n < 4; ++n) acc[m][n] = dbl4{0.0, 0.0, 0.0, 0.0};
     // rows past nd read row nd - 1; the epilogue skips them
-    mfma_panel<kPanelBuffers>(H, (uint32_t)(fk * nd + min(r0, nd - 1)), (uint32_t)(fk * nd + min(r1, nd - 1)),
-                              (uint32_t)(4 * nd), S, bo, (uint32_t)(4 * P), nd4 / 4, acc);
+    uint32_t ao[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * nd + min(i0 + 16 * m + fi, nd - 1));
+    mfma_panel_t<MT, SMCF_LIKE_NB>(H, ao, (uint32_t)(4 * nd), S, bo, (uint32_t)(4 * P), nd4 / 4, acc);
     if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
       bool in = nd4 + fk < nd;
-      double a0 = (in && v0) ? ha[(size_t)nd4 * nd + r0] : 0.0, a1 = (in && v1) ? ha[(size_t)nd4 * nd + r1] : 0.0;
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
-        double bn = in ? sb[(size_t)nd4 * P + pcol[n]] : 0.0;
-        acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bn, acc[0][n], 0, 0, 0);
-        acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bn, acc[1][n], 0, 0, 0);
+        double bn = in ? S[(size_t)(nd4 + fk) * P + pcol[n]] : 0.0;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          int r = i0 + 16 * m + fi;
+          double a = (in && r < nd) ? H[(size_t)(nd4 + fk) * nd + r] : 0.0;
+          acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bn, acc[m][n], 0, 0, 0);
+        }
       }
     }
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         int row = i0 + 16 * m + 4 * t + fk;
@@ -477,13 +523,11 @@ __device__ __forceinline__ void wave_like(int nd, int P, const double* __restric
 
 // one workgroup of 4 wavefronts per 256 particles of an instance; all chunks of an instance on one
 // XCD (workgroup w runs on XCD w % 8), so that its H is read into one L2
-__global__ void __launch_bounds__(256, 4) k_like_mfma(int nd, int P, int B, const double* __restrict__ Hall,
-                                                      const double* __restrict__ bvec, const double* __restrict__ dsd,
-                                                      const double* __restrict__ scal, const double* __restrict__ S0,
-                                                      const double* __restrict__ S1, const int* __restrict__ par,
-                                                      const int* __restrict__ active, int use_active,
-                                                      const int* __restrict__ sel, double* __restrict__ L0,
-                                                      double* __restrict__ L1) {
+__global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
+    k_like_mfma(int nd, int P, int B, const double* __restrict__ Hall, const double* __restrict__ bvec,
+                const double* __restrict__ dsd, const double* __restrict__ scal, const double* __restrict__ S0,
+                const double* __restrict__ S1, const int* __restrict__ par, const int* __restrict__ active,
+                int use_active, const int* __restrict__ sel, double* __restrict__ L0, double* __restrict__ L1) {
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % 8, r = w / 8;
   int bi = (r / chunks) * 8 + xcd, chunk = r % chunks;
@@ -657,16 +701,19 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
 //    r + 16k, B(k, c) at lane c + 16k; lane l holds D(4 (l/16) + t, l % 16) in register t (checked on
 //    the device before first use; the FP64 instruction orders its rows differently);
 //  * q of each tile converted to double in LDS; the conditional mean, the in-tile corrections and the
-//    old and new values in FP64; the draw in single precision (trunc_normal_prec_f32, inlined), with the
-//    FP64 draw where that declines (bounds beyond 8 sigma on the far side of the mean);
-//  * occupancy 2 (256 registers, 66 KB of LDS per workgroup): the tile's current values are loaded
-//    together before its draws into LDS beside q, and the new values stored together after them, so no
-//    draw waits on a load or a store;
+//    old and new values in FP64; the draw in single precision (trunc_normal_prec_f32), with the FP64
+//    draw where that declines (bounds beyond 8 sigma on the far side of the mean);
 //  * the corrections of the later tiles by a tile's deltas on the FP32 matrix cores;
 //  * every redrawn value stored to S and to its FP32 copy.
 // ---------------------------------------------------------------------------
-constexpr int kMixedMT = 4;  // 16-row tiles per block
-constexpr int kMixedNB = 3;  // register buffers of the matrix-phase pipeline
+#ifndef SMCF_MIXED_MT
+#define SMCF_MIXED_MT 3
+#define SMCF_MIXED_NB 3
+#define SMCF_MIXED_OCC 3
+#endif
+constexpr int kMixedMT = SMCF_MIXED_MT;    // 16-row tiles per block
+constexpr int kMixedNB = SMCF_MIXED_NB;    // register buffers of the matrix-phase pipeline
+constexpr int kMixedOcc = SMCF_MIXED_OCC;  // waves per SIMD the register budget is sized for
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -707,22 +754,19 @@ __device__ __forceinline__ void mfma_panel_f32(const float* __restrict__ abase, 
     if (k < nsteps - full) prod(k);
 }
 
-// one coordinate redraw in single precision, inlined (a call would drain every outstanding memory
-// operation first); NaN where trunc_normal_prec_f32 declines
-__device__ __forceinline__ double gibbs_draw_f32(double mu, double s, double is, double max_slip, float u) {
+// one coordinate redraw in single precision; NaN where trunc_normal_prec_f32 declines
+__device__ __noinline__ double gibbs_draw_f32(double mu, double s, double is, double max_slip, float u) {
   return trunc_normal_prec_f32(mu, s, is, 0.0, max_slip, u);
 }
 
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, kMixedOcc)
     k_gibbs_mixed(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
                   const float* __restrict__ Q32all, double* __restrict__ S0, double* __restrict__ S1,
                   float* __restrict__ T0, float* __restrict__ T1, const double* __restrict__ cdall,
                   const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
                   const int* __restrict__ par) {
   constexpr int MT = kMixedMT, KB = 16 * MT;
-  // per wavefront: q of the tile being drawn (rows 0..15) and the current values of its coordinates
-  // (rows 16..31), lane = particle
-  __shared__ double T[4][32][65];
+  __shared__ double T[4][16][65];
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
   int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
@@ -786,20 +830,10 @@ __global__ void __launch_bounds__(256, 2)
         for (int n = 0; n < 4; ++n)
 #pragma unroll
           for (int t = 0; t < 4; ++t) Tw[4 * fk + t][16 * n + fi] = (double)acc[m][n][t];
-        // the tile's current values of the lane's particle, requested together before the draws (coalesced,
-        // in the accumulator layout) and kept in Tw[16 + k]; the new values are stored after the tile
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            int row = i0 + 16 * m + 4 * fk + t;
-            Tw[16 + 4 * fk + t][16 * n + fi] = row < nd ? S[(size_t)row * P + pcol[n]] : 0.0;
-          }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint64_t wodd = 0;
-#pragma unroll 1
         for (int kk = 0; kk < 16; ++kk) {
           int i = i0 + 16 * m + kk;
           if (i >= nd) {
@@ -816,12 +850,15 @@ __global__ void __launch_bounds__(256, 2)
           // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
           const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
           double qi = Tw[kk][lane];
-          double so = Tw[16 + kk][lane];
+          double so = S[(size_t)i * P + pc];
           double mu = so - (qi - cd[i]) * cd[nd + i];
           double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
           if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open(wc));
           double dlt = sn - so;
-          Tw[16 + kk][lane] = sn;
+          if (valid) {
+            S[(size_t)i * P + p] = sn;
+            S32[(size_t)i * P + p] = (float)sn;
+          }
 #pragma unroll
           for (int k2 = 0; k2 < 16; ++k2) Tw[k2][lane] = fma(k2 > kk ? qrow[k2] : 0.0, dlt, Tw[k2][lane]);
           Tw[kk][lane] = dlt;
@@ -829,18 +866,6 @@ __global__ void __launch_bounds__(256, 2)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // the new values, coalesced, to S and its FP32 copy
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            int row = i0 + 16 * m + 4 * fk + t, pp = pw0 + 16 * n + fi;
-            if (row < nd && pp < P) {
-              double v = Tw[16 + 4 * fk + t][16 * n + fi];
-              S[(size_t)row * P + pp] = v;
-              S32[(size_t)row * P + pp] = (float)v;
-            }
-          }
         // corrections of the later tiles by this tile's deltas (in Tw[k][p]), FP32 matrix cores:
         //   acc[m2](r, p) += sum_k Q(i0 + 16m + k, i0 + 16m2 + r) delta_k(p)
 #pragma unroll
