@@ -64,7 +64,8 @@ typedef struct {
   const double* obs_unitvec;  /* 3 x nobs */
   const double* obs_sigma;    /* nobs */
   const double* dvec;         /* nobs */
-  const double* sar_sinv;     /* inverse SAR covariance blocks; only nsar_total == 0 is supported */
+  const double* sar_sinv;     /* inverse SAR covariance per path (calc_sigma_sar_mat, init.f90:82-119), each block
+                                 column-major n_p x n_p, concatenated in path order; may be NULL when nsar_total == 0 */
   double max_slip;
   int nparticle_slip;
   int mover;                  /* SMCF_MOVER_* */
@@ -135,6 +136,11 @@ int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, 
  * gamma_trace: >= max_stages entries (may be NULL).  Inner evaluations of one wave are issued as
  * smcf_fault_calc_likelihood_batch calls of at most max_batch geometries (any value >= 1; each call
  * streams through the problem's slots, so a whole wave per call keeps the device fullest). */
+/* Per-stage dumps of the outer SMC (smc_fault.f90:923-936, 1054-1067): with a directory set, smcf_fault_smc_exec
+ * and smcf_fault_smc_exec_gsmc write <output_dir><iter>.csv for iter = 0 (initial particles) and every stage, one
+ * line per particle: its coordinates and its -log Z, each as Fortran F20.5.  output_dir is used as given (the
+ * reference appends the stage number directly, so it normally ends in '/'); NULL or "" stops the dumps. */
+int smcf_fault_smc_set_output_dir(smcf_problem* pb, const char* output_dir);
 int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
                         double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace);
 /* Outer SMC with the Gaussian proposal q = N(qmean, qcov) (the gsmc branch of fault_smc_exec,

@@ -93,6 +93,8 @@ def _declare(L):
     L.orc_trunc_normal.argtypes = [C.c_double] * 5
     L.orc_trunc_normal_prec.argtypes = [C.c_double] * 5
     L.orc_trunc_normal_prec.restype = C.c_double
+    L.orc_calc_sigma_sar_mat.argtypes = [C.c_int, I, C.c_int, D, D, D]
+    L.orc_calc_sigma_sar_mat.restype = C.c_int
     L.orc_trunc_normal_prec_f32.argtypes = [C.c_double] * 4 + [C.c_float]
     L.orc_trunc_normal_prec_f32.restype = C.c_double
     L.orc_ndtri7_lower.argtypes = [C.c_float]
@@ -318,6 +320,18 @@ def grad(s, nnode, ndof, li, lv, lti, ltv, gamma, alpha2, b, H):
     lib().orc_slip_calc_grad(dp(g), dp(s), s.shape[0], nnode, ndof, ip(i32(li)), dp(f64(lv)), ip(i32(lti)),
                              dp(f64(ltv)), gamma, dp(f64(alpha2)), dp(f64(b)), dp(f64(H)), dp(lsv))
     return g
+
+
+def calc_sigma_sar_mat(nsar_index, obs_points, obs_sigma):
+    """Inverse InSAR covariance per path (init.f90:82-119), column-major blocks concatenated."""
+    nsi = i32(nsar_index)
+    sizes = np.diff(nsi)
+    out = np.zeros(max(1, int((sizes.astype(np.int64) ** 2).sum())))
+    pts = f64(obs_points)
+    rc = lib().orc_calc_sigma_sar_mat(len(nsi) - 1, ip(nsi), pts.shape[1], dp(pts), dp(f64(obs_sigma)), dp(out))
+    if rc != 0:
+        raise ValueError("calc_sigma_sar_mat: sigma index past nobs or singular covariance")
+    return out
 
 
 def gsd_gsg(gmat, dvec, obs_sigma, sigma_sar2, sigma_gnss2, nsar_index=(1, 1), sar_sinv=None):

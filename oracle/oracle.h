@@ -101,6 +101,9 @@ void orc_slip_calc_cov_particles(const double* particles, const double* w, const
 void orc_resample_quad(int P, const double* w, double u_hi, double u_lo, int* assigned);
 double orc_ndtri(double p);
 double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u);
+/* InSAR inverse covariance blocks of calc_sigma_sar_mat (init.f90:82-119), column-major, concatenated */
+int orc_calc_sigma_sar_mat(int npath, const int* nsar_index, int nobs, const double* obs_points,
+                           const double* obs_sigma, double* sinv_out);
 /* single-precision draw of the mixed-precision Gibbs mover (NaN where it declines) and its PPND7 */
 double orc_trunc_normal_prec_f32(double mu, double s, double lo, double hi, float u);
 float orc_ndtri7_lower(float p);

@@ -48,6 +48,69 @@ static void draw_quad(uint64_t key, uint32_t site, uint32_t stage, double* hi, d
     *lo = (double)(w[1] >> 11) * 0x1.0p-106;
 }
 
+/* ---------------- InSAR covariance, init.f90:82-119 ----------------
+ * Per path p (rows a .. a+n-1, a = nsar_index(p) - 1): C(i, j) = s_i s_j exp(-dist_ij / 10) with dist the
+ * distance of the two points, and s_i = obs_sigma(nsar_index(p) + i) for 1-based i -- one row past the
+ * observation's own (init.f90:107-108), so the last row of a path takes the next row's sigma.  The
+ * inverse, column-major, is written to sinv_out block after block; it is formed by LU with partial
+ * pivoting and one forward/back substitution per unit column (the factors of dgetrf; dgetri applies the
+ * same factors).  Returns 0, or -1 when a path's last sigma index lies past nobs (the reference reads
+ * past the array there) or a pivot is zero. */
+int orc_calc_sigma_sar_mat(int npath, const int* nsar_index, int nobs, const double* obs_points,
+                           const double* obs_sigma, double* sinv_out) {
+    size_t off = 0;
+    for (int ip = 0; ip < npath; ++ip) {
+        int a = nsar_index[ip] - 1, n = nsar_index[ip + 1] - nsar_index[ip];
+        if (n <= 0) continue;
+        if (a + n >= nobs) return -1;
+        double* A = (double*)malloc(sizeof(double) * n * n);
+        int* piv = (int*)malloc(sizeof(int) * n);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                double dx = obs_points[2 * (a + i)] - obs_points[2 * (a + j)];
+                double dy = obs_points[2 * (a + i) + 1] - obs_points[2 * (a + j) + 1];
+                double dist = sqrt(dx * dx + dy * dy);
+                A[(size_t)j * n + i] = obs_sigma[a + 1 + i] * obs_sigma[a + 1 + j] * exp(-dist / 1e1);
+            }
+        /* LU, row pivoting: P A = L U, unit L below the diagonal */
+        for (int k = 0; k < n; ++k) {
+            int pr = k;
+            double best = fabs(A[(size_t)k * n + k]);
+            for (int i = k + 1; i < n; ++i)
+                if (fabs(A[(size_t)k * n + i]) > best) { best = fabs(A[(size_t)k * n + i]); pr = i; }
+            piv[k] = pr;
+            if (best == 0.0) { free(A); free(piv); return -1; }
+            if (pr != k)
+                for (int j = 0; j < n; ++j) {
+                    double t = A[(size_t)j * n + k]; A[(size_t)j * n + k] = A[(size_t)j * n + pr]; A[(size_t)j * n + pr] = t;
+                }
+            double rp = 1.0 / A[(size_t)k * n + k];
+            for (int i = k + 1; i < n; ++i) A[(size_t)k * n + i] *= rp;
+            for (int j = k + 1; j < n; ++j) {
+                double u = A[(size_t)j * n + k];
+                if (u != 0.0)
+                    for (int i = k + 1; i < n; ++i) A[(size_t)j * n + i] -= A[(size_t)k * n + i] * u;
+            }
+        }
+        double* X = sinv_out + off;
+        for (int c = 0; c < n; ++c) {
+            double* x = X + (size_t)c * n;
+            for (int i = 0; i < n; ++i) x[i] = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < n; ++k)
+                if (piv[k] != k) { double t = x[k]; x[k] = x[piv[k]]; x[piv[k]] = t; }
+            for (int j = 0; j < n; ++j)           /* L y = P e_c */
+                for (int i = j + 1; i < n; ++i) x[i] -= A[(size_t)j * n + i] * x[j];
+            for (int j = n - 1; j >= 0; --j) {    /* U x = y */
+                x[j] /= A[(size_t)j * n + j];
+                for (int i = 0; i < j; ++i) x[i] -= A[(size_t)j * n + i] * x[j];
+            }
+        }
+        free(A); free(piv);
+        off += (size_t)n * n;
+    }
+    return 0;
+}
+
 /* ---------------- Gram quantities, smc_slip.f90:1347-1443 ---------------- */
 void orc_slip_calc_gsd_gsg(const double* gmat, const double* dvec, int nobs, int ndim, int npath,
                            const int* nsar_index, int nsar_total, const double* sar_sinv,

@@ -54,7 +54,7 @@ EXPORTS = [
     "smcf_problem_create", "smcf_problem_destroy", "smcf_problem_dims", "smcf_dc3d0",
     "smcf_calc_greens_func", "smcf_laplacian", "smcf_slip_calc_gsd_gsg", "smcf_slip_smc_exec",
     "smcf_fault_calc_likelihood_batch", "smcf_find_next_gamma", "smcf_normalize_resample",
-    "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
+    "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
 ]
 
 
@@ -90,6 +90,7 @@ def _declare(L):
     L.smcf_normalize_resample.argtypes = [C.c_void_p, C.c_int, _D, C.c_double, C.c_double, _D, _I]
     L.smcf_trunc_normal.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
     L.smcf_trunc_normal_f32.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, C.POINTER(C.c_float), _D]
+    L.smcf_fault_smc_set_output_dir.argtypes = [C.c_void_p, C.c_char_p]
     L.smcf_philox.argtypes = [C.c_void_p, C.c_int, _U32, _U32, _U32]
     L.smcf_fault_smc_exec.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _I, _D, _D]
     L.smcf_fault_smc_exec_gsmc.argtypes = [C.c_void_p, _D, C.c_int, C.c_uint64, C.c_int, C.c_int, _D, _D, _D,
@@ -225,7 +226,8 @@ class Problem:
             nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
             nsar_index=_i32(obs.get("nsar_index", [1, 1 + int(obs.get("nsar_total", 0))])),
             obs_points=_f64(obs["obs_points"]), obs_unitvec=_f64(obs["obs_unitvec"]),
-            obs_sigma=_f64(obs["obs_sigma"]), dvec=_f64(obs["dvec"]), sar=_f64(np.zeros(1)))
+            obs_sigma=_f64(obs["obs_sigma"]), dvec=_f64(obs["dvec"]),
+            sar=_f64(obs["sar_sinv"] if obs.get("sar_sinv") is not None else np.zeros(1)))
         k = self._keep
         nobs = k["dvec"].shape[0]
         self.desc = ProblemDesc(len(k["nxi"]), _ip(k["nxi"]), _ip(k["neta"]), xmin, xmax, zmin, int(bool(fix_xbend)),
@@ -303,6 +305,10 @@ class Problem:
                                         C.byref(ns), C.byref(nl), _dp(gt)))
         return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=ns.value, neglog=nl.value,
                     gamma=gt[:ns.value].copy())
+
+    def set_output_dir(self, output_dir):
+        """Per-stage dumps <output_dir><iter>.csv of fault_smc_exec (None stops them)."""
+        _check(lib().smcf_fault_smc_set_output_dir(self.h, output_dir.encode() if output_dir else None))
 
     def fault_smc_exec(self, range_, N, seed, max_stages=100, gsmc=None, call_batch=None):
         """Outer SMC over the geometry (smc_fault.f90:785-1075, one rank).

@@ -132,7 +132,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
   CHECK_LAUNCH(launch_laplacian(m, bt, n, s, sel));
   {
     ProfScope ps(ctx, "gram", s);
-    CHECK_LAUNCH(launch_gram(m, bt, n, s, sel));
+    CHECK_LAUNCH(launch_gram(P, m, bt, n, s, sel));
   }
   {
     ProfScope ps(ctx, "init_chain", s);
@@ -160,10 +160,6 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
   if (N < 1) {
     set_error("no geometries");
     return SMCF_EINVAL;
-  }
-  if (m.nsar > 0) {
-    set_error("InSAR rows (dense per-path covariance) are not supported by the device Gram yet");
-    return SMCF_EUNSUPPORTED;
   }
   bool hmc = P.mover == SMCF_MOVER_HMC;
   const int nf = m.ndim_fault;
@@ -479,6 +475,40 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= upload(P, &m.unitvec, uv);
   rc |= upload(P, &m.obs_sigma, os);
   rc |= upload(P, &m.dvec, dv);
+  // InSAR paths: rows nsar_index(p) .. nsar_index(p+1)-1 (1-based) with the inverse covariance blocks the
+  // caller computed (calc_sigma_sar_mat, init.f90:82-119), column-major, concatenated in path order
+  m.npath = 0;
+  if (m.nsar > 0) {
+    if (d->npath < 1 || !d->nsar_index || !d->sar_sinv) {
+      set_error("InSAR rows need npath >= 1, nsar_index and sar_sinv");
+      return SMCF_EINVAL;
+    }
+    std::vector<int> r0v, nv, rowp(m.nsar);
+    std::vector<long> offv;
+    long off = 0;
+    for (int p = 0; p < d->npath; ++p) {
+      int a = d->nsar_index[p] - 1, e = d->nsar_index[p + 1] - 1;
+      if (a != (p == 0 ? 0 : r0v.back() + nv.back()) || e < a || e > m.nsar) {
+        set_error("nsar_index must split the SAR rows 1..nsar_total into consecutive paths");
+        return SMCF_EINVAL;
+      }
+      r0v.push_back(a); nv.push_back(e - a); offv.push_back(off);
+      for (int r = a; r < e; ++r) rowp[r] = p;
+      off += (long)(e - a) * (e - a);
+    }
+    if (r0v.back() + nv.back() != m.nsar) {
+      set_error("nsar_index(npath+1) must be nsar_total + 1");
+      return SMCF_EINVAL;
+    }
+    std::vector<double> sinv(d->sar_sinv, d->sar_sinv + off);
+    m.npath = d->npath;
+    rc |= upload(P, &m.sinv, sinv);
+    rc |= upload(P, &m.sar_r0, r0v);
+    rc |= upload(P, &m.sar_n, nv);
+    rc |= upload(P, &m.sar_off, offv);
+    rc |= upload(P, &m.row_path, rowp);
+    P.sar_r0 = r0v; P.sar_n = nv; P.sar_off = offv;
+  }
   if (rc) return rc;
   // batch buffers
   BatchDev& bt = P.bt;
@@ -498,6 +528,9 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.ltv, (size_t)9 * 2 * ndof * C);
   rc |= dalloc(P, &bt.llc_idx, (size_t)25 * nd * C);
   rc |= dalloc(P, &bt.llc_val, (size_t)25 * nd * C);
+  bt.Gw = nullptr;
+  if (m.nsar > 0) rc |= dalloc(P, &bt.Gw, (size_t)m.nsar * nd * C);
+  rc |= dalloc(P, &bt.sdv, (size_t)m.nobs * C);
   rc |= dalloc(P, &bt.H, nd * nd * C);
   rc |= dalloc(P, &bt.bvec, nd * C);
   rc |= dalloc(P, &bt.dsd, C);
@@ -580,6 +613,7 @@ int smcf_problem_destroy(smcf_problem* pb) {
   (void)hipSetDevice(pb->p.ctx->device);
   (void)hipStreamSynchronize(pb->p.ctx->stream);
   for (void* p : pb->p.allocs) (void)hipFree(p);
+  (void)smcf_fault_smc_set_output_dir(pb, nullptr);  // a later problem at the same address starts without dumps
   delete pb;
   return SMCF_OK;
 }
@@ -665,12 +699,11 @@ int smcf_slip_calc_gsd_gsg(smcf_problem* pb, int B, const double* theta, double*
   Problem& P = pb->p;
   MeshDev& m = P.m;
   if (B < 1 || B > P.bt.cap) { set_error("batch size outside capacity"); return SMCF_EINVAL; }
-  if (m.nsar > 0) { set_error("InSAR Gram not supported on the device yet"); return SMCF_EUNSUPPORTED; }
   hipStream_t s = P.ctx->stream;
   SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * m.ndim_fault * B, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_geometry(m, P.bt, B, s));
   CHECK_LAUNCH(launch_greens(m, P.bt, B, s));
-  CHECK_LAUNCH(launch_gram(m, P.bt, B, s));
+  CHECK_LAUNCH(launch_gram(P, m, P.bt, B, s));
   size_t nd = m.ndim;
   if (gsgmat) SMCF_HIP(hipMemcpyAsync(gsgmat, P.bt.H, sizeof(double) * nd * nd * B, hipMemcpyDeviceToHost, s));
   if (gsdvec) SMCF_HIP(hipMemcpyAsync(gsdvec, P.bt.bvec, sizeof(double) * nd * B, hipMemcpyDeviceToHost, s));

@@ -2,8 +2,11 @@
 //
 //  k_scal      theta -> alpha^2, sigma^2 and the likelihood constant (init.f90:175-196,
 //              smc_fault.f90:148-173, smc_slip.f90:84-87)
-//  k_gram      H = G^T diag(1/(sigma_i^2 sigma_gnss^2)) G          (smc_slip.f90:1401-1409)
-//  k_gram_vec  b = G^T sd, dsd = d^T sd                              (smc_slip.f90:1371-1388)
+//  k_sar_weight  SAR rows of Sigma^-1 G: (1/sigma_sar^2) Sinv_p^T G_p per path (smc_slip.f90:1390-1398)
+//  k_sd        sd = Sigma^-1 d (SAR: column-oriented Sinv d; GNSS: d / (sigma_gnss^2 sigma_i^2))
+//                                                                    (smc_slip.f90:1365-1377)
+//  k_gram      H = (Sigma^-1 G)^T G                                  (smc_slip.f90:1390-1409)
+//  k_gram_vec  b = G^T sd, dsd = d^T sd                              (smc_slip.f90:1379-1388)
 //  k_prior     ELL prior per particle                                (smc_slip.f90:106-137)
 //  k_form_q    Q = gamma H + LL / alpha^2 for the Gibbs mover
 // The two GEMMs use 64x64 LDS tiles with a 4x4 register block per thread.
@@ -39,16 +42,96 @@ __global__ void k_scal(MeshDev m, const double* __restrict__ theta, double* __re
 
 constexpr int TM = 64, TN = 64, TK = 16;
 
-// H(i, j) = sum_k (G(k, i) / c_k) G(k, j), c_k = obs_sigma_k^2 * sigma_gnss^2 (GNSS rows)
+// SAR rows of the weighted design matrix for path p (rows r0 .. r0+n-1):
+//   Gw(i, r0 + c) = (1 / sigma_sar^2) sum_l G(r0 + l, i) Sinv_p(l, c)
+// (the reference's gtsmat columns of the path, dgemm('t', 'n'), smc_slip.f90:1390-1398), as a tiled GEMM over
+// (i, c) with K = l; Gw is stored like G, column-major over the nsar SAR rows.
+__global__ void __launch_bounds__(256) k_sar_weight(MeshDev m, const double* __restrict__ Gall,
+                                                    const double* __restrict__ scal, double* __restrict__ Gwall,
+                                                    int r0, int n, long off, const int* __restrict__ sel) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
+  int i0 = blockIdx.x * TM, c0 = blockIdx.y * TN;
+  int nobs = m.nobs, nd = m.ndim, nsar = m.nsar;
+  const double* G = Gall + (size_t)b * nobs * nd;
+  const double* A = m.sinv + off;
+  double rs2 = 1.0 / scal[8 * (size_t)b + 6];
+  int t = threadIdx.x, ty = t / 16, tx = t % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int k0 = 0; k0 < n; k0 += TK) {
+#pragma unroll
+    for (int rep = 0; rep < 4; ++rep) {
+      int kk = t % 16, ii = t / 16 + rep * 16;
+      int l = k0 + kk;
+      int i = i0 + ii, c = c0 + ii;
+      As[kk][ii] = (l < n && i < nd) ? G[(size_t)i * nobs + r0 + l] : 0.0;
+      Bs[kk][ii] = (l < n && c < n) ? A[(size_t)c * n + l] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      double a4[4], b4[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b4[c] = Bs[kk][tx * 4 + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] += a4[a] * b4[c];
+    }
+    __syncthreads();
+  }
+  double* Gw = Gwall + (size_t)b * nsar * nd;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int i = i0 + ty * 4 + a, cc = c0 + tx * 4 + c;
+      if (i < nd && cc < n) Gw[(size_t)i * nsar + r0 + cc] = rs2 * acc[a][c];
+    }
+}
+
+// sd = Sigma^-1 d per observation row: SAR rows of path p as dgemv('n') with alpha = 1/sigma_sar^2
+// accumulated column by column (sd(r0 + i) += (alpha d(r0 + j)) Sinv_p(i, j), j ascending); GNSS rows
+// d / (sigma_gnss^2 sigma_i^2) (smc_slip.f90:1365-1377)
+__global__ void k_sd(MeshDev m, const double* __restrict__ scal, double* __restrict__ sdall,
+                     const int* __restrict__ sel) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
+  if (k >= m.nobs) return;
+  double v;
+  if (k < m.nsar) {
+    int p = m.row_path[k];
+    int r0 = m.sar_r0[p], n = m.sar_n[p], i = k - r0;
+    const double* A = m.sinv + m.sar_off[p];
+    double alpha = 1.0 / scal[8 * (size_t)b + 6];
+    v = 0.0;
+    for (int j = 0; j < n; ++j) v += (alpha * m.dvec[r0 + j]) * A[(size_t)j * n + i];
+  } else {
+    double sk = m.obs_sigma[k];
+    v = m.dvec[k] / (scal[8 * (size_t)b + 5] * (sk * sk));
+  }
+  sdall[(size_t)b * m.nobs + k] = v;
+}
+
+// H(i, j) = sum_k W(k, i) G(k, j) with W = Sigma^-1 G: SAR rows from Gw, GNSS rows G(k, i) / c_k,
+// c_k = obs_sigma_k^2 * sigma_gnss^2
 __global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restrict__ Gall,
-                                              const double* __restrict__ scal, double* __restrict__ Hall,
-                                              const int* __restrict__ sel) {
+                                              const double* __restrict__ Gwall, const double* __restrict__ scal,
+                                              double* __restrict__ Hall, const int* __restrict__ sel) {
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   int b = sel ? sel[blockIdx.z] : (int)blockIdx.z;
   int i0 = blockIdx.x * TM, j0 = blockIdx.y * TN;
   int nobs = m.nobs, nd = m.ndim;
   const double* G = Gall + (size_t)b * nobs * nd;
+  const double* Gw = Gwall ? Gwall + (size_t)b * m.nsar * nd : nullptr;
   double sg2 = scal[8 * (size_t)b + 5];
   int t = threadIdx.x, ty = t / 16, tx = t % 16;
   double acc[4][4];
@@ -64,9 +147,13 @@ __global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restric
       int i = i0 + ii, j = j0 + ii;
       double av = 0.0, bv = 0.0;
       if (k < nobs) {
-        double sk = m.obs_sigma[k];
-        double ck = (sk * sk) * sg2;
-        if (i < nd) av = G[(size_t)i * nobs + k] / ck;
+        if (k < m.nsar) {
+          if (i < nd) av = Gw[(size_t)i * m.nsar + k];
+        } else {
+          double sk = m.obs_sigma[k];
+          double ck = (sk * sk) * sg2;
+          if (i < nd) av = G[(size_t)i * nobs + k] / ck;
+        }
         if (j < nd) bv = G[(size_t)j * nobs + k];
       }
       As[kk][ii] = av;
@@ -97,37 +184,37 @@ __global__ void __launch_bounds__(256) k_gram(MeshDev m, const double* __restric
     }
 }
 
-__global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const double* __restrict__ scal,
+__global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const double* __restrict__ sdall,
                            double* __restrict__ bvec, double* __restrict__ dsd, int B, const int* __restrict__ sel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   int nobs = m.nobs, nd = m.ndim;
-  double sg2 = scal[8 * (size_t)b + 5];
+  const double* sd = sdall + (size_t)b * nobs;
   if (i < nd) {
     const double* g = Gall + ((size_t)b * nd + i) * nobs;
     double s = 0.0;
-    for (int k = 0; k < nobs; ++k) {
-      double sk = m.obs_sigma[k];
-      double sdk = m.dvec[k] / (sg2 * (sk * sk));
-      s = s + g[k] * sdk;
-    }
+    for (int k = 0; k < nobs; ++k) s = s + g[k] * sd[k];
     bvec[(size_t)b * nd + i] = s;
   } else if (i == nd) {
     double s = 0.0;
-    for (int k = 0; k < nobs; ++k) {
-      double sk = m.obs_sigma[k];
-      double sdk = m.dvec[k] / (sg2 * (sk * sk));
-      s = s + m.dvec[k] * sdk;
-    }
+    for (int k = 0; k < nobs; ++k) s = s + m.dvec[k] * sd[k];
     dsd[b] = s;
   }
 }
 
-hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
   hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B, sel);
+  for (int p = 0; p < m.npath && m.nsar > 0; ++p) {
+    int n = P.sar_n[p];
+    if (n == 0) continue;
+    dim3 gw((m.ndim + TM - 1) / TM, (n + TN - 1) / TN, B);
+    hipLaunchKernelGGL(k_sar_weight, gw, dim3(256), 0, s, m, bt.G, bt.scal, bt.Gw, P.sar_r0[p], n, P.sar_off[p],
+                       sel);
+  }
+  hipLaunchKernelGGL(k_sd, dim3((m.nobs + 127) / 128, B), dim3(128), 0, s, m, bt.scal, bt.sdv, sel);
   dim3 g((m.ndim + TM - 1) / TM, (m.ndim + TN - 1) / TN, B);
-  hipLaunchKernelGGL(k_gram, g, dim3(256), 0, s, m, bt.G, bt.scal, bt.H, sel);
-  hipLaunchKernelGGL(k_gram_vec, dim3((m.ndim + 1 + 127) / 128, B), dim3(128), 0, s, m, bt.G, bt.scal, bt.bvec,
+  hipLaunchKernelGGL(k_gram, g, dim3(256), 0, s, m, bt.G, m.nsar > 0 ? bt.Gw : nullptr, bt.scal, bt.H, sel);
+  hipLaunchKernelGGL(k_gram_vec, dim3((m.ndim + 1 + 127) / 128, B), dim3(128), 0, s, m, bt.G, bt.sdv, bt.bvec,
                      bt.dsd, B, sel);
   return hipGetLastError();
 }

@@ -13,7 +13,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -129,6 +132,30 @@ double neglog_q(const double* x, const Gsmc& g, int nd) {
   return tmp / 2.0 + std::log(2.0 * pi) * nd / 2.0 + g.log_det / 2.0;
 }
 
+// per-stage dumps of fault_smc_exec (smc_fault.f90:923-936, 1054-1067): <output_dir><iter>.csv, one line per
+// particle, its ndim coordinates then its likelihood, each in Fortran F20.5 (C "%20.5f" for values that fit)
+std::mutex g_out_mu;
+std::map<const smcf_problem*, std::string> g_out_dir;
+
+std::string output_dir_of(const smcf_problem* pb) {
+  std::lock_guard<std::mutex> lk(g_out_mu);
+  auto it = g_out_dir.find(pb);
+  return it == g_out_dir.end() ? std::string() : it->second;
+}
+
+int write_stage(const std::string& dir, int iter, int N, int nd, const std::vector<double>& X,
+                const std::vector<double>& L) {
+  if (dir.empty()) return SMCF_OK;
+  std::string name = dir + std::to_string(iter) + ".csv";
+  FILE* f = std::fopen(name.c_str(), "w");
+  if (!f) return SMCF_EINVAL;
+  for (int i = 0; i < N; ++i) {
+    for (int d = 0; d < nd; ++d) std::fprintf(f, "%20.5f", X[(size_t)i * nd + d]);
+    std::fprintf(f, "%20.5f\n", L[i]);
+  }
+  return std::fclose(f) == 0 ? SMCF_OK : SMCF_EINVAL;
+}
+
 int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages, int max_batch,
               double* particles, double* likelihood, int* nstage, double* evidence, double* gamma_trace,
               const Gsmc* g) {
@@ -182,6 +209,8 @@ int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int m
   }
   if (g)
     for (int i = 0; i < N; ++i) Q[i] = neglog_q(&X[(size_t)i * nd], *g, nd);
+  const std::string out_dir = output_dir_of(pb);
+  if ((rc = write_stage(out_dir, 0, N, nd, X, L))) return rc;
   double gamma = 0.0, ev_total = 0.0;
   int iter = 1;
   while (1.0 - gamma > 1e-8 && iter <= max_stages) {
@@ -324,6 +353,7 @@ int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int m
     X = Xn;
     L = Ln;
     Q = Qn;
+    if ((rc = write_stage(out_dir, iter, N, nd, X, L))) return rc;
     iter++;
   }
   std::memcpy(particles, X.data(), sizeof(double) * X.size());
@@ -336,6 +366,16 @@ int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int m
 }  // namespace
 
 extern "C" {
+
+int smcf_fault_smc_set_output_dir(smcf_problem* pb, const char* output_dir) {
+  if (!pb) return SMCF_EINVAL;
+  std::lock_guard<std::mutex> lk(g_out_mu);
+  if (output_dir && output_dir[0])
+    g_out_dir[pb] = output_dir;
+  else
+    g_out_dir.erase(pb);
+  return SMCF_OK;
+}
 
 int smcf_fault_smc_exec(smcf_problem* pb, const double* range, int N, uint64_t seed, int max_stages,
                         int max_batch, double* particles, double* likelihood, int* nstage, double* evidence,

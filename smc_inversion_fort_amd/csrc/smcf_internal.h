@@ -52,6 +52,13 @@ struct MeshDev {
   const double* unitvec;    // 3 x nobs
   const double* obs_sigma;  // nobs
   const double* dvec;       // nobs
+  // InSAR paths (rows 0 .. nsar-1): inverse covariance blocks, column-major per path, concatenated
+  int npath;
+  const double* sinv;       // sum over paths of n_p^2
+  const int* sar_r0;        // npath: first row (0-based)
+  const int* sar_n;         // npath: rows
+  const long* sar_off;      // npath: offset of the block in sinv
+  const int* row_path;      // nsar: path of each SAR row
 };
 
 // Per-batch device state.  Index b is the instance (geometry particle) slot.
@@ -70,6 +77,8 @@ struct BatchDev {
   double* ltv;
   int* llc_idx;             // 25 x ndim x cap (sparse columns of L^T L, 0-based rows, -1 pad)
   double* llc_val;
+  double* Gw;               // ndim x nsar x cap: SAR rows of Sigma^-1 G, (1/sigma_sar^2) Sinv_p^T G_p (nsar > 0)
+  double* sdv;              // nobs x cap: Sigma^-1 d (the reference's sdvec)
   double* H;                // ndim x ndim x cap
   double* bvec;             // ndim x cap
   double* dsd;              // cap
@@ -120,6 +129,8 @@ struct Problem {
   HmcDev hm;
   double max_slip;
   int P, mover, sweeps, precision;
+  std::vector<int> sar_r0, sar_n;  // host copies of the InSAR path layout (launch geometry)
+  std::vector<long> sar_off;
   std::vector<void*> allocs;
 };
 
@@ -133,7 +144,9 @@ struct StageCfg {
 hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
-hipError_t launch_gram(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+// Gram quantities H, b, dsd (InSAR paths need the host copies of the path layout in P)
+hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s,
+                       const int* sel = nullptr);
 hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s,
                              const int* sel = nullptr);
 // likelihood of the current particles (matrix cores; Gram form)

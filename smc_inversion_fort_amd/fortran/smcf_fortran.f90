@@ -22,6 +22,7 @@ end module type_mat
 
 module smcf_c
     use iso_c_binding
+    use type_mat
     implicit none
 
     type, bind(C) :: smcf_problem_desc
@@ -137,27 +138,36 @@ contains
     end subroutine
 
     ! (re)create the device problem when the observation set or mesh changes
+    ! sigma_sar_mat (optional): the inverse InSAR covariance per path (calc_sigma_sar_mat); without it the
+    ! blocks are identities, which is enough for Green's matrices alone
     subroutine smcf_ensure_problem(nplane, nxi_ls, neta_ls, xmin, xmax, zmin, fix_xbend, xbend, nobs, &
                                    nsar_total, ngnss, npath, nsar_index, obs_points, obs_unitvec, &
-                                   obs_sigma, dvec, max_slip, nparticle_slip)
+                                   obs_sigma, dvec, max_slip, nparticle_slip, sigma_sar_mat)
         integer, intent(in) :: nplane, nxi_ls(:), neta_ls(:), nobs, nsar_total, ngnss, npath, nsar_index(:), &
                                nparticle_slip
         double precision, intent(in) :: xmin, xmax, zmin, xbend(:), obs_points(:, :), obs_unitvec(:, :), &
                                         obs_sigma(:), dvec(:), max_slip
         logical, intent(in) :: fix_xbend
+        type(mat), intent(in), optional :: sigma_sar_mat(:)
         type(smcf_problem_desc) :: d
-        integer :: sig(8)
+        integer :: sig(8), ip, n, off, k, nsinv
         double precision :: s
         sig = [nplane, nobs, nsar_total, ngnss, npath, nparticle_slip, sum(nxi_ls) + 1000*sum(neta_ls), &
                merge(1, 0, fix_xbend)]
         s = sum(obs_points) + 3d0*sum(obs_unitvec) + 7d0*sum(obs_sigma) + 11d0*sum(dvec) + xmin + 2d0*xmax &
             + 5d0*zmin + max_slip
+        nsinv = 0
+        do ip = 1, npath
+            n = nsar_index(ip + 1) - nsar_index(ip)
+            nsinv = nsinv + n*n
+            if (present(sigma_sar_mat) .and. n > 0) s = s + 13d0*sum(sigma_sar_mat(ip)%body(1:n, 1:n))
+        end do
         if (c_associated(g_pb) .and. all(sig == g_sig) .and. s == g_sum) return
         if (.not. c_associated(g_ctx)) call smcf_check(smcf_ctx_create(0_c_int, g_ctx), "smcf_ctx_create")
         if (c_associated(g_pb)) call smcf_check(smcf_problem_destroy(g_pb), "smcf_problem_destroy")
         if (allocated(t_nxi)) deallocate (t_nxi, t_neta, t_nsi, t_xb, t_pts, t_uv, t_sig, t_d, t_sar)
         allocate (t_nxi(nplane), t_neta(nplane), t_nsi(npath + 1), t_xb(max(nplane - 1, 1)), &
-                  t_pts(2*nobs), t_uv(3*nobs), t_sig(nobs), t_d(nobs), t_sar(1))
+                  t_pts(2*nobs), t_uv(3*nobs), t_sig(nobs), t_d(nobs), t_sar(max(nsinv, 1)))
         t_nxi = nxi_ls(1:nplane)
         t_neta = neta_ls(1:nplane)
         t_nsi = nsar_index(1:npath + 1)
@@ -168,6 +178,19 @@ contains
         t_sig = obs_sigma(1:nobs)
         t_d = dvec(1:nobs)
         t_sar = 0d0
+        off = 0
+        do ip = 1, npath
+            n = nsar_index(ip + 1) - nsar_index(ip)
+            if (n <= 0) cycle
+            if (present(sigma_sar_mat)) then
+                t_sar(off + 1:off + n*n) = reshape(sigma_sar_mat(ip)%body(1:n, 1:n), [n*n])
+            else
+                do k = 1, n
+                    t_sar(off + (k - 1)*n + k) = 1d0
+                end do
+            end if
+            off = off + n*n
+        end do
         d%nplane = nplane
         d%nxi_ls = c_loc(t_nxi)
         d%neta_ls = c_loc(t_neta)
@@ -321,7 +344,7 @@ contains
         ndim = 2*ndof_total
         call smcf_ensure_problem(nplane, nxi_ls, neta_ls, xmin, xmax, zmin, fix_xbend, xbend, nobs, nsar_total, &
                                  ngnss, npath, nsar_index, obs_points, obs_unitvec, obs_sigma, dvec, max_slip, &
-                                 nparticle_slip)
+                                 nparticle_slip, sigma_sar_mat)
         allocate (th(size(theta)), x(ndim*nparticle_slip), lk(nparticle_slip), pr(nparticle_slip), mn(ndim), &
                   gt(202))
         th = theta
@@ -398,7 +421,7 @@ contains
         integer :: i0, nb, k
         call smcf_ensure_problem(nplane, nxi_ls, neta_ls, xmin, xmax, zmin, fix_xbend, xbend, nobs, nsar_total, &
                                  ngnss, npath, nsar_index, obs_points, obs_unitvec, obs_sigma, dvec, max_slip, &
-                                 nparticle_slip)
+                                 nparticle_slip, sigma_sar_mat)
         i0 = 1
         do while (i0 <= work_size)
             nb = min(g_max_batch, work_size - i0 + 1)

@@ -92,11 +92,33 @@ def truth_slip(nxi, neta, theta):
     return np.array(vals)
 
 
-def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, neta=None):
+# line-of-sight unit vectors (east, north, up) of two InSAR tracks, ascending and descending
+SAR_LOS = ((-0.62, -0.11, 0.78), (0.62, -0.11, 0.78))
+SAR_SIGMA = 1.0  # cm
+
+
+def sar_points(npath, nper, rng):
+    """npath InSAR paths of nper pixels each, uniform in the box; rows path after path (SAR rows first)."""
+    n = npath * nper
+    pts = np.zeros((2, n), order="F")
+    uv = np.zeros((3, n), order="F")
+    pts[:] = rng.uniform(-BOX_KM / 2, BOX_KM / 2, size=(2, n))
+    for p in range(npath):
+        los = np.array(SAR_LOS[p % len(SAR_LOS)])
+        uv[:, p * nper:(p + 1) * nper] = (los / np.linalg.norm(los))[:, None]
+    return pts, uv
+
+
+def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, neta=None, npath=0, nsar_per_path=0):
     """Build the observation dict of config `cfg_name` (sizes overridable for small tests).
 
     greens(theta, obs_points, obs_unitvec, nxi, neta) -> G (nobs x ndim) at the true geometry.
+    npath > 0 adds npath InSAR paths of nsar_per_path pixels before the GNSS rows (the reference's row
+    order), with noise correlated as the path covariance of calc_sigma_sar_mat, and stores the inverse
+    covariance blocks under "sar_sinv"; greens is then called as greens(theta, pts, uv, nxi, neta, nsar).
     """
+    if npath:
+        return _make_observations_sar(cfg_name, seed, greens, nsta, sigma, nxi, neta, npath, nsar_per_path)
     cfg = dict(CONFIGS[cfg_name])
     nsta = nsta or cfg["nsta"]
     sigma = sigma if sigma is not None else cfg["sigma"]
@@ -111,6 +133,36 @@ def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, n
     return dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.full(G.shape[0], sigma), dvec=d,
                 nsar_total=0, ngnss=nsta, nsar_index=np.array([1, 1], np.int32), nxi=nxi, neta=neta,
                 slip_true=s)
+
+
+def _make_observations_sar(cfg_name, seed, greens, nsta, sigma, nxi, neta, npath, nper):
+    from .sar import calc_sigma_sar_mat
+    cfg = dict(CONFIGS[cfg_name])
+    nsta = nsta or cfg["nsta"]
+    sigma = sigma if sigma is not None else cfg["sigma"]
+    nxi = nxi or cfg["nxi"]
+    neta = neta or cfg["neta"]
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0x5A17, 0x5A2]))
+    spts, suv = sar_points(npath, nper, rng)
+    gpts, guv = stations(nsta, rng)
+    pts = np.asfortranarray(np.concatenate([spts, gpts], axis=1))
+    uv = np.asfortranarray(np.concatenate([suv, guv], axis=1))
+    nsar = npath * nper
+    nsar_index = np.array([1 + p * nper for p in range(npath + 1)], np.int32)
+    obs_sigma = np.concatenate([np.full(nsar, SAR_SIGMA), np.full(3 * nsta, sigma)])
+    th = theta_true()
+    G = greens(th, pts, uv, nxi, neta, nsar)
+    s = truth_slip(nxi, neta, th)
+    noise = np.concatenate([np.zeros(nsar), sigma * rng.standard_normal(3 * nsta)])
+    sinv = calc_sigma_sar_mat(nsar_index, pts, obs_sigma)
+    off = 0
+    for p in range(npath):  # correlated path noise with the covariance the likelihood uses
+        c = np.linalg.inv(sinv[off:off + nper * nper].reshape(nper, nper, order="F"))
+        noise[p * nper:(p + 1) * nper] = np.linalg.cholesky((c + c.T) / 2) @ rng.standard_normal(nper)
+        off += nper * nper
+    d = G @ s + noise
+    return dict(obs_points=pts, obs_unitvec=uv, obs_sigma=obs_sigma, dvec=d, nsar_total=nsar, ngnss=nsta,
+                nsar_index=nsar_index, sar_sinv=sinv, nxi=nxi, neta=neta, slip_true=s)
 
 
 def sample_thetas(B, seed, frac=0.2):
