@@ -187,7 +187,11 @@ def main():
            "roofline": head["roofline"], "cpu_baseline": cpu}
     if len(forms) == 2:
         m, f64 = res["mixed"], res["fp64"]
-        rel = np.abs(m["neglog"] - f64["neglog"]) / np.abs(f64["neglog"])
+        # geometries whose inner SMC ran past 200 stages return the reference's sentinel -log Z = 1e10
+        # (smc_slip.f90:1557-1560); near that limit one form can cross it and the other not
+        sent_m, sent_f = m["neglog"] >= 1e10, f64["neglog"] >= 1e10
+        both = ~sent_m & ~sent_f
+        rel = np.abs(m["neglog"][both] - f64["neglog"][both]) / np.abs(f64["neglog"][both])
         m50, m99 = latency(m)
         out["mixed"] = {"value": round(m["value"], 4), "unit": "outer particles/s",
                         "speedup_vs_fp64": round(m["value"] / f64["value"], 3),
@@ -195,7 +199,9 @@ def main():
                         "mean_inner_stages": round(float(m["stages"].mean()), 2),
                         "p50_wave_latency_s": m50, "p99_wave_latency_s": m99,
                         "neglogz_rel_diff_vs_fp64": {"median": float("%.3g" % np.median(rel)),
-                                                     "max": float("%.3g" % rel.max())},
+                                                     "p99": float("%.3g" % np.percentile(rel, 99)),
+                                                     "max": float("%.3g" % rel.max()),
+                                                     "sentinel_in_one_form_only": int((sent_m != sent_f).sum())},
                         "roofline": m["roofline"]}
     print(json.dumps(out), flush=True)
 
