@@ -54,18 +54,32 @@ __global__ void __launch_bounds__(64) k_init_chain(MeshDev m, int P, double max_
       int d = (int)dl, i = 2 * d + c;
       const int* ix = IX + (size_t)i * 25;
       const double* vx = VX + (size_t)i * 25;
-      double lii = 0.0, mu = 0.0;
+      // the column's 25 slots are loaded together (the -1 padding sits at the end, so skipping j < 0 is
+      // the reference's exit at the first pad); the sums run in slot order as in the sequential chain
+      int jj[25];
+      double vv[25];
+#pragma unroll
       for (int w = 0; w < 25; ++w) {
-        if (ix[w] == i) { lii = vx[w]; break; }
+        jj[w] = ix[w];
+        vv[w] = vx[w];
       }
+      double lii = 0.0, mu = 0.0;
+      bool found = false;
+#pragma unroll
+      for (int w = 0; w < 25; ++w)
+        if (!found && jj[w] == i) {
+          lii = vv[w];
+          found = true;
+        }
       double s2 = 1.0 / (lii / a2);
+#pragma unroll
       for (int w = 0; w < 25; ++w) {
-        int j = ix[w];
-        if (j < 0) break;
-        if (j == i) continue;
-        int dj = j >> 1;  // same component as i
-        double cj = j < i ? mine[dj & (W - 1)] : prev[dj & (W - 1)];
-        mu = mu - vx[w] / a2 * cj * s2;
+        int j = jj[w];
+        if (j >= 0 && j != i) {
+          int dj = j >> 1;  // same component as i
+          double cj = j < i ? mine[dj & (W - 1)] : prev[dj & (W - 1)];
+          mu = mu - vv[w] / a2 * cj * s2;
+        }
       }
       double v1 = draw_normal(key, kSiteGibbsInit, 0, (uint32_t)k, (uint32_t)i);
       double x = mu + v1 * sqrt(s2);

@@ -27,7 +27,7 @@ def write_run(root, obs, nxi, neta, npart_fault, npart_slip, theta):
         f.write("number of fault planes\n1\nfix-xbend\nF\nxbend\n\n\n"
                 "number of samples for fault\n%d\nnumber of samples for slip\n%d\n\n"
                 "observation path\ndata/observation.csv\nnumber of SAR paths\n%d\noutput directory path\noutput/\n\n"
-                "max slip(m)\n%r\n\nnxi\n%d\nneta\n%d\n\nxmin\n%r\nxmax\n%r\nzmin\n%r\n\nGSMC\nF\n"
+                "max slip(m)\n%.17g\n\nnxi\n%d\nneta\n%d\n\nxmin\n%.17g\nxmax\n%.17g\nzmin\n%.17g\n\nGSMC\nF\n"
                 % (npart_fault, npart_slip, npath, syn.MAX_SLIP, nxi, neta, xmin, xmax, zmin))
     with open(os.path.join(root, "data", "observation.csv"), "w") as f:
         f.write("x,y,ex,ey,ez,dlos,sigma,type\n")
@@ -39,15 +39,14 @@ def write_run(root, obs, nxi, neta, npart_fault, npart_slip, theta):
                 typ = "InSAR_%d" % p
             else:
                 typ = "GNSS"
-            x, y = obs["obs_points"][:, k]
-            ex, ey, ez = obs["obs_unitvec"][:, k]
-            f.write("%r, %r, %r, %r, %r, %r, %r, %s\n" % (x, y, ex, ey, ez, obs["dvec"][k], obs["obs_sigma"][k], typ))
+            vals = list(obs["obs_points"][:, k]) + list(obs["obs_unitvec"][:, k]) + [obs["dvec"][k], obs["obs_sigma"][k]]
+            f.write(", ".join("%.17g" % v for v in vals) + ", %s\n" % typ)
     with open(os.path.join(root, "peak_fault.dat"), "w") as f:
-        f.write("".join("%r\n" % v for v in theta))
+        f.write("".join("%.17g\n" % v for v in theta))
     r = syn.prior_range()
     with open(os.path.join(root, "data", "range.dat"), "w") as f:
         for i, name in enumerate(["ymin", "ymax", "dip", "log_alpha2", "log_sigma_sar2", "log_sigma_gnss2"]):
-            f.write("%s\n%r, %r\n" % (name, r[0, i], r[1, i]))
+            f.write("%s\n%.17g, %.17g\n" % (name, r[0, i], r[1, i]))
 
 
 def sar_obs():
