@@ -7,11 +7,13 @@ matrix, the Gram matrices and a full inner slip SMC (P = 2000 particles, truncat
 5 sweeps per stage) tempered to gamma = 1.  Observations stay resident on the device; each
 step uploads only the thetas and reads back one -log Z per geometry.
 
-Two arithmetic forms of the same workload, on the same geometries and keys:
+Arithmetic forms of the same workload, on the same geometries and keys:
   fp64   every operation in FP64 (the headline `value`);
-  mixed  Gibbs products Q S in FP32 on the matrix cores and single-precision truncated-normal
-         draws; conditional means, likelihood, weights and evidence in FP64 (reported under
-         "mixed", with its speed-up and its -log Z difference from the FP64 form).
+  mixed  Gibbs products Q S in FP32 on the matrix cores, single-precision truncated-normal draws;
+         conditional means, likelihood, weights and evidence in FP64;
+  bf16   as mixed, with Q S formed from split-BF16 operands (hi.hi + hi.lo + lo.hi, FP32 accumulation).
+The mixed forms are reported under "mixed" (the faster one) and "mixed_forms", each with its speed-up
+over FP64 and its -log Z difference from the FP64 form.
 
 Prints one JSON line with `roofline` (Gibbs-sweep kernel, timed with HIP events on the
 library's stream) and `cpu_baseline` (the C oracle on this host's cores, a bounded sample of
@@ -31,6 +33,7 @@ import numpy as np  # noqa: E402
 METRIC = "outer-SMC particles/s at 16x16 patches, mixed vs FP64; p50/p99 solve latency"
 FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector = FP64 matrix
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 
 
 def parse():
@@ -43,7 +46,7 @@ def parse():
     ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+    ap.add_argument("--precision", choices=["all", "fp64", "mixed", "bf16"], nargs="+", default=["all"],
                     help="arithmetic forms to run (the first one run gives `value`)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
@@ -63,6 +66,9 @@ def traffic_from_profiles(form):
     return None
 
 
+FORM_NAMES = {0: "fp64", 1: "mixed", 2: "bf16"}
+
+
 def run_form(a, S, syn, obs, precision, keys_for):
     """Warm-up and timed steps of one arithmetic form; returns its figures."""
     cfg = syn.CONFIGS["c2"]
@@ -72,7 +78,7 @@ def run_form(a, S, syn, obs, precision, keys_for):
     pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
                    mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots, precision=precision)
     ndim, P, sweeps = pb.ndim, cfg["P"], cfg["sweeps"]
-    name = "fp64" if precision == S.PRECISION_FP64 else "mixed"
+    name = FORM_NAMES[precision]
     print("bench[%s]: c2 16x16 patches ndim=%d nobs=%d P=%d evals/step=%d slots=%d"
           % (name, ndim, pb.nobs, P, B, a.slots), flush=True)
     for w in range(a.warmup):
@@ -96,10 +102,19 @@ def run_form(a, S, syn, obs, precision, keys_for):
     stages = np.concatenate(stages)
     # algorithmic flops of the Gibbs kernel: per particle-sweep 2*ndim^2 (the products Q S)
     flops = float(stages.sum()) * P * sweeps * 2.0 * ndim * ndim
+    name = FORM_NAMES[precision]
+    if name == "fp64":
+        peak, kernel = FP64_PEAK_TFLOPS, "k_gibbs (FP64 matrix cores + FP64 VALU draws)"
+    elif name == "mixed":
+        peak, kernel = (FP32_MFMA_PEAK_TFLOPS,
+                        "k_gibbs_mixed (FP32 matrix cores + FP32 VALU draws; peak = dense FP32 matrix)")
+    else:
+        # three BF16 products per FP32-equivalent product: the comparable peak is a third of dense BF16
+        flops *= 3.0
+        peak, kernel = (BF16_MFMA_PEAK_TFLOPS,
+                        "k_gibbs_bf (split-BF16 matrix cores, 3 products per term, + FP32 VALU draws; "
+                        "achieved counts the 3 BF16 products; peak = dense BF16 matrix)")
     achieved = flops / (ms_gibbs * 1e-3) / 1e12 if ms_gibbs > 0 else 0.0
-    peak = FP64_PEAK_TFLOPS if precision == S.PRECISION_FP64 else FP32_MFMA_PEAK_TFLOPS
-    kernel = ("k_gibbs (FP64 matrix cores + FP64 VALU draws)" if precision == S.PRECISION_FP64 else
-              "k_gibbs_mixed (FP32 matrix cores + FP32 VALU draws; peak = dense FP32 matrix)")
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(name), "kernel": kernel,
             "launches": n_gibbs, "avg_launch_ms": round(ms_gibbs / max(n_gibbs, 1), 4)}
@@ -146,12 +161,18 @@ def main():
     B = a.evals
 
     def keys_for(step):
-        base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(step + 1)
-        return (np.arange(B, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
+        # the 64-bit products wrap modulo 2^64 by design (seed mixing); numpy would warn on the scalar wrap
+        with np.errstate(over="ignore"):
+            base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(step + 1)
+            return (np.arange(B, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
 
-    forms = {"both": ["fp64", "mixed"], "fp64": ["fp64"], "mixed": ["mixed"]}[a.precision]
-    res = {f: run_form(a, S, syn, obs, S.PRECISION_FP64 if f == "fp64" else S.PRECISION_MIXED, keys_for)
-           for f in forms}
+    forms = []
+    for f in a.precision:
+        for g in (["fp64", "mixed", "bf16"] if f == "all" else [f]):
+            if g not in forms:
+                forms.append(g)
+    prec = {"fp64": S.PRECISION_FP64, "mixed": S.PRECISION_MIXED, "bf16": S.PRECISION_MIXED_BF16}
+    res = {f: run_form(a, S, syn, obs, prec[f], keys_for) for f in forms}
     head = res[forms[0]]
     cpu = None
     if not a.no_cpu:
@@ -175,7 +196,8 @@ def main():
     out = {"metric": METRIC, "value": round(head["value"], 4), "unit": "outer particles/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64" if forms[0] == "fp64" else "mixed (fp32 matrix + fp32 draws; fp64 means/likelihood)",
+           "dtype": {"fp64": "f64", "mixed": "mixed (fp32 matrix + fp32 draws; fp64 means/likelihood)",
+                     "bf16": "mixed (split-bf16 matrix + fp32 draws; fp64 means/likelihood)"}[forms[0]],
            "data": "synthetic (seeded GNSS stations, Gaussian slip bump, 5 mm noise)",
            "config": {"workload": "c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, "
                                   "truncated-MVN Gibbs x5 per stage; waves of %d geometry particles "
@@ -185,24 +207,33 @@ def main():
                       "mean_inner_stages": round(float(head["stages"].mean()), 2),
                       "p50_wave_latency_s": p50, "p99_wave_latency_s": p99},
            "roofline": head["roofline"], "cpu_baseline": cpu}
-    if len(forms) == 2:
-        m, f64 = res["mixed"], res["fp64"]
-        # geometries whose inner SMC ran past 200 stages return the reference's sentinel -log Z = 1e10
-        # (smc_slip.f90:1557-1560); near that limit one form can cross it and the other not
-        sent_m, sent_f = m["neglog"] >= 1e10, f64["neglog"] >= 1e10
-        both = ~sent_m & ~sent_f
-        rel = np.abs(m["neglog"][both] - f64["neglog"][both]) / np.abs(f64["neglog"][both])
-        m50, m99 = latency(m)
-        out["mixed"] = {"value": round(m["value"], 4), "unit": "outer particles/s",
-                        "speedup_vs_fp64": round(m["value"] / f64["value"], 3),
-                        "ms_per_step": round(m["ms_per_step"], 2),
-                        "mean_inner_stages": round(float(m["stages"].mean()), 2),
-                        "p50_wave_latency_s": m50, "p99_wave_latency_s": m99,
-                        "neglogz_rel_diff_vs_fp64": {"median": float("%.3g" % np.median(rel)),
-                                                     "p99": float("%.3g" % np.percentile(rel, 99)),
-                                                     "max": float("%.3g" % rel.max()),
-                                                     "sentinel_in_one_form_only": int((sent_m != sent_f).sum())},
-                        "roofline": m["roofline"]}
+    if "fp64" in res and len(res) > 1:
+        f64 = res["fp64"]
+        mixed = {}
+        for name in forms:
+            if name == "fp64":
+                continue
+            m = res[name]
+            # geometries whose inner SMC ran past 200 stages return the reference's sentinel -log Z = 1e10
+            # (smc_slip.f90:1557-1560); near that limit one form can cross it and the other not
+            sent_m, sent_f = m["neglog"] >= 1e10, f64["neglog"] >= 1e10
+            both = ~sent_m & ~sent_f
+            rel = np.abs(m["neglog"][both] - f64["neglog"][both]) / np.abs(f64["neglog"][both])
+            m50, m99 = latency(m)
+            mixed[name] = {"value": round(m["value"], 4), "unit": "outer particles/s",
+                           "speedup_vs_fp64": round(m["value"] / f64["value"], 3),
+                           "ms_per_step": round(m["ms_per_step"], 2),
+                           "mean_inner_stages": round(float(m["stages"].mean()), 2),
+                           "p50_wave_latency_s": m50, "p99_wave_latency_s": m99,
+                           "neglogz_rel_diff_vs_fp64": {"median": float("%.3g" % np.median(rel)),
+                                                        "p99": float("%.3g" % np.percentile(rel, 99)),
+                                                        "max": float("%.3g" % rel.max()),
+                                                        "sentinel_in_one_form_only": int((sent_m != sent_f).sum())},
+                           "roofline": m["roofline"]}
+        best = max(mixed, key=lambda k: mixed[k]["value"])
+        out["mixed"] = dict(form=best, **mixed[best])
+        out["mixed_forms"] = {k: {kk: v[kk] for kk in ("value", "speedup_vs_fp64", "ms_per_step",
+                                                        "neglogz_rel_diff_vs_fp64")} for k, v in mixed.items()}
     print(json.dumps(out), flush=True)
 
 

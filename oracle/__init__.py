@@ -99,6 +99,8 @@ def _declare(L):
     L.orc_trunc_normal_prec_f32.restype = C.c_double
     L.orc_ndtri7_lower.argtypes = [C.c_float]
     L.orc_ndtri7_lower.restype = C.c_float
+    L.orc_erfc_cheb32.argtypes = [C.c_float]
+    L.orc_erfc_cheb32.restype = C.c_float
     L.orc_trunc_normal.restype = C.c_double
     L.orc_slip_smc_exec.argtypes = [C.POINTER(Problem), D, D, C.c_uint64, D, D, D, D, C.POINTER(SlipResult)]
     L.orc_fault_calc_likelihood.argtypes = [C.POINTER(Problem), D, C.c_uint64, C.POINTER(SlipResult)]
@@ -403,6 +405,10 @@ def trunc_normal_prec(mu, s, lo, hi, u):
 def trunc_normal_prec_f32(mu, s, lo, hi, u):
     """Single-precision draw of the mixed-precision Gibbs mover (NaN where it declines)."""
     return lib().orc_trunc_normal_prec_f32(mu, s, lo, hi, u)
+
+
+def erfc_cheb32(x):
+    return lib().orc_erfc_cheb32(x)
 
 
 def ndtri7_lower(p):

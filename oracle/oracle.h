@@ -107,6 +107,7 @@ int orc_calc_sigma_sar_mat(int npath, const int* nsar_index, int nobs, const dou
 /* single-precision draw of the mixed-precision Gibbs mover (NaN where it declines) and its PPND7 */
 double orc_trunc_normal_prec_f32(double mu, double s, double lo, double hi, float u);
 float orc_ndtri7_lower(float p);
+float orc_erfc_cheb32(float x);
 double orc_trunc_normal(double mu, double sigma, double lo, double hi, double u);
 
 typedef struct {

@@ -29,6 +29,7 @@ MOVER_HMC = 0
 MOVER_GIBBS = 1
 PRECISION_FP64 = 0   # every operation in FP64
 PRECISION_MIXED = 1  # Gibbs Q.S products in FP32 on the matrix cores, single-precision draws; means/weights FP64
+PRECISION_MIXED_BF16 = 2  # as PRECISION_MIXED with the Q.S products from split-BF16 operands (hi.hi + hi.lo + lo.hi)
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)

@@ -139,6 +139,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
     CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
   }
   if (bt.S32) CHECK_LAUNCH(launch_s32(m, bt, n, s, sel));
+  if (bt.SX) CHECK_LAUNCH(launch_sbf(m, bt, n, s, sel));
   CHECK_LAUNCH(launch_like(m, bt, n, 0, s, sel));
   CHECK_LAUNCH(launch_prior(m, bt, n, s, sel));
   return SMCF_OK;
@@ -365,11 +366,12 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   P.mover = d->mover;
   P.sweeps = d->gibbs_sweeps > 0 ? d->gibbs_sweeps : 5;
   P.precision = d->precision;
-  if (P.precision != SMCF_PRECISION_FP64 && P.precision != SMCF_PRECISION_MIXED) {
+  if (P.precision != SMCF_PRECISION_FP64 && P.precision != SMCF_PRECISION_MIXED &&
+      P.precision != SMCF_PRECISION_MIXED_BF16) {
     set_error("unknown precision " + std::to_string(P.precision));
     return SMCF_EINVAL;
   }
-  if (P.precision == SMCF_PRECISION_MIXED && P.mover != SMCF_MOVER_GIBBS) {
+  if (P.precision != SMCF_PRECISION_FP64 && P.mover != SMCF_MOVER_GIBBS) {
     set_error("mixed precision is implemented for the Gibbs mover only");
     return SMCF_EUNSUPPORTED;
   }
@@ -538,10 +540,16 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
   rc |= dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked Gibbs layout, padded to whole blocks
   bt.Q32 = nullptr;
   bt.S32 = bt.S32b = nullptr;
+  bt.SX = bt.SXb = bt.QX = nullptr;
   if (P.precision == SMCF_PRECISION_MIXED) {
     rc |= dalloc(P, &bt.Q32, (nd + bt.kb) * nd * C);
     rc |= dalloc(P, &bt.S32, nd * Pn * C);
     rc |= dalloc(P, &bt.S32b, nd * Pn * C);
+  }
+  if (P.precision == SMCF_PRECISION_MIXED_BF16) {
+    rc |= dalloc(P, &bt.QX, bf_q_elems((int)nd, bt.kb) * C);
+    rc |= dalloc(P, &bt.SX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    rc |= dalloc(P, &bt.SXb, (size_t)bf_rows((int)nd) * Pn * 2 * C);
   }
   rc |= dalloc(P, &bt.qdiag, 4 * nd * C);
   rc |= dalloc(P, &bt.S, nd * Pn * C);

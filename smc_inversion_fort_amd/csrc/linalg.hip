@@ -349,6 +349,7 @@ hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
   hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.Q, bt.gamma, bt.bvec,
                      bt.active, bt.qdiag);
   if (bt.Q32) hipLaunchKernelGGL(k_q_to_f32, dim3(blocks, B), dim3(256), 0, s, n, bt.Q, bt.active, bt.Q32);
+  if (bt.QX) return launch_q_bf(m, bt, B, s);
   return hipGetLastError();
 }
 

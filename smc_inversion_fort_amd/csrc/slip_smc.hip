@@ -309,6 +309,15 @@ __global__ void k_gather(int nd, int P, BatchDev bt) {
   const double* Ss = (cur ? bt.S : bt.S2) + (size_t)b * nd * P;
   double* Sd = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
   int pa = bt.parent[(size_t)b * P + j];
+  if (bt.SX) {  // split-BF16 copy of the children as well: 16-value groups [hi 8 | lo 8]
+    int ng = bf_rows(nd) / 8;
+    const uint4* Xs = (const uint4*)(cur ? bt.SX : bt.SXb) + (size_t)b * ng * P * 2;
+    uint4* Xd = (uint4*)(cur ? bt.SXb : bt.SX) + (size_t)b * ng * P * 2;
+    for (int g8 = blockIdx.y; g8 < ng; g8 += gridDim.y) {
+      Xd[((size_t)g8 * P + j) * 2] = Xs[((size_t)g8 * P + pa) * 2];
+      Xd[((size_t)g8 * P + j) * 2 + 1] = Xs[((size_t)g8 * P + pa) * 2 + 1];
+    }
+  }
   if (bt.S32) {  // mixed precision: the FP32 copy of the children as well
     float* Td = (cur ? bt.S32b : bt.S32) + (size_t)b * nd * P;
     for (int i = blockIdx.y; i < nd; i += gridDim.y) {
@@ -667,6 +676,7 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint64_t wodd = 0;
+#pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
           int i = i0 + 16 * m + kk;
           if (i >= nd) {
@@ -682,9 +692,13 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
           } else {
             u = u01_open(wodd);
           }
-          // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
-          const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
+          // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
+          //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
+          //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
+          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
           double qi = Tw[kk][lane];
+#pragma unroll
+          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
           double so = S[(size_t)i * P + pc];
           double mu = so - (qi - cd[i]) * cd[nd + i];
           // lean draw for the common regions; the full one (tails beyond 10 sigma) when it declines
@@ -692,11 +706,7 @@ __global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int swee
           if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u);
           double dlt = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
-          // right-looking correction of the later coordinates of the tile, Tw[kk'] += Q(i, i0+16m+kk') delta;
-          // the other entries get a zero coefficient, and fma(0, x, t) = t leaves them unchanged
-#pragma unroll
-          for (int k2 = 0; k2 < 16; ++k2) Tw[k2][lane] = fma(k2 > kk ? qrow[k2] : 0.0, dlt, Tw[k2][lane]);
-          Tw[kk][lane] = dlt;
+          Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -848,6 +858,7 @@ __global__ void __launch_bounds__(256, kMixedOcc)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint64_t wodd = 0;
+#pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
           int i = i0 + 16 * m + kk;
           if (i >= nd) {
@@ -861,9 +872,13 @@ __global__ void __launch_bounds__(256, kMixedOcc)
           } else {
             wc = wodd;
           }
-          // Tw[kk][lane] already holds the corrections by the coordinates redrawn before it
-          const double* qrow = Qr + (size_t)i * KB + 16 * m;  // Q(i, i0 + 16m + kk'), kk' = 0..15
+          // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
+          //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
+          //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
+          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
           double qi = Tw[kk][lane];
+#pragma unroll
+          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
           double so = S[(size_t)i * P + pc];
           double mu = so - (qi - cd[i]) * cd[nd + i];
           double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
@@ -873,9 +888,7 @@ __global__ void __launch_bounds__(256, kMixedOcc)
             S[(size_t)i * P + p] = sn;
             S32[(size_t)i * P + p] = (float)sn;
           }
-#pragma unroll
-          for (int k2 = 0; k2 < 16; ++k2) Tw[k2][lane] = fma(k2 > kk ? qrow[k2] : 0.0, dlt, Tw[k2][lane]);
-          Tw[kk][lane] = dlt;
+          Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -899,14 +912,336 @@ __global__ void __launch_bounds__(256, kMixedOcc)
   }
 }
 
-int gibbs_kb(int precision) { return precision == SMCF_PRECISION_MIXED ? 16 * kMixedMT : kGibbsKB; }
+// ---------------------------------------------------------------------------
+// Mixed-precision Gibbs sweeps with split-BF16 products (SMCF_PRECISION_MIXED_BF16).  The sweep of
+// k_gibbs_mixed, with the products q = Q S formed on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16):
+// every operand is split as x = hi + lo with hi = bf16(x), lo = bf16(x - hi), and
+//   Q S ~ Qhi Shi + Qhi Slo + Qlo Shi        (FP32 accumulation),
+// which leaves about 2^-16 relative error per term (the dropped Qlo Slo and the split residues are
+// below 2^-17 each) at three BF16 products per FP32-equivalent one, each 16x the FP32 matrix rate.
+//  * Operands in matrix-core fragment order, 16 bytes per lane and part: QX holds, for block blk,
+//    32-column step st, 16-row tile m and lane l = r + 16g, the 8 values Q(32 st + 8g + e, blk KB +
+//    16m + r), e = 0..7 (hi, then lo); SX holds the particles as [j/8][p][hi 8 | lo 8], so lane
+//    c + 16g of a particle tile reads coordinates 32 st + 8g .. +7 of particle p0 + c.  A and B
+//    assign each contraction index to the same (lane group, element) slot, so the products do not
+//    depend on the instruction's internal k order; lane l holds D(4 (l/16) + t, l % 16) in register t
+//    (the FP32 accumulator layout of k_gibbs_mixed).
+//  * Draw phase as in k_gibbs_mixed (FP64 conditional means and in-tile corrections, single-precision
+//    draw); each lane parks its tile's 16 new values in LDS and writes them to SX as two 16-value
+//    groups at the end of the tile.
+//  * Corrections of the later tiles of the block by a tile's deltas: the same BF16 products with the
+//    half of the 32-wide contraction that the tile does not cover set to zero.
+// ---------------------------------------------------------------------------
+#ifndef SMCF_BF_MT
+#define SMCF_BF_MT 4
+#define SMCF_BF_NB 2
+#define SMCF_BF_OCC 2
+#endif
+constexpr int kBfMT = SMCF_BF_MT;    // 16-row tiles per block
+constexpr int kBfNB = SMCF_BF_NB;    // register buffers of the matrix-phase pipeline
+constexpr int kBfOcc = SMCF_BF_OCC;  // waves per SIMD the register budget is sized for
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+SMCF_DEV void split_bf16(float x, __bf16& h, __bf16& l) {
+  h = (__bf16)x;
+  l = (__bf16)(x - (float)h);
+}
+
+// acc[m][n] += sum over nst 32-wide steps of A_m B_n in split BF16.  A fragment (st, m) at
+// qa[ao[m] + st astep] (hi) and +1 (lo); B fragment (st, n) at sb[bo[n] + st bstep] (hi) and +1 (lo);
+// offsets in 16-byte units.  Same software pipeline as mfma_panel_f32.
+template <int MT, int NB>
+__device__ __forceinline__ void mfma_panel_bf3(const bf16x8* __restrict__ qa, const uint32_t (&ao)[MT], uint32_t astep,
+                                               const bf16x8* __restrict__ sb, const uint32_t (&bo)[4],
+                                               uint32_t bstep, int nst, flt4 (&acc)[MT][4]) {
+  if (nst <= 0) return;
+  bf16x8 ah[NB][MT], al[NB][MT], bh[NB][4], bl[NB][4];
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nst - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      uint32_t o = ao[m] + q * astep;
+      ah[k][m] = qa[o];
+      al[k][m] = qa[o + 1];
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      uint32_t o = bo[n] + q * bstep;
+      bh[k][n] = sb[o];
+      bl[k][n] = sb[o + 1];
+    }
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[k][m], bh[k][n], acc[m][n], 0, 0, 0);
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[k][m], bl[k][n], acc[m][n], 0, 0, 0);
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[k][m], bh[k][n], acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nst - nst % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nst - full) prod(k);
+}
+
+__global__ void __launch_bounds__(256, kBfOcc)
+    k_gibbs_bf(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
+               const bf16x8* __restrict__ QXall, double* __restrict__ S0, double* __restrict__ S1,
+               bf16x8* __restrict__ X0, bf16x8* __restrict__ X1, const double* __restrict__ cdall,
+               const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
+               const int* __restrict__ par) {
+  constexpr int MT = kBfMT, KB = 16 * MT;
+  __shared__ double T[4][16][65];
+  __shared__ float NV[4][16][64];
+  int chunks = (P + 255) / 256;
+  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
+  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
+  if (b >= B || !active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int pw0 = chunk * 256 + wave * 64;
+  int p = pw0 + lane;
+  bool valid = p < P;
+  int pc = valid ? p : P - 1;
+  int pb = par[b];
+  int ndp = bf_rows(nd), nst = ndp / 32;
+  int nblk = (nd + KB - 1) / KB;
+  double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
+  bf16x8* SX = (pb ? X1 : X0) + (size_t)b * (ndp / 8) * P * 2;
+  const double* Qb = Qball + (size_t)b * nd * nblk * KB;
+  const bf16x8* QX = QXall + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  // per-coordinate constants of the stage: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  const double* cd = cdall + (size_t)b * 4 * nd;
+  uint64_t key = keys[b];
+  uint32_t stage = (uint32_t)(iters[b] - 1);
+  int fi = lane % 16, fk = lane / 16;
+  uint32_t bo[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)((fk * P + min(pw0 + 16 * n + fi, P - 1)) * 2);
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)((m * 64 + lane) * 2);
+  const uint32_t astep = MT * 128, bstep = 8 * (uint32_t)P;
+  double (*Tw)[65] = T[wave];
+  float (*Nw)[64] = NV[wave];
+  const bf16x8 zero8 = {};
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int blk = 0; blk < nblk; ++blk) {
+      int i0 = blk * KB;
+      const double* Qr = Qb + (size_t)blk * nd * KB;
+      const bf16x8* QXb = QX + (size_t)blk * nst * MT * 128;
+      flt4 acc[MT][4];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = flt4{0.f, 0.f, 0.f, 0.f};
+      mfma_panel_bf3<MT, kBfNB>(QXb, ao, astep, SX, bo, bstep, nst, acc);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
+        // transpose tile m through LDS in double: Tw[row][particle]
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) Tw[4 * fk + t][16 * n + fi] = (double)acc[m][n][t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t wodd = 0;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+          int i = i0 + 16 * m + kk;
+          if (i >= nd) {
+            Tw[kk][lane] = 0.0;
+            Nw[kk][lane] = 0.f;
+            continue;
+          }
+          // coordinates 2m' and 2m'+1 share one Philox block (word i & 1); i0 and the tile offset are even
+          uint64_t wc;
+          if ((kk & 1) == 0) {
+            philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wodd);
+          } else {
+            wc = wodd;
+          }
+          // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
+          //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
+          //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
+          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
+          double qi = Tw[kk][lane];
+#pragma unroll
+          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
+          double so = S[(size_t)i * P + pc];
+          double mu = so - (qi - cd[i]) * cd[nd + i];
+          double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
+          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open(wc));
+          double dlt = sn - so;
+          if (valid) S[(size_t)i * P + p] = sn;
+          Nw[kk][lane] = (float)sn;
+          Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the tile's new values into the split copy: coordinate groups (i0 + 16m) / 8 and the next
+        if (valid) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            bf16x8 hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              __bf16 a, c;
+              split_bf16(Nw[8 * h + e][lane], a, c);
+              hi[e] = a;
+              lo[e] = c;
+            }
+            size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + p) * 2;
+            SX[o] = hi;
+            SX[o + 1] = lo;
+          }
+        }
+        // corrections of the later tiles by this tile's deltas (in Tw[k][p]):
+        //   acc[m2](r, p) += sum_k Q(i0 + 16m + k, i0 + 16m2 + r) delta_k(p)
+        // The tile's coordinates are columns 16 hh .. 16 hh + 15 of 32-wide step stc, held by lane
+        // groups 2 hh and 2 hh + 1 of a fragment; the other two groups contribute zero.
+        if (m + 1 < MT) {
+          int jt = i0 + 16 * m, stc = jt / 32, hh = (jt / 16) & 1;
+          bool mine = (fk >> 1) == hh;
+          int kb8 = 8 * (fk - 2 * hh);
+          bf16x8 dh[4], dl[4];
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              __bf16 a, c;
+              split_bf16(mine ? (float)Tw[kb8 + e][16 * n + fi] : 0.f, a, c);
+              dh[n][e] = a;
+              dl[n][e] = c;
+            }
+          }
+#pragma unroll
+          for (int m2 = m + 1; m2 < MT; ++m2) {
+            if (i0 + 16 * m2 >= nd) break;
+            uint32_t o = (uint32_t)(((stc * MT + m2) * 64 + lane) * 2);
+            bf16x8 qh = mine ? QXb[o] : zero8, ql = mine ? QXb[o + 1] : zero8;
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh, dh[n], acc[m2][n], 0, 0, 0);
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh, dl[n], acc[m2][n], 0, 0, 0);
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ql, dh[n], acc[m2][n], 0, 0, 0);
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // every lane has read the deltas and values before the next tile
+      }
+    }
+  }
+}
+
+// Split-BF16 copy of the blocked precision in fragment order: fragment f = ((blk nst + st) MT + m) 64 + lane
+// of lane = r + 16g holds Q(32 st + 8g + e, blk KB + 16m + r), e = 0..7, hi at QX[2f], lo at QX[2f + 1].
+__global__ void k_q_to_bf(int nd, int KB, const double* __restrict__ Qball, const int* __restrict__ active,
+                          bf16x8* __restrict__ QXall) {
+  int b = blockIdx.y;
+  if (!active[b]) return;
+  int MT = KB / 16, nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
+  size_t nfrag = (size_t)nblk * nst * MT * 64;
+  const double* Qb = Qball + (size_t)b * nblk * nd * KB;
+  bf16x8* QX = QXall + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nfrag; f += (size_t)gridDim.x * blockDim.x) {
+    int lane = (int)(f % 64);
+    size_t rest = f / 64;
+    int m = (int)(rest % MT);
+    rest /= MT;
+    int st = (int)(rest % nst), blk = (int)(rest / nst);
+    int col = 16 * m + lane % 16, g = lane / 16;
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int j = 32 * st + 8 * g + e;
+      float x = j < nd ? (float)Qb[((size_t)blk * nd + j) * KB + col] : 0.f;
+      __bf16 a, c;
+      split_bf16(x, a, c);
+      hi[e] = a;
+      lo[e] = c;
+    }
+    QX[2 * f] = hi;
+    QX[2 * f + 1] = lo;
+  }
+}
+
+// split-BF16 copy of the current particles of the given slots: [j/8][p][hi 8 | lo 8], zero past nd
+__global__ void k_s_to_bf(int nd, int P, BatchDev bt, const int* __restrict__ sel) {
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
+  int cur = bt.par[b];
+  int ndp = bf_rows(nd);
+  const double* S = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  bf16x8* X = (bf16x8*)(cur ? bt.SXb : bt.SX) + (size_t)b * (ndp / 8) * P * 2;
+  size_t n = (size_t)(ndp / 8) * P;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int p = (int)(e % P), g8 = (int)(e / P);
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      int j = 8 * g8 + k;
+      float x = j < nd ? (float)S[(size_t)j * P + p] : 0.f;
+      __bf16 a, c;
+      split_bf16(x, a, c);
+      hi[k] = a;
+      lo[k] = c;
+    }
+    X[2 * e] = hi;
+    X[2 * e + 1] = lo;
+  }
+}
+
+hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+  hipLaunchKernelGGL(k_s_to_bf, dim3(128, B), dim3(256), 0, s, m.ndim, bt.P, bt, sel);
+  return hipGetLastError();
+}
+
+hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  int nd = m.ndim, KB = bt.kb;
+  size_t nfrag = bf_q_elems(nd, KB) / 16;
+  int blocks = (int)std::min<size_t>((nfrag + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_q_to_bf, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.Q, bt.active, (bf16x8*)bt.QX);
+  return hipGetLastError();
+}
+
+int gibbs_kb(int precision) {
+  return precision == SMCF_PRECISION_MIXED ? 16 * kMixedMT : precision == SMCF_PRECISION_MIXED_BF16 ? 16 * kBfMT
+                                                                                                     : kGibbsKB;
+}
 
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
   int nd = m.ndim;
   int chunks = (bt.P + 255) / 256;
   int groups = (B + kXcd - 1) / kXcd;
   dim3 g(groups * kXcd * chunks);
-  if (bt.Q32) {
+  if (bt.QX) {
+    if (bt.kb != 16 * kBfMT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gibbs_bf, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q,
+                       (const bf16x8*)bt.QX, bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, bt.qdiag, bt.key, bt.iter,
+                       bt.active, bt.par);
+  } else if (bt.Q32) {
     if (bt.kb != 16 * kMixedMT) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.Q32, bt.S,
                        bt.S2, bt.S32, bt.S32b, bt.qdiag, bt.key, bt.iter, bt.active, bt.par);

@@ -90,6 +90,12 @@ struct BatchDev {
   double* S2;
   float* S32;               // FP32 copies of S and S2 read by the mixed-precision Gibbs products (else null)
   float* S32b;
+  // split-BF16 copies (SMCF_PRECISION_MIXED_BF16 only, else null), bf16 bits:
+  //   SX, SXb  particles of S and S2 as [ndp/8][P][hi 8 | lo 8], ndp = nd rounded up to 32 (pad rows zero)
+  //   QX       the stage's blocked precision in matrix-core fragment order (k_q_to_bf)
+  uint16_t* SX;
+  uint16_t* SXb;
+  uint16_t* QX;
   double* Y;                // ndim x P x cap (H X of the HMC leapfrog; null for the Gibbs mover)
   double* like;             // P x cap
   double* prior;
@@ -162,6 +168,14 @@ hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& c
 int gibbs_kb(int precision);
 // FP32 copies of the current particles of the given slots (mixed precision)
 hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+// split-BF16 copies of the current particles of the given slots (SMCF_PRECISION_MIXED_BF16)
+hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+// split-BF16 fragment-ordered copy of the stage's blocked precision (SMCF_PRECISION_MIXED_BF16)
+hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+// coordinates of the split-BF16 particle copies (nd rounded up to whole 32-row matrix-core steps)
+__host__ __device__ inline int bf_rows(int nd) { return (nd + 31) & ~31; }
+// bf16 values of one instance's fragment-ordered precision QX (KB-row blocks x 32-column steps)
+__host__ __device__ inline size_t bf_q_elems(int nd, int kb) { return (size_t)((nd + kb - 1) / kb) * kb * bf_rows(nd) * 2; }
 hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 
 hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel = nullptr);

@@ -1,8 +1,9 @@
-"""Mixed-precision Gibbs mover (SMCF_PRECISION_MIXED) against the FP64 oracle and the FP64 device path.
+"""Mixed-precision Gibbs movers against the FP64 oracle and the FP64 device path.
 
-The mixed form computes the Gibbs products Q S in FP32 on the matrix cores and draws the truncated
-normals in single precision; conditional means, corrections, likelihood, weights and evidence stay
-FP64.  It is a different rounding of the same sampler, so it is compared statistically: -log Z and
+SMCF_PRECISION_MIXED computes the Gibbs products Q S in FP32 on the matrix cores;
+SMCF_PRECISION_MIXED_BF16 forms them from split-BF16 operands (hi.hi + hi.lo + lo.hi, FP32
+accumulation).  Both draw the truncated normals in single precision; conditional means, corrections,
+likelihood, weights and evidence stay FP64.  It is a different rounding of the same sampler, so it is compared statistically: -log Z and
 posterior means over independent keys, within Monte Carlo error plus the stated tolerance (posterior
 means within 1% of the largest FP64 mean).  Each test prints the measured error.  Needs an MI355X."""
 import sys
@@ -65,14 +66,18 @@ def test_trunc_normal_f32_vs_oracle(ctx):
     assert np.all(dz <= 2e-6 * zmag)
 
 
-def test_mixed_slip_smc_vs_fp64_oracle(ctx):
+MIXED_FORMS = [pytest.param(S.PRECISION_MIXED, id="fp32"), pytest.param(S.PRECISION_MIXED_BF16, id="bf16x3")]
+
+
+@pytest.mark.parametrize("precision", MIXED_FORMS)
+def test_mixed_slip_smc_vs_fp64_oracle(ctx, precision):
     """Inner SMC on a small mesh, 24 keys: mixed device vs FP64 oracle.  The first gamma is identical
     (FP64 likelihood of the same prior particles); -log Z and the posterior means agree within Monte
     Carlo error, posterior means also within 1% of the largest FP64 mean."""
-    say("case: mixed inner SMC vs FP64 oracle, 24 keys")
+    say("case: mixed inner SMC (precision %d) vs FP64 oracle, 24 keys" % precision)
     obs = small_obs()
     P = 256
-    pb = device_problem(ctx, obs, P, S.PRECISION_MIXED)
+    pb = device_problem(ctx, obs, P, precision)
     xmin, xmax, zmin, _ = syn.fault_frame()
     f = O.Fault([obs["nxi"]], [obs["neta"]], xmin, xmax, zmin)
     po = O.ProblemHolder(f, obs, syn.MAX_SLIP, P, mover=1, gibbs_sweeps=3)
@@ -100,14 +105,15 @@ def test_mixed_slip_smc_vs_fp64_oracle(ctx):
     pb.close()
 
 
-def test_mixed_streamed_slots_match_one_wide_batch(ctx):
-    """The mixed path is deterministic per (geometry, key) whatever the slot count."""
-    say("case: mixed streamed slots vs wide batch")
+@pytest.mark.parametrize("precision", MIXED_FORMS)
+def test_mixed_streamed_slots_match_one_wide_batch(ctx, precision):
+    """The mixed paths are deterministic per (geometry, key) whatever the slot count."""
+    say("case: mixed (precision %d) streamed slots vs wide batch" % precision)
     obs = small_obs(seed=8)
     th = syn.sample_thetas(8, 31)
     keys = [O.inner_key(5, 0, 0, i) for i in range(8)]
-    narrow = device_problem(ctx, obs, 128, S.PRECISION_MIXED, max_batch=3)
-    wide = device_problem(ctx, obs, 128, S.PRECISION_MIXED, max_batch=8)
+    narrow = device_problem(ctx, obs, 128, precision, max_batch=3)
+    wide = device_problem(ctx, obs, 128, precision, max_batch=8)
     a, na = narrow.fault_calc_likelihood(th, keys)
     b, nb = wide.fault_calc_likelihood(th, keys)
     say("stages", na.tolist(), "neglog", a.tolist())
@@ -118,11 +124,11 @@ def test_mixed_streamed_slots_match_one_wide_batch(ctx):
 
 
 def test_mixed_c2_vs_fp64_device():
-    """Benchmark size (16x16 patches, 400 stations, P = 2000, 5 sweeps): mixed vs FP64 device path
-    on 8 geometries.  The stage counts and -log Z agree within Monte Carlo spread; the posterior means
-    at the true geometry, averaged over 6 keys, agree within Monte Carlo error plus 1% of the largest
-    mean."""
-    say("case: mixed vs fp64 at 16x16 patches")
+    """Benchmark size (16x16 patches, 400 stations, P = 2000, 5 sweeps): each mixed form vs the FP64
+    device path on 8 geometries.  The stage counts and -log Z agree within Monte Carlo spread; the
+    posterior means at the true geometry, averaged over 6 keys, agree within Monte Carlo error plus 1% of
+    the largest mean."""
+    say("case: mixed forms vs fp64 at 16x16 patches")
     ctx = S.Context(0)
     xmin, xmax, zmin, _ = syn.fault_frame()
     cfg = syn.CONFIGS["c2"]
@@ -135,35 +141,39 @@ def test_mixed_c2_vs_fp64_device():
         p.close()
         return g
     obs = syn.make_observations("c2", 0, greens)
+    precs = (S.PRECISION_FP64, S.PRECISION_MIXED, S.PRECISION_MIXED_BF16)
     pbs = {prec: S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
                            mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=8, precision=prec)
-           for prec in (S.PRECISION_FP64, S.PRECISION_MIXED)}
+           for prec in precs}
     th = syn.sample_thetas(8, 3)
     keys = [O.inner_key(2, 0, 0, i) for i in range(8)]
-    nl64, ns64 = pbs[S.PRECISION_FP64].fault_calc_likelihood(th, keys)
-    nlm, nsm = pbs[S.PRECISION_MIXED].fault_calc_likelihood(th, keys)
+    res = {prec: pbs[prec].fault_calc_likelihood(th, keys) for prec in precs}
+    nl64, ns64 = res[S.PRECISION_FP64]
     say("fp64 -logZ", np.round(nl64, 3).tolist(), "stages", ns64.tolist())
-    say("mixed -logZ", np.round(nlm, 3).tolist(), "stages", nsm.tolist())
-    rel = np.abs(nlm - nl64) / np.abs(nl64)
-    say("-log Z relative difference: max %.2e" % rel.max())
-    assert rel.max() < 1e-3
-    assert np.all(np.abs(nsm - ns64) <= 3)
-    # posterior means at the true geometry over 6 keys: the difference of the key averages against its
-    # Monte Carlo standard error, plus 1% of the largest mean (the stated tolerance)
     t0 = syn.theta_true()
     K = 6
     mk = {prec: np.array([pbs[prec].slip_smc_exec(t0, O.inner_key(9, 0, 0, k))["mean"] for k in range(K)])
-          for prec in pbs}
-    m64, mmx = mk[S.PRECISION_FP64], mk[S.PRECISION_MIXED]
+          for prec in precs}
+    m64 = mk[S.PRECISION_FP64]
     scale = np.abs(m64.mean(0)).max()
-    diff = mmx.mean(0) - m64.mean(0)
-    se = np.sqrt(m64.var(0, ddof=1) / K + mmx.var(0, ddof=1) / K)
     say("posterior mean, one key: fp64 key0 vs key1 max |diff| = %.2f%% of max mean (Monte Carlo spread)"
         % (100 * np.abs(m64[0] - m64[1]).max() / scale))
-    say("posterior mean, %d-key averages: max |mixed - fp64| = %.3e = %.3f%% of max mean; max |diff|/se = %.2f"
-        % (K, np.abs(diff).max(), 100 * np.abs(diff).max() / scale, float((np.abs(diff) / se).max())))
-    # 5 standard errors: the largest of 480 coordinates' |N(0, 1)| exceeds 5 with probability 3e-4
-    assert np.all(np.abs(diff) <= 5 * se + 0.01 * scale)
+    for prec in precs[1:]:
+        nlm, nsm = res[prec]
+        say("precision %d -logZ" % prec, np.round(nlm, 3).tolist(), "stages", nsm.tolist())
+        rel = np.abs(nlm - nl64) / np.abs(nl64)
+        say("-log Z relative difference: max %.2e" % rel.max())
+        assert rel.max() < 1e-3
+        assert np.all(np.abs(nsm - ns64) <= 3)
+        # posterior means at the true geometry over 6 keys: the difference of the key averages against its
+        # Monte Carlo standard error, plus 1% of the largest mean (the stated tolerance)
+        mmx = mk[prec]
+        diff = mmx.mean(0) - m64.mean(0)
+        se = np.sqrt(m64.var(0, ddof=1) / K + mmx.var(0, ddof=1) / K)
+        say("posterior mean, %d-key averages: max |mixed - fp64| = %.3e = %.3f%% of max mean; max |diff|/se = %.2f"
+            % (K, np.abs(diff).max(), 100 * np.abs(diff).max() / scale, float((np.abs(diff) / se).max())))
+        # 5 standard errors: the largest of 480 coordinates' |N(0, 1)| exceeds 5 with probability 3e-4
+        assert np.all(np.abs(diff) <= 5 * se + 0.01 * scale)
     for p in pbs.values():
         p.close()
     ctx.close()

@@ -211,6 +211,22 @@ def test_ndtri7_matches_scipy():
         assert abs(z - ref) <= 1e-6 * max(1.0, abs(ref)), (p, z, ref)
 
 
+def test_erfc_cheb32_matches_scipy():
+    """The single-precision erfc of the mixed Gibbs draw (Numerical Recipes' Chebyshev-fitted erfcc)
+    against scipy's erfc from 0 to 9 (erfc(9) = 4e-37, still a normal float): relative error below
+    4e-7 + 1.5e-7 x^2 (the fit's 1.2e-7, single-precision rounding, and the rounding of the exponent
+    argument -x^2, which grows like x^2 2^-24).  A wrong coefficient shows up as an error of 1e-4 or more."""
+    xs = np.concatenate([np.linspace(0.0, 2.0, 401), np.linspace(2.0, 9.0, 701)]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        v = O.erfc_cheb32(float(x))
+        ref = sp.erfc(float(x))
+        rel = abs(v - ref) / ref
+        worst = max(worst, rel)
+        assert rel <= 4e-7 + 1.5e-7 * float(x) ** 2, (x, v, ref)
+    assert worst < 1.1e-5
+
+
 @pytest.mark.parametrize("a,b", [(-7.5, 0.5), (-3.0, 3.0), (-0.5, 20.0), (0.5, 7.9), (2.0, 2.5), (-7.9, -7.0),
                                  (-30.0, 1.0), (6.0, 30.0), (-1e3, 1e3)])
 def test_trunc_normal_prec_f32_inverts_cdf(a, b):
