@@ -594,8 +594,12 @@ hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hi
 // chunks of one instance get the same w % 8, so an instance's Q blocks are fetched into one L2.
 constexpr int kXcd = 8;
 constexpr int kGibbsKB = 32;
+#ifndef SMCF_GIBBS_OCC
+#define SMCF_GIBBS_OCC 4
+#endif
+constexpr int kGibbsOcc = SMCF_GIBBS_OCC;  // waves per SIMD the register budget of k_gibbs is sized for
 
-__global__ void __launch_bounds__(256, 4) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
+__global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
                                                const double* __restrict__ Qball, double* __restrict__ S0,
                                                double* __restrict__ S1, const double* __restrict__ cdall,
                                                const uint64_t* __restrict__ keys,
@@ -927,15 +931,18 @@ __global__ void __launch_bounds__(256, kMixedOcc)
 //    depend on the instruction's internal k order; lane l holds D(4 (l/16) + t, l % 16) in register t
 //    (the FP32 accumulator layout of k_gibbs_mixed).
 //  * Draw phase as in k_gibbs_mixed (FP64 conditional means and in-tile corrections, single-precision
-//    draw); each lane parks its tile's 16 new values in LDS and writes them to SX as two 16-value
-//    groups at the end of the tile.
+//    draw); each lane keeps its tile's 16 values in registers and writes the new ones to SX as two
+//    16-value groups at the end of the tile.
 //  * Corrections of the later tiles of the block by a tile's deltas: the same BF16 products with the
 //    half of the 32-wide contraction that the tile does not cover set to zero.
 // ---------------------------------------------------------------------------
 #ifndef SMCF_BF_MT
-#define SMCF_BF_MT 4
+#define SMCF_BF_MT 3
 #define SMCF_BF_NB 2
 #define SMCF_BF_OCC 2
+#endif
+#ifndef SMCF_BF_PREPHILOX
+#define SMCF_BF_PREPHILOX 0
 #endif
 constexpr int kBfMT = SMCF_BF_MT;    // 16-row tiles per block
 constexpr int kBfNB = SMCF_BF_NB;    // register buffers of the matrix-phase pipeline
@@ -1012,7 +1019,11 @@ __global__ void __launch_bounds__(256, kBfOcc)
                const int* __restrict__ par) {
   constexpr int MT = kBfMT, KB = 16 * MT;
   __shared__ double T[4][16][65];
-  __shared__ float NV[4][16][64];
+  // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
+  //   CQ[kk][k] = Q(i0 + 16m + k, i0 + 16m + kk), the tile's diagonal block (in-tile corrections)
+  //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  __shared__ double CQ[4][16][16];
+  __shared__ double CD[4][4][16];
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
   int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
@@ -1042,7 +1053,8 @@ __global__ void __launch_bounds__(256, kBfOcc)
   for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)((m * 64 + lane) * 2);
   const uint32_t astep = MT * 128, bstep = 8 * (uint32_t)P;
   double (*Tw)[65] = T[wave];
-  float (*Nw)[64] = NV[wave];
+  double (*CQw)[16] = CQ[wave];
+  double (*CDw)[16] = CD[wave];
   const bf16x8 zero8 = {};
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int blk = 0; blk < nblk; ++blk) {
@@ -1058,6 +1070,13 @@ __global__ void __launch_bounds__(256, kBfOcc)
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
+        {  // the tile's constants into LDS: lane (k, c4) = (lane % 16, lane / 16) brings 4 + 1 values
+          int ik = i0 + 16 * m + fi;
+          const double* qsrc = Qr + (size_t)min(ik, nd - 1) * KB + 16 * m + 4 * fk;  // Q(ik, i0 + 16m + 4c4 + t)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) CQw[4 * fk + t][fi] = ik < nd ? qsrc[t] : 0.0;
+          CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
+        }
         // transpose tile m through LDS in double: Tw[row][particle]
 #pragma unroll
         for (int n = 0; n < 4; ++n)
@@ -1066,36 +1085,55 @@ __global__ void __launch_bounds__(256, kBfOcc)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the tile's 16 old values, requested together: one memory latency per tile, not per coordinate
+        double sv[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) sv[kk] = i0 + 16 * m + kk < nd ? S[(size_t)(i0 + 16 * m + kk) * P + pc] : 0.0;
+#if SMCF_BF_PREPHILOX
+        // the tile's 8 Philox blocks up front: independent chains the scheduler can interleave
+        uint64_t wv[16];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+          int i = i0 + 16 * m + 2 * h;
+          wv[2 * h] = wv[2 * h + 1] = 0;
+          if (i < nd)
+            philox_draw_u(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wv[2 * h],
+                        wv[2 * h + 1]);
+        }
+#else
         uint64_t wodd = 0;
+#endif
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
           int i = i0 + 16 * m + kk;
           if (i >= nd) {
             Tw[kk][lane] = 0.0;
-            Nw[kk][lane] = 0.f;
             continue;
           }
           // coordinates 2m' and 2m'+1 share one Philox block (word i & 1); i0 and the tile offset are even
           uint64_t wc;
+#if SMCF_BF_PREPHILOX
+          wc = wv[kk];
+#else
           if ((kk & 1) == 0) {
-            philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wodd);
+            philox_draw_u(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wodd);
           } else {
             wc = wodd;
           }
+#endif
           // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
           //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
           //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
-          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
           double qi = Tw[kk][lane];
 #pragma unroll
-          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
-          double so = S[(size_t)i * P + pc];
-          double mu = so - (qi - cd[i]) * cd[nd + i];
-          double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
-          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open(wc));
+          for (int k = 0; k < kk; ++k) qi = fma(CQw[kk][k], Tw[k][lane], qi);
+          double so = sv[kk];
+          double mu = so - (qi - CDw[0][kk]) * CDw[1][kk];
+          double sn = gibbs_draw_f32(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open_f32(wc));
+          if (sn != sn) sn = gibbs_draw(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open(wc));
           double dlt = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
-          Nw[kk][lane] = (float)sn;
+          sv[kk] = sn;  // the new value, for the split copy
           Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1109,7 +1147,7 @@ __global__ void __launch_bounds__(256, kBfOcc)
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
               __bf16 a, c;
-              split_bf16(Nw[8 * h + e][lane], a, c);
+              split_bf16((float)sv[8 * h + e], a, c);
               hi[e] = a;
               lo[e] = c;
             }
