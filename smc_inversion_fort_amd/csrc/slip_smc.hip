@@ -607,6 +607,12 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
                                                const int* __restrict__ par) {
   constexpr int KB = kGibbsKB;
   __shared__ double T[4][16][65];
+  // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
+  //   CQ[kk (kk - 1) / 2 + k] = Q(i0 + 16m + k, i0 + 16m + kk), k < kk: the strictly lower part of the tile's
+  //   diagonal block (in-tile corrections), packed so that four workgroups still fit one CU's LDS
+  //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  __shared__ double CQ[4][120];
+  __shared__ double CD[4][4][16];
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
   int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
@@ -633,6 +639,17 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
   for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
   int nd4 = nd & ~3;
   double (*Tw)[65] = T[wave];
+  double* CQw = CQ[wave];
+  double (*CDw)[16] = CD[wave];
+  // the packed entries e = lane and lane + 64 (< 120) this lane stages: (row k, column kk) of the block
+  int ek[2], ekk[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int e = lane + 64 * h, kk = 1;
+    while (kk < 16 && kk * (kk + 1) / 2 <= e) ++kk;
+    ekk[h] = kk;
+    ek[h] = e - kk * (kk - 1) / 2;
+  }
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int blk = 0; blk < nblk; ++blk) {
       int i0 = blk * KB;
@@ -671,6 +688,15 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
           }
           __builtin_amdgcn_wave_barrier();  // every lane has read tile 0's deltas before they are overwritten
         }
+        {  // the tile's constants into LDS
+          int ik = i0 + 16 * m + fi;
+          CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            int e = lane + 64 * h, r = i0 + 16 * m + ek[h];
+            if (e < 120) CQw[e] = r < nd ? Qr[(size_t)r * KB + 16 * m + ekk[h]] : 0.0;
+          }
+        }
         // transpose tile m through LDS: Tw[row][particle]
 #pragma unroll
         for (int n = 0; n < 4; ++n)
@@ -699,15 +725,14 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
           // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
           //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
           //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
-          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
           double qi = Tw[kk][lane];
 #pragma unroll
-          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
+          for (int k = 0; k < kk; ++k) qi = fma(CQw[kk * (kk - 1) / 2 + k], Tw[k][lane], qi);
           double so = S[(size_t)i * P + pc];
-          double mu = so - (qi - cd[i]) * cd[nd + i];
+          double mu = so - (qi - CDw[0][kk]) * CDw[1][kk];
           // lean draw for the common regions; the full one (tails beyond 10 sigma) when it declines
-          double sn = gibbs_draw_fast(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u);
-          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u);
+          double sn = gibbs_draw_fast(mu, CDw[2][kk], CDw[3][kk], max_slip, u);
+          if (sn != sn) sn = gibbs_draw(mu, CDw[2][kk], CDw[3][kk], max_slip, u);
           double dlt = sn - so;
           if (valid) S[(size_t)i * P + p] = sn;
           Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
@@ -795,6 +820,11 @@ __global__ void __launch_bounds__(256, kMixedOcc)
                   const int* __restrict__ par) {
   constexpr int MT = kMixedMT, KB = 16 * MT;
   __shared__ double T[4][16][65];
+  // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
+  //   CQ[kk][k] = Q(i0 + 16m + k, i0 + 16m + kk), the tile's diagonal block (in-tile corrections)
+  //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  __shared__ double CQ[4][16][16];
+  __shared__ double CD[4][4][16];
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
   int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
@@ -827,6 +857,8 @@ __global__ void __launch_bounds__(256, kMixedOcc)
   for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * KB + 16 * m + fi);
   int nd4 = nd & ~3;
   double (*Tw)[65] = T[wave];
+  double (*CQw)[16] = CQ[wave];
+  double (*CDw)[16] = CD[wave];
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int blk = 0; blk < nblk; ++blk) {
       int i0 = blk * KB;
@@ -853,6 +885,13 @@ __global__ void __launch_bounds__(256, kMixedOcc)
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
+        {  // the tile's constants into LDS: lane (k, c4) = (lane % 16, lane / 16) brings 4 + 1 values
+          int ik = i0 + 16 * m + fi;
+          const double* qsrc = Qr + (size_t)min(ik, nd - 1) * KB + 16 * m + 4 * fk;  // Q(ik, i0 + 16m + 4c4 + t)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) CQw[4 * fk + t][fi] = ik < nd ? qsrc[t] : 0.0;
+          CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
+        }
         // transpose tile m through LDS in double: Tw[row][particle]
 #pragma unroll
         for (int n = 0; n < 4; ++n)
@@ -879,14 +918,13 @@ __global__ void __launch_bounds__(256, kMixedOcc)
           // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
           //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
           //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
-          const double* qcol = Qr + (size_t)(i0 + 16 * m) * KB + 16 * m + kk;  // Q(i0 + 16m + k, i) at qcol[k KB]
           double qi = Tw[kk][lane];
 #pragma unroll
-          for (int k = 0; k < kk; ++k) qi = fma(qcol[(size_t)k * KB], Tw[k][lane], qi);
+          for (int k = 0; k < kk; ++k) qi = fma(CQw[kk][k], Tw[k][lane], qi);
           double so = S[(size_t)i * P + pc];
-          double mu = so - (qi - cd[i]) * cd[nd + i];
-          double sn = gibbs_draw_f32(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open_f32(wc));
-          if (sn != sn) sn = gibbs_draw(mu, cd[2 * nd + i], cd[3 * nd + i], max_slip, u01_open(wc));
+          double mu = so - (qi - CDw[0][kk]) * CDw[1][kk];
+          double sn = gibbs_draw_f32(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open_f32(wc));
+          if (sn != sn) sn = gibbs_draw(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open(wc));
           double dlt = sn - so;
           if (valid) {
             S[(size_t)i * P + p] = sn;
