@@ -134,6 +134,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
     ProfScope ps(ctx, "gram", s);
     CHECK_LAUNCH(launch_gram(P, m, bt, n, s, sel));
   }
+  if (bt.HX) CHECK_LAUNCH(launch_h_bf(m, bt, n, s, sel));
   {
     ProfScope ps(ctx, "init_chain", s);
     CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
@@ -204,6 +205,7 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
       } else {
         if (launch_gather(m, bt, W, s) != hipSuccess || launch_form_q(m, bt, W, s) != hipSuccess)
           return fail(SMCF_EHIP);
+        if (bt.HX && launch_like_ref(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
         {
           ProfScope ps(ctx, "gibbs");
           if (launch_gibbs(m, bt, W, cfg, s) != hipSuccess) return fail(SMCF_EHIP);
@@ -211,7 +213,8 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
         {
           ProfScope ps(ctx, "likelihood");
           // the Gibbs mover reads no prior values; they are formed once for the final particles
-          if (launch_like(m, bt, W, 1, s) != hipSuccess) return fail(SMCF_EHIP);
+          hipError_t e = bt.HX ? launch_like_bf(m, bt, W, s) : launch_like(m, bt, W, 1, s);
+          if (e != hipSuccess) return fail(SMCF_EHIP);
         }
       }
       if (hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -546,10 +549,22 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
     rc |= dalloc(P, &bt.S32, nd * Pn * C);
     rc |= dalloc(P, &bt.S32b, nd * Pn * C);
   }
+  bt.HX = bt.DX = nullptr;
+  bt.mref = bt.gref = bt.lref = nullptr;
   if (P.precision == SMCF_PRECISION_MIXED_BF16) {
     rc |= dalloc(P, &bt.QX, bf_q_elems((int)nd, bt.kb) * C);
     rc |= dalloc(P, &bt.SX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
     rc |= dalloc(P, &bt.SXb, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    rc |= dalloc(P, &bt.HX, bf_q_elems((int)nd, 64) * C);
+    rc |= dalloc(P, &bt.DX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    rc |= dalloc(P, &bt.mref, nd * C);
+    rc |= dalloc(P, &bt.gref, nd * C);
+    rc |= dalloc(P, &bt.lref, C);
+    // rows past ndim of the centred copy are read by the likelihood's matrix products and never written
+    if (!rc && hipMemset(bt.DX, 0, sizeof(uint16_t) * (size_t)bf_rows((int)nd) * Pn * 2 * C) != hipSuccess) {
+      set_error("hipMemset failed");
+      rc = SMCF_EHIP;
+    }
   }
   rc |= dalloc(P, &bt.qdiag, 4 * nd * C);
   rc |= dalloc(P, &bt.S, nd * Pn * C);

@@ -979,9 +979,6 @@ __global__ void __launch_bounds__(256, kMixedOcc)
 #define SMCF_BF_NB 2
 #define SMCF_BF_OCC 2
 #endif
-#ifndef SMCF_BF_PREPHILOX
-#define SMCF_BF_PREPHILOX 0
-#endif
 constexpr int kBfMT = SMCF_BF_MT;    // 16-row tiles per block
 constexpr int kBfNB = SMCF_BF_NB;    // register buffers of the matrix-phase pipeline
 constexpr int kBfOcc = SMCF_BF_OCC;  // waves per SIMD the register budget is sized for
@@ -1052,16 +1049,18 @@ __device__ __forceinline__ void mfma_panel_bf3(const bf16x8* __restrict__ qa, co
 __global__ void __launch_bounds__(256, kBfOcc)
     k_gibbs_bf(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
                const bf16x8* __restrict__ QXall, double* __restrict__ S0, double* __restrict__ S1,
-               bf16x8* __restrict__ X0, bf16x8* __restrict__ X1, const double* __restrict__ cdall,
+               bf16x8* __restrict__ X0, bf16x8* __restrict__ X1, bf16x8* __restrict__ DXall,
+               const double* __restrict__ mrefall, const double* __restrict__ cdall,
                const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
                const int* __restrict__ par) {
   constexpr int MT = kBfMT, KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
   //   CQ[kk][k] = Q(i0 + 16m + k, i0 + 16m + kk), the tile's diagonal block (in-tile corrections)
-  //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
+  //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii); CD[4][kk] the
+  //   reference particle mref(i0 + 16m + kk) of the centred likelihood
   __shared__ double CQ[4][16][16];
-  __shared__ double CD[4][4][16];
+  __shared__ double CD[4][5][16];
   int chunks = (P + 255) / 256;
   int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
   int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
@@ -1076,6 +1075,8 @@ __global__ void __launch_bounds__(256, kBfOcc)
   int nblk = (nd + KB - 1) / KB;
   double* S = (pb ? S1 : S0) + (size_t)b * nd * P;
   bf16x8* SX = (pb ? X1 : X0) + (size_t)b * (ndp / 8) * P * 2;
+  bf16x8* DX = DXall + (size_t)b * (ndp / 8) * P * 2;
+  const double* mref = mrefall + (size_t)b * nd;
   const double* Qb = Qball + (size_t)b * nd * nblk * KB;
   const bf16x8* QX = QXall + (size_t)b * (bf_q_elems(nd, KB) / 8);
   // per-coordinate constants of the stage: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
@@ -1114,6 +1115,7 @@ __global__ void __launch_bounds__(256, kBfOcc)
 #pragma unroll
           for (int t = 0; t < 4; ++t) CQw[4 * fk + t][fi] = ik < nd ? qsrc[t] : 0.0;
           CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
+          if (fk == 0) CDw[4][fi] = ik < nd ? mref[ik] : 0.0;
         }
         // transpose tile m through LDS in double: Tw[row][particle]
 #pragma unroll
@@ -1127,20 +1129,7 @@ __global__ void __launch_bounds__(256, kBfOcc)
         double sv[16];
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) sv[kk] = i0 + 16 * m + kk < nd ? S[(size_t)(i0 + 16 * m + kk) * P + pc] : 0.0;
-#if SMCF_BF_PREPHILOX
-        // the tile's 8 Philox blocks up front: independent chains the scheduler can interleave
-        uint64_t wv[16];
-#pragma unroll
-        for (int h = 0; h < 8; ++h) {
-          int i = i0 + 16 * m + 2 * h;
-          wv[2 * h] = wv[2 * h + 1] = 0;
-          if (i < nd)
-            philox_draw_u(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wv[2 * h],
-                        wv[2 * h + 1]);
-        }
-#else
-        uint64_t wodd = 0;
-#endif
+        uint64_t wnext = 0;  // the second word of the current Philox block
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
           int i = i0 + 16 * m + kk;
@@ -1150,15 +1139,11 @@ __global__ void __launch_bounds__(256, kBfOcc)
           }
           // coordinates 2m' and 2m'+1 share one Philox block (word i & 1); i0 and the tile offset are even
           uint64_t wc;
-#if SMCF_BF_PREPHILOX
-          wc = wv[kk];
-#else
           if ((kk & 1) == 0) {
-            philox_draw_u(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wodd);
+            philox_draw(key, kSiteGibbsMove, stage, (uint32_t)pc, (uint32_t)(i >> 1), (uint32_t)sw, wc, wnext);
           } else {
-            wc = wodd;
+            wc = wnext;
           }
-#endif
           // left-looking corrections by the tile's coordinates redrawn before it (deltas in Tw[k][lane]):
           //   qi = T(kk) + Q(i0 + 16m + k, i) delta_k, k = 0 .. kk - 1, fused in ascending k (the operations and
           //   order of updating every later entry of the tile after each draw; Q read from row i0 + 16m + k)
@@ -1192,6 +1177,22 @@ __global__ void __launch_bounds__(256, kBfOcc)
             size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + p) * 2;
             SX[o] = hi;
             SX[o + 1] = lo;
+          }
+          if (sw == sweeps - 1) {  // the moved particles relative to the reference, for the likelihood
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              bf16x8 hi, lo;
+#pragma unroll
+              for (int e = 0; e < 8; ++e) {
+                __bf16 a, c;
+                split_bf16((float)(sv[8 * h + e] - CDw[4][8 * h + e]), a, c);
+                hi[e] = a;
+                lo[e] = c;
+              }
+              size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + p) * 2;
+              DX[o] = hi;
+              DX[o + 1] = lo;
+            }
           }
         }
         // corrections of the later tiles by this tile's deltas (in Tw[k][p]):
@@ -1302,6 +1303,160 @@ hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Likelihood of the split-BF16 form (SMCF_PRECISION_MIXED_BF16), centred on a reference particle m of the
+// stage (the first particle after resampling, FP64):
+//   L(s) = L(m) + sum_i D_i (g_i + Y_i / 2),   D = s - m,  g = H m - b,  Y = H D,
+// which is slip_calc_likelihood's (s^T H s - 2 b^T s + dsd) / 2 + c (smc_slip.f90:56-104) rearranged exactly.
+// L(m) and g are FP64 (k_like_ref); D and the epilogue are FP64; only Y = H D is formed from split-BF16
+// operands on the matrix cores (HX: H in fragment order, formed once per instance; DX: D, written by the last
+// Gibbs sweep of the stage).  Centring keeps the products' 2^-16 relative error on the spread of the particle
+// cloud, not on s itself: |error in L| ~ 2^-16 sum |D_i H_ij D_j|, which tempering multiplies by the stage's
+// increment of gamma (the increment is chosen so that gamma times the spread of L is O(1)).
+// ---------------------------------------------------------------------------
+constexpr int kLikeBfMT = 4;  // 16-row tiles per block of H (64 rows)
+
+// m = particle 0 of the current buffer; g = H m - b; L(m) (one workgroup of GT threads per instance)
+__global__ void __launch_bounds__(GT) k_like_ref(int nd, int P, BatchDev bt) {
+  __shared__ double red[GT];
+  int b = blockIdx.x;
+  if (!bt.active[b]) return;
+  const double* S = (bt.par[b] ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  const double* H = bt.H + (size_t)b * nd * nd;
+  const double* bv = bt.bvec + (size_t)b * nd;
+  double* mref = bt.mref + (size_t)b * nd;
+  double* g = bt.gref + (size_t)b * nd;
+  for (int i = threadIdx.x; i < nd; i += GT) mref[i] = S[(size_t)i * P];
+  __syncthreads();
+  double part = 0.0;
+  for (int i = threadIdx.x; i < nd; i += GT) {
+    double hm = 0.0;
+    for (int j = 0; j < nd; ++j) hm = fma(H[(size_t)j * nd + i], mref[j], hm);  // (H m)_i = sum_j H(i, j) m_j
+    g[i] = hm - bv[i];
+    part = fma(mref[i], 0.5 * hm - bv[i], part);
+  }
+  double tot = block_sum<GT>(part, red);
+  if (threadIdx.x == 0) {
+    const double* sc = bt.scal + 8 * (size_t)b;
+    bt.lref[b] = tot + bt.dsd[b] / 2.0 + (sc[3] + sc[4]);
+  }
+}
+
+// HX: split-BF16 copy of H in fragment order, fragment f = ((blk nst + st) MT + m) 64 + lane of lane = r + 16g
+// holding H(blk KB + 16m + r, 32 st + 8g + e), e = 0..7 (hi at HX[2f], lo at HX[2f + 1]; zero past nd)
+__global__ void k_h_to_bf(int nd, const double* __restrict__ Hall, bf16x8* __restrict__ HXall,
+                          const int* __restrict__ sel) {
+  constexpr int MT = kLikeBfMT, KB = 16 * MT;
+  int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
+  int nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
+  size_t nfrag = (size_t)nblk * nst * MT * 64;
+  const double* H = Hall + (size_t)b * nd * nd;
+  bf16x8* HX = HXall + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nfrag; f += (size_t)gridDim.x * blockDim.x) {
+    int lane = (int)(f % 64);
+    size_t rest = f / 64;
+    int m = (int)(rest % MT);
+    rest /= MT;
+    int st = (int)(rest % nst), blk = (int)(rest / nst);
+    int i = blk * KB + 16 * m + lane % 16, g = lane / 16;
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int j = 32 * st + 8 * g + e;
+      float x = (i < nd && j < nd) ? (float)H[(size_t)j * nd + i] : 0.f;
+      __bf16 a, c;
+      split_bf16(x, a, c);
+      hi[e] = a;
+      lo[e] = c;
+    }
+    HX[2 * f] = hi;
+    HX[2 * f + 1] = lo;
+  }
+}
+
+// one workgroup of 4 wavefronts per 256 particles of an instance, XCD-aware as k_like_mfma
+__global__ void __launch_bounds__(256, 2)
+    k_like_bf(int nd, int P, int B, BatchDev bt) {
+  constexpr int MT = kLikeBfMT, KB = 16 * MT;
+  int chunks = (P + 255) / 256;
+  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
+  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
+  if (b >= B || !bt.active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int p0 = chunk * 256 + wave * 64;
+  if (p0 >= P) return;
+  int pb = bt.par[b];
+  int ndp = bf_rows(nd), nst = ndp / 32, nblk = (nd + KB - 1) / KB;
+  const double* S = (pb ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  const bf16x8* DX = (const bf16x8*)bt.DX + (size_t)b * (ndp / 8) * P * 2;
+  const bf16x8* HX = (const bf16x8*)bt.HX + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  const double* mref = bt.mref + (size_t)b * nd;
+  const double* g = bt.gref + (size_t)b * nd;
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) pcol[n] = min(p0 + 16 * n + fi, P - 1);
+  uint32_t bo[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)((fk * P + pcol[n]) * 2);
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)((m * 64 + lane) * 2);
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int blk = 0; blk < nblk; ++blk) {
+    int i0 = blk * KB;
+    flt4 acc[MT][4];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = flt4{0.f, 0.f, 0.f, 0.f};
+    mfma_panel_bf3<MT, 2>(HX + (size_t)blk * nst * MT * 128, ao, MT * 128, DX, bo, 8 * (uint32_t)P, nst, acc);
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int row = i0 + 16 * m + 4 * fk + t;
+        if (row < nd) {
+          double mr = mref[row], gr = g[row];
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+            double d = S[(size_t)row * P + pcol[n]] - mr;
+            part[n] = fma(d, fma(0.5, (double)acc[m][n][t], gr), part[n]);
+          }
+        }
+      }
+  }
+  double lm = bt.lref[b];
+  double* Lout = (pb ? bt.like2 : bt.like) + (size_t)b * P;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    double v = part[n];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    int p = p0 + 16 * n + fi;
+    if (fk == 0 && p < P) Lout[p] = lm + v;
+  }
+}
+
+hipError_t launch_h_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
+  size_t nfrag = bf_q_elems(m.ndim, 16 * kLikeBfMT) / 16;
+  int blocks = (int)std::min<size_t>((nfrag + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_h_to_bf, dim3(blocks, B), dim3(256), 0, s, m.ndim, bt.H, (bf16x8*)bt.HX, sel);
+  return hipGetLastError();
+}
+
+hipError_t launch_like_ref(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  hipLaunchKernelGGL(k_like_ref, dim3(B), dim3(GT), 0, s, m.ndim, bt.P, bt);
+  return hipGetLastError();
+}
+
+hipError_t launch_like_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
+  int chunks = (bt.P + 255) / 256;
+  int groups = (B + kXcd - 1) / kXcd;
+  hipLaunchKernelGGL(k_like_bf, dim3(groups * kXcd * chunks), dim3(256), 0, s, m.ndim, bt.P, B, bt);
+  return hipGetLastError();
+}
+
 int gibbs_kb(int precision) {
   return precision == SMCF_PRECISION_MIXED ? 16 * kMixedMT : precision == SMCF_PRECISION_MIXED_BF16 ? 16 * kBfMT
                                                                                                      : kGibbsKB;
@@ -1315,8 +1470,8 @@ hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& c
   if (bt.QX) {
     if (bt.kb != 16 * kBfMT) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gibbs_bf, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q,
-                       (const bf16x8*)bt.QX, bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, bt.qdiag, bt.key, bt.iter,
-                       bt.active, bt.par);
+                       (const bf16x8*)bt.QX, bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, (bf16x8*)bt.DX, bt.mref,
+                       bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
   } else if (bt.Q32) {
     if (bt.kb != 16 * kMixedMT) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.Q32, bt.S,

@@ -5,9 +5,6 @@
 #include <stdint.h>
 
 #define SMCF_DEV __device__ __forceinline__
-#ifndef SMCF_PHILOX_KEY_BARRIER
-#define SMCF_PHILOX_KEY_BARRIER 0
-#endif
 
 namespace smcf {
 
@@ -51,18 +48,6 @@ SMCF_DEV void philox_draw(uint64_t key, uint32_t site, uint32_t stage, uint32_t 
   }
   w0 = ((uint64_t)x1 << 32) | x0;
   w1 = ((uint64_t)x3 << 32) | x2;
-}
-
-// philox_draw for a wave-uniform key: the round keys are re-derived at every call (two scalar adds per
-// round) rather than hoisted out of the caller's loops as 20 loop-invariant scalar registers
-SMCF_DEV void philox_draw_u(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1, uint32_t c0,
-                            uint64_t& w0, uint64_t& w1) {
-#if SMCF_PHILOX_KEY_BARRIER
-  uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
-  asm volatile("" : "+s"(k0), "+s"(k1));
-  key = ((uint64_t)k1 << 32) | k0;
-#endif
-  philox_draw(key, site, stage, c2, c1, c0, w0, w1);
 }
 
 SMCF_DEV double u01(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
@@ -283,6 +268,7 @@ SMCF_DEV float erfc_cheb32(float x) {
 }
 // uniform strictly inside (0, 1) from the top 23 bits of a Philox word (exact in single precision)
 SMCF_DEV float u01_open_f32(uint64_t w) { return ((float)(w >> 41) + 0.5f) * 0x1.0p-23f; }
+
 // Same construction as tn_std: p = Phi(a) + u m and q = Q(b) + (1 - u) m, the smaller inverted.  NaN
 // (the caller then takes the double-precision draw) when a bound lies beyond 8 sigma on the far side of
 // the mean, or when the interval holds less than 1e-30 of the mass.

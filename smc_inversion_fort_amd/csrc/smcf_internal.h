@@ -96,6 +96,13 @@ struct BatchDev {
   uint16_t* SX;
   uint16_t* SXb;
   uint16_t* QX;
+  //   HX       H in fragment order (64-row blocks), formed once per instance for the centred likelihood
+  //   DX       D = S - mref of the stage's moved particles, [ndp/8][P][hi 8 | lo 8] (pad rows zero)
+  uint16_t* HX;
+  uint16_t* DX;
+  double* mref;             // ndim x cap: reference particle of the stage (centred likelihood)
+  double* gref;             // ndim x cap: H mref - b
+  double* lref;             // cap: L(mref)
   double* Y;                // ndim x P x cap (H X of the HMC leapfrog; null for the Gibbs mover)
   double* like;             // P x cap
   double* prior;
@@ -172,6 +179,11 @@ hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, cons
 hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 // split-BF16 fragment-ordered copy of the stage's blocked precision (SMCF_PRECISION_MIXED_BF16)
 hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+// centred split-BF16 likelihood (SMCF_PRECISION_MIXED_BF16): H in fragment order for the given slots (set-up),
+// the stage's reference particle, H m - b and L(m) (before the Gibbs sweeps), and the likelihood (after them)
+hipError_t launch_h_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
+hipError_t launch_like_ref(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+hipError_t launch_like_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 // coordinates of the split-BF16 particle copies (nd rounded up to whole 32-row matrix-core steps)
 __host__ __device__ inline int bf_rows(int nd) { return (nd + 31) & ~31; }
 // bf16 values of one instance's fragment-ordered precision QX (KB-row blocks x 32-column steps)
