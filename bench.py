@@ -46,8 +46,11 @@ def parse():
     ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--precision", choices=["all", "fp64", "mixed", "bf16"], nargs="+", default=["all"],
+    ap.add_argument("--precision", choices=["all", "fp64", "mixed", "bf16"], nargs="+", default=["fp64", "bf16"],
                     help="arithmetic forms to run (the first one run gives `value`)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the serving run (Poisson arrivals of small inversions); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean request arrivals per second")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
@@ -115,14 +118,83 @@ def run_form(a, S, syn, obs, precision, keys_for):
                         "k_gibbs_bf (split-BF16 matrix cores, 3 products per term, + FP32 VALU draws; "
                         "achieved counts the 3 BF16 products; peak = dense BF16 matrix)")
     achieved = flops / (ms_gibbs * 1e-3) / 1e12 if ms_gibbs > 0 else 0.0
+    traffic = traffic_from_profiles(name)
+    avg_ms = ms_gibbs / max(n_gibbs, 1)
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(name), "kernel": kernel,
-            "launches": n_gibbs, "avg_launch_ms": round(ms_gibbs / max(n_gibbs, 1), 4)}
+            "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kernel,
+            "launches": n_gibbs, "avg_launch_ms": round(avg_ms, 4),
+            # the measured HBM bytes per launch (rocprofv3, profiles/) over this run's average launch time
+            "hbm_tb_per_s": round(traffic / (avg_ms * 1e-3) / 1e12, 3) if traffic and avg_ms > 0 else None}
     out = dict(value=B * a.steps / elapsed, ms_per_step=1e3 * elapsed / a.steps, stages=stages, lat=lat,
                neglog=np.concatenate(neglog), roofline=roof, ndim=ndim, nobs=pb.nobs, P=P, sweeps=sweeps)
     pb.close()
     ctx.close()
     return out
+
+
+def run_serving(a, S, syn):
+    """Serving form, scaled to a bench run: inversion requests arrive as a Poisson stream for
+    --serve-seconds at --serve-rate per second; each carries its own mesh (log-uniform 4x4 .. 12x8 patches),
+    station set (log-uniform 50 .. 400 GNSS stations), noise (2 .. 10 mm) and outer particle count
+    (log-uniform 64 .. 512), and runs an outer SMC of 3 stages (inner P = 512, Gibbs x3, split-BF16 form).
+    Requests are admitted by a 200 GB device-memory budget and overlap on the GPU.  Latency is submission to
+    posterior-ready, on the server's clock.  Observation sets are generated before the clock starts."""
+    rng = np.random.default_rng(np.random.SeedSequence([a.seed, 0x5E7E]))
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / a.serve_rate)
+        if t > a.serve_seconds:
+            break
+        arrivals.append(t)
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    gctx = S.Context(0)
+    reqs = []
+    for k, _t in enumerate(arrivals):
+        nxi = int(round(np.exp(rng.uniform(np.log(4), np.log(12)))))
+        neta = int(round(np.exp(rng.uniform(np.log(4), np.log(8)))))
+        nsta = int(round(np.exp(rng.uniform(np.log(50), np.log(400)))))
+        sigma = rng.uniform(0.2, 1.0)
+        N = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
+
+        def greens(th, pts, uv, nx, ne):
+            obs0 = dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.ones(pts.shape[1]),
+                        dvec=np.zeros(pts.shape[1]), nsar_total=0, ngnss=pts.shape[1] // 3)
+            p = S.Problem(gctx, [nx], [ne], xmin, xmax, zmin, obs0, syn.MAX_SLIP, 8)
+            g = p.calc_greens_func(th)
+            p.close()
+            return g
+        obs = syn.make_observations("c2", 1000 + k, greens, nsta=nsta, sigma=sigma, nxi=nxi, neta=neta)
+        desc = S.problem_desc([nxi], [neta], xmin, xmax, zmin, obs, syn.MAX_SLIP, 512, mover=S.MOVER_GIBBS,
+                              gibbs_sweeps=3, precision=S.PRECISION_MIXED_BF16)
+        reqs.append(dict(desc=desc, N=N, nsta=nsta))
+    gctx.close()
+    srv = S.Server(0, hbm_budget_bytes=200 << 30)
+    rr = syn.prior_range()
+    print("serving: %d requests over %.0f s" % (len(reqs), a.serve_seconds), flush=True)
+    t0 = time.perf_counter()
+    ids = []
+    for k, r in enumerate(reqs):
+        dt = arrivals[k] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(r["desc"][0], rr, r["N"], 7000 + k, max_stages=3, max_batch=r["N"]))
+    res = [srv.wait(i) for i in ids]
+    srv.close()
+    lat = np.array([x["t_done"] - x["t_submit"] for x in res])
+    span = max(x["t_done"] for x in res) - min(x["t_submit"] for x in res)
+    # outer-particle evaluations: N initial + 2N per completed stage (parents re-evaluated, children)
+    evals = sum(r["N"] * (1 + 2 * max(x["nstage"] - 1, 0)) for r, x in zip(reqs, res))
+    print("serving: p50 %.3f s  p99 %.3f s  over %d requests" % (np.percentile(lat, 50), np.percentile(lat, 99),
+                                                                   len(lat)), flush=True)
+    return {"requests": len(res), "seconds": a.serve_seconds, "arrival_rate_per_s": a.serve_rate,
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+            "max_latency_s": round(float(lat.max()), 3),
+            "requests_per_s": round(len(res) / span, 3), "outer_evals_per_s": round(evals / span, 2),
+            "form": "bf16", "budget_gb": 200,
+            "workload": "Poisson arrivals; per request 4x4..12x8 patches, 50..400 stations, 2..10 mm noise, "
+                        "N 64..512 outer particles, 3 outer stages, inner P=512 Gibbs x3"}
 
 
 def main():
@@ -190,6 +262,8 @@ def main():
                "sample": "%d c2 outer-particle evaluation(s) (same workload, P=2000, Gibbs x5, FP64) in %.1f s"
                          % (a.cpu_evals, ct)}
 
+    serving = run_serving(a, S, syn) if a.serve_seconds > 0 else None
+
     def latency(r):
         return (round(float(np.percentile(r["lat"], 50)), 3), round(float(np.percentile(r["lat"], 99)), 3))
     p50, p99 = latency(head)
@@ -206,7 +280,7 @@ def main():
                       "nobs": head["nobs"], "inner_particles": head["P"],
                       "mean_inner_stages": round(float(head["stages"].mean()), 2),
                       "p50_wave_latency_s": p50, "p99_wave_latency_s": p99},
-           "roofline": head["roofline"], "cpu_baseline": cpu}
+           "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving}
     if "fp64" in res and len(res) > 1:
         f64 = res["fp64"]
         mixed = {}

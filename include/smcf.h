@@ -14,6 +14,7 @@
  */
 #ifndef SMCF_H
 #define SMCF_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -78,6 +79,10 @@ typedef struct {
 
 /* Upload a problem; device buffers are sized for up to max_batch geometries in flight. */
 int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* desc, int max_batch, smcf_problem** out);
+/* Device bytes smcf_problem_create would allocate for this description (no device work; no counterpart in the
+ * reference, used for admission by memory budget), and the bytes a created problem holds. */
+int smcf_problem_bytes(const smcf_problem_desc* desc, int max_batch, size_t* bytes);
+int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes);
 int smcf_problem_destroy(smcf_problem* pb);
 int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_slip, int* ndim_fault,
                       int* nobs);
@@ -157,6 +162,25 @@ int smcf_fault_smc_exec_gsmc(smcf_problem* pb, const double* range, int N, uint6
                              double* evidence, double* gamma_trace);
 /* RNG key of the inner SMC of one outer evaluation: (seed, outer stage, role 0 parent / 1 child, slot) */
 uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint32_t slot);
+
+/*
+ * Serving (no counterpart in the reference, whose main.f90 runs one inversion per job).  A server admits
+ * inversion requests - a problem description plus the outer-SMC settings of smcf_fault_smc_exec - against a
+ * device-memory budget as they arrive (first fit in arrival order, sized by smcf_problem_bytes), and runs each
+ * admitted request on its own context and host thread, so requests overlap on the device; the problem stays
+ * resident for the whole outer SMC.  A request's result does not depend on what runs beside it (keyed RNG).
+ * smcf_server_submit copies its inputs; smcf_server_wait blocks until the request is done, returns its results
+ * as smcf_fault_smc_exec would (gamma_trace: max_stages + 1 entries) and times[3] = seconds from server start
+ * to submission, admission and completion.  Each request id is waited for once.
+ */
+typedef struct smcf_server smcf_server;
+int smcf_server_create(int device, size_t hbm_budget_bytes, smcf_server** out);
+int smcf_server_submit(smcf_server* s, const smcf_problem_desc* desc, int max_batch, const double* range, int N,
+                       uint64_t seed, int max_stages, int64_t* id);
+int smcf_server_wait(smcf_server* s, int64_t id, double* particles, double* likelihood, int* nstage,
+                     double* evidence, double* gamma_trace, double* times);
+int smcf_server_status(smcf_server* s, int* queued, int* running, size_t* bytes_in_use);
+int smcf_server_destroy(smcf_server* s);
 
 /* sort.cpp:3-7 - 1-based index sorted by descending arr(index) (std::sort, same tie order). */
 void qsort_(int* arr, int* index, int* n);

@@ -55,7 +55,9 @@ EXPORTS = [
     "smcf_problem_create", "smcf_problem_destroy", "smcf_problem_dims", "smcf_dc3d0",
     "smcf_calc_greens_func", "smcf_laplacian", "smcf_slip_calc_gsd_gsg", "smcf_slip_smc_exec",
     "smcf_fault_calc_likelihood_batch", "smcf_find_next_gamma", "smcf_normalize_resample",
-    "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec", "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_",
+    "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec",
+    "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_", "smcf_problem_bytes", "smcf_problem_device_bytes",
+    "smcf_server_create", "smcf_server_submit", "smcf_server_wait", "smcf_server_status", "smcf_server_destroy",
 ]
 
 
@@ -99,6 +101,14 @@ def _declare(L):
     L.smcf_inner_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
     L.smcf_inner_key.restype = C.c_uint64
     L.qsort_.argtypes = [_I, _I, _I]
+    L.smcf_problem_bytes.argtypes = [C.POINTER(ProblemDesc), C.c_int, C.POINTER(C.c_size_t)]
+    L.smcf_problem_device_bytes.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.smcf_server_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+    L.smcf_server_submit.argtypes = [C.c_void_p, C.POINTER(ProblemDesc), C.c_int, _D, C.c_int, C.c_uint64, C.c_int,
+                                     C.POINTER(C.c_int64)]
+    L.smcf_server_wait.argtypes = [C.c_void_p, C.c_int64, _D, _D, _I, C.POINTER(C.c_double), _D, _D]
+    L.smcf_server_status.argtypes = [C.c_void_p, _I, _I, C.POINTER(C.c_size_t)]
+    L.smcf_server_destroy.argtypes = [C.c_void_p]
 
 
 def _check(rc):
@@ -216,6 +226,83 @@ class Context:
         return out
 
 
+def problem_desc(nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip, mover=MOVER_GIBBS,
+                 gibbs_sweeps=5, fix_xbend=False, xbend=(), precision=PRECISION_FP64):
+    """smcf_problem_desc of an observation set and fault mesh; returns (desc, arrays the desc points into)."""
+    keep = dict(
+        nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
+        nsar_index=_i32(obs.get("nsar_index", [1, 1 + int(obs.get("nsar_total", 0))])),
+        obs_points=_f64(obs["obs_points"]), obs_unitvec=_f64(obs["obs_unitvec"]),
+        obs_sigma=_f64(obs["obs_sigma"]), dvec=_f64(obs["dvec"]),
+        sar=_f64(obs["sar_sinv"] if obs.get("sar_sinv") is not None else np.zeros(1)))
+    k = keep
+    nobs = k["dvec"].shape[0]
+    desc = ProblemDesc(len(k["nxi"]), _ip(k["nxi"]), _ip(k["neta"]), xmin, xmax, zmin, int(bool(fix_xbend)),
+                       _dp(k["xbend"]), nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
+                       len(k["nsar_index"]) - 1, _ip(k["nsar_index"]), _dp(k["obs_points"]),
+                       _dp(k["obs_unitvec"]), _dp(k["obs_sigma"]), _dp(k["dvec"]), _dp(k["sar"]),
+                       float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps), int(precision))
+    return desc, keep
+
+
+def problem_bytes(desc, max_batch):
+    """Device bytes a problem of this description allocates (smcf_problem_bytes; no device work)."""
+    n = C.c_size_t()
+    _check(lib().smcf_problem_bytes(C.byref(desc), int(max_batch), C.byref(n)))
+    return n.value
+
+
+class Server:
+    """Serving loop of libsmcf (smcf_server_*): inversion requests admitted by a device-memory budget as
+    they arrive, each run on its own context and host thread, overlapping on the device."""
+
+    def __init__(self, device=0, hbm_budget_bytes=200 << 30):
+        h = C.c_void_p()
+        _check(lib().smcf_server_create(int(device), C.c_size_t(int(hbm_budget_bytes)), C.byref(h)))
+        self.h = h
+        self._req = {}
+
+    def submit(self, desc, rng, N, seed, max_stages=50, max_batch=None):
+        """Queue one outer SMC (smcf_fault_smc_exec settings); returns the request id."""
+        rng = _f64(rng)
+        if max_batch is None:
+            max_batch = min(int(N), 512)
+        rid = C.c_int64()
+        _check(lib().smcf_server_submit(self.h, C.byref(desc), int(max_batch), _dp(rng), int(N), C.c_uint64(seed),
+                                        int(max_stages), C.byref(rid)))
+        self._req[rid.value] = (rng.shape[1], int(N), int(max_stages))
+        return rid.value
+
+    def wait(self, rid):
+        nf, N, max_stages = self._req.pop(rid)
+        X = np.zeros((nf, N), order="F")
+        L = np.zeros(N)
+        ns = C.c_int()
+        ev = C.c_double()
+        gt = np.zeros(max_stages + 1)
+        times = np.zeros(3)
+        _check(lib().smcf_server_wait(self.h, C.c_int64(rid), _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt),
+                                      _dp(times)))
+        return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy(),
+                    t_submit=times[0], t_start=times[1], t_done=times[2])
+
+    def status(self):
+        q, r, b = C.c_int(), C.c_int(), C.c_size_t()
+        _check(lib().smcf_server_status(self.h, C.byref(q), C.byref(r), C.byref(b)))
+        return dict(queued=q.value, running=r.value, bytes_in_use=b.value)
+
+    def close(self):
+        if self.h:
+            lib().smcf_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Problem:
     """Observations + fault mesh + inner-SMC settings resident on the device."""
 
@@ -223,20 +310,8 @@ class Problem:
                  mover=MOVER_GIBBS, gibbs_sweeps=5, max_batch=1, fix_xbend=False, xbend=(),
                  precision=PRECISION_FP64):
         self.ctx = ctx
-        self._keep = dict(
-            nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
-            nsar_index=_i32(obs.get("nsar_index", [1, 1 + int(obs.get("nsar_total", 0))])),
-            obs_points=_f64(obs["obs_points"]), obs_unitvec=_f64(obs["obs_unitvec"]),
-            obs_sigma=_f64(obs["obs_sigma"]), dvec=_f64(obs["dvec"]),
-            sar=_f64(obs["sar_sinv"] if obs.get("sar_sinv") is not None else np.zeros(1)))
-        k = self._keep
-        nobs = k["dvec"].shape[0]
-        self.desc = ProblemDesc(len(k["nxi"]), _ip(k["nxi"]), _ip(k["neta"]), xmin, xmax, zmin, int(bool(fix_xbend)),
-                                _dp(k["xbend"]), nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
-                                len(k["nsar_index"]) - 1, _ip(k["nsar_index"]), _dp(k["obs_points"]),
-                                _dp(k["obs_unitvec"]), _dp(k["obs_sigma"]), _dp(k["dvec"]), _dp(k["sar"]),
-                                float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps),
-                                int(precision))
+        self.desc, self._keep = problem_desc(nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip,
+                                             mover, gibbs_sweeps, fix_xbend, xbend, precision)
         h = C.c_void_p()
         _check(lib().smcf_problem_create(ctx.h, C.byref(self.desc), int(max_batch), C.byref(h)))
         self.h = h

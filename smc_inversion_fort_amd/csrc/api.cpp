@@ -77,6 +77,11 @@ template <class T>
 int dalloc(Problem& P, T** ptr, size_t n) {
   void* p = nullptr;
   if (n == 0) n = 1;
+  P.bytes += (n * sizeof(T) + 255) & ~(size_t)255;
+  if (P.sizing) {
+    *ptr = nullptr;
+    return SMCF_OK;
+  }
   hipError_t e = hipMalloc(&p, n * sizeof(T));
   if (e != hipSuccess) {
     set_error(std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " bytes): " + hipGetErrorString(e));
@@ -92,7 +97,7 @@ int upload(Problem& P, const T** dst, const std::vector<T>& v) {
   T* d = nullptr;
   int rc = dalloc(P, &d, v.size());
   if (rc) return rc;
-  if (!v.empty()) SMCF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty() && !P.sizing) SMCF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   *dst = d;
   return SMCF_OK;
 }
@@ -350,8 +355,10 @@ int smcf_kernel_stats(smcf_ctx* ctx, const char* name, long* count, double* tota
   return SMCF_OK;
 }
 
-int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch, smcf_problem** out) {
-  if (!ctx || !d || !out || max_batch < 1 || d->nplane < 1 || d->nparticle_slip < 1) {
+// Builds a problem on ctx's device, or (sizing) only counts the device bytes it would allocate.
+static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch, bool sizing, smcf_problem** out,
+                         size_t* bytes) {
+  if ((!ctx && !sizing) || !d || (!out && !sizing) || max_batch < 1 || d->nplane < 1 || d->nparticle_slip < 1) {
     set_error("invalid problem description");
     return SMCF_EINVAL;
   }
@@ -360,10 +367,11 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
               ": the tempering workgroup keeps all weights in 64 KiB of LDS");
     return SMCF_EINVAL;
   }
-  SMCF_HIP(hipSetDevice(ctx->device));
+  if (!sizing) SMCF_HIP(hipSetDevice(ctx->device));
   auto holder = std::make_unique<smcf_problem>();
   Problem& P = holder->p;
   P.ctx = ctx;
+  P.sizing = sizing;
   P.max_slip = d->max_slip;
   P.P = d->nparticle_slip;
   P.mover = d->mover;
@@ -561,7 +569,7 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
     rc |= dalloc(P, &bt.gref, nd * C);
     rc |= dalloc(P, &bt.lref, C);
     // rows past ndim of the centred copy are read by the likelihood's matrix products and never written
-    if (!rc && hipMemset(bt.DX, 0, sizeof(uint16_t) * (size_t)bf_rows((int)nd) * Pn * 2 * C) != hipSuccess) {
+    if (!rc && !sizing && hipMemset(bt.DX, 0, sizeof(uint16_t) * (size_t)bf_rows((int)nd) * Pn * 2 * C) != hipSuccess) {
       set_error("hipMemset failed");
       rc = SMCF_EHIP;
     }
@@ -625,9 +633,28 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
     for (void* p : P.allocs) (void)hipFree(p);
     return rc;
   }
+  if (sizing) {
+    *bytes = P.bytes;
+    return SMCF_OK;
+  }
   SMCF_HIP(hipMemset(bt.parent, 0, sizeof(int) * Pn * C));
   SMCF_HIP(hipMemset(bt.gtrace, 0, sizeof(double) * 202 * C));
   *out = holder.release();
+  return SMCF_OK;
+}
+
+int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch, smcf_problem** out) {
+  return problem_build(ctx, d, max_batch, false, out, nullptr);
+}
+
+int smcf_problem_bytes(const smcf_problem_desc* d, int max_batch, size_t* bytes) {
+  if (!bytes) return SMCF_EINVAL;
+  return problem_build(nullptr, d, max_batch, true, nullptr, bytes);
+}
+
+int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes) {
+  if (!pb || !bytes) return SMCF_EINVAL;
+  *bytes = pb->p.bytes;
   return SMCF_OK;
 }
 

@@ -145,6 +145,8 @@ struct Problem {
   std::vector<int> sar_r0, sar_n;  // host copies of the InSAR path layout (launch geometry)
   std::vector<long> sar_off;
   std::vector<void*> allocs;
+  bool sizing = false;  // count the device bytes of a description without allocating (smcf_problem_bytes)
+  size_t bytes = 0;     // device bytes allocated (or counted) for the problem
 };
 
 struct StageCfg {
