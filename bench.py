@@ -50,7 +50,7 @@ def parse():
                     help="arithmetic forms to run (the first one run gives `value`)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the serving run (Poisson arrivals of small inversions); 0 skips it")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean request arrivals per second")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")

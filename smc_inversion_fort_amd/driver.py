@@ -341,12 +341,13 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mover", choices=["hmc", "gibbs"], default="hmc")
     ap.add_argument("--sweeps", type=int, default=5)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "bf16"], default="fp64",
+                    help="Gibbs arithmetic: fp64, mixed (FP32 products) or bf16 (split-BF16 products)")
     ap.add_argument("--max-stages", type=int, default=100)
     ap.add_argument("--slots", type=int, default=256, help="device slots (inner SMC instances in flight)")
     a = ap.parse_args(argv)
     mover = 0 if a.mover == "hmc" else 1
-    prec = 0 if a.precision == "fp64" else 1
+    prec = {"fp64": 0, "mixed": 1, "bf16": 2}[a.precision]
     if a.mode == "likelihood":
         run_likelihood(a.dir, a.seed, mover, a.sweeps, prec)
     else:
