@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--evals", type=int, default=1024, help="geometry evaluations per step (one wave)")
+    ap.add_argument("--warmup-evals", type=int, default=0,
+                    help="geometry evaluations per warm-up step (default: as --evals)")
     ap.add_argument("--slots", type=int, default=512, help="device slots (instances in flight)")
     ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -84,9 +86,10 @@ def run_form(a, S, syn, obs, precision, keys_for):
     name = FORM_NAMES[precision]
     print("bench[%s]: c2 16x16 patches ndim=%d nobs=%d P=%d evals/step=%d slots=%d"
           % (name, ndim, pb.nobs, P, B, a.slots), flush=True)
+    BW = a.warmup_evals or B
     for w in range(a.warmup):
         t0 = time.time()
-        nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(B, 1000 + w), keys_for(1000 + w))
+        nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(BW, 1000 + w), keys_for(1000 + w, BW))
         print("warmup step %d: %.2f s, stages mean %.1f max %d" % (w, time.time() - t0, ns.mean(), ns.max()),
               flush=True)
     ctx.set_profiling(True)
@@ -232,11 +235,11 @@ def main():
     gctx.close()
     B = a.evals
 
-    def keys_for(step):
+    def keys_for(step, n=B):
         # the 64-bit products wrap modulo 2^64 by design (seed mixing); numpy would warn on the scalar wrap
         with np.errstate(over="ignore"):
             base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(step + 1)
-            return (np.arange(B, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
+            return (np.arange(n, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
 
     forms = []
     for f in a.precision:

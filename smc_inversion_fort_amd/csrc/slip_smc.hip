@@ -575,25 +575,30 @@ hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hi
   return hipGetLastError();
 }
 
-// Q is stored per stage in blocks of KB = 32 columns, row-major inside the block:
-//   Qb[(blk*nd + j)*32 + k] = Q(j, blk*32 + k)   (zero for columns past nd)
+// Q is stored per stage in blocks of KB columns, row-major inside the block:
+//   Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k)   (zero for columns past nd)
 //
 // One wavefront owns 64 particles; a workgroup of 4 wavefronts owns 256 particles of one instance.
-// Per block of 32 coordinates (two 16-coordinate tiles m) and sweep:
+// Per block of KB = 16 MT coordinates (MT 16-coordinate tiles m) and sweep:
 //  * matrix phase: q(i0 + 16m + r, p) = sum_j Q(j, i0 + 16m + r) s_j(p) for the wave's 64 particles
-//    (four 16-particle tiles n) as 8 accumulators of v_mfma_f64_16x16x4f64 over j in steps of 4,
-//    software-pipelined (mfma_panel).  Operand fragments are plain coalesced loads: A(r, k) =
+//    (four 16-particle tiles n) as 4 MT accumulators of v_mfma_f64_16x16x4f64 over j in steps of 4,
+//    software-pipelined (mfma_panel_t).  Operand fragments are plain coalesced loads: A(r, k) =
 //    Q(j0 + k, col) at lane r + 16k, B(k, c) = s_{j0+k}(p0 + c) at lane c + 16k; lane l holds
 //    D(4t + l/16, l%16) in register t.
-//  * draw phase, per tile: the tile's accumulators are transposed through 8 KB of wave-private LDS so
-//    that lane l finds q of its own particle; the 16 coordinates are redrawn in order with right-looking
-//    in-tile corrections (VALU), and tile 0's deltas, left in LDS, correct tile 1 by 16 MFMAs.
+//  * draw phase, per tile: the tile's constants are staged in LDS, its accumulators are transposed through
+//    8 KB of wave-private LDS so that lane l finds q of its own particle; the 16 coordinates are redrawn in
+//    order with left-looking in-tile corrections (VALU), and the tile's deltas, left in LDS, correct the
+//    block's later tiles by 16 MFMAs each.
 // While one wavefront draws (VALU) another can run its matrix phase on the matrix cores.
 //
 // Workgroup -> (instance, particle chunk) is XCD-aware: workgroup w runs on XCD w % 8, and all
 // chunks of one instance get the same w % 8, so an instance's Q blocks are fetched into one L2.
 constexpr int kXcd = 8;
-constexpr int kGibbsKB = 32;
+#ifndef SMCF_GIBBS_MT
+#define SMCF_GIBBS_MT 2
+#endif
+constexpr int kGibbsMT = SMCF_GIBBS_MT;  // 16-row tiles per block of the FP64 Gibbs kernel
+constexpr int kGibbsKB = 16 * kGibbsMT;
 #ifndef SMCF_GIBBS_OCC
 #define SMCF_GIBBS_OCC 4
 #endif
@@ -605,7 +610,7 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
                                                const uint64_t* __restrict__ keys,
                                                const int* __restrict__ iters, const int* __restrict__ active,
                                                const int* __restrict__ par) {
-  constexpr int KB = kGibbsKB;
+  constexpr int MT = kGibbsMT, KB = kGibbsKB;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
   //   CQ[kk (kk - 1) / 2 + k] = Q(i0 + 16m + k, i0 + 16m + kk), k < kk: the strictly lower part of the tile's
@@ -638,6 +643,9 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
 #pragma unroll
   for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
   int nd4 = nd & ~3;
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * KB + 16 * m + fi);
   double (*Tw)[65] = T[wave];
   double* CQw = CQ[wave];
   double (*CDw)[16] = CD[wave];
@@ -654,40 +662,29 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
     for (int blk = 0; blk < nblk; ++blk) {
       int i0 = blk * KB;
       const double* Qr = Qb + (size_t)blk * nd * KB;
-      dbl4 acc[2][4];
+      dbl4 acc[MT][4];
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int n = 0; n < 4; ++n) acc[m][n] = dbl4{0.0, 0.0, 0.0, 0.0};
       const double* qa = Qr + (size_t)fk * KB + fi;
       const double* sb = S + (size_t)fk * P;
-      mfma_panel<kPanelBuffers>(Qr, (uint32_t)(fk * KB + fi), (uint32_t)(fk * KB + fi + 16), (uint32_t)(4 * KB), S, bo,
-                                (uint32_t)(4 * P), nd4 / 4, acc);
+      mfma_panel_t<MT, kPanelBuffers>(Qr, ao, (uint32_t)(4 * KB), S, bo, (uint32_t)(4 * P), nd4 / 4, acc);
       if (nd4 < nd) {  // last rows (nd = 2 mod 4): rows past nd contribute zero
         bool in = nd4 + fk < nd;
-        double a0 = in ? qa[(size_t)nd4 * KB] : 0.0, a1 = in ? qa[(size_t)nd4 * KB + 16] : 0.0;
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
           double bn = in ? sb[(size_t)nd4 * P + pcol[n]] : 0.0;
-          acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bn, acc[0][n], 0, 0, 0);
-          acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bn, acc[1][n], 0, 0, 0);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            double am = in ? qa[(size_t)nd4 * KB + 16 * m] : 0.0;
+            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(am, bn, acc[m][n], 0, 0, 0);
+          }
         }
       }
 #pragma unroll
-      for (int m = 0; m < 2; ++m) {
-        if (m == 1) {
-          // corrections of tile 1 by the redrawn tile 0: acc1(r, p) += sum_k Q(i0 + k, i0 + 16 + r) delta_k(p),
-          // with delta_k(p) left in Tw[k][p] by the draws of tile 0
-#pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4) {
-            int kq = 4 * s4 + fk;
-            double a = (i0 + kq < nd) ? Qr[(size_t)(i0 + kq) * KB + 16 + fi] : 0.0;
-#pragma unroll
-            for (int n = 0; n < 4; ++n)
-              acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Tw[kq][16 * n + fi], acc[1][n], 0, 0, 0);
-          }
-          __builtin_amdgcn_wave_barrier();  // every lane has read tile 0's deltas before they are overwritten
-        }
+      for (int m = 0; m < MT; ++m) {
+        if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
         {  // the tile's constants into LDS
           int ik = i0 + 16 * m + fi;
           CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
@@ -740,6 +737,20 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // corrections of the later tiles by this tile's deltas (in Tw[k][p]):
+        //   acc[m2](r, p) += sum_k Q(i0 + 16m + k, i0 + 16m2 + r) delta_k(p)
+#pragma unroll
+        for (int m2 = m + 1; m2 < MT; ++m2) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            int kq = 4 * s4 + fk;
+            double a = (i0 + 16 * m + kq < nd) ? Qr[(size_t)(i0 + 16 * m + kq) * KB + 16 * m2 + fi] : 0.0;
+#pragma unroll
+            for (int n = 0; n < 4; ++n)
+              acc[m2][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Tw[kq][16 * n + fi], acc[m2][n], 0, 0, 0);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // every lane has read the tile's deltas before the next tile overwrites them
       }
     }
   }
