@@ -25,6 +25,12 @@ import os
 import sys
 import time
 
+# The serving run overlaps many requests, each with its own two HIP streams; the HIP runtime maps streams onto
+# GPU_MAX_HW_QUEUES hardware queues per process (4 by default), so with more requests in flight than queues their
+# kernels serialize.  16 queues let eight requests run side by side (read once, at HIP initialisation, which
+# happens below this line).  The single-problem forms use two streams and are unaffected.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
