@@ -121,6 +121,12 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
 int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys,
                                      double* neglog, int* nstage);
 
+/* Plain mean of each geometry's final slip particles (ndim_slip x B, column b for theta column b) and its -log Z:
+ * the per-particle loop of calc_slip_map (main.f90:600-640, fault_calc_likelihood then the average over the
+ * nparticle_slip particles), batched.  B <= max_batch. */
+int smcf_slip_mean_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys, double* mean,
+                         double* neglog);
+
 /* Inner-SMC components on the device (for parity checks and reuse). */
 /* slip_find_next_gamma / fault_find_next_gamma (smc_slip.f90:314-387, smc_fault.f90:392-485) */
 int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gamma_prev, const double* likelihood, double cv,

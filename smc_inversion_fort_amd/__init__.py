@@ -58,6 +58,7 @@ EXPORTS = [
     "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec",
     "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_", "smcf_problem_bytes", "smcf_problem_device_bytes",
     "smcf_server_create", "smcf_server_submit", "smcf_server_wait", "smcf_server_status", "smcf_server_destroy",
+    "smcf_slip_mean_batch",
 ]
 
 
@@ -109,6 +110,7 @@ def _declare(L):
     L.smcf_server_wait.argtypes = [C.c_void_p, C.c_int64, _D, _D, _I, C.POINTER(C.c_double), _D, _D]
     L.smcf_server_status.argtypes = [C.c_void_p, _I, _I, C.POINTER(C.c_size_t)]
     L.smcf_server_destroy.argtypes = [C.c_void_p]
+    L.smcf_slip_mean_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _D]
 
 
 def _check(rc):
@@ -411,6 +413,18 @@ class Problem:
                                                   cb, _dp(qm), _dp(qc), _dp(qi), float(gsmc[3]),
                                                   _dp(X), _dp(L), C.byref(ns), C.byref(ev), _dp(gt)))
         return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy())
+
+    def slip_mean(self, thetas, keys):
+        """Plain mean of the final slip particles and -log Z for each column of `thetas` (at most max_batch
+        columns per call; calc_slip_map's loop, main.f90:600-640)."""
+        th = self._theta(thetas)
+        B = th.shape[1]
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        mean = np.zeros((self.ndim, B), order="F")
+        nl = np.zeros(B)
+        _check(lib().smcf_slip_mean_batch(self.h, B, _dp(np.asfortranarray(th)), k.ctypes.data_as(_U64), _dp(mean),
+                                          _dp(nl)))
+        return mean, nl
 
     def fault_calc_likelihood(self, thetas, keys):
         """-log Z for each column of `thetas` (ndim_fault x B), RNG key per column."""

@@ -762,6 +762,47 @@ int smcf_slip_calc_gsd_gsg(smcf_problem* pb, int B, const double* theta, double*
   return SMCF_OK;
 }
 
+// plain mean over the P final particles of instances 0 .. B-1 (the loop of calc_slip_map, main.f90:620-633:
+// ascending particle order, then one division)
+__global__ void k_plain_mean(int nd, int P, int B, BatchDev bt, double* __restrict__ mean) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (i >= nd || b >= B) return;
+  const double* S = (bt.par[b] ? bt.S2 : bt.S) + ((size_t)b * nd + i) * P;
+  double acc = 0.0;
+  for (int p = 0; p < P; ++p) acc += S[p];
+  mean[(size_t)b * nd + i] = acc / P;
+}
+
+int smcf_slip_mean_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys, double* mean,
+                         double* neglog) {
+  if (!pb || !theta || !keys || !mean || B < 1) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  if (B > P.bt.cap) {
+    set_error("smcf_slip_mean_batch: B exceeds the problem's max_batch");
+    return SMCF_EINVAL;
+  }
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  std::vector<double> nl(B);
+  int rc = run_stream(P, B, theta, keys, false, nl.data(), nullptr);  // B <= cap: slot b ends holding geometry b
+  if (rc) return rc;
+  hipStream_t s = P.ctx->stream;
+  size_t nd = P.m.ndim;
+  double* dm = nullptr;
+  SMCF_HIP(hipMalloc(&dm, sizeof(double) * nd * B));
+  hipLaunchKernelGGL(k_plain_mean, dim3((unsigned)((nd + 127) / 128), B), dim3(128), 0, s, (int)nd, P.bt.P, B, P.bt,
+                     dm);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(mean, dm, sizeof(double) * nd * B, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(dm);
+  if (e != hipSuccess) {
+    set_error(std::string("smcf_slip_mean_batch: ") + hipGetErrorString(e));
+    return SMCF_EHIP;
+  }
+  if (neglog) std::copy(nl.begin(), nl.end(), neglog);
+  return SMCF_OK;
+}
+
 int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, double* particles, double* likelihood,
                        double* prior, double* mean, int* nstage, double* neglog, double* gamma_trace) {
   if (!pb || !theta) return SMCF_EINVAL;

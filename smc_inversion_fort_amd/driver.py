@@ -16,7 +16,9 @@ and runs either of main.f90's two paths:
   python -m smc_inversion_fort_amd.driver likelihood [--dir D]   fault_calc_likelihood at peak_fault.dat with
       the slip particles written to output/slip_from_mean_fault.dat (main.f90:447-483, the active path);
   python -m smc_inversion_fort_amd.driver smc [--dir D]          fault_smc_exec over the prior box of
-      data/range.dat, plain or gsmc as setting.dat says (main.f90:513-549).
+      data/range.dat, plain or gsmc as setting.dat says (main.f90:513-549);
+  python -m smc_inversion_fort_amd.driver slipmap --fault-file F  calc_slip_map (main.f90:553-666);
+  python -m smc_inversion_fort_amd.driver points --fault-file F   point_cloud (main.f90:668-830).
 
 Files are read with the semantics of Fortran list-directed input: every READ starts at a new record, its
 items may continue over following records (blank records are skipped while values are sought), the rest
@@ -334,9 +336,53 @@ def run_smc(workdir, seed=0, mover=0, sweeps=5, precision=0, max_stages=100, max
     return r
 
 
+def run_slip_map(workdir, fault_path, out_path, seed=0, mover=0, sweeps=5, precision=0, max_batch=256,
+                 offset_sample=10, stride=2, out=sys.stdout):
+    """calc_slip_map (main.f90:553-666): geometry samples of an outer-SMC stage file (offset_sample records
+    skipped, then every stride-th record, nparticle_fault of them), the inner SMC of each on the device, the plain
+    mean of its final slip particles on every node written to out_path."""
+    from . import Context
+    from . import lib
+    from . import postprocess as PP
+    st, obs = load_run(workdir)
+    n = st["nparticle_fault"]
+    th = PP.read_samples(fault_path, n, st["ndim_fault"] + 1, offset_sample, stride)[:st["ndim_fault"]]
+    keys = [int(lib().smcf_inner_key(seed, 0, 2, i)) for i in range(n)]
+    ctx = Context(0)
+    pb = make_problem(ctx, st, obs, mover, sweeps, min(max_batch, n), precision)
+    means, nl = PP.calc_slip_map(pb, th, keys, out_path, st["nxi_ls"], st["neta_ls"])
+    print("slip map of %d geometries written to %s" % (n, out_path), file=out)
+    pb.close()
+    ctx.close()
+    return means, nl
+
+
+def run_point_cloud(workdir, fault_path, slip_path, out_prefix, offset_sample=10, stride=100, layout="current",
+                    out=sys.stdout):
+    """point_cloud (main.f90:668-830): geometry samples (every stride-th record after offset_sample) and their
+    mean slips (calc_slip_map's records), 8 x 8 points per patch, one file per plane (<out_prefix><plane>)."""
+    from . import postprocess as PP
+    st = read_settings(os.path.join(workdir, "data", "setting.dat"))
+    n_pl = st["nplane"]
+    nd = 2 * n_pl + 4 if st["fix_xbend"] else 3 * n_pl + 3
+    n = st["nparticle_fault"]
+    th = PP.read_samples(fault_path, n, nd + 1, offset_sample, stride)[:nd]
+    nnode = sum((a + 1) * (b + 1) for a, b in zip(st["nxi_ls"], st["neta_ls"]))
+    slips = PP.read_slip_map(slip_path, n, nnode)
+    pts = PP.point_cloud(th, slips, st["nxi_ls"], st["neta_ls"], st["xmin"], st["xmax"], st["zmin"], st["fix_xbend"],
+                         st["xbend"], layout=layout)
+    PP.write_point_cloud(pts, out_prefix)
+    print("point clouds of %d geometries written to %s<plane>" % (n, out_prefix), file=out)
+    return pts
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("mode", choices=["likelihood", "smc"])
+    ap.add_argument("mode", choices=["likelihood", "smc", "slipmap", "points"])
+    ap.add_argument("--fault-file", help="outer-SMC stage file of geometry samples (slipmap, points)")
+    ap.add_argument("--slip-file", default="output/more/mapslip.dat", help="slip map written (slipmap) or read "
+                    "(points)")
+    ap.add_argument("--points-prefix", default="output/shuf/points.dat", help="point files <prefix><plane>")
     ap.add_argument("--dir", default=".", help="run directory holding data/ (the reference's working directory)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mover", choices=["hmc", "gibbs"], default="hmc")
@@ -350,8 +396,16 @@ def main(argv=None):
     prec = {"fp64": 0, "mixed": 1, "bf16": 2}[a.precision]
     if a.mode == "likelihood":
         run_likelihood(a.dir, a.seed, mover, a.sweeps, prec)
-    else:
+    elif a.mode == "smc":
         run_smc(a.dir, a.seed, mover, a.sweeps, prec, a.max_stages, a.slots)
+    elif a.mode == "slipmap":
+        out_path = os.path.join(a.dir, a.slip_file)
+        os.makedirs(os.path.dirname(out_path) or ".", exist_ok=True)
+        run_slip_map(a.dir, a.fault_file, out_path, a.seed, mover, a.sweeps, prec, a.slots)
+    else:
+        prefix = os.path.join(a.dir, a.points_prefix)
+        os.makedirs(os.path.dirname(prefix) or ".", exist_ok=True)
+        run_point_cloud(a.dir, a.fault_file, os.path.join(a.dir, a.slip_file), prefix)
 
 
 if __name__ == "__main__":

@@ -87,3 +87,15 @@ def test_driver_likelihood_and_smc(tmp_path):
                      open(os.path.join(root, "output", "%d.csv" % s["nstage"])).read().splitlines()])
     np.testing.assert_allclose(last[:, :6], s["particles"].T, atol=6e-6)
     np.testing.assert_allclose(last[:, 6], s["likelihood"], atol=6e-6)
+    # post-processing of the last stage's samples: calc_slip_map on the device, then point_cloud
+    fault_file = os.path.join(root, "output", "%d.csv" % s["nstage"])
+    slip_file = os.path.join(root, "output", "mapslip.dat")
+    means, nl = D.run_slip_map(root, fault_file, slip_file, seed=5, mover=S.MOVER_GIBBS, sweeps=3, max_batch=4,
+                               offset_sample=0, stride=1)
+    rows = open(slip_file).read().splitlines()
+    assert len(rows) == 8 and all(len(x) == 12 * 2 * 20 for x in rows)
+    assert np.all(np.isfinite(nl)) and np.all(means >= 0) and np.all(means <= syn.MAX_SLIP)
+    pts = D.run_point_cloud(root, fault_file, slip_file, os.path.join(root, "output", "points.dat"), offset_sample=0,
+                            stride=1)
+    rows = open(os.path.join(root, "output", "points.dat1")).read().splitlines()
+    assert len(rows) == 8 * 12 * 64 == pts[0].shape[0] and all(len(x) == 6 * 12 for x in rows)
