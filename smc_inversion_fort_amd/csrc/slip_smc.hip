@@ -388,53 +388,15 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 #endif
 constexpr int kPanelBuffers = SMCF_PANEL_BUFFERS;
 
-// One 32-row block of a product Y = A B on the matrix cores (v_mfma_f64_16x16x4f64), 64 columns per
-// wavefront:  acc[m][n] += sum_j A(16m + r, j) B(j, c_n) over j = 0 .. 4 nsteps - 1, where lane
-// (r, k) = (lane % 16, lane / 16) reads A(r, 4st + k) at abase[ao0 + st lda4] (row 16 + r at ao1) and
-// B(4st + k, c_n) at bbase[bo[n] + st ldb4].  Bases are wave-uniform and offsets 32-bit, so the loads
-// take the scalar-base + 32-bit-offset form.  The loop is software-pipelined over NB register buffers:
-// the operands of step st + NB are requested as soon as the products of step st are issued.  The main
-// loop runs whole groups of NB steps and has no branches, so the wait before each group's products
-// covers only that buffer's loads; the last nsteps % NB steps follow it.  (The compiler does not unroll
-// a loop holding MFMA instructions when that needs a remainder loop, so a plain loop would wait for
-// its own loads on every step.)
-template <int NB>
-__device__ __forceinline__ void mfma_panel(const double* __restrict__ abase, uint32_t ao0, uint32_t ao1, uint32_t lda4,
-                                           const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
-                                           int nsteps, dbl4 (&acc)[2][4]) {
-  if (nsteps <= 0) return;
-  double xa[NB], xb[NB], y[NB][4];
-  // loads of step st (clamped to the last step: a group's prefetch past the end re-reads it, unused)
-  auto load = [&](int k, int st) {
-    uint32_t q = (uint32_t)min(st, nsteps - 1);
-    xa[k] = abase[ao0 + q * lda4];
-    xb[k] = abase[ao1 + q * lda4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
-  };
-  auto prod = [&](int k) {
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      acc[0][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[k], y[k][n], acc[0][n], 0, 0, 0);
-      acc[1][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[k], y[k][n], acc[1][n], 0, 0, 0);
-    }
-  };
-#pragma unroll
-  for (int k = 0; k < NB; ++k) load(k, k);
-  int full = nsteps - nsteps % NB;
-  for (int st = 0; st < full; st += NB) {
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      prod(k);
-      load(k, st + k + NB);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < NB - 1; ++k)
-    if (k < nsteps - full) prod(k);
-}
-
-// mfma_panel for MT 16-row tiles (A(16m + r, 4st + k) at abase[ao[m] + st lda4]); same pipeline.
+// One block of MT 16-row tiles of a product Y = A B on the matrix cores (v_mfma_f64_16x16x4f64), 64 columns
+// per wavefront:  acc[m][n] += sum_j A(16m + r, j) B(j, c_n) over j = 0 .. 4 nsteps - 1, where lane
+// (r, k) = (lane % 16, lane / 16) reads A(16m + r, 4st + k) at abase[ao[m] + st lda4] and B(4st + k, c_n) at
+// bbase[bo[n] + st ldb4].  Bases are wave-uniform and offsets 32-bit, so the loads take the scalar-base +
+// 32-bit-offset form.  The loop is software-pipelined over NB register buffers: the operands of step st + NB
+// are requested as soon as the products of step st are issued.  The main loop runs whole groups of NB steps and
+// has no branches, so the wait before each group's products covers only that buffer's loads; the last
+// nsteps % NB steps follow it.  (The compiler does not unroll a loop holding MFMA instructions when that needs a
+// remainder loop, so a plain loop would wait for its own loads on every step.)
 template <int MT, int NB>
 __device__ __forceinline__ void mfma_panel_t(const double* __restrict__ abase, const uint32_t (&ao)[MT], uint32_t lda4,
                                            const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
@@ -781,7 +743,7 @@ constexpr int kMixedOcc = SMCF_MIXED_OCC;  // waves per SIMD the register budget
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
-// mfma_panel in FP32: acc[m][n] += sum_j A(16m + r, j) B(j, c_n), same operand addressing and pipeline
+// mfma_panel_t in FP32: acc[m][n] += sum_j A(16m + r, j) B(j, c_n), same operand addressing and pipeline
 template <int MT, int NB>
 __device__ __forceinline__ void mfma_panel_f32(const float* __restrict__ abase, const uint32_t (&ao)[MT],
                                                uint32_t lda4, const float* __restrict__ bbase,
