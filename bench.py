@@ -99,12 +99,13 @@ def run_form(a, S, syn, obs, precision, keys_for):
         print("warmup step %d: %.2f s, stages mean %.1f max %d" % (w, time.time() - t0, ns.mean(), ns.max()),
               flush=True)
     ctx.set_profiling(True)
-    stages, lat, neglog = [], [], []
+    stages, lat, neglog, evl = [], [], [], []
     t_start = time.perf_counter()
     for k in range(a.steps):
         t0 = time.perf_counter()
         nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(B, k), keys_for(k))
         lat.append(time.perf_counter() - t0)
+        evl.append(pb.batch_times(B))
         stages.append(ns)
         neglog.append(nl)
         print("step %d: %.3f s, stages mean %.1f, -logZ median %.3f" % (k, lat[-1], ns.mean(), np.median(nl)),
@@ -134,7 +135,9 @@ def run_form(a, S, syn, obs, precision, keys_for):
             "launches": n_gibbs, "avg_launch_ms": round(avg_ms, 4),
             # the measured HBM bytes per launch (rocprofv3, profiles/) over this run's average launch time
             "hbm_tb_per_s": round(traffic / (avg_ms * 1e-3) / 1e12, 3) if traffic and avg_ms > 0 else None}
+    evl = np.concatenate(evl)
     out = dict(value=B * a.steps / elapsed, ms_per_step=1e3 * elapsed / a.steps, stages=stages, lat=lat,
+               eval_lat=(round(float(np.percentile(evl, 50)), 3), round(float(np.percentile(evl, 99)), 3)),
                neglog=np.concatenate(neglog), roofline=roof, ndim=ndim, nobs=pb.nobs, P=P, sweeps=sweeps)
     pb.close()
     ctx.close()
@@ -288,7 +291,9 @@ def main():
                       "form": forms[0], "evals_per_step": B, "slots": a.slots, "ndim_slip": head["ndim"],
                       "nobs": head["nobs"], "inner_particles": head["P"],
                       "mean_inner_stages": round(float(head["stages"].mean()), 2),
-                      "p50_wave_latency_s": p50, "p99_wave_latency_s": p99},
+                      "p50_wave_latency_s": p50, "p99_wave_latency_s": p99,
+                      # per geometry: from the wave's submission to the stage boundary that returned its -log Z
+                      "p50_eval_latency_s": head["eval_lat"][0], "p99_eval_latency_s": head["eval_lat"][1]},
            "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving}
     if "fp64" in res and len(res) > 1:
         f64 = res["fp64"]
@@ -308,6 +313,7 @@ def main():
                            "ms_per_step": round(m["ms_per_step"], 2),
                            "mean_inner_stages": round(float(m["stages"].mean()), 2),
                            "p50_wave_latency_s": m50, "p99_wave_latency_s": m99,
+                           "p50_eval_latency_s": m["eval_lat"][0], "p99_eval_latency_s": m["eval_lat"][1],
                            "neglogz_rel_diff_vs_fp64": {"median": float("%.3g" % np.median(rel)),
                                                         "p99": float("%.3g" % np.percentile(rel, 99)),
                                                         "max": float("%.3g" % rel.max()),

@@ -120,6 +120,9 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
  * taking the next geometry; results do not depend on B, the slot count or the order. */
 int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys,
                                      double* neglog, int* nstage);
+/* Per-geometry latency of the last batch call on this problem (no counterpart in the reference): seconds from
+ * the call's start to the stage boundary at which geometry j's result was read back, for j < B. */
+int smcf_batch_times(const smcf_problem* pb, int B, double* seconds);
 
 /* Plain mean of each geometry's final slip particles (ndim_slip x B, column b for theta column b) and its -log Z:
  * the per-particle loop of calc_slip_map (main.f90:600-640, fault_calc_likelihood then the average over the

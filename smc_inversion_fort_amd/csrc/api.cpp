@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -170,6 +171,8 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
   }
   bool hmc = P.mover == SMCF_MOVER_HMC;
   const int nf = m.ndim_fault;
+  const auto t_call = std::chrono::steady_clock::now();
+  P.t_done.assign(N, 0.0);
   const int W = std::min(N, bt.cap);  // slots in use
   SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * nf * W, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * W, hipMemcpyHostToDevice, s));
@@ -230,6 +233,7 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
       for (int b = 0; b < W; ++b)
         if (job[b] >= 0 && done[b]) {
           if (neglog) neglog[job[b]] = nl[b];
+          P.t_done[job[b]] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_call).count();
           if (nstage) nstage[job[b]] = std::min(iter[b] - 1, 201);
           job[b] = -1;
           --busy;
@@ -650,6 +654,12 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batch
 int smcf_problem_bytes(const smcf_problem_desc* d, int max_batch, size_t* bytes) {
   if (!bytes) return SMCF_EINVAL;
   return problem_build(nullptr, d, max_batch, true, nullptr, bytes);
+}
+
+int smcf_batch_times(const smcf_problem* pb, int B, double* seconds) {
+  if (!pb || !seconds || B < 0 || (size_t)B > pb->p.t_done.size()) return SMCF_EINVAL;
+  std::copy(pb->p.t_done.begin(), pb->p.t_done.begin() + B, seconds);
+  return SMCF_OK;
 }
 
 int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes) {

@@ -147,6 +147,7 @@ struct Problem {
   std::vector<void*> allocs;
   bool sizing = false;  // count the device bytes of a description without allocating (smcf_problem_bytes)
   size_t bytes = 0;     // device bytes allocated (or counted) for the problem
+  std::vector<double> t_done;  // seconds from the start of the last batch call to each geometry's result
 };
 
 struct StageCfg {

@@ -83,7 +83,7 @@ def test_mixed_slip_smc_vs_fp64_oracle(ctx, precision):
     po = O.ProblemHolder(f, obs, syn.MAX_SLIP, P, mover=1, gibbs_sweeps=3)
     theta = syn.theta_true()
     G = pb.calc_greens_func(theta)
-    nl_d, nl_o, m_d, m_o = [], [], [], []
+    nl_d, nl_o, m_d, m_o, s_d, s_o = [], [], [], [], [], []
     for k in range(24):
         key = O.inner_key(40 + k, 1, 1, 3)
         d = pb.slip_smc_exec(theta, key)
@@ -91,7 +91,8 @@ def test_mixed_slip_smc_vs_fp64_oracle(ctx, precision):
         assert d["gamma"][0] == pytest.approx(o["gamma"][0], rel=0, abs=1e-9)
         nl_d.append(d["neglog"]); nl_o.append(o["neglog"])
         m_d.append(d["mean"]); m_o.append(o["mean"])
-    nl_d, nl_o, m_d, m_o = map(np.array, (nl_d, nl_o, m_d, m_o))
+        s_d.append(d["particles"].std(axis=1)); s_o.append(o["particles"].std(axis=1))
+    nl_d, nl_o, m_d, m_o, s_d, s_o = map(np.array, (nl_d, nl_o, m_d, m_o, s_d, s_o))
     dl = nl_d - nl_o
     se = dl.std(ddof=1) / np.sqrt(len(dl))
     say("-log Z: mixed %.5f fp64 %.5f  diff %.5f  se %.5f" % (nl_d.mean(), nl_o.mean(), dl.mean(), se))
@@ -102,6 +103,13 @@ def test_mixed_slip_smc_vs_fp64_oracle(ctx, precision):
     say("posterior mean: max |diff| %.3e (%.3f%% of max mean), max |diff|/se %.2f"
         % (np.abs(dm).max(), 100 * np.abs(dm).max() / scale, float((np.abs(dm) / np.maximum(sem, 1e-300)).max())))
     assert np.all(np.abs(dm) <= 4 * sem + 0.01 * scale)
+    # posterior standard deviations (particle spread per coordinate), averaged over the keys
+    ds = s_d.mean(0) - s_o.mean(0)
+    ses = (s_d - s_o).std(0, ddof=1) / np.sqrt(len(s_d))
+    sscale = s_o.mean(0).max()
+    say("posterior std: max |diff| %.3e (%.3f%% of max std), max |diff|/se %.2f"
+        % (np.abs(ds).max(), 100 * np.abs(ds).max() / sscale, float((np.abs(ds) / np.maximum(ses, 1e-300)).max())))
+    assert np.all(np.abs(ds) <= 4 * ses + 0.01 * sscale)
     pb.close()
 
 
