@@ -48,9 +48,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--evals", type=int, default=1024, help="geometry evaluations per step (one wave)")
-    ap.add_argument("--warmup-evals", type=int, default=0,
-                    help="geometry evaluations per warm-up step (default: as --evals)")
-    ap.add_argument("--slots", type=int, default=512, help="device slots (instances in flight)")
+    ap.add_argument("--warmup-evals", type=int, default=128,
+                    help="geometry evaluations per warm-up step (loads the kernels and touches every code path; 0: "
+                         "as --evals)")
+    ap.add_argument("--slots", type=int, default=1024, help="device slots (instances in flight)")
     ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)

@@ -31,3 +31,22 @@ def test_problem_bytes_grows_with_slots_and_sizes():
     # more particles, more bytes
     dp, _k3 = _desc(P=192)
     assert S.problem_bytes(dp, 4) > b[2]
+
+
+def test_problem_bytes_rejects_invalid_descriptions():
+    """The sizing pass applies problem_create's checks (no device needed): mixed forms need the Gibbs mover,
+    unknown precisions and oversized particle counts are refused with the C ABI's error codes."""
+    import pytest
+    for kw, code in ((dict(precision=S.PRECISION_MIXED_BF16), 4), (dict(precision=7), 1)):
+        xmin, xmax, zmin, _ = syn.fault_frame()
+
+        def greens(th, pts, uv, nx, ne):
+            return O.Fault([nx], [ne], xmin, xmax, zmin).greens(th, pts, uv, 0, pts.shape[1] // 3)
+        obs = syn.make_observations("c2", 3, greens, nsta=12, nxi=4, neta=3)
+        d, _k = S.problem_desc([4], [3], xmin, xmax, zmin, obs, syn.MAX_SLIP, 96, mover=S.MOVER_HMC, gibbs_sweeps=3,
+                               **kw)
+        with pytest.raises(S.SmcfError, match="error %d" % code):
+            S.problem_bytes(d, 4)
+    d, _k = _desc(P=100000)
+    with pytest.raises(S.SmcfError, match="nparticle_slip"):
+        S.problem_bytes(d, 4)
