@@ -942,8 +942,12 @@ __global__ void __launch_bounds__(256, kMixedOcc)
 //    depend on the instruction's internal k order; lane l holds D(4 (l/16) + t, l % 16) in register t
 //    (the FP32 accumulator layout of k_gibbs_mixed).
 //  * Draw phase as in k_gibbs_mixed (FP64 conditional means and in-tile corrections, single-precision
-//    draw); each lane keeps its tile's 16 values in registers and writes the new ones to SX as two
-//    16-value groups at the end of the tile.
+//    draw); each lane reads its tile's 16 current values from SX (hi + lo, exact in double: the values
+//    the matrix phase multiplied, so the diagonal term Q_ii s_i of q_i cancels exactly in the
+//    conditional mean), keeps them in registers and writes the new ones to SX as two 16-value groups
+//    at the end of the tile.  The FP64 particles S are written in the last sweep only: between sweeps
+//    the chain's state is its split copy (16 significant bits, below the 2^-16 error the products
+//    already carry), and S, SX and DX agree when the kernel ends.
 //  * Corrections of the later tiles of the block by a tile's deltas: the same BF16 products with the
 //    half of the 32-wide contraction that the tile does not cover set to zero.
 // ---------------------------------------------------------------------------
@@ -1098,10 +1102,16 @@ __global__ void __launch_bounds__(256, kBfOcc)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // the tile's 16 old values, requested together: one memory latency per tile, not per coordinate
+        // the tile's 16 current values from the split copy (coordinate groups (i0 + 16m) / 8 and the next),
+        // requested together: one memory latency per tile, not per coordinate
         double sv[16];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) sv[kk] = i0 + 16 * m + kk < nd ? S[(size_t)(i0 + 16 * m + kk) * P + pc] : 0.0;
+        for (int h = 0; h < 2; ++h) {
+          size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + pc) * 2;
+          bf16x8 hi = SX[o], lo = SX[o + 1];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) sv[8 * h + e] = (double)(float)hi[e] + (double)(float)lo[e];
+        }
         uint64_t wnext = 0;  // the second word of the current Philox block
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
@@ -1128,15 +1138,20 @@ __global__ void __launch_bounds__(256, kBfOcc)
           double sn = gibbs_draw_f32(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open_f32(wc));
           if (sn != sn) sn = gibbs_draw(mu, CDw[2][kk], CDw[3][kk], max_slip, u01_open(wc));
           double dlt = sn - so;
-          if (valid) S[(size_t)i * P + p] = sn;
-          sv[kk] = sn;  // the new value, for the split copy
+          sv[kk] = sn;  // the new value, stored with the tile's others below
           Tw[kk][lane] = dlt;  // the delta, read by the later coordinates of the tile
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // the tile's new values into the split copy: coordinate groups (i0 + 16m) / 8 and the next
+        // the tile's new values into the split copy (coordinate groups (i0 + 16m) / 8 and the next) and, in
+        // the last sweep, into S, stored together at the end of the tile rather than one after each draw
         if (valid) {
+          if (sw == sweeps - 1) {
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+              if (i0 + 16 * m + kk < nd) S[(size_t)(i0 + 16 * m + kk) * P + p] = sv[kk];
+          }
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             bf16x8 hi, lo;
