@@ -1083,17 +1083,42 @@ __global__ void __launch_bounds__(256, kBfOcc)
 #pragma unroll
         for (int n = 0; n < 4; ++n) acc[m][n] = flt4{0.f, 0.f, 0.f, 0.f};
       mfma_panel_bf3<MT, kBfNB>(QXb, ao, astep, SX, bo, bstep, nst, acc);
+      // What a tile needs from memory is requested one tile ahead, while the previous tile is drawn: its
+      // constants (this lane's 4 entries of the diagonal block, one per-coordinate constant, the reference
+      // particle's entry) and its current values in split form.  The draw is an out-of-line call, whose
+      // entry waits for every outstanding memory operation of the wave, so each tile then waits for
+      // memory once instead of once per dependent request.
+      double cq[4], cdv, mrv;
+      bf16x8 xv[4];  // hi, lo of coordinate group (i0 + 16m) / 8, then of the next
+      auto fetch = [&](int m) {
+        int ik = i0 + 16 * m + fi;
+        const double* qsrc = Qr + (size_t)min(ik, nd - 1) * KB + 16 * m + 4 * fk;  // Q(ik, i0 + 16m + 4c4 + t)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cq[t] = ik < nd ? qsrc[t] : 0.0;
+        cdv = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
+        mrv = (fk == 0 && ik < nd) ? mref[ik] : 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + pc) * 2;
+          xv[2 * h] = SX[o];
+          xv[2 * h + 1] = SX[o + 1];
+        }
+      };
+      fetch(0);
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
         if (i0 + 16 * m >= nd) break;  // tiles past the last coordinate (wave-uniform)
-        {  // the tile's constants into LDS: lane (k, c4) = (lane % 16, lane / 16) brings 4 + 1 values
-          int ik = i0 + 16 * m + fi;
-          const double* qsrc = Qr + (size_t)min(ik, nd - 1) * KB + 16 * m + 4 * fk;  // Q(ik, i0 + 16m + 4c4 + t)
+        // the tile's constants into LDS: lane (k, c4) = (lane % 16, lane / 16) brings 4 + 1 values
 #pragma unroll
-          for (int t = 0; t < 4; ++t) CQw[4 * fk + t][fi] = ik < nd ? qsrc[t] : 0.0;
-          CDw[fk][fi] = ik < nd ? cd[(size_t)fk * nd + ik] : 0.0;
-          if (fk == 0) CDw[4][fi] = ik < nd ? mref[ik] : 0.0;
-        }
+        for (int t = 0; t < 4; ++t) CQw[4 * fk + t][fi] = cq[t];
+        CDw[fk][fi] = cdv;
+        if (fk == 0) CDw[4][fi] = mrv;
+        // the tile's 16 current values (hi + lo is exact in double)
+        double sv[16];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) sv[8 * h + e] = (double)(float)xv[2 * h][e] + (double)(float)xv[2 * h + 1][e];
         // transpose tile m through LDS in double: Tw[row][particle]
 #pragma unroll
         for (int n = 0; n < 4; ++n)
@@ -1102,15 +1127,19 @@ __global__ void __launch_bounds__(256, kBfOcc)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // the tile's 16 current values from the split copy (coordinate groups (i0 + 16m) / 8 and the next),
-        // requested together: one memory latency per tile, not per coordinate
-        double sv[16];
+        // the next tile's requests, and the Q fragments of this tile's corrections of the later tiles: the
+        // tile's coordinates are columns 16 hh .. 16 hh + 15 of 32-wide step stc, held by lane groups 2 hh
+        // and 2 hh + 1 of a fragment; the other two groups contribute zero
+        if (m + 1 < MT && i0 + 16 * (m + 1) < nd) fetch(m + 1);
+        const int jt = i0 + 16 * m, stc = jt / 32, hh = (jt / 16) & 1;
+        const bool mine = (fk >> 1) == hh;
+        bf16x8 cqh[MT], cql[MT];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          size_t o = ((size_t)((i0 + 16 * m) / 8 + h) * P + pc) * 2;
-          bf16x8 hi = SX[o], lo = SX[o + 1];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) sv[8 * h + e] = (double)(float)hi[e] + (double)(float)lo[e];
+        for (int m2 = m + 1; m2 < MT; ++m2) {
+          uint32_t o = (uint32_t)(((stc * MT + m2) * 64 + lane) * 2);
+          bool use = mine && i0 + 16 * m2 < nd;
+          cqh[m2] = use ? QXb[o] : zero8;
+          cql[m2] = use ? QXb[o + 1] : zero8;
         }
         uint64_t wnext = 0;  // the second word of the current Philox block
 #pragma unroll
@@ -1185,11 +1214,7 @@ __global__ void __launch_bounds__(256, kBfOcc)
         }
         // corrections of the later tiles by this tile's deltas (in Tw[k][p]):
         //   acc[m2](r, p) += sum_k Q(i0 + 16m + k, i0 + 16m2 + r) delta_k(p)
-        // The tile's coordinates are columns 16 hh .. 16 hh + 15 of 32-wide step stc, held by lane
-        // groups 2 hh and 2 hh + 1 of a fragment; the other two groups contribute zero.
         if (m + 1 < MT) {
-          int jt = i0 + 16 * m, stc = jt / 32, hh = (jt / 16) & 1;
-          bool mine = (fk >> 1) == hh;
           int kb8 = 8 * (fk - 2 * hh);
           bf16x8 dh[4], dl[4];
 #pragma unroll
@@ -1205,13 +1230,11 @@ __global__ void __launch_bounds__(256, kBfOcc)
 #pragma unroll
           for (int m2 = m + 1; m2 < MT; ++m2) {
             if (i0 + 16 * m2 >= nd) break;
-            uint32_t o = (uint32_t)(((stc * MT + m2) * 64 + lane) * 2);
-            bf16x8 qh = mine ? QXb[o] : zero8, ql = mine ? QXb[o + 1] : zero8;
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
-              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh, dh[n], acc[m2][n], 0, 0, 0);
-              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qh, dl[n], acc[m2][n], 0, 0, 0);
-              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ql, dh[n], acc[m2][n], 0, 0, 0);
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cqh[m2], dh[n], acc[m2][n], 0, 0, 0);
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cqh[m2], dl[n], acc[m2][n], 0, 0, 0);
+              acc[m2][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cql[m2], dh[n], acc[m2][n], 0, 0, 0);
             }
           }
         }
