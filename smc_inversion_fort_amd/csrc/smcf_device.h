@@ -239,7 +239,9 @@ SMCF_DEV float ndtri7_lower(float p) {  // p in (0, 1/2]: z <= 0 with Phi(z) = p
     return q * (((59.109374720f * r + 159.29113202f) * r + 50.434271938f) * r + 3.3871327179f) *
            __builtin_amdgcn_rcpf(((67.187563600f * r + 78.757757664f) * r + 17.895169469f) * r + 1.0f);
   }
-  float r = sqrtf(-logf(p));
+  // r = sqrt(-ln p) from the hardware log2 and square root (1 ulp each): p >= 2^-126 wherever a draw is
+  // accepted (the interval's mass is at least 1e-30), so neither needs the library's denormal scaling
+  float r = __builtin_amdgcn_sqrtf(-0.69314718056f * __builtin_amdgcn_logf(p));
   float z;
   if (r <= 5.0f) {
     r -= 1.6f;
@@ -252,9 +254,21 @@ SMCF_DEV float ndtri7_lower(float p) {  // p in (0, 1/2]: z <= 0 with Phi(z) = p
   }
   return -z;
 }
+// e^y for y <= 1 in single precision: y log2(e) carried as ph + pl (pl from an FMA and the low part of
+// log2(e)), e = rint(ph), then 2^((ph - e) + pl) from the hardware exp2 (1 ulp on [-1/2, 1/2]) scaled by
+// 2^e.  Results below 2^-149 flush to zero.  The library expf adds overflow and underflow tests that
+// cannot trigger for y <= 1.
+SMCF_DEV float exp_le1_f32(float y) {
+  const float l2e_hi = 1.44269502163f, l2e_lo = 1.92596299e-8f;
+  float ph = y * l2e_hi;
+  float pl = fmaf(y, l2e_hi, -ph) + y * l2e_lo;
+  float e = __builtin_rintf(ph);
+  return __builtin_ldexpf(__builtin_amdgcn_exp2f((ph - e) + pl), (int)e);
+}
 // erfc(x) for x >= 0 in single precision: the Chebyshev-fitted form of Numerical Recipes in C (2nd ed.,
 // section 6.2, erfcc), t exp(-x^2 + c(t)) with t = 1 / (1 + x/2) and c a degree-9 polynomial in t,
 // fractional error below 1.2e-7 for every x >= 0 (one reciprocal, one exponential, nine FMAs).
+// c(t) <= 3e-8 on (0, 1] (its largest value, at t = 1, where erfc(0) = 1), so the exponent is below 1.
 SMCF_DEV float erfc_cheb32(float x) {
   float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
   float c = -1.26551223f +
@@ -264,7 +278,7 @@ SMCF_DEV float erfc_cheb32(float x) {
                            t * (-0.18628806f +
                                 t * (0.27886807f +
                                      t * (-1.13520398f + t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
-  return t * expf(c - x * x);
+  return t * exp_le1_f32(c - x * x);
 }
 // uniform strictly inside (0, 1) from the top 23 bits of a Philox word (exact in single precision)
 SMCF_DEV float u01_open_f32(uint64_t w) { return ((float)(w >> 41) + 0.5f) * 0x1.0p-23f; }
