@@ -308,6 +308,8 @@ def main():
             sent_m, sent_f = m["neglog"] >= 1e10, f64["neglog"] >= 1e10
             both = ~sent_m & ~sent_f
             rel = np.abs(m["neglog"][both] - f64["neglog"][both]) / np.abs(f64["neglog"][both])
+            if rel.size == 0:  # every geometry at the sentinel in one form or the other
+                rel = np.array([np.nan])
             m50, m99 = latency(m)
             mixed[name] = {"value": round(m["value"], 4), "unit": "outer particles/s",
                            "speedup_vs_fp64": round(m["value"] / f64["value"], 3),
