@@ -459,7 +459,8 @@ float orc_erfc_cheb32(float x) {
 }
 
 /* The tn_std construction in single precision: a = (lo - mu) s and b = (hi - mu) s formed in double
- * and rounded; one erfc (orc_erfc_cheb32) per bound; p = Phi(a) + u m, q = Q(b) + (1 - u) m, the smaller inverted.
+ * and rounded; one erfc (orc_erfc_cheb32) per bound within 10 sigma; p = Phi(a) + u m, q = Q(b) + (1 - u) m, the
+ * smaller inverted.
  * Returns NaN (the mover then takes orc_trunc_normal_prec) when a bound lies beyond 8 sigma on the
  * far side of the mean or the interval holds less than 1e-30 of the mass. */
 double orc_trunc_normal_prec_f32(double mu, double s, double lo, double hi, float u) {
@@ -467,7 +468,10 @@ double orc_trunc_normal_prec_f32(double mu, double s, double lo, double hi, floa
     if (a64 > 8.0 || b64 < -8.0) return NAN;
     float a = (float)fmax(a64, -40.0), b = (float)fmin(b64, 40.0);
     const float r2 = 0.70710678118654752f;
-    float ta = 0.5f * orc_erfc_cheb32(fabsf(a) * r2), tb = 0.5f * orc_erfc_cheb32(fabsf(b) * r2);
+    /* a bound 10 or more standard deviations out: its tail (< 7.7e-24) is taken as zero, below 2^-24 of
+     * the interval's mass since the other bound lies within 8 sigma */
+    float ta = fabsf(a) < 10.0f ? 0.5f * orc_erfc_cheb32(fabsf(a) * r2) : 0.0f;
+    float tb = fabsf(b) < 10.0f ? 0.5f * orc_erfc_cheb32(fabsf(b) * r2) : 0.0f;
     float pa = a < 0.0f ? ta : 1.0f - ta, qa = a < 0.0f ? 1.0f - ta : ta;
     float pb = b < 0.0f ? tb : 1.0f - tb, qb = b < 0.0f ? 1.0f - tb : tb;
     float m = a >= 0.0f ? qa - qb : pb - pa;
