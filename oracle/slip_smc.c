@@ -418,14 +418,25 @@ double orc_trunc_normal_prec(double mu, double s, double lo, double hi, double u
 }
 
 /* Single-precision draw of the mixed-precision Gibbs mover (no reference counterpart; the device
- * restates it in trunc_normal_prec_f32).  Inverse CDF: AS 241 PPND7 (Wichura 1988, Appl. Statist.
- * 37, 477-484), the 7-digit version of orc_ndtri.  p in (0, 1/2]; returns z <= 0 with Phi(z) = p. */
+ * restates it in trunc_normal_prec_f32).  Inverse CDF, p in (0, 1/2], returning z <= 0 with Phi(z) = p:
+ * for p > 0.0017 (w = -ln(4 p (1 - p)) < 5), z = sqrt(2) erfinv(2p - 1) from the central polynomial of
+ * Giles' single-precision erfinv (M. Giles, "Approximating the erfinv function", GPU Computing Gems Jade
+ * Edition, 2011); below that the tail of AS 241 PPND7 (Wichura 1988, Appl. Statist. 37, 477-484), the
+ * 7-digit version of orc_ndtri. */
 float orc_ndtri7_lower(float p) {
-    float q = p - 0.5f;
-    if (q >= -0.425f) {
-        float r = 0.180625f - q * q;
-        return q * (((59.109374720f * r + 159.29113202f) * r + 50.434271938f) * r + 3.3871327179f) /
-               (((67.187563600f * r + 78.757757664f) * r + 17.895169469f) * r + 1.0f);
+    float w = -logf(4.0f * p * (1.0f - p));
+    if (w < 5.0f) {
+        float t = w - 2.5f;
+        float c = 2.81022636e-08f;
+        c = fmaf(c, t, 3.43273939e-07f);
+        c = fmaf(c, t, -3.5233877e-06f);
+        c = fmaf(c, t, -4.39150654e-06f);
+        c = fmaf(c, t, 0.00021858087f);
+        c = fmaf(c, t, -0.00125372503f);
+        c = fmaf(c, t, -0.00417768164f);
+        c = fmaf(c, t, 0.246640727f);
+        c = fmaf(c, t, 1.50140941f);
+        return 1.41421356237f * c * (2.0f * p - 1.0f);
     }
     float r = sqrtf(-logf(p)), z;
     if (r <= 5.0f) {

@@ -229,16 +229,30 @@ SMCF_DEV double trunc_normal_prec(double mu, double s, double is, double lo, dou
 // ---------------------------------------------------------------------------
 // Single-precision truncated-normal draw of the mixed-precision Gibbs mover.  The conditional mean
 // and the returned value stay double; the standardised bounds, the two tail masses and the inverse
-// CDF are single precision.  Inverse CDF: AS 241 PPND7 (Wichura 1988, Appl. Statist. 37, 477-484),
-// the 7-digit version of the algorithm used in double precision above, its quotients formed with the
-// hardware reciprocal (1 ulp, below the 7-digit accuracy of the rational forms).
+// CDF are single precision.  Inverse CDF: Giles' single-precision erfinv polynomial for p > 0.0017, and
+// below that the tail of AS 241 PPND7 (Wichura 1988, Appl. Statist. 37, 477-484), the 7-digit version of
+// the algorithm used in double precision above, its quotients formed with the hardware reciprocal (1 ulp,
+// below the 7-digit accuracy of the rational forms).
 SMCF_DEV float ndtri7_lower(float p) {  // p in (0, 1/2]: z <= 0 with Phi(z) = p
-  float q = p - 0.5f;
-  if (q >= -0.425f) {
-    float r = 0.180625f - q * q;
-    return q * (((59.109374720f * r + 159.29113202f) * r + 50.434271938f) * r + 3.3871327179f) *
-           __builtin_amdgcn_rcpf(((67.187563600f * r + 78.757757664f) * r + 17.895169469f) * r + 1.0f);
+  // p > 0.0017 (w = -ln(4 p (1 - p)) < 5): z = sqrt(2) erfinv(2p - 1) from the central polynomial of Giles'
+  // single-precision erfinv (M. Giles, "Approximating the erfinv function", GPU Computing Gems Jade Edition,
+  // 2011; relative error below 1.3e-7), one branch where AS 241 needs two (p < 0.075 is its tail), so fewer
+  // waves run both.  2p - 1 is exact for p >= 1/4.
+  float w = -0.69314718056f * __builtin_amdgcn_logf(4.0f * p * (1.0f - p));
+  if (w < 5.0f) {
+    float t = w - 2.5f;
+    float c = 2.81022636e-08f;
+    c = fmaf(c, t, 3.43273939e-07f);
+    c = fmaf(c, t, -3.5233877e-06f);
+    c = fmaf(c, t, -4.39150654e-06f);
+    c = fmaf(c, t, 0.00021858087f);
+    c = fmaf(c, t, -0.00125372503f);
+    c = fmaf(c, t, -0.00417768164f);
+    c = fmaf(c, t, 0.246640727f);
+    c = fmaf(c, t, 1.50140941f);
+    return 1.41421356237f * c * (2.0f * p - 1.0f);
   }
+  // p <= 0.0017: AS 241 PPND7's tail (r = sqrt(-ln p) > 2.5)
   // r = sqrt(-ln p) from the hardware log2 and square root (1 ulp each): p >= 2^-126 wherever a draw is
   // accepted (the interval's mass is at least 1e-30), so neither needs the library's denormal scaling
   float r = __builtin_amdgcn_sqrtf(-0.69314718056f * __builtin_amdgcn_logf(p));
