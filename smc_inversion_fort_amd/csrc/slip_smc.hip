@@ -969,26 +969,33 @@ SMCF_DEV void split_bf16(float x, __bf16& h, __bf16& l) {
 
 // acc[m][n] += sum over nst 32-wide steps of A_m B_n in split BF16.  A fragment (st, m) at
 // qa[ao[m] + st astep] (hi) and +1 (lo); B fragment (st, n) at sb[bo[n] + st bstep] (hi) and +1 (lo);
-// offsets in 16-byte units.  Same software pipeline as mfma_panel_f32.
+// offsets in 16-byte units.  Same software pipeline as mfma_panel_f32.  Each load is a wave-uniform base
+// for the step plus the lane's fixed 32-bit byte offset, the scalar-base form of the global load, so the
+// steps spend no vector instructions on addresses.
 template <int MT, int NB>
 __device__ __forceinline__ void mfma_panel_bf3(const bf16x8* __restrict__ qa, const uint32_t (&ao)[MT], uint32_t astep,
                                                const bf16x8* __restrict__ sb, const uint32_t (&bo)[4],
                                                uint32_t bstep, int nst, flt4 (&acc)[MT][4]) {
   if (nst <= 0) return;
   bf16x8 ah[NB][MT], al[NB][MT], bh[NB][4], bl[NB][4];
+  uint32_t aoff[MT], boff[4];  // byte offsets of the lane's fragments
+#pragma unroll
+  for (int m = 0; m < MT; ++m) aoff[m] = ao[m] * 16u;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) boff[n] = bo[n] * 16u;
   auto load = [&](int k, int st) {
     uint32_t q = (uint32_t)min(st, nst - 1);
+    const char* qs = (const char*)(qa + (size_t)q * astep);
+    const char* ss = (const char*)(sb + (size_t)q * bstep);
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-      uint32_t o = ao[m] + q * astep;
-      ah[k][m] = qa[o];
-      al[k][m] = qa[o + 1];
+      ah[k][m] = *(const bf16x8*)(qs + aoff[m]);
+      al[k][m] = *(const bf16x8*)(qs + aoff[m] + 16u);
     }
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      uint32_t o = bo[n] + q * bstep;
-      bh[k][n] = sb[o];
-      bl[k][n] = sb[o + 1];
+      bh[k][n] = *(const bf16x8*)(ss + boff[n]);
+      bl[k][n] = *(const bf16x8*)(ss + boff[n] + 16u);
     }
   };
   auto prod = [&](int k) {
