@@ -294,8 +294,13 @@ SMCF_DEV float erfc_cheb32(float x) {
                                      t * (-1.13520398f + t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
   return t * exp_le1_f32(c - x * x);
 }
-// uniform strictly inside (0, 1) from the top 23 bits of a Philox word (exact in single precision)
-SMCF_DEV float u01_open_f32(uint64_t w) { return ((float)(w >> 41) + 0.5f) * 0x1.0p-23f; }
+// uniform strictly inside (0, 1) from the top 23 bits h of a Philox word: (h + 1/2) 2^-23, exact in single
+// precision.  Formed as the float with mantissa h and exponent 0, minus 1, plus 2^-24 (each step exact),
+// which avoids the compiler's 64-bit integer conversion sequence.
+SMCF_DEV float u01_open_f32(uint64_t w) {
+  uint32_t h = (uint32_t)(w >> 41);
+  return (__uint_as_float(0x3F800000u | h) - 1.0f) + 0x1.0p-24f;
+}
 
 // Same construction as tn_std: p = Phi(a) + u m and q = Q(b) + (1 - u) m, the smaller inverted.  NaN
 // (the caller then takes the double-precision draw) when a bound lies beyond 8 sigma on the far side of
