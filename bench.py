@@ -1,23 +1,34 @@
 """Benchmark: outer-particle evaluations per second of the nested SMC at 16x16 patches, FP64 and mixed.
 
-A step is one wave of fault_calc_likelihood (smc_fault.f90:66-292) evaluations, as the outer
-SMC issues them: E geometry particles drawn from the prior, streamed through S device slots
-(a slot whose inner SMC has finished takes the next geometry).  Per geometry: the Green's
-matrix, the Gram matrices and a full inner slip SMC (P = 2000 particles, truncated-MVN Gibbs,
-5 sweeps per stage) tempered to gamma = 1.  Observations stay resident on the device; each
-step uploads only the thetas and reads back one -log Z per geometry.
+The unit of work is one fault_calc_likelihood (smc_fault.f90:66-292) evaluation, as the outer SMC issues
+them in its waves (2N per stage, smc_fault.f90:695,734): for one geometry particle the Green's matrix, the
+Gram matrices and a full inner slip SMC (P = 2000 particles, truncated-MVN Gibbs, 5 sweeps per stage) tempered
+to gamma = 1.  At 16x16 patches and 1200 observations (N = 10 000 outer particles) one outer stage is a wave of
+20 000 evaluations streamed through the device's slots (a slot whose inner SMC has finished takes the next
+geometry).  The bench times a stretch of such a stream:
+
+  one library call streams  W*Ew + K*E + T  geometries (drawn from the prior) through S slots;
+  warm-up step w  = the next Ew completions (Ew >= S/W, so the warm-up spans one fill of every slot);
+  timed step k    = the next E completions, k = 1..K;
+  tail            = T more geometries (default T = S) that keep every slot busy until the last timed
+                    completion, as the rest of a 20 000-evaluation wave would.
+
+Step boundaries are stage boundaries at which the host has synchronised with the device and read the finished
+geometries' -log Z back, so `ms_per_step * steps` is wall time between two device synchronisations, and `value`
+= K*E / that time.  The whole stream's average (ramp-up and drain included) is reported beside it.
 
 Arithmetic forms of the same workload, on the same geometries and keys:
   fp64   every operation in FP64 (the headline `value`);
   mixed  Gibbs products Q S in FP32 on the matrix cores, single-precision truncated-normal draws;
          conditional means, likelihood, weights and evidence in FP64;
   bf16   as mixed, with Q S formed from split-BF16 operands (hi.hi + hi.lo + lo.hi, FP32 accumulation).
-The mixed forms are reported under "mixed" (the faster one) and "mixed_forms", each with its speed-up
-over FP64 and its -log Z difference from the FP64 form.
+The mixed forms are reported under "mixed" (the faster one) and "mixed_forms", each with its speed-up over
+FP64 and its -log Z difference from the FP64 form.
 
-Prints one JSON line with `roofline` (Gibbs-sweep kernel, timed with HIP events on the
-library's stream) and `cpu_baseline` (the C oracle on this host's cores, a bounded sample of
-the same workload).
+Prints one JSON line with `roofline` (the Gibbs-sweep kernel, timed with HIP events on the library's stream),
+`cpu_baseline` (the C oracle on this host's cores, a time-bounded sample of whole evaluations of the same
+workload) and `serving` (Poisson arrivals of inversion requests admitted by a device-memory budget).  A
+wall-clock guard skips legs that would not fit --budget-s and still prints the line.
 """
 import argparse
 import json
@@ -25,7 +36,7 @@ import os
 import sys
 import time
 
-# The serving run overlaps many requests, each with its own two HIP streams; the HIP runtime maps streams onto
+# The serving run overlaps several requests, each with its own two HIP streams; the HIP runtime maps streams onto
 # GPU_MAX_HW_QUEUES hardware queues per process (4 by default), so with more requests in flight than queues their
 # kernels serialize.  16 queues let eight requests run side by side (read once, at HIP initialisation, which
 # happens below this line).  The single-problem forms use two streams and are unaffected.
@@ -40,30 +51,38 @@ METRIC = "outer-SMC particles/s at 16x16 patches, mixed vs FP64; p50/p99 solve l
 FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector = FP64 matrix
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
+T0 = time.perf_counter()
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--evals", type=int, default=1024, help="geometry evaluations per step (one wave)")
-    ap.add_argument("--warmup-evals", type=int, default=128,
-                    help="geometry evaluations per warm-up step (loads the kernels and touches every code path; 0: "
-                         "as --evals)")
-    ap.add_argument("--slots", type=int, default=1024, help="device slots (instances in flight)")
-    ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--evals", type=int, default=64, help="geometry evaluations completed per timed step")
+    ap.add_argument("--slots", type=int, default=1024, help="device slots (inner SMCs in flight)")
+    ap.add_argument("--tail", type=int, default=-1,
+                    help="geometries streamed after the last timed one to keep the slots busy (-1: --slots)")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="time budget of the CPU baseline: whole oracle evaluations are run until it is spent")
+    ap.add_argument("--cpu-min-evals", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["all", "fp64", "mixed", "bf16"], nargs="+", default=["fp64", "bf16"],
                     help="arithmetic forms to run (the first one run gives `value`)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="length of the serving run (Poisson arrivals of small inversions); 0 skips it")
+                    help="length of the serving run's arrival stream; 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-clock budget of the whole run: a leg whose estimate does not fit is skipped")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
     return ap.parse_args()
+
+
+def elapsed():
+    return time.perf_counter() - T0
 
 
 def traffic_from_profiles(form):
@@ -81,49 +100,63 @@ def traffic_from_profiles(form):
 FORM_NAMES = {0: "fp64", 1: "mixed", 2: "bf16"}
 
 
-def run_form(a, S, syn, obs, precision, keys_for):
-    """Warm-up and timed steps of one arithmetic form; returns its figures."""
+def keys_for(stream, n):
+    """RNG keys of n geometries of one stream (the 64-bit products wrap modulo 2^64 by design)."""
+    with np.errstate(over="ignore"):
+        base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(stream + 1)
+        return (np.arange(n, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
+
+
+def stream_plan(a):
+    W, K, E = a.warmup, a.steps, a.evals
+    Ew = max(E, -(-a.slots // W)) if W > 0 else 0
+    tail = a.slots if a.tail < 0 else a.tail
+    return W, K, E, Ew, tail, W * Ew + K * E + tail
+
+
+def run_form(a, S, syn, obs, precision):
+    """One arithmetic form: one streamed batch call, cut into warm-up and timed steps by completion order."""
     cfg = syn.CONFIGS["c2"]
     xmin, xmax, zmin, _ = syn.fault_frame()
+    name = FORM_NAMES[precision]
+    W, K, E, Ew, tail, total = stream_plan(a)
     ctx = S.Context(0)
-    B = a.evals
     pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
                    mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots, precision=precision)
     ndim, P, sweeps = pb.ndim, cfg["P"], cfg["sweeps"]
-    name = FORM_NAMES[precision]
-    print("bench[%s]: c2 16x16 patches ndim=%d nobs=%d P=%d evals/step=%d slots=%d"
-          % (name, ndim, pb.nobs, P, B, a.slots), flush=True)
-    BW = a.warmup_evals or B
-    for w in range(a.warmup):
-        t0 = time.time()
-        nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(BW, 1000 + w), keys_for(1000 + w, BW))
-        print("warmup step %d: %.2f s, stages mean %.1f max %d" % (w, time.time() - t0, ns.mean(), ns.max()),
-              flush=True)
+    print("bench[%s]: c2 16x16 patches ndim=%d nobs=%d P=%d; streaming %d geometries through %d slots "
+          "(warm-up %d x %d, timed %d x %d, tail %d)" % (name, ndim, pb.nobs, P, total, a.slots, W, Ew, K, E, tail),
+          flush=True)
+    th = syn.sample_thetas(total, a.seed)
+    keys = keys_for(a.seed, total)
     ctx.set_profiling(True)
-    stages, lat, neglog, evl = [], [], [], []
-    t_start = time.perf_counter()
-    for k in range(a.steps):
-        t0 = time.perf_counter()
-        nl, ns = pb.fault_calc_likelihood(syn.sample_thetas(B, k), keys_for(k))
-        lat.append(time.perf_counter() - t0)
-        evl.append(pb.batch_times(B))
-        stages.append(ns)
-        neglog.append(nl)
-        print("step %d: %.3f s, stages mean %.1f, -logZ median %.3f" % (k, lat[-1], ns.mean(), np.median(nl)),
-              flush=True)
-    elapsed = time.perf_counter() - t_start
+    t0 = time.perf_counter()
+    nl, ns = pb.fault_calc_likelihood(th, keys)
+    wall = time.perf_counter() - t0
+    t_done = pb.batch_times(total)
+    t_adm = pb.batch_admit_times(total)
     n_gibbs, ms_gibbs = ctx.kernel_stats("gibbs")
-    stages = np.concatenate(stages)
-    # algorithmic flops of the Gibbs kernel: per particle-sweep 2*ndim^2 (the products Q S)
-    flops = float(stages.sum()) * P * sweeps * 2.0 * ndim * ndim
-    name = FORM_NAMES[precision]
+    order = np.argsort(t_done, kind="stable")
+    c = t_done[order]
+    n_warm = W * Ew
+    t_begin = c[n_warm - 1] if n_warm > 0 else 0.0
+    ends = np.array([c[n_warm + (k + 1) * E - 1] for k in range(K)])
+    step_s = np.diff(np.concatenate([[t_begin], ends]))
+    for k in range(K):
+        print("step %d: %.3f s (geometries %d..%d completed)" % (k, step_s[k], n_warm + k * E, n_warm + (k + 1) * E - 1),
+              flush=True)
+    timed_s = float(ends[-1] - t_begin)
+    timed = order[n_warm:n_warm + K * E]
+    lat = t_done[timed] - t_adm[timed]
+    # algorithmic flops of the Gibbs kernel over the whole call: per particle-sweep 2*ndim^2 (the products Q S)
+    flops = float(ns.sum()) * P * sweeps * 2.0 * ndim * ndim
     if name == "fp64":
         peak, kernel = FP64_PEAK_TFLOPS, "k_gibbs (FP64 matrix cores + FP64 VALU draws)"
     elif name == "mixed":
         peak, kernel = (FP32_MFMA_PEAK_TFLOPS,
                         "k_gibbs_mixed (FP32 matrix cores + FP32 VALU draws; peak = dense FP32 matrix)")
     else:
-        # three BF16 products per FP32-equivalent product: the comparable peak is a third of dense BF16
+        # three BF16 products per FP32-equivalent product
         flops *= 3.0
         peak, kernel = (BF16_MFMA_PEAK_TFLOPS,
                         "k_gibbs_bf (split-BF16 matrix cores, 3 products per term, + FP32 VALU draws; "
@@ -134,15 +167,62 @@ def run_form(a, S, syn, obs, precision, keys_for):
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kernel,
             "launches": n_gibbs, "avg_launch_ms": round(avg_ms, 4),
+            "mean_instances_per_launch": round(float(ns.sum()) / max(n_gibbs, 1), 1),
             # the measured HBM bytes per launch (rocprofv3, profiles/) over this run's average launch time
             "hbm_tb_per_s": round(traffic / (avg_ms * 1e-3) / 1e12, 3) if traffic and avg_ms > 0 else None}
-    evl = np.concatenate(evl)
-    out = dict(value=B * a.steps / elapsed, ms_per_step=1e3 * elapsed / a.steps, stages=stages, lat=lat,
-               eval_lat=(round(float(np.percentile(evl, 50)), 3), round(float(np.percentile(evl, 99)), 3)),
-               neglog=np.concatenate(neglog), roofline=roof, ndim=ndim, nobs=pb.nobs, P=P, sweeps=sweeps)
+    out = dict(value=K * E / timed_s, ms_per_step=1e3 * timed_s / K, step_s=step_s, stages=ns, neglog=nl,
+               stages_timed=ns[timed], eval_lat=(round(float(np.percentile(lat, 50)), 3),
+                                                 round(float(np.percentile(lat, 99)), 3)),
+               stream={"geometries": total, "wall_s": round(wall, 2), "evals_per_s": round(total / wall, 3)},
+               roofline=roof, ndim=ndim, nobs=pb.nobs, P=P, sweeps=sweeps)
     pb.close()
     ctx.close()
+    print("bench[%s]: %.2f outer particles/s over %d timed evaluations (stream %.1f s, %.2f/s overall)"
+          % (name, out["value"], K * E, wall, total / wall), flush=True)
     return out
+
+
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return model, os.cpu_count(), aff
+
+
+def run_cpu(a, syn, obs, head):
+    """The C oracle (the same FP64 workload: per-DOF Green's loop, dense LtL, Gibbs x5, OpenMP over the inner
+    particles) timed on whole evaluations until --cpu-seconds is spent."""
+    model, ncpu, aff = cpu_info()
+    # this job's share of the host: OMP_NUM_THREADS when set (16 per GPU on the GPU hosts), else the affinity set
+    threads = int(os.environ.get("OMP_NUM_THREADS") or min(aff or ncpu or 1, 16))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import oracle as O  # loads libgomp, which reads OMP_NUM_THREADS now
+    cfg = syn.CONFIGS["c2"]
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    f = O.Fault([cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin)
+    po = O.ProblemHolder(f, obs, syn.MAX_SLIP, head["P"], mover=1, gibbs_sweeps=head["sweeps"])
+    th = syn.sample_thetas(max(a.cpu_min_evals, 64), a.seed)
+    keys = keys_for(a.seed, th.shape[1])
+    t0 = time.perf_counter()
+    n = 0
+    while n < th.shape[1] and (n < a.cpu_min_evals or time.perf_counter() - t0 < a.cpu_seconds):
+        po.fault_likelihood(th[:, n], int(keys[n]))
+        n += 1
+        print("cpu: %d evaluation(s) in %.1f s" % (n, time.perf_counter() - t0), flush=True)
+    ct = time.perf_counter() - t0
+    return {"value": round(n / ct, 5), "unit": "outer particles/s", "cores": threads, "kind": "port",
+            "sample": "the first %d geometries of the FP64 stream (c2, P=2000, Gibbs x5), whole evaluations, "
+                      "%.1f s on %d OpenMP threads" % (n, ct, threads),
+            "cpu_model": model, "nproc": ncpu, "affinity_cpus": aff}
 
 
 def run_serving(a, S, syn):
@@ -216,6 +296,9 @@ def main():
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.steps < 1 or a.evals < 1 or a.slots < 1:
+        print("--steps, --evals and --slots must be >= 1", file=sys.stderr)
+        sys.exit(2)
     import smc_inversion_fort_amd as S
     from smc_inversion_fort_amd import synthetic as syn
 
@@ -243,13 +326,6 @@ def main():
         return g
     obs = syn.make_observations("c2", a.seed, greens)
     gctx.close()
-    B = a.evals
-
-    def keys_for(step, n=B):
-        # the 64-bit products wrap modulo 2^64 by design (seed mixing); numpy would warn on the scalar wrap
-        with np.errstate(over="ignore"):
-            base = np.uint64(0x9E3779B97F4A7C15) * np.uint64(step + 1)
-            return (np.arange(n, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)) ^ base
 
     forms = []
     for f in a.precision:
@@ -257,45 +333,52 @@ def main():
             if g not in forms:
                 forms.append(g)
     prec = {"fp64": S.PRECISION_FP64, "mixed": S.PRECISION_MIXED, "bf16": S.PRECISION_MIXED_BF16}
-    res = {f: run_form(a, S, syn, obs, prec[f], keys_for) for f in forms}
+    skipped = []
+    res = {}
+    for i, f in enumerate(forms):
+        # the first form is the headline and always runs; a later one is skipped when the one before it, scaled
+        # by the same stream, would overrun the budget
+        if i > 0 and elapsed() + res[forms[0]]["stream"]["wall_s"] * 1.1 > a.budget_s:
+            skipped.append(f)
+            continue
+        res[f] = run_form(a, S, syn, obs, prec[f])
+    forms = [f for f in forms if f in res]
     head = res[forms[0]]
     cpu = None
     if not a.no_cpu:
-        import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
-        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-        f = O.Fault([cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin)
-        po = O.ProblemHolder(f, obs, syn.MAX_SLIP, head["P"], mover=1, gibbs_sweeps=head["sweeps"])
-        th = syn.sample_thetas(a.cpu_evals, 0)
-        t0 = time.perf_counter()
-        for k in range(a.cpu_evals):
-            po.fault_likelihood(th[:, k], int(keys_for(0)[k]))
-        ct = time.perf_counter() - t0
-        cpu = {"value": round(a.cpu_evals / ct, 5), "unit": "outer particles/s", "cores": threads, "kind": "port",
-               "sample": "%d c2 outer-particle evaluation(s) (same workload, P=2000, Gibbs x5, FP64) in %.1f s"
-                         % (a.cpu_evals, ct)}
+        if elapsed() + a.cpu_seconds + 30 <= a.budget_s:
+            cpu = run_cpu(a, syn, obs, head)
+        else:
+            skipped.append("cpu_baseline")
+    serving = None
+    if a.serve_seconds > 0:
+        if elapsed() + 3 * a.serve_seconds + 20 <= a.budget_s:
+            serving = run_serving(a, S, syn)
+        else:
+            skipped.append("serving")
 
-    serving = run_serving(a, S, syn) if a.serve_seconds > 0 else None
-
-    def latency(r):
-        return (round(float(np.percentile(r["lat"], 50)), 3), round(float(np.percentile(r["lat"], 99)), 3))
-    p50, p99 = latency(head)
+    W, K, E, Ew, tail, total = stream_plan(a)
     out = {"metric": METRIC, "value": round(head["value"], 4), "unit": "outer particles/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": {"fp64": "f64", "mixed": "mixed (fp32 matrix + fp32 draws; fp64 means/likelihood)",
                      "bf16": "mixed (split-bf16 matrix + fp32 draws; fp64 means/likelihood)"}[forms[0]],
-           "data": "synthetic (seeded GNSS stations, Gaussian slip bump, 5 mm noise)",
-           "config": {"workload": "c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, "
-                                  "truncated-MVN Gibbs x5 per stage; waves of %d geometry particles "
-                                  "streamed through %d device slots" % (B, a.slots),
-                      "form": forms[0], "evals_per_step": B, "slots": a.slots, "ndim_slip": head["ndim"],
-                      "nobs": head["nobs"], "inner_particles": head["P"],
+           "data": "synthetic (seeded GNSS stations, Gaussian slip bump, 5 mm noise; thetas from the prior)",
+           "config": {"workload": "c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, truncated-MVN "
+                                  "Gibbs x5 per stage; one outer-SMC wave stream of %d geometries through %d "
+                                  "device slots, step = %d completed evaluations" % (total, a.slots, E),
+                      "form": forms[0], "evals_per_step": E, "warmup_evals_per_step": Ew, "tail_evals": tail,
+                      "slots": a.slots, "ndim_slip": head["ndim"], "nobs": head["nobs"],
+                      "inner_particles": head["P"],
                       "mean_inner_stages": round(float(head["stages"].mean()), 2),
-                      "p50_wave_latency_s": p50, "p99_wave_latency_s": p99,
-                      # per geometry: from the wave's submission to the stage boundary that returned its -log Z
-                      "p50_eval_latency_s": head["eval_lat"][0], "p99_eval_latency_s": head["eval_lat"][1]},
-           "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving}
+                      "mean_inner_stages_timed": round(float(head["stages_timed"].mean()), 2),
+                      "p50_step_s": round(float(np.percentile(head["step_s"], 50)), 3),
+                      "p99_step_s": round(float(np.percentile(head["step_s"], 99)), 3),
+                      # per geometry: from its set-up launch to the stage boundary that returned its -log Z
+                      "p50_eval_latency_s": head["eval_lat"][0], "p99_eval_latency_s": head["eval_lat"][1],
+                      "stream": head["stream"]},
+           "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving,
+           "skipped_legs": skipped, "wall_s": round(elapsed(), 1)}
     if "fp64" in res and len(res) > 1:
         f64 = res["fp64"]
         mixed = {}
@@ -310,13 +393,12 @@ def main():
             rel = np.abs(m["neglog"][both] - f64["neglog"][both]) / np.abs(f64["neglog"][both])
             if rel.size == 0:  # every geometry at the sentinel in one form or the other
                 rel = np.array([np.nan])
-            m50, m99 = latency(m)
             mixed[name] = {"value": round(m["value"], 4), "unit": "outer particles/s",
                            "speedup_vs_fp64": round(m["value"] / f64["value"], 3),
                            "ms_per_step": round(m["ms_per_step"], 2),
                            "mean_inner_stages": round(float(m["stages"].mean()), 2),
-                           "p50_wave_latency_s": m50, "p99_wave_latency_s": m99,
                            "p50_eval_latency_s": m["eval_lat"][0], "p99_eval_latency_s": m["eval_lat"][1],
+                           "stream": m["stream"],
                            "neglogz_rel_diff_vs_fp64": {"median": float("%.3g" % np.median(rel)),
                                                         "p99": float("%.3g" % np.percentile(rel, 99)),
                                                         "max": float("%.3g" % rel.max()),

@@ -123,6 +123,9 @@ int smcf_fault_calc_likelihood_batch(smcf_problem* pb, int B, const double* thet
 /* Per-geometry latency of the last batch call on this problem (no counterpart in the reference): seconds from
  * the call's start to the stage boundary at which geometry j's result was read back, for j < B. */
 int smcf_batch_times(const smcf_problem* pb, int B, double* seconds);
+/* Seconds from the start of the last batch call to the launch of geometry j's set-up in a free slot (0 for the
+ * geometries of the first fill); smcf_batch_times minus this is the geometry's own latency in the stream. */
+int smcf_batch_admit_times(const smcf_problem* pb, int B, double* seconds);
 
 /* Plain mean of each geometry's final slip particles (ndim_slip x B, column b for theta column b) and its -log Z:
  * the per-particle loop of calc_slip_map (main.f90:600-640, fault_calc_likelihood then the average over the

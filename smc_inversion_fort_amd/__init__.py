@@ -58,7 +58,7 @@ EXPORTS = [
     "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec",
     "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_", "smcf_problem_bytes", "smcf_problem_device_bytes",
     "smcf_server_create", "smcf_server_submit", "smcf_server_wait", "smcf_server_status", "smcf_server_destroy",
-    "smcf_slip_mean_batch", "smcf_batch_times",
+    "smcf_slip_mean_batch", "smcf_batch_times", "smcf_batch_admit_times",
 ]
 
 
@@ -112,6 +112,7 @@ def _declare(L):
     L.smcf_server_destroy.argtypes = [C.c_void_p]
     L.smcf_slip_mean_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _D]
     L.smcf_batch_times.argtypes = [C.c_void_p, C.c_int, _D]
+    L.smcf_batch_admit_times.argtypes = [C.c_void_p, C.c_int, _D]
 
 
 def _check(rc):
@@ -431,6 +432,12 @@ class Problem:
         """Seconds from the start of the last batch call to each of its B geometries' results."""
         t = np.zeros(B)
         _check(lib().smcf_batch_times(self.h, int(B), _dp(t)))
+        return t
+
+    def batch_admit_times(self, B):
+        """Seconds from the start of the last batch call to each geometry's set-up launch (0: first fill)."""
+        t = np.zeros(B)
+        _check(lib().smcf_batch_admit_times(self.h, int(B), _dp(t)))
         return t
 
     def fault_calc_likelihood(self, thetas, keys):

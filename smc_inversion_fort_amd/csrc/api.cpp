@@ -74,9 +74,13 @@ void resolve_profiles(smcf_ctx* ctx) {
   ctx->pending.clear();
 }
 
+// Allocations of a problem.  After the first failure every later call returns that failure's code without
+// allocating, so problem_build can issue its allocations in sequence and check P.alloc_rc once.
 template <class T>
 int dalloc(Problem& P, T** ptr, size_t n) {
   void* p = nullptr;
+  *ptr = nullptr;
+  if (P.alloc_rc) return P.alloc_rc;
   if (n == 0) n = 1;
   P.bytes += (n * sizeof(T) + 255) & ~(size_t)255;
   if (P.sizing) {
@@ -86,6 +90,7 @@ int dalloc(Problem& P, T** ptr, size_t n) {
   hipError_t e = hipMalloc(&p, n * sizeof(T));
   if (e != hipSuccess) {
     set_error(std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + " bytes): " + hipGetErrorString(e));
+    P.alloc_rc = SMCF_ENOMEM;
     return SMCF_ENOMEM;
   }
   P.allocs.push_back(p);
@@ -93,12 +98,39 @@ int dalloc(Problem& P, T** ptr, size_t n) {
   return SMCF_OK;
 }
 
+// Temporary device buffers of one component entry point's call, freed on every return path (hipFree waits for
+// the device, so kernels still reading them have finished).
+struct TmpBufs {
+  std::vector<void*> p;
+  TmpBufs() = default;
+  TmpBufs(const TmpBufs&) = delete;
+  TmpBufs& operator=(const TmpBufs&) = delete;
+  ~TmpBufs() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
+    if (e == hipSuccess) p.push_back(q);
+    *out = (T*)q;
+    return e;
+  }
+};
+
 template <class T>
 int upload(Problem& P, const T** dst, const std::vector<T>& v) {
   T* d = nullptr;
   int rc = dalloc(P, &d, v.size());
   if (rc) return rc;
-  if (!v.empty() && !P.sizing) SMCF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty() && !P.sizing) {
+    hipError_t e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMemcpy (problem upload): ") + hipGetErrorString(e));
+      P.alloc_rc = SMCF_EHIP;
+      return SMCF_EHIP;
+    }
+  }
   *dst = d;
   return SMCF_OK;
 }
@@ -173,6 +205,7 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
   const int nf = m.ndim_fault;
   const auto t_call = std::chrono::steady_clock::now();
   P.t_done.assign(N, 0.0);
+  P.t_admit.assign(N, 0.0);
   const int W = std::min(N, bt.cap);  // slots in use
   SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * nf * W, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * W, hipMemcpyHostToDevice, s));
@@ -263,8 +296,10 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
       adm.jobs.resize(n);
       adm.th.resize((size_t)nf * n);
       adm.key.resize(n);
+      const double t_adm = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_call).count();
       for (int q = 0; q < n; ++q) {
         adm.jobs[q] = next + q;
+        P.t_admit[next + q] = t_adm;
         std::copy(theta + (size_t)(next + q) * nf, theta + (size_t)(next + q + 1) * nf,
                   adm.th.begin() + (size_t)q * nf);
         adm.key[q] = keys[next + q];
@@ -471,27 +506,34 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   for (int k = 0; k < np - 1; ++k) xb[k] = d->xbend ? d->xbend[k] : 0.0;
   std::vector<double> uv(d->obs_unitvec, d->obs_unitvec + 3 * (size_t)m.nobs);
   std::vector<double> os(d->obs_sigma, d->obs_sigma + m.nobs), dv(d->dvec, d->dvec + m.nobs);
-  int rc = 0;
-  rc |= upload(P, &m.xbend, xb);
-  rc |= upload(P, &m.plane_nxi, pnxi);
-  rc |= upload(P, &m.plane_neta, pneta);
-  rc |= upload(P, &m.plane_node0, pnode0);
-  rc |= upload(P, &m.cny, cny);
-  rc |= upload(P, &m.patch_plane, patch_plane);
-  rc |= upload(P, &m.node_plane, node_plane);
-  rc |= upload(P, &m.node_k, node_k);
-  rc |= upload(P, &m.node_l, node_l);
-  rc |= upload(P, &m.id_dof, id_dof);
-  rc |= upload(P, &m.dof_of_node, dof_of_node);
-  rc |= upload(P, &m.n2e, n2e);
-  rc |= upload(P, &m.n2e_slot, n2e_slot);
-  rc |= upload(P, &m.n2es, n2es);
-  rc |= upload(P, &m.pts, pts);
-  rc |= upload(P, &m.pt_row0, row0);
-  rc |= upload(P, &m.pt_nrow, nrow);
-  rc |= upload(P, &m.unitvec, uv);
-  rc |= upload(P, &m.obs_sigma, os);
-  rc |= upload(P, &m.dvec, dv);
+  // every device buffer allocated so far is freed on any return before the problem is handed out
+  struct FreeOnFail {
+    Problem* p;
+    ~FreeOnFail() {
+      if (p)
+        for (void* q : p->allocs) (void)hipFree(q);
+    }
+  } guard{&P};
+  upload(P, &m.xbend, xb);
+  upload(P, &m.plane_nxi, pnxi);
+  upload(P, &m.plane_neta, pneta);
+  upload(P, &m.plane_node0, pnode0);
+  upload(P, &m.cny, cny);
+  upload(P, &m.patch_plane, patch_plane);
+  upload(P, &m.node_plane, node_plane);
+  upload(P, &m.node_k, node_k);
+  upload(P, &m.node_l, node_l);
+  upload(P, &m.id_dof, id_dof);
+  upload(P, &m.dof_of_node, dof_of_node);
+  upload(P, &m.n2e, n2e);
+  upload(P, &m.n2e_slot, n2e_slot);
+  upload(P, &m.n2es, n2es);
+  upload(P, &m.pts, pts);
+  upload(P, &m.pt_row0, row0);
+  upload(P, &m.pt_nrow, nrow);
+  upload(P, &m.unitvec, uv);
+  upload(P, &m.obs_sigma, os);
+  upload(P, &m.dvec, dv);
   // InSAR paths: rows nsar_index(p) .. nsar_index(p+1)-1 (1-based) with the inverse covariance blocks the
   // caller computed (calc_sigma_sar_mat, init.f90:82-119), column-major, concatenated in path order
   m.npath = 0;
@@ -519,130 +561,129 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
     }
     std::vector<double> sinv(d->sar_sinv, d->sar_sinv + off);
     m.npath = d->npath;
-    rc |= upload(P, &m.sinv, sinv);
-    rc |= upload(P, &m.sar_r0, r0v);
-    rc |= upload(P, &m.sar_n, nv);
-    rc |= upload(P, &m.sar_off, offv);
-    rc |= upload(P, &m.row_path, rowp);
+    upload(P, &m.sinv, sinv);
+    upload(P, &m.sar_r0, r0v);
+    upload(P, &m.sar_n, nv);
+    upload(P, &m.sar_off, offv);
+    upload(P, &m.row_path, rowp);
     P.sar_r0 = r0v; P.sar_n = nv; P.sar_off = offv;
   }
-  if (rc) return rc;
+  if (P.alloc_rc) return P.alloc_rc;
   // batch buffers
   BatchDev& bt = P.bt;
   memset(&bt, 0, sizeof(bt));
   size_t C = (size_t)max_batch, nd = m.ndim, Pn = P.P;
   bt.cap = max_batch;
   bt.P = P.P;
-  rc |= dalloc(P, &bt.theta, m.ndim_fault * C);
-  rc |= dalloc(P, &bt.geo, 16 * np * C);
-  rc |= dalloc(P, &bt.scal, 8 * C);
-  rc |= dalloc(P, &bt.G, (size_t)m.nobs * nd * C);
-  rc |= dalloc(P, &bt.gscr, (size_t)24 * m.npts * npatch * C);
-  rc |= dalloc(P, &bt.luni, (size_t)9 * nnode * C);
-  rc |= dalloc(P, &bt.li, (size_t)9 * 2 * nnode * C);
-  rc |= dalloc(P, &bt.lv, (size_t)9 * 2 * nnode * C);
-  rc |= dalloc(P, &bt.lti, (size_t)9 * 2 * ndof * C);
-  rc |= dalloc(P, &bt.ltv, (size_t)9 * 2 * ndof * C);
-  rc |= dalloc(P, &bt.llc_idx, (size_t)25 * nd * C);
-  rc |= dalloc(P, &bt.llc_val, (size_t)25 * nd * C);
+  dalloc(P, &bt.theta, m.ndim_fault * C);
+  dalloc(P, &bt.geo, 16 * np * C);
+  dalloc(P, &bt.scal, 8 * C);
+  dalloc(P, &bt.G, (size_t)m.nobs * nd * C);
+  dalloc(P, &bt.gscr, (size_t)24 * m.npts * npatch * C);
+  dalloc(P, &bt.luni, (size_t)9 * nnode * C);
+  dalloc(P, &bt.li, (size_t)9 * 2 * nnode * C);
+  dalloc(P, &bt.lv, (size_t)9 * 2 * nnode * C);
+  dalloc(P, &bt.lti, (size_t)9 * 2 * ndof * C);
+  dalloc(P, &bt.ltv, (size_t)9 * 2 * ndof * C);
+  dalloc(P, &bt.llc_idx, (size_t)25 * nd * C);
+  dalloc(P, &bt.llc_val, (size_t)25 * nd * C);
   bt.Gw = nullptr;
-  if (m.nsar > 0) rc |= dalloc(P, &bt.Gw, (size_t)m.nsar * nd * C);
-  rc |= dalloc(P, &bt.sdv, (size_t)m.nobs * C);
-  rc |= dalloc(P, &bt.H, nd * nd * C);
-  rc |= dalloc(P, &bt.bvec, nd * C);
-  rc |= dalloc(P, &bt.dsd, C);
+  if (m.nsar > 0) dalloc(P, &bt.Gw, (size_t)m.nsar * nd * C);
+  dalloc(P, &bt.sdv, (size_t)m.nobs * C);
+  dalloc(P, &bt.H, nd * nd * C);
+  dalloc(P, &bt.bvec, nd * C);
+  dalloc(P, &bt.dsd, C);
   bt.kb = gibbs_kb(P.precision);
-  rc |= dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked Gibbs layout, padded to whole blocks
   bt.Q32 = nullptr;
   bt.S32 = bt.S32b = nullptr;
   bt.SX = bt.SXb = bt.QX = nullptr;
   if (P.precision == SMCF_PRECISION_MIXED) {
-    rc |= dalloc(P, &bt.Q32, (nd + bt.kb) * nd * C);
-    rc |= dalloc(P, &bt.S32, nd * Pn * C);
-    rc |= dalloc(P, &bt.S32b, nd * Pn * C);
+    dalloc(P, &bt.Q32, (nd + bt.kb) * nd * C);
+    dalloc(P, &bt.S32, nd * Pn * C);
+    dalloc(P, &bt.S32b, nd * Pn * C);
   }
   bt.HX = bt.DX = nullptr;
   bt.mref = bt.gref = bt.lref = nullptr;
   if (P.precision == SMCF_PRECISION_MIXED_BF16) {
-    rc |= dalloc(P, &bt.QX, bf_q_elems((int)nd, bt.kb) * C);
-    rc |= dalloc(P, &bt.SX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
-    rc |= dalloc(P, &bt.SXb, (size_t)bf_rows((int)nd) * Pn * 2 * C);
-    rc |= dalloc(P, &bt.HX, bf_q_elems((int)nd, 64) * C);
-    rc |= dalloc(P, &bt.DX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
-    rc |= dalloc(P, &bt.mref, nd * C);
-    rc |= dalloc(P, &bt.gref, nd * C);
-    rc |= dalloc(P, &bt.lref, C);
+    dalloc(P, &bt.QX, bf_q_elems((int)nd, bt.kb) * C);
+    dalloc(P, &bt.SX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    dalloc(P, &bt.SXb, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    dalloc(P, &bt.HX, bf_q_elems((int)nd, 64) * C);
+    dalloc(P, &bt.DX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
+    dalloc(P, &bt.mref, nd * C);
+    dalloc(P, &bt.gref, nd * C);
+    dalloc(P, &bt.lref, C);
     // rows past ndim of the centred copy are read by the likelihood's matrix products and never written
-    if (!rc && !sizing && hipMemset(bt.DX, 0, sizeof(uint16_t) * (size_t)bf_rows((int)nd) * Pn * 2 * C) != hipSuccess) {
+    if (!P.alloc_rc && !sizing &&
+        hipMemset(bt.DX, 0, sizeof(uint16_t) * (size_t)bf_rows((int)nd) * Pn * 2 * C) != hipSuccess) {
       set_error("hipMemset failed");
-      rc = SMCF_EHIP;
+      P.alloc_rc = SMCF_EHIP;
     }
   }
-  rc |= dalloc(P, &bt.qdiag, 4 * nd * C);
-  rc |= dalloc(P, &bt.S, nd * Pn * C);
-  rc |= dalloc(P, &bt.S2, nd * Pn * C);
+  dalloc(P, &bt.qdiag, 4 * nd * C);
+  dalloc(P, &bt.S, nd * Pn * C);
+  dalloc(P, &bt.S2, nd * Pn * C);
   bt.Y = nullptr;
-  if (P.mover == SMCF_MOVER_HMC) rc |= dalloc(P, &bt.Y, nd * Pn * C);  // H X of the leapfrog (HMC only)
-  rc |= dalloc(P, &bt.like, Pn * C);
-  rc |= dalloc(P, &bt.prior, Pn * C);
-  rc |= dalloc(P, &bt.like2, Pn * C);
-  rc |= dalloc(P, &bt.prior2, Pn * C);
-  rc |= dalloc(P, &bt.w, Pn * C);
-  rc |= dalloc(P, &bt.parent, Pn * C);
-  rc |= dalloc(P, &bt.assigned, Pn * C);
-  rc |= dalloc(P, &bt.mean, nd * C);
-  rc |= dalloc(P, &bt.cov, nd * C);
-  rc |= dalloc(P, &bt.key, C);
-  rc |= dalloc(P, &bt.gamma, C);
-  rc |= dalloc(P, &bt.neglog, C);
-  rc |= dalloc(P, &bt.iter, C);
-  rc |= dalloc(P, &bt.active, C);
-  rc |= dalloc(P, &bt.done, C);
-  rc |= dalloc(P, &bt.sel, 2 * C);
-  rc |= dalloc(P, &bt.par, C);
-  rc |= dalloc(P, &bt.gtrace, 202 * C);
+  if (P.mover == SMCF_MOVER_HMC) dalloc(P, &bt.Y, nd * Pn * C);  // H X of the leapfrog (HMC only)
+  dalloc(P, &bt.like, Pn * C);
+  dalloc(P, &bt.prior, Pn * C);
+  dalloc(P, &bt.like2, Pn * C);
+  dalloc(P, &bt.prior2, Pn * C);
+  dalloc(P, &bt.w, Pn * C);
+  dalloc(P, &bt.parent, Pn * C);
+  dalloc(P, &bt.assigned, Pn * C);
+  dalloc(P, &bt.mean, nd * C);
+  dalloc(P, &bt.cov, nd * C);
+  dalloc(P, &bt.key, C);
+  dalloc(P, &bt.gamma, C);
+  dalloc(P, &bt.neglog, C);
+  dalloc(P, &bt.iter, C);
+  dalloc(P, &bt.active, C);
+  dalloc(P, &bt.done, C);
+  dalloc(P, &bt.sel, 2 * C);
+  dalloc(P, &bt.par, C);
+  dalloc(P, &bt.gtrace, 202 * C);
   if (P.mover == SMCF_MOVER_HMC) {
     HmcDev& h = P.hm;
     size_t nn2 = 2 * (size_t)nnode;
-    rc |= dalloc(P, &h.Xc, nd * Pn * C);
-    rc |= dalloc(P, &h.Xt, nd * Pn * C);
-    rc |= dalloc(P, &h.Pm, nd * Pn * C);
-    rc |= dalloc(P, &h.LS, nn2 * Pn * C);
-    rc |= dalloc(P, &h.Lc, Pn * C);
-    rc |= dalloc(P, &h.Rc, Pn * C);
-    rc |= dalloc(P, &h.postc, Pn * C);
-    rc |= dalloc(P, &h.hcur, Pn * C);
-    rc |= dalloc(P, &h.dtau, Pn * C);
-    rc |= dalloc(P, &h.dtmp, Pn * C);
-    rc |= dalloc(P, &h.score, Pn * C);
-    rc |= dalloc(P, &h.ldlo, C);
-    rc |= dalloc(P, &h.ldhi, C);
-    rc |= dalloc(P, &h.wptr, (Pn + 1) * C);
-    rc |= dalloc(P, &h.tptr, (Pn + 1) * C);
-    rc |= dalloc(P, &h.wcols, Pn * C);
-    rc |= dalloc(P, &h.tcols, Pn * C);
-    rc |= dalloc(P, &h.wamax, C);
-    rc |= dalloc(P, &h.tamax, C);
-    rc |= dalloc(P, &h.ncol, C);
-    rc |= dalloc(P, &h.cpar, Pn * C);
-    rc |= dalloc(P, &h.cchild, Pn * C);
-    rc |= dalloc(P, &h.dead, Pn * C);
-    rc |= dalloc(P, &h.idstart, Pn * C);
-    rc |= dalloc(P, &h.flag, C);
+    dalloc(P, &h.Xc, nd * Pn * C);
+    dalloc(P, &h.Xt, nd * Pn * C);
+    dalloc(P, &h.Pm, nd * Pn * C);
+    dalloc(P, &h.LS, nn2 * Pn * C);
+    dalloc(P, &h.Lc, Pn * C);
+    dalloc(P, &h.Rc, Pn * C);
+    dalloc(P, &h.postc, Pn * C);
+    dalloc(P, &h.hcur, Pn * C);
+    dalloc(P, &h.dtau, Pn * C);
+    dalloc(P, &h.dtmp, Pn * C);
+    dalloc(P, &h.score, Pn * C);
+    dalloc(P, &h.ldlo, C);
+    dalloc(P, &h.ldhi, C);
+    dalloc(P, &h.wptr, (Pn + 1) * C);
+    dalloc(P, &h.tptr, (Pn + 1) * C);
+    dalloc(P, &h.wcols, Pn * C);
+    dalloc(P, &h.tcols, Pn * C);
+    dalloc(P, &h.wamax, C);
+    dalloc(P, &h.tamax, C);
+    dalloc(P, &h.ncol, C);
+    dalloc(P, &h.cpar, Pn * C);
+    dalloc(P, &h.cchild, Pn * C);
+    dalloc(P, &h.dead, Pn * C);
+    dalloc(P, &h.idstart, Pn * C);
+    dalloc(P, &h.flag, C);
   } else if (P.mover != SMCF_MOVER_GIBBS) {
-    rc = SMCF_EINVAL;
     set_error("unknown mover");
+    return SMCF_EINVAL;
   }
-  if (rc) {
-    for (void* p : P.allocs) (void)hipFree(p);
-    return rc;
-  }
+  if (P.alloc_rc) return P.alloc_rc;
   if (sizing) {
     *bytes = P.bytes;
     return SMCF_OK;
   }
   SMCF_HIP(hipMemset(bt.parent, 0, sizeof(int) * Pn * C));
   SMCF_HIP(hipMemset(bt.gtrace, 0, sizeof(double) * 202 * C));
+  guard.p = nullptr;
   *out = holder.release();
   return SMCF_OK;
 }
@@ -659,6 +700,12 @@ int smcf_problem_bytes(const smcf_problem_desc* d, int max_batch, size_t* bytes)
 int smcf_batch_times(const smcf_problem* pb, int B, double* seconds) {
   if (!pb || !seconds || B < 0 || (size_t)B > pb->p.t_done.size()) return SMCF_EINVAL;
   std::copy(pb->p.t_done.begin(), pb->p.t_done.begin() + B, seconds);
+  return SMCF_OK;
+}
+
+int smcf_batch_admit_times(const smcf_problem* pb, int B, double* seconds) {
+  if (!pb || !seconds || B < 0 || (size_t)B > pb->p.t_admit.size()) return SMCF_EINVAL;
+  std::copy(pb->p.t_admit.begin(), pb->p.t_admit.begin() + B, seconds);
   return SMCF_OK;
 }
 
@@ -691,19 +738,20 @@ int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_s
 int smcf_dc3d0(smcf_ctx* ctx, int n, const double* in, double* u, int* iret) {
   if (!ctx || n < 0 || (n > 0 && (!in || !u || !iret))) return SMCF_EINVAL;
   if (n == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   double *din = nullptr, *du = nullptr;
   int* dir = nullptr;
-  SMCF_HIP(hipMalloc(&din, sizeof(double) * 10 * n));
-  SMCF_HIP(hipMalloc(&du, sizeof(double) * 12 * n));
-  SMCF_HIP(hipMalloc(&dir, sizeof(int) * n));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&din, 10 * (size_t)n));
+  SMCF_HIP(tb.alloc(&du, 12 * (size_t)n));
+  SMCF_HIP(tb.alloc(&dir, (size_t)n));
   SMCF_HIP(hipMemcpyAsync(din, in, sizeof(double) * 10 * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(dir, iret, sizeof(int) * n, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_dc3d0_batch(n, din, du, dir, s));
   SMCF_HIP(hipMemcpyAsync(u, du, sizeof(double) * 12 * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipMemcpyAsync(iret, dir, sizeof(int) * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
-  (void)(void)hipFree(din); (void)hipFree(du); (void)hipFree(dir);
   return SMCF_OK;
 }
 
@@ -860,11 +908,13 @@ int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, do
                          double* gamma, double* w, double* nev) {
   if (!ctx || P < 1 || !L) return SMCF_EINVAL;
   if (P > kMaxInnerParticles) { set_error("P exceeds the tempering workgroup's LDS"); return SMCF_EINVAL; }
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   double *dL, *dw, *dout;
-  SMCF_HIP(hipMalloc(&dL, sizeof(double) * P));
-  SMCF_HIP(hipMalloc(&dw, sizeof(double) * P));
-  SMCF_HIP(hipMalloc(&dout, sizeof(double) * 3));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&dL, (size_t)P));
+  SMCF_HIP(tb.alloc(&dw, (size_t)P));
+  SMCF_HIP(tb.alloc(&dout, 3));
   SMCF_HIP(hipMemcpyAsync(dL, L, sizeof(double) * P, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_find_next_gamma(P, gprev, dL, cv, tol, cap, dout, dw, s));
   double out3[3];
@@ -873,25 +923,25 @@ int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, do
   SMCF_HIP(hipStreamSynchronize(s));
   if (gamma) *gamma = out3[0];
   if (nev) *nev = out3[1];
-  (void)(void)hipFree(dL); (void)hipFree(dw); (void)hipFree(dout);
   return SMCF_OK;
 }
 
 int smcf_normalize_resample(smcf_ctx* ctx, int P, const double* w, double uh, double ul, double* wn, int* assigned) {
   if (!ctx || P < 1 || !w) return SMCF_EINVAL;
   if (P > kMaxInnerParticles) { set_error("P exceeds the tempering workgroup's LDS"); return SMCF_EINVAL; }
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   double *dw, *dwn;
   int* da;
-  SMCF_HIP(hipMalloc(&dw, sizeof(double) * P));
-  SMCF_HIP(hipMalloc(&dwn, sizeof(double) * P));
-  SMCF_HIP(hipMalloc(&da, sizeof(int) * 2 * P));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&dw, (size_t)P));
+  SMCF_HIP(tb.alloc(&dwn, (size_t)P));
+  SMCF_HIP(tb.alloc(&da, 2 * (size_t)P));
   SMCF_HIP(hipMemcpyAsync(dw, w, sizeof(double) * P, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_normalize_resample(P, dw, uh, ul, dwn, da, s));
   if (wn) SMCF_HIP(hipMemcpyAsync(wn, dwn, sizeof(double) * P, hipMemcpyDeviceToHost, s));
   if (assigned) SMCF_HIP(hipMemcpyAsync(assigned, da, sizeof(int) * P, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
-  (void)(void)hipFree(dw); (void)hipFree(dwn); (void)hipFree(da);
   return SMCF_OK;
 }
 
@@ -899,19 +949,20 @@ int smcf_trunc_normal(smcf_ctx* ctx, int n, const double* mu, const double* sg, 
                       double* out) {
   if (!ctx || n < 0 || (n > 0 && (!mu || !sg || !u || !out))) return SMCF_EINVAL;
   if (n == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   double *dm, *ds, *du, *dout;
-  SMCF_HIP(hipMalloc(&dm, sizeof(double) * n));
-  SMCF_HIP(hipMalloc(&ds, sizeof(double) * n));
-  SMCF_HIP(hipMalloc(&du, sizeof(double) * n));
-  SMCF_HIP(hipMalloc(&dout, sizeof(double) * n));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&dm, (size_t)n));
+  SMCF_HIP(tb.alloc(&ds, (size_t)n));
+  SMCF_HIP(tb.alloc(&du, (size_t)n));
+  SMCF_HIP(tb.alloc(&dout, (size_t)n));
   SMCF_HIP(hipMemcpyAsync(dm, mu, sizeof(double) * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(ds, sg, sizeof(double) * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(du, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_trunc_normal(n, dm, ds, lo, hi, du, dout, s));
   SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
-  (void)(void)hipFree(dm); (void)hipFree(ds); (void)hipFree(du); (void)hipFree(dout);
   return SMCF_OK;
 }
 
@@ -919,37 +970,39 @@ int smcf_trunc_normal_f32(smcf_ctx* ctx, int n, const double* mu, const double* 
                           const float* u, double* out) {
   if (!ctx || n < 0 || (n > 0 && (!mu || !sq || !u || !out))) return SMCF_EINVAL;
   if (n == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   double *dm, *ds, *dout;
   float* du;
-  SMCF_HIP(hipMalloc(&dm, sizeof(double) * n));
-  SMCF_HIP(hipMalloc(&ds, sizeof(double) * n));
-  SMCF_HIP(hipMalloc(&du, sizeof(float) * n));
-  SMCF_HIP(hipMalloc(&dout, sizeof(double) * n));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&dm, (size_t)n));
+  SMCF_HIP(tb.alloc(&ds, (size_t)n));
+  SMCF_HIP(tb.alloc(&du, (size_t)n));
+  SMCF_HIP(tb.alloc(&dout, (size_t)n));
   SMCF_HIP(hipMemcpyAsync(dm, mu, sizeof(double) * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(ds, sq, sizeof(double) * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(du, u, sizeof(float) * n, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_trunc_normal_f32(n, dm, ds, lo, hi, du, dout, s));
   SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
-  (void)hipFree(dm); (void)hipFree(ds); (void)hipFree(du); (void)hipFree(dout);
   return SMCF_OK;
 }
 
 int smcf_philox(smcf_ctx* ctx, int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
   if (!ctx || n < 0 || (n > 0 && (!ctr || !key || !out))) return SMCF_EINVAL;
   if (n == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   uint32_t *dc, *dk, *dout;
-  SMCF_HIP(hipMalloc(&dc, sizeof(uint32_t) * 4 * n));
-  SMCF_HIP(hipMalloc(&dk, sizeof(uint32_t) * 2 * n));
-  SMCF_HIP(hipMalloc(&dout, sizeof(uint32_t) * 4 * n));
+  TmpBufs tb;
+  SMCF_HIP(tb.alloc(&dc, 4 * (size_t)n));
+  SMCF_HIP(tb.alloc(&dk, 2 * (size_t)n));
+  SMCF_HIP(tb.alloc(&dout, 4 * (size_t)n));
   SMCF_HIP(hipMemcpyAsync(dc, ctr, sizeof(uint32_t) * 4 * n, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(dk, key, sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice, s));
   CHECK_LAUNCH(launch_philox(n, dc, dk, dout, s));
   SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
-  (void)(void)hipFree(dc); (void)hipFree(dk); (void)hipFree(dout);
   return SMCF_OK;
 }
 

@@ -147,7 +147,9 @@ struct Problem {
   std::vector<void*> allocs;
   bool sizing = false;  // count the device bytes of a description without allocating (smcf_problem_bytes)
   size_t bytes = 0;     // device bytes allocated (or counted) for the problem
+  int alloc_rc = 0;     // first failure of a problem allocation or upload (SMCF_* code), 0 while none
   std::vector<double> t_done;  // seconds from the start of the last batch call to each geometry's result
+  std::vector<double> t_admit; // seconds from the start of the last batch call to each geometry's set-up launch
 };
 
 struct StageCfg {
