@@ -137,6 +137,14 @@ int smcf_slip_mean_batch(smcf_problem* pb, int B, const double* theta, const uin
 /* slip_find_next_gamma / fault_find_next_gamma (smc_slip.f90:314-387, smc_fault.f90:392-485) */
 int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gamma_prev, const double* likelihood, double cv,
                          double tol, int cap, double* gamma, double* weights, double* neglog_evidence);
+/* Allocation-free smcf_find_next_gamma.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops and harnesses that forbid hipMalloc).  Same arguments
+ * and results, bit for bit, and no hipMalloc/hipFree: the device copies of the likelihoods and weights live in a
+ * fixed scratch array that the library holds on each device for the life of the process (its address is resolved
+ * by smcf_ctx_create).  P <= 3584, else SMCF_EINVAL.  Calls on one GPU share that array: calls that overlap on one
+ * GPU (from several contexts or threads) race and must be serialised by the caller. */
+int smcf_find_next_gamma_noalloc(smcf_ctx* ctx, int P, double gamma_prev, const double* likelihood, double cv,
+                                 double tol, int cap, double* gamma, double* weights, double* neglog_evidence);
 /* slip_normalize_weights + slip_resample_particles (smc_slip.f90:465-483, 572-589);
  * u_hi + u_lo is the unclamped real(16) uniform. */
 int smcf_normalize_resample(smcf_ctx* ctx, int P, const double* weights, double u_hi, double u_lo,

@@ -58,7 +58,7 @@ EXPORTS = [
     "smcf_trunc_normal", "smcf_trunc_normal_f32", "smcf_philox", "smcf_fault_smc_set_output_dir", "smcf_fault_smc_exec",
     "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_", "smcf_problem_bytes", "smcf_problem_device_bytes",
     "smcf_server_create", "smcf_server_submit", "smcf_server_wait", "smcf_server_status", "smcf_server_destroy",
-    "smcf_slip_mean_batch", "smcf_batch_times", "smcf_batch_admit_times",
+    "smcf_slip_mean_batch", "smcf_batch_times", "smcf_batch_admit_times", "smcf_find_next_gamma_noalloc",
 ]
 
 
@@ -91,6 +91,8 @@ def _declare(L):
     L.smcf_fault_calc_likelihood_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _I]
     L.smcf_find_next_gamma.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, C.c_double, C.c_double, C.c_int,
                                        _D, _D, _D]
+    L.smcf_find_next_gamma_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, C.c_double, C.c_double,
+                                               C.c_int, _D, _D, _D]
     L.smcf_normalize_resample.argtypes = [C.c_void_p, C.c_int, _D, C.c_double, C.c_double, _D, _I]
     L.smcf_trunc_normal.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
     L.smcf_trunc_normal_f32.argtypes = [C.c_void_p, C.c_int, _D, _D, C.c_double, C.c_double, C.POINTER(C.c_float), _D]
@@ -194,6 +196,16 @@ class Context:
         ev = C.c_double()
         _check(lib().smcf_find_next_gamma(self.h, L.shape[0], gamma_prev, _dp(L), cv, tol, cap, C.byref(g),
                                           _dp(w), C.byref(ev)))
+        return g.value, w, ev.value
+
+    def find_next_gamma_noalloc(self, gamma_prev, likelihood, cv=2.0, tol=1e-10, cap=100000):
+        """find_next_gamma without device allocations (the library's per-device scratch; P <= 3584)."""
+        L = np.ascontiguousarray(likelihood, dtype=np.float64)
+        w = np.zeros_like(L)
+        g = C.c_double()
+        ev = C.c_double()
+        _check(lib().smcf_find_next_gamma_noalloc(self.h, L.shape[0], gamma_prev, _dp(L), cv, tol, cap,
+                                                  C.byref(g), _dp(w), C.byref(ev)))
         return g.value, w, ev.value
 
     def normalize_resample(self, weights, u_hi, u_lo):

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,33 @@ void set_error(const std::string& msg) { g_err = msg; }
 using namespace smcf;
 
 static constexpr int kMaxInnerParticles = 3584;
+
+// Scratch of smcf_find_next_gamma_noalloc: the device copies of the likelihoods and the weights (kNoAllocMaxP
+// doubles each) and the two results, in one array that every device holds for the life of the process.  Host
+// code never passes `scratch` itself to a kernel or a copy: its address on a device comes from
+// hipGetSymbolAddress, resolved once per device by smcf_ctx_create and kept in g_scratch.
+static constexpr int kNoAllocMaxP = kMaxInnerParticles;
+static constexpr size_t kScratchDoubles = 2 * (size_t)kNoAllocMaxP + 4;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch;  // per device index; nullptr until resolved
+
+// Device address of `scratch` on `device`, which must be the current device.
+static int scratch_resolve(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch.size() <= device) g_scratch.resize((size_t)device + 1, nullptr);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return SMCF_EHIP;
+    }
+    g_scratch[device] = (double*)p;
+  }
+  *out = g_scratch[device];
+  return SMCF_OK;
+}
 
 struct smcf_problem {
   Problem p;
@@ -347,9 +375,13 @@ int smcf_ctx_create(int device, smcf_ctx** out) {
     return SMCF_EINVAL;
   }
   SMCF_HIP(hipSetDevice(device));
+  double* scr = nullptr;
+  int rs = scratch_resolve(device, &scr);
+  if (rs) return rs;
   auto* c = new smcf_ctx();
   c->device = device;
   c->profiling = false;
+  c->scratch = scr;
   hipError_t es = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (es == hipSuccess) {
     es = hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking);
@@ -923,6 +955,34 @@ int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, do
   SMCF_HIP(hipStreamSynchronize(s));
   if (gamma) *gamma = out3[0];
   if (nev) *nev = out3[1];
+  return SMCF_OK;
+}
+
+int smcf_find_next_gamma_noalloc(smcf_ctx* ctx, int P, double gprev, const double* L, double cv, double tol,
+                                 int cap, double* gamma, double* w, double* nev) {
+  if (!ctx || P < 1 || !L) return SMCF_EINVAL;
+  if (P > kNoAllocMaxP) {
+    set_error("smcf_find_next_gamma_noalloc: P = " + std::to_string(P) + " exceeds the scratch maximum of " +
+              std::to_string(kNoAllocMaxP) + " particles");
+    return SMCF_EINVAL;
+  }
+  if (!ctx->scratch) {
+    set_error("smcf_find_next_gamma_noalloc: the context has no scratch address");
+    return SMCF_ESTATE;
+  }
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  double* dL = ctx->scratch;          // kNoAllocMaxP likelihoods
+  double* dw = dL + kNoAllocMaxP;     // kNoAllocMaxP weights
+  double* dout = dw + kNoAllocMaxP;   // gamma, -log evidence increment
+  SMCF_HIP(hipMemcpyAsync(dL, L, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_find_next_gamma(P, gprev, dL, cv, tol, cap, dout, dw, s));
+  double out2[2];
+  SMCF_HIP(hipMemcpyAsync(out2, dout, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  if (w) SMCF_HIP(hipMemcpyAsync(w, dw, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  if (gamma) *gamma = out2[0];
+  if (nev) *nev = out2[1];
   return SMCF_OK;
 }
 

@@ -216,6 +216,7 @@ struct smcf_ctx {
   int device;
   hipStream_t stream;
   hipStream_t setup_stream;  // set-up of streamed-in instances, overlapping the stage loop
+  double* scratch = nullptr;  // device address of the no-allocation entry point's scratch array on this device
   bool profiling;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   struct Stat {
