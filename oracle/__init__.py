@@ -43,11 +43,13 @@ class Problem(C.Structure):
     _fields_ = [("nobs", C.c_int), ("nsar_total", C.c_int), ("ngnss", C.c_int), ("npath", C.c_int),
                 ("nsar_index", I), ("obs_points", D), ("obs_unitvec", D), ("obs_sigma", D),
                 ("dvec", D), ("sar_sinv", D), ("fault", FaultDesc), ("max_slip", C.c_double),
-                ("nparticle_slip", C.c_int), ("mover", C.c_int), ("gibbs_sweeps", C.c_int)]
+                ("nparticle_slip", C.c_int), ("mover", C.c_int), ("gibbs_sweeps", C.c_int),
+                ("hmc_ntau", C.c_int), ("hmc_accept", C.c_double), ("stage_cap", C.c_int)]
 
 
 class SlipResult(C.Structure):
-    _fields_ = [("nstage", C.c_int), ("neglog", C.c_double), ("gamma_trace", D), ("evid_trace", D)]
+    _fields_ = [("nstage", C.c_int), ("neglog", C.c_double), ("gamma_trace", D), ("evid_trace", D),
+                ("accept_trace", D), ("dtau_trace", D)]
 
 
 class FaultResult(C.Structure):
@@ -111,12 +113,20 @@ def _declare(L):
     L.orc_fault_calc_mean_particles.argtypes = [D, D, D, C.c_int, C.c_int]
     L.orc_fault_calc_cov_particles.argtypes = [D, D, D, D, C.c_int, C.c_int, I]
     L.orc_qsort.argtypes = [I, I, C.c_int]
+    L.orc_fault_propose.argtypes = [C.c_int, D, D, D, D, D]
     L.orc_fault_reorder.argtypes = [I, D, D, C.c_int, C.c_int, I]
     L.orc_fault_smc_exec.argtypes = [C.POINTER(Problem), D, C.c_int, C.c_uint64, D, D, C.c_int,
                                      C.POINTER(FaultResult)]
     L.orc_fault_smc_exec_gsmc.argtypes = [C.POINTER(Problem), D, C.c_int, C.c_uint64, D, D, C.c_int,
                                           C.POINTER(FaultResult), C.POINTER(Gsmc)]
     L.orc_inst_grad.argtypes = [C.POINTER(Problem), D, D, C.c_double, D, D]
+    L.orc_hmc_tuning_step.argtypes = [C.POINTER(Problem), D, D, C.c_uint64, C.c_int, C.c_double, D, D, D, D, I, D,
+                                      D, D, D]
+    L.orc_hmc_tuning_step.restype = C.c_int
+    L.orc_draw_normal.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_draw_normal.restype = C.c_double
+    L.orc_draw_clamped_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_draw_clamped_uniform.restype = C.c_double
     L.orc_hmc_leapfrog.argtypes = [C.POINTER(Problem), D, D, C.c_double, D, D, D, C.c_double, C.c_int, D, D, D, D]
     L.orc_fault_calc_neglog_q.argtypes = [D, D, D, C.c_double, C.c_int]
     L.orc_fault_calc_neglog_q.restype = C.c_double
@@ -148,6 +158,15 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().orc_philox4x32(c, k, o)
     return list(o)
+
+
+def draw_normal(key, site, stage, c2, c1):
+    """Box-Muller normal of the keyed stream (sin branch, clamped uniforms)."""
+    return lib().orc_draw_normal(key, site, stage, c2, c1)
+
+
+def draw_clamped_uniform(key, site, stage, c2, c1, word=0):
+    return lib().orc_draw_clamped_uniform(key, site, stage, c2, c1, word)
 
 
 def inner_key(seed, stage, role, slot):
@@ -220,7 +239,8 @@ class Fault:
 class ProblemHolder:
     """Problem description for the end-to-end oracle (GNSS rows and optional SAR paths)."""
 
-    def __init__(self, fault, obs, max_slip, nparticle_slip, mover, gibbs_sweeps=5):
+    def __init__(self, fault, obs, max_slip, nparticle_slip, mover, gibbs_sweeps=5, hmc_ntau=10, hmc_accept=0.65,
+                 stage_cap=0):
         self.fault = fault
         self.obs_points = f64(obs["obs_points"])
         self.obs_unitvec = f64(obs["obs_unitvec"])
@@ -232,7 +252,8 @@ class ProblemHolder:
         self.pb = Problem(nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
                           len(self.nsar_index) - 1, ip(self.nsar_index), dp(self.obs_points),
                           dp(self.obs_unitvec), dp(self.obs_sigma), dp(self.dvec), dp(self.sar_sinv),
-                          fault.desc, float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps))
+                          fault.desc, float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps),
+                          int(hmc_ntau), float(hmc_accept), int(stage_cap))
 
     def slip_smc(self, theta, gmat, key):
         f = self.fault
@@ -243,11 +264,25 @@ class ProblemHolder:
         mean = np.zeros(f.ndim)
         gt = np.zeros(202)
         et = np.zeros(202)
-        res = SlipResult(0, 0.0, dp(gt), dp(et))
+        at = np.zeros(202)
+        dt = np.zeros(202)
+        res = SlipResult(0, 0.0, dp(gt), dp(et), dp(at), dp(dt))
         lib().orc_slip_smc_exec(C.byref(self.pb), dp(f64(theta)), dp(f64(gmat)), key, dp(X), dp(L), dp(PR),
                                 dp(mean), C.byref(res))
-        return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=res.nstage, neglog=res.neglog,
-                    gamma=gt[:res.nstage].copy(), evid=et[:res.nstage].copy())
+        n = res.nstage
+        return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=n, neglog=res.neglog,
+                    gamma=gt[:n].copy(), evid=et[:n].copy(), accept=at[:n].copy(), dtau=dt[:n].copy())
+
+    def hmc_tuning_step(self, theta, gmat, key, stage, gamma, X, L, PR, cov, assigned, dtau, log_lo, log_hi):
+        """One slip_hmc_tuning call; returns (rc, dtau list, log_lo, log_hi, scores)."""
+        P = self.pb.nparticle_slip
+        d = f64(dtau).copy()
+        lo, hi = C.c_double(log_lo), C.c_double(log_hi)
+        sc = np.zeros(P)
+        rc = lib().orc_hmc_tuning_step(C.byref(self.pb), dp(f64(theta)), dp(f64(gmat)), key, stage, gamma,
+                                       dp(f64(X)), dp(f64(L)), dp(f64(PR)), dp(f64(cov)), ip(i32(assigned)), dp(d),
+                                       C.byref(lo), C.byref(hi), dp(sc))
+        return rc, d, lo.value, hi.value, sc
 
     def grad(self, theta, gmat, gamma, s):
         """Tempered gradient of gamma * likelihood + prior on this geometry (slip_calc_grad)."""
@@ -270,7 +305,7 @@ class ProblemHolder:
     def fault_likelihood(self, theta, key):
         gt = np.zeros(202)
         et = np.zeros(202)
-        res = SlipResult(0, 0.0, dp(gt), dp(et))
+        res = SlipResult(0, 0.0, dp(gt), dp(et), None, None)
         v = lib().orc_fault_calc_likelihood(C.byref(self.pb), dp(f64(theta)), key, C.byref(res))
         return v, res.nstage
 
@@ -438,6 +473,15 @@ def fault_mean_cov(X, w):
     lib().orc_fault_calc_mean_particles(dp(X), dp(f64(w)), dp(m), N, nd)
     lib().orc_fault_calc_cov_particles(dp(X), dp(f64(w)), dp(m), dp(cov), N, nd, C.byref(info))
     return m, cov, info.value
+
+
+def fault_propose(chol, cur, z, rng):
+    """work_mcmc_sampling's proposal: cur + chol z, reflected into the range (smc_fault.f90:711-747)."""
+    cur = f64(cur)
+    zz = f64(z).copy()
+    cand = np.zeros_like(cur)
+    lib().orc_fault_propose(cur.shape[0], dp(f64(chol)), dp(cur), dp(zz), dp(f64(rng)), dp(cand))
+    return cand
 
 
 def fault_neglog_q(x, qmean, qcov_inv, log_det):

@@ -95,6 +95,21 @@ static void trmv_lower(int n, const double* A, double* x) {
     }
 }
 
+/* Outer proposal of work_mcmc_sampling (smc_fault.f90:711-747): cand = cur + L z (dtrmv of the lower
+ * Cholesky factor of 0.04 cov), then each dimension whose range is wider than 1e-3 is reflected once into
+ * [lo, hi] (and clipped there); z is overwritten with L z. */
+void orc_fault_propose(int nd, const double* chol, const double* cur, double* z, const double* range, double* cand) {
+    trmv_lower(nd, chol, z);
+    for (int d = 0; d < nd; ++d) cand[d] = cur[d] + z[d];
+    for (int d = 0; d < nd; ++d) {
+        double lo = range[2 * d], hi = range[2 * d + 1];
+        if (hi - lo > 1e-3) {
+            if (cand[d] < lo) cand[d] = fmin(2.0 * lo - cand[d], hi);
+            if (cand[d] > hi) cand[d] = fmax(2.0 * hi - cand[d], lo);
+        }
+    }
+}
+
 /* fault_calc_neglog_q, smc_fault.f90:9-27 */
 double orc_fault_calc_neglog_q(const double* x, const double* qmean, const double* qinv, double log_det, int nd) {
     double pi = 2.0 * asin(1.0), tmp = 0.0, delta[64];
@@ -189,15 +204,7 @@ void orc_fault_smc_exec_gsmc(const orc_problem* pb, const double* range, int N, 
                     z[d] = orc_box_muller_sin(orc_clamp_uniform(orc_uniform_raw(wz[0])),
                                               orc_clamp_uniform(orc_uniform_raw(wz[1])));
                 }
-                trmv_lower(nd, cov, z);
-                for (int d = 0; d < nd; ++d) cand[d] = cur[d] + z[d];
-                for (int d = 0; d < nd; ++d) {
-                    double lo = range[2 * d], hi = range[2 * d + 1];
-                    if (hi - lo > 1e-3) {
-                        if (cand[d] < lo) cand[d] = fmin(2.0 * lo - cand[d], hi);
-                        if (cand[d] > hi) cand[d] = fmax(2.0 * hi - cand[d], lo);
-                    }
-                }
+                orc_fault_propose(nd, cov, cur, z, range, cand);
                 double lcand = orc_fault_calc_likelihood(pb, cand, orc_inner_key(seed, iter, 1, j), NULL);
                 double qcand = g ? orc_fault_calc_neglog_q(cand, g->qmean, g->qcov_inv, g->log_det, nd) : 0.0;
                 double mh = orc_clamp_uniform(draw_u(okey, SITE_OUTER_MH, iter, j, 0, 0));

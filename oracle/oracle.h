@@ -122,15 +122,27 @@ typedef struct {
     orc_fault_desc fault;
     double max_slip;
     int nparticle_slip;
-    int mover;                 /* 0 = reference HMC (ntau 50, ESJD tuning), 1 = Gibbs sweeps */
+    int mover;                 /* 0 = reference HMC (ntau 50, ESJD tuning), 1 = Gibbs sweeps,
+                                  2 = HMC with hmc_ntau steps and a step size adapted toward hmc_accept */
     int gibbs_sweeps;
+    int hmc_ntau;              /* leapfrog steps of mover 2 */
+    double hmc_accept;         /* target mean acceptance probability of mover 2 */
+    int stage_cap;             /* test hook: stop after this many stages (0: run to gamma = 1) */
 } orc_problem;
+
+/* Step-size adaptation of mover 2 (no reference counterpart): the stage's children are moved with
+ * dtau = exp(log_dtau); afterwards log_dtau += ORC_HMC_ADAPT_GAIN * (abar - hmc_accept), abar = the mean over
+ * the P children of min(1, exp(H_cur - H_cand)) summed in child order.  First stage: log_dtau = log(0.1). */
+#define ORC_HMC_ADAPT_GAIN 1.0
+#define ORC_HMC_LOG_DTAU0 (-2.302585092994045684)
 
 typedef struct {
     int nstage;            /* stages executed */
     double neglog;         /* -log Z estimate (or 1e10 sentinel) */
     double* gamma_trace;   /* optional, >= 202 entries */
     double* evid_trace;    /* optional, per-stage neglog evidence increment */
+    double* accept_trace;  /* optional (mover 2), per-stage mean acceptance probability */
+    double* dtau_trace;    /* optional (movers 0 and 2), per-stage step size (mover 0: mean over children) */
 } orc_slip_result;
 
 /* slip_smc_exec for one theta with prepared G etc.  particles (ndim x P) out. */
@@ -150,6 +162,8 @@ void orc_fault_calc_mean_particles(const double* particles, const double* w, dou
 void orc_fault_calc_cov_particles(const double* particles, const double* w, const double* mean,
                                   double* cov, int N, int nd, int* info);
 void orc_qsort(int* arr, int* index, int n);   /* sort.cpp:3-7 */
+void orc_fault_propose(int nd, const double* chol, const double* cur, double* z, const double* range,
+                       double* cand);
 void orc_fault_reorder(int* assigned, double* particles, double* likelihood, int N, int nd,
                        int* sorted_idx);
 typedef struct {
@@ -159,6 +173,18 @@ typedef struct {
 } orc_fault_result;
 void orc_inst_grad(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
                    const double* s, double* grad);
+/* One slip_hmc_tuning call (smc_slip.f90:919-1214) on one instance with given parents: draws the tuning
+ * children's dtau from [log_lo, log_hi], runs their trajectories (ntau 50) in the keyed streams of `stage`,
+ * scores them, resamples the scores into dtau_io (P values in, P out), updates the log-range and returns 0,
+ * 1 when there were no tuning children (dtau_io and the range unchanged), -1 when the resampled count was
+ * not P.  score_out (P, may be NULL) receives each child's score (-1 for children that were not scored). */
+int orc_hmc_tuning_step(const orc_problem* pb, const double* theta, const double* gmat, uint64_t key, int stage,
+                        double gamma, const double* X, const double* L, const double* PR, const double* cov,
+                        const int* assigned, double* dtau_io, double* log_lo, double* log_hi, double* score_out);
+/* Box-Muller normal of the keyed stream (sites of the reference's draws) */
+double orc_draw_normal(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1);
+double orc_draw_clamped_uniform(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1,
+                                uint32_t word);
 void orc_hmc_leapfrog(const orc_problem* pb, const double* theta, const double* gmat, double gamma,
                       const double* cur, const double* p0, const double* cov, double dtau, int ntau,
                       double* cand, double* p, double* ham_cur, double* ham_cand);

@@ -14,7 +14,10 @@
  *   smc_fault.f90:66-292 fault_calc_likelihood
  * The Gibbs-sweep mover (mover=1) has no reference counterpart: it is the
  * truncated-MVN Gibbs mover of the benchmark configuration, specified in
- * orc_gibbs_move below and restated identically on the device.
+ * orc_gibbs_move below and restated identically on the device.  Mover 2 (HMC with
+ * hmc_ntau leapfrog steps of the reference's slip_leapfrog and a step size adapted
+ * toward a target acceptance, ORC_HMC_ADAPT_GAIN in oracle.h) has no reference
+ * counterpart either: it replaces the ESJD tuning pass of slip_hmc_tuning.
  */
 #include "oracle.h"
 #include <math.h>
@@ -40,6 +43,13 @@ static double draw_bm(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, 
     orc_draw_u64x2(key, site, stage, c2, c1, 0, w);
     return orc_box_muller_sin(orc_clamp_uniform(orc_uniform_raw(w[0])),
                               orc_clamp_uniform(orc_uniform_raw(w[1])));
+}
+double orc_draw_normal(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1) {
+    return draw_bm(key, site, stage, c2, c1);
+}
+double orc_draw_clamped_uniform(uint64_t key, uint32_t site, uint32_t stage, uint32_t c2, uint32_t c1,
+                                uint32_t word) {
+    return orc_clamp_uniform(draw_uniform(key, site, stage, c2, c1, (int)word));
 }
 static void draw_quad(uint64_t key, uint32_t site, uint32_t stage, double* hi, double* lo) {
     uint64_t w[2];
@@ -506,6 +516,9 @@ typedef struct {
     double *lv, *ltv;
 } inst_t;
 
+static void inst_build(inst_t* I, const orc_problem* pb, const double* theta, const double* gmat);
+static void inst_free(inst_t* I);
+
 static double inst_likelihood(const inst_t* I, const double* s) {
     return orc_slip_calc_likelihood(s, I->ndim, I->H, I->b, I->dsd, I->pb->nsar_total,
                                     I->pb->ngnss, I->lss2, I->lsg2);
@@ -558,6 +571,7 @@ typedef struct {
     int ntau_upper, ntau_lower;
     double* dtau_ls;
     int* ntau_ls;
+    double* score;  /* optional copy of the children's scores (-1 where not scored) */
 } hmc_tune_t;
 
 static int hmc_tuning(const inst_t* I, uint64_t key, int stage, double gamma, const double* X,
@@ -604,6 +618,7 @@ static int hmc_tuning(const inst_t* I, uint64_t key, int stage, double gamma, co
         }
         free(cur); free(cand); free(p); free(grad); free(lsv);
     }
+    if (T->score) memcpy(T->score, score, sizeof(double) * P);
     double ss = 0.0;
     for (int i = 0; i < P; i += tf)
         for (int j = id_start[i]; j < id_start[i] + assigned[i]; ++j) ss = ss + score[j];
@@ -639,11 +654,35 @@ static int hmc_tuning(const inst_t* I, uint64_t key, int stage, double gamma, co
     return ok ? 0 : -1;
 }
 
+int orc_hmc_tuning_step(const orc_problem* pb, const double* theta, const double* gmat, uint64_t key, int stage,
+                        double gamma, const double* X, const double* L, const double* PR, const double* cov,
+                        const int* assigned, double* dtau_io, double* log_lo, double* log_hi, double* score_out) {
+    inst_t I;
+    inst_build(&I, pb, theta, gmat);
+    int P = I.P;
+    int* id_start = (int*)malloc(sizeof(int) * P);
+    id_start[0] = 0;
+    for (int k = 0; k < P - 1; ++k) id_start[k + 1] = id_start[k] + assigned[k];
+    hmc_tune_t T;
+    T.log_dtau_upper = *log_hi; T.log_dtau_lower = *log_lo; T.ntau_lower = 1; T.ntau_upper = 100;
+    T.dtau_ls = dtau_io;
+    T.ntau_ls = (int*)malloc(sizeof(int) * P);
+    for (int k = 0; k < P; ++k) T.ntau_ls[k] = 50;
+    T.score = score_out;
+    int cnt = 0;
+    for (int i = 0; i < P; i += 5) cnt += assigned[i];
+    int rc = hmc_tuning(&I, key, stage, gamma, X, L, PR, cov, assigned, id_start, &T);
+    *log_hi = T.log_dtau_upper; *log_lo = T.log_dtau_lower;
+    free(T.ntau_ls); free(id_start);
+    inst_free(&I);
+    return cnt == 0 ? 1 : rc;
+}
+
 /* ---------------- HMC sampling over all parents, smc_slip.f90:591-730 ---------------- */
 static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma, const double* X,
                          const double* L, const double* PR, double* Xn, double* Ln, double* PRn,
                          const double* cov, const int* assigned, const int* id_start,
-                         const double* dtau_ls) {
+                         const double* dtau_ls, int ntau, double* acc) {
     int P = I->P, nd = I->ndim;
     #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < P; ++i) {
@@ -659,8 +698,9 @@ static void hmc_sampling(const inst_t* I, uint64_t key, int stage, double gamma,
             memcpy(cand, cur, sizeof(double) * nd);
             for (int d = 0; d < nd; ++d) p[d] = draw_bm(key, SITE_HMC_MOM, stage, j, d) / sqrt(cov[d]);
             double hc, lcand = 0, pcand = 0, postcand = 0, hcand;
-            leapfrog(I, gamma, postc, p, cov, cand, cur, dtau_ls[j], 50, &hc, &lcand, &pcand, &postcand,
+            leapfrog(I, gamma, postc, p, cov, cand, cur, dtau_ls[j], ntau, &hc, &lcand, &pcand, &postcand,
                      &hcand, grad, lsv);
+            if (acc) acc[j] = exp(fmin(0.0, hc - hcand));
             double mh = orc_clamp_uniform(draw_uniform(key, SITE_HMC_MH, stage, j, 0, 0));
             if ((hc - hcand) > log(mh)) {
                 memcpy(cur, cand, sizeof(double) * nd);
@@ -799,10 +839,14 @@ static void smc_run(const inst_t* I, uint64_t key, double* X, double* L, double*
     T.log_dtau_upper = 0.0; T.log_dtau_lower = -1e1; T.ntau_lower = 1; T.ntau_upper = 100;
     T.dtau_ls = (double*)malloc(sizeof(double) * P);
     T.ntau_ls = (int*)malloc(sizeof(int) * P);
+    T.score = NULL;
     for (int k = 0; k < P; ++k) { T.dtau_ls[k] = exp(-5.0); T.ntau_ls[k] = 50; }
+    double* acc = (double*)malloc(sizeof(double) * P);
+    double log_dtau = ORC_HMC_LOG_DTAU0;
     int iter = 1;
     double gamma = 0.0, neglog = 0.0;
     while (1.0 - gamma > 10e-8) {
+        if (I->pb->stage_cap > 0 && iter > I->pb->stage_cap) break;
         double ev;
         gamma = orc_find_next_gamma(gamma, L, P, w, &ev, 2.0, 1e-10, 100000);
         neglog = neglog + ev;
@@ -819,9 +863,26 @@ static void smc_run(const inst_t* I, uint64_t key, double* X, double* L, double*
         for (int k = 0; k < P - 1; ++k) id_start[k + 1] = id_start[k] + assigned[k];
         if (I->pb->mover == 1) {
             gibbs_move(I, key, iter, gamma, I->pb->gibbs_sweeps, X, Xn, Ln, PRn, assigned, id_start);
+        } else if (I->pb->mover == 2) {
+            double dt = exp(log_dtau);
+            for (int k = 0; k < P; ++k) T.dtau_ls[k] = dt;
+            hmc_sampling(I, key, iter, gamma, X, L, PR, Xn, Ln, PRn, cov, assigned, id_start, T.dtau_ls,
+                         I->pb->hmc_ntau, acc);
+            double sa = 0.0;
+            for (int k = 0; k < P; ++k) sa = sa + acc[k];
+            double abar = sa / (double)P;
+            if (res && res->accept_trace && iter <= 201) res->accept_trace[iter - 1] = abar;
+            if (res && res->dtau_trace && iter <= 201) res->dtau_trace[iter - 1] = dt;
+            log_dtau = log_dtau + ORC_HMC_ADAPT_GAIN * (abar - I->pb->hmc_accept);
         } else {
             hmc_tuning(I, key, iter, gamma, X, L, PR, cov, assigned, id_start, &T);
-            hmc_sampling(I, key, iter, gamma, X, L, PR, Xn, Ln, PRn, cov, assigned, id_start, T.dtau_ls);
+            if (res && res->dtau_trace && iter <= 201) {
+                double m = 0.0;
+                for (int k = 0; k < P; ++k) m = m + T.dtau_ls[k];
+                res->dtau_trace[iter - 1] = m / (double)P;
+            }
+            hmc_sampling(I, key, iter, gamma, X, L, PR, Xn, Ln, PRn, cov, assigned, id_start, T.dtau_ls, 50,
+                         NULL);
         }
         memcpy(X, Xn, sizeof(double) * (size_t)P * nd);
         memcpy(L, Ln, sizeof(double) * P);
@@ -830,7 +891,7 @@ static void smc_run(const inst_t* I, uint64_t key, double* X, double* L, double*
     }
     if (res) { res->nstage = iter - 1; res->neglog = neglog; }
     free(w); free(cov); free(assigned); free(id_start); free(Xn); free(Ln); free(PRn);
-    free(T.dtau_ls); free(T.ntau_ls);
+    free(T.dtau_ls); free(T.ntau_ls); free(acc);
 }
 
 void orc_slip_smc_exec(const orc_problem* pb, const double* theta, const double* gmat,
