@@ -30,6 +30,10 @@ extern "C" {
 /* inner movers */
 #define SMCF_MOVER_HMC 0   /* reference HMC: ntau 50 + ESJD dtau tuning (smc_slip.f90:591-1214) */
 #define SMCF_MOVER_GIBBS 1 /* truncated-MVN Gibbs sweeps (benchmark configuration) */
+#define SMCF_MOVER_HMC_ACCEPT 2 /* HMC with hmc_ntau leapfrog steps (the reference's slip_leapfrog) and one step size
+                                   per inversion, adapted after every stage toward the mean acceptance probability
+                                   hmc_accept (log dtau += 2.0 * (mean acceptance - hmc_accept), first dtau 0.1);
+                                   no reference counterpart: it replaces the ESJD tuning pass */
 
 /* arithmetic of the Gibbs mover (no counterpart in the reference, which is FP64 throughout) */
 #define SMCF_PRECISION_FP64 0  /* every operation in FP64 (default; parity with the reference's arithmetic) */
@@ -75,6 +79,8 @@ typedef struct {
   int mover;                  /* SMCF_MOVER_* */
   int gibbs_sweeps;
   int precision;              /* SMCF_PRECISION_* (MIXED needs the Gibbs mover) */
+  int hmc_ntau;               /* SMCF_MOVER_HMC_ACCEPT: leapfrog steps per trajectory (0: 10) */
+  double hmc_accept;          /* SMCF_MOVER_HMC_ACCEPT: target mean acceptance probability (0: 0.65) */
 } smcf_problem_desc;
 
 /* Upload a problem; device buffers are sized for up to max_batch geometries in flight. */
@@ -132,6 +138,36 @@ int smcf_batch_admit_times(const smcf_problem* pb, int B, double* seconds);
  * nparticle_slip particles), batched.  B <= max_batch. */
 int smcf_slip_mean_batch(smcf_problem* pb, int B, const double* theta, const uint64_t* keys, double* mean,
                          double* neglog);
+
+/* Component procedures of the inner SMC for one geometry theta, on the device (the geometry is set up in the
+ * problem's first slot; particles in the reference layout, ndim x n column per particle, n <= nparticle_slip).
+ *   smcf_slip_calc_likelihood   slip_calc_likelihood   (smc_slip.f90:56-104), Gram form
+ *   smcf_slip_calc_prior        slip_calc_prior        (smc_slip.f90:106-137)
+ *   smcf_slip_calc_grad         slip_calc_grad         (smc_slip.f90:139-182), tempered by gamma; grad ndim x n
+ *   smcf_slip_leapfrog          slip_leapfrog          (smc_slip.f90:732-917) for n independent trajectories from
+ *                               (cur, p0) with mass diagonal cov_diag; cand / p_out ndim x n, ham_cur / ham_cand n
+ *                               (1e20 where the trajectory escaped |x| > 1e3; the candidate is then cur)
+ *   smcf_slip_gen_init_particles slip_gen_init_particles (smc_slip.f90:184-287): the sequential prior Gibbs chain
+ *                               of all nparticle_slip particles in the keyed stream of `key`, with likelihood
+ *                               and prior
+ * The gradient and leapfrog need a problem created with an HMC mover (SMCF_EUNSUPPORTED otherwise). */
+int smcf_slip_calc_likelihood(smcf_problem* pb, const double* theta, int n, const double* particles, double* out);
+int smcf_slip_calc_prior(smcf_problem* pb, const double* theta, int n, const double* particles, double* out);
+int smcf_slip_calc_grad(smcf_problem* pb, const double* theta, double gamma, int n, const double* particles,
+                        double* grad);
+int smcf_slip_leapfrog(smcf_problem* pb, const double* theta, double gamma, int n, const double* cur,
+                       const double* p0, const double* cov_diag, double dtau, int ntau, double* cand, double* p_out,
+                       double* ham_cur, double* ham_cand);
+int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t key, double* particles,
+                                 double* likelihood, double* prior);
+/* SMCF_MOVER_HMC_ACCEPT (no reference counterpart): after smcf_slip_smc_exec, the step size the next stage of
+ * that inversion would use and the mean acceptance probability of its last stage */
+int smcf_slip_hmc_accept_state(const smcf_problem* pb, double* next_dtau, double* mean_accept);
+/* slip_calc_mean_particles + slip_calc_cov_particles (smc_slip.f90:485-570): weighted mean and variance diagonal
+ * (real(16) sums as double-double, variances below 1e-3 floored at 1e-3) of P particles (ndim x P) with
+ * normalised weights */
+int smcf_slip_calc_moments(smcf_ctx* ctx, int ndim, int P, const double* particles, const double* weights,
+                           double* mean, double* cov_diag);
 
 /* Inner-SMC components on the device (for parity checks and reuse). */
 /* slip_find_next_gamma / fault_find_next_gamma (smc_slip.f90:314-387, smc_fault.f90:392-485) */

@@ -133,7 +133,7 @@ typedef struct {
 /* Step-size adaptation of mover 2 (no reference counterpart): the stage's children are moved with
  * dtau = exp(log_dtau); afterwards log_dtau += ORC_HMC_ADAPT_GAIN * (abar - hmc_accept), abar = the mean over
  * the P children of min(1, exp(H_cur - H_cand)) summed in child order.  First stage: log_dtau = log(0.1). */
-#define ORC_HMC_ADAPT_GAIN 1.0
+#define ORC_HMC_ADAPT_GAIN 2.0
 #define ORC_HMC_LOG_DTAU0 (-2.302585092994045684)
 
 typedef struct {

@@ -25,8 +25,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsmcf.so")
 _LIB = None
 
-MOVER_HMC = 0
-MOVER_GIBBS = 1
+MOVER_HMC = 0         # reference HMC: ntau 50, ESJD step-size tuning
+MOVER_GIBBS = 1       # truncated-MVN Gibbs sweeps
+MOVER_HMC_ACCEPT = 2  # HMC with hmc_ntau steps, step size adapted toward the hmc_accept mean acceptance
 PRECISION_FP64 = 0   # every operation in FP64
 PRECISION_MIXED = 1  # Gibbs Q.S products in FP32 on the matrix cores, single-precision draws; means/weights FP64
 PRECISION_MIXED_BF16 = 2  # as PRECISION_MIXED with the Q.S products from split-BF16 operands (hi.hi + hi.lo + lo.hi)
@@ -47,7 +48,8 @@ class ProblemDesc(C.Structure):
                 ("nobs", C.c_int), ("nsar_total", C.c_int), ("ngnss", C.c_int), ("npath", C.c_int),
                 ("nsar_index", _I), ("obs_points", _D), ("obs_unitvec", _D), ("obs_sigma", _D),
                 ("dvec", _D), ("sar_sinv", _D), ("max_slip", C.c_double), ("nparticle_slip", C.c_int),
-                ("mover", C.c_int), ("gibbs_sweeps", C.c_int), ("precision", C.c_int)]
+                ("mover", C.c_int), ("gibbs_sweeps", C.c_int), ("precision", C.c_int), ("hmc_ntau", C.c_int),
+                ("hmc_accept", C.c_double)]
 
 
 EXPORTS = [
@@ -59,6 +61,8 @@ EXPORTS = [
     "smcf_fault_smc_exec_gsmc", "smcf_inner_key", "qsort_", "smcf_problem_bytes", "smcf_problem_device_bytes",
     "smcf_server_create", "smcf_server_submit", "smcf_server_wait", "smcf_server_status", "smcf_server_destroy",
     "smcf_slip_mean_batch", "smcf_batch_times", "smcf_batch_admit_times", "smcf_find_next_gamma_noalloc",
+    "smcf_slip_calc_likelihood", "smcf_slip_calc_prior", "smcf_slip_calc_grad", "smcf_slip_leapfrog",
+    "smcf_slip_gen_init_particles", "smcf_slip_calc_moments", "smcf_slip_hmc_accept_state",
 ]
 
 
@@ -115,6 +119,14 @@ def _declare(L):
     L.smcf_slip_mean_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _D]
     L.smcf_batch_times.argtypes = [C.c_void_p, C.c_int, _D]
     L.smcf_batch_admit_times.argtypes = [C.c_void_p, C.c_int, _D]
+    L.smcf_slip_calc_likelihood.argtypes = [C.c_void_p, _D, C.c_int, _D, _D]
+    L.smcf_slip_calc_prior.argtypes = [C.c_void_p, _D, C.c_int, _D, _D]
+    L.smcf_slip_calc_grad.argtypes = [C.c_void_p, _D, C.c_double, C.c_int, _D, _D]
+    L.smcf_slip_leapfrog.argtypes = [C.c_void_p, _D, C.c_double, C.c_int, _D, _D, _D, C.c_double, C.c_int, _D, _D,
+                                     _D, _D]
+    L.smcf_slip_gen_init_particles.argtypes = [C.c_void_p, _D, C.c_uint64, _D, _D, _D]
+    L.smcf_slip_calc_moments.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D, _D, _D]
+    L.smcf_slip_hmc_accept_state.argtypes = [C.c_void_p, _D, _D]
 
 
 def _check(rc):
@@ -208,6 +220,15 @@ class Context:
                                                   C.byref(g), _dp(w), C.byref(ev)))
         return g.value, w, ev.value
 
+    def slip_calc_moments(self, particles, weights):
+        """Weighted mean and variance diagonal (floored at 1e-3) of particles (ndim x P)."""
+        X = _f64(particles)
+        nd, P = X.shape
+        m = np.zeros(nd)
+        c = np.zeros(nd)
+        _check(lib().smcf_slip_calc_moments(self.h, nd, P, _dp(X), _dp(_f64(weights)), _dp(m), _dp(c)))
+        return m, c
+
     def normalize_resample(self, weights, u_hi, u_lo):
         w = np.ascontiguousarray(weights, dtype=np.float64)
         wn = np.zeros_like(w)
@@ -243,7 +264,7 @@ class Context:
 
 
 def problem_desc(nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip, mover=MOVER_GIBBS,
-                 gibbs_sweeps=5, fix_xbend=False, xbend=(), precision=PRECISION_FP64):
+                 gibbs_sweeps=5, fix_xbend=False, xbend=(), precision=PRECISION_FP64, hmc_ntau=10, hmc_accept=0.65):
     """smcf_problem_desc of an observation set and fault mesh; returns (desc, arrays the desc points into)."""
     keep = dict(
         nxi=_i32(nxi_ls), neta=_i32(neta_ls), xbend=_f64(list(xbend) + [0.0]),
@@ -257,7 +278,8 @@ def problem_desc(nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_sli
                        _dp(k["xbend"]), nobs, int(obs.get("nsar_total", 0)), int(obs["ngnss"]),
                        len(k["nsar_index"]) - 1, _ip(k["nsar_index"]), _dp(k["obs_points"]),
                        _dp(k["obs_unitvec"]), _dp(k["obs_sigma"]), _dp(k["dvec"]), _dp(k["sar"]),
-                       float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps), int(precision))
+                       float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps), int(precision),
+                       int(hmc_ntau), float(hmc_accept))
     return desc, keep
 
 
@@ -324,10 +346,10 @@ class Problem:
 
     def __init__(self, ctx, nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip,
                  mover=MOVER_GIBBS, gibbs_sweeps=5, max_batch=1, fix_xbend=False, xbend=(),
-                 precision=PRECISION_FP64):
+                 precision=PRECISION_FP64, hmc_ntau=10, hmc_accept=0.65):
         self.ctx = ctx
         self.desc, self._keep = problem_desc(nxi_ls, neta_ls, xmin, xmax, zmin, obs, max_slip, nparticle_slip,
-                                             mover, gibbs_sweeps, fix_xbend, xbend, precision)
+                                             mover, gibbs_sweeps, fix_xbend, xbend, precision, hmc_ntau, hmc_accept)
         h = C.c_void_p()
         _check(lib().smcf_problem_create(ctx.h, C.byref(self.desc), int(max_batch), C.byref(h)))
         self.h = h
@@ -397,6 +419,62 @@ class Problem:
                                         C.byref(ns), C.byref(nl), _dp(gt)))
         return dict(particles=X, likelihood=L, prior=PR, mean=mean, nstage=ns.value, neglog=nl.value,
                     gamma=gt[:ns.value].copy())
+
+    def _cols(self, particles):
+        X = _f64(particles)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        if X.shape[0] != self.ndim:
+            raise ValueError("particles must have %d rows" % self.ndim)
+        return X
+
+    def slip_calc_likelihood(self, theta, particles):
+        """slip_calc_likelihood for each column of particles (ndim x n) on geometry theta."""
+        X = self._cols(particles)
+        out = np.zeros(X.shape[1])
+        _check(lib().smcf_slip_calc_likelihood(self.h, _dp(self._theta(theta)), X.shape[1], _dp(X), _dp(out)))
+        return out
+
+    def slip_calc_prior(self, theta, particles):
+        X = self._cols(particles)
+        out = np.zeros(X.shape[1])
+        _check(lib().smcf_slip_calc_prior(self.h, _dp(self._theta(theta)), X.shape[1], _dp(X), _dp(out)))
+        return out
+
+    def slip_calc_grad(self, theta, gamma, particles):
+        X = self._cols(particles)
+        g = np.zeros(X.shape, order="F")
+        _check(lib().smcf_slip_calc_grad(self.h, _dp(self._theta(theta)), float(gamma), X.shape[1], _dp(X), _dp(g)))
+        return g
+
+    def slip_leapfrog(self, theta, gamma, cur, p0, cov_diag, dtau, ntau):
+        """n independent slip_leapfrog trajectories; returns (cand, p, ham_cur, ham_cand)."""
+        X = self._cols(cur)
+        Pm = self._cols(p0)
+        n = X.shape[1]
+        cand = np.zeros(X.shape, order="F")
+        pout = np.zeros(X.shape, order="F")
+        hc = np.zeros(n)
+        hn = np.zeros(n)
+        _check(lib().smcf_slip_leapfrog(self.h, _dp(self._theta(theta)), float(gamma), n, _dp(X), _dp(Pm),
+                                        _dp(_f64(cov_diag)), float(dtau), int(ntau), _dp(cand), _dp(pout), _dp(hc),
+                                        _dp(hn)))
+        return cand, pout, hc, hn
+
+    def slip_gen_init_particles(self, theta, key):
+        """The prior Gibbs chain of slip_gen_init_particles: (particles ndim x P, likelihood, prior)."""
+        X = np.zeros((self.ndim, self.P), order="F")
+        L = np.zeros(self.P)
+        R = np.zeros(self.P)
+        _check(lib().smcf_slip_gen_init_particles(self.h, _dp(self._theta(theta)), C.c_uint64(key), _dp(X), _dp(L),
+                                                  _dp(R)))
+        return X, L, R
+
+    def hmc_accept_state(self):
+        """MOVER_HMC_ACCEPT: (next stage's step size, mean acceptance of the last stage) after slip_smc_exec."""
+        dt, ab = C.c_double(), C.c_double()
+        _check(lib().smcf_slip_hmc_accept_state(self.h, C.byref(dt), C.byref(ab)))
+        return dt.value, ab.value
 
     def set_output_dir(self, output_dir):
         """Per-stage dumps <output_dir><iter>.csv of fault_smc_exec (None stops them)."""

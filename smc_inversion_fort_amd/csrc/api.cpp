@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -189,7 +190,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
   MeshDev& m = P.m;
   BatchDev& bt = P.bt;
   hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel, hold);
-  if (P.mover == SMCF_MOVER_HMC) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, n, s, sel));
+  if (P.mover != SMCF_MOVER_GIBBS) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, n, s, sel, P.mover));
   {
     ProfScope ps(ctx, "geometry_greens", s);
     CHECK_LAUNCH(launch_geometry(m, bt, n, s, sel));
@@ -229,7 +230,8 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
     set_error("no geometries");
     return SMCF_EINVAL;
   }
-  bool hmc = P.mover == SMCF_MOVER_HMC;
+  bool hmc = P.mover != SMCF_MOVER_GIBBS;
+  const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept};
   const int nf = m.ndim_fault;
   const auto t_call = std::chrono::steady_clock::now();
   P.t_done.assign(N, 0.0);
@@ -270,7 +272,8 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
       if ((want_moments || hmc) && launch_moments(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
       if (hmc) {
         ProfScope ps(ctx, "hmc");
-        if (launch_hmc_stage(m, bt, P.hm, W, P.max_slip, s, amax.data()) != hipSuccess) return fail(SMCF_EHIP);
+        if (launch_hmc_stage(m, bt, P.hm, W, P.max_slip, hcfg, s, amax.data()) != hipSuccess)
+          return fail(SMCF_EHIP);
       } else {
         if (launch_gather(m, bt, W, s) != hipSuccess || launch_form_q(m, bt, W, s) != hipSuccess)
           return fail(SMCF_EHIP);
@@ -448,6 +451,16 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   P.mover = d->mover;
   P.sweeps = d->gibbs_sweeps > 0 ? d->gibbs_sweeps : 5;
   P.precision = d->precision;
+  P.hmc_ntau = P.mover == SMCF_MOVER_HMC ? 50 : (d->hmc_ntau > 0 ? d->hmc_ntau : 10);
+  P.hmc_accept = d->hmc_accept > 0.0 ? d->hmc_accept : 0.65;
+  if (P.mover != SMCF_MOVER_HMC && P.mover != SMCF_MOVER_GIBBS && P.mover != SMCF_MOVER_HMC_ACCEPT) {
+    set_error("unknown mover " + std::to_string(P.mover));
+    return SMCF_EINVAL;
+  }
+  if (P.mover == SMCF_MOVER_HMC_ACCEPT && (P.hmc_ntau > 1000 || P.hmc_accept >= 1.0)) {
+    set_error("hmc_ntau must be 1 .. 1000 and hmc_accept in (0, 1)");
+    return SMCF_EINVAL;
+  }
   if (P.precision != SMCF_PRECISION_FP64 && P.precision != SMCF_PRECISION_MIXED &&
       P.precision != SMCF_PRECISION_MIXED_BF16) {
     set_error("unknown precision " + std::to_string(P.precision));
@@ -657,7 +670,7 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   dalloc(P, &bt.S, nd * Pn * C);
   dalloc(P, &bt.S2, nd * Pn * C);
   bt.Y = nullptr;
-  if (P.mover == SMCF_MOVER_HMC) dalloc(P, &bt.Y, nd * Pn * C);  // H X of the leapfrog (HMC only)
+  if (P.mover != SMCF_MOVER_GIBBS) dalloc(P, &bt.Y, nd * Pn * C);  // H X of the leapfrog (HMC movers only)
   dalloc(P, &bt.like, Pn * C);
   dalloc(P, &bt.prior, Pn * C);
   dalloc(P, &bt.like2, Pn * C);
@@ -676,7 +689,7 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   dalloc(P, &bt.sel, 2 * C);
   dalloc(P, &bt.par, C);
   dalloc(P, &bt.gtrace, 202 * C);
-  if (P.mover == SMCF_MOVER_HMC) {
+  if (P.mover != SMCF_MOVER_GIBBS) {
     HmcDev& h = P.hm;
     size_t nn2 = 2 * (size_t)nnode;
     dalloc(P, &h.Xc, nd * Pn * C);
@@ -704,9 +717,6 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
     dalloc(P, &h.dead, Pn * C);
     dalloc(P, &h.idstart, Pn * C);
     dalloc(P, &h.flag, C);
-  } else if (P.mover != SMCF_MOVER_GIBBS) {
-    set_error("unknown mover");
-    return SMCF_EINVAL;
   }
   if (P.alloc_rc) return P.alloc_rc;
   if (sizing) {
@@ -925,6 +935,201 @@ int smcf_slip_smc_exec(smcf_problem* pb, const double* theta, uint64_t key, doub
   if (gamma_trace) SMCF_HIP(hipMemcpy(gamma_trace, bt.gtrace, sizeof(double) * 202, hipMemcpyDeviceToHost));
   if (nstage) *nstage = std::min(it - 1, 201);
   (void)s;
+  return SMCF_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Component procedures of one geometry (slot 0 of the problem).  Particles are passed in the reference layout
+// (ndim x n, one column per particle, n <= nparticle_slip) and transposed into the dimension-major device layout.
+// ---------------------------------------------------------------------------------------------------------------
+static int slot0_geometry(Problem& P, const double* theta, hipStream_t s) {
+  MeshDev& m = P.m;
+  BatchDev& bt = P.bt;
+  SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * m.ndim_fault, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, bt, 1, (const int*)nullptr, 0);
+  CHECK_LAUNCH(launch_geometry(m, bt, 1, s));
+  CHECK_LAUNCH(launch_greens(m, bt, 1, s));
+  CHECK_LAUNCH(launch_laplacian(m, bt, 1, s));
+  CHECK_LAUNCH(launch_gram(P, m, bt, 1, s));
+  return SMCF_OK;
+}
+
+static int check_component(smcf_problem* pb, const double* theta, int n) {
+  if (!pb || !theta || n < 1) {
+    set_error("component call: missing problem or theta, or n < 1");
+    return SMCF_EINVAL;
+  }
+  if (n > pb->p.P) {
+    set_error("component call: n = " + std::to_string(n) + " exceeds nparticle_slip = " + std::to_string(pb->p.P));
+    return SMCF_EINVAL;
+  }
+  return SMCF_OK;
+}
+
+// particles (ndim x n, column per particle) -> dimension-major rows of length P at dst
+static int upload_columns(Problem& P, const double* cols, int n, double* dst, hipStream_t s) {
+  size_t nd = P.m.ndim, Pn = P.P;
+  std::vector<double> t(nd * Pn, 0.0);
+  for (int p = 0; p < n; ++p)
+    for (size_t i = 0; i < nd; ++i) t[i * Pn + p] = cols[(size_t)p * nd + i];
+  SMCF_HIP(hipMemcpyAsync(dst, t.data(), sizeof(double) * nd * Pn, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipStreamSynchronize(s));  // t is a stack buffer
+  return SMCF_OK;
+}
+
+static int download_columns(Problem& P, const double* src, int n, double* cols, hipStream_t s) {
+  size_t nd = P.m.ndim, Pn = P.P;
+  std::vector<double> t(nd * Pn);
+  SMCF_HIP(hipMemcpyAsync(t.data(), src, sizeof(double) * nd * Pn, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < n; ++p)
+    for (size_t i = 0; i < nd; ++i) cols[(size_t)p * nd + i] = t[i * Pn + p];
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_likelihood(smcf_problem* pb, const double* theta, int n, const double* particles, double* out) {
+  int rc = check_component(pb, theta, n);
+  if (rc) return rc;
+  if (!particles || !out) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  if ((rc = slot0_geometry(P, theta, s))) return rc;
+  if ((rc = upload_columns(P, particles, n, P.bt.S, s))) return rc;  // k_state_init left par = 0: S is current
+  CHECK_LAUNCH(launch_like(P.m, P.bt, 1, 0, s));
+  SMCF_HIP(hipMemcpyAsync(out, P.bt.like, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_prior(smcf_problem* pb, const double* theta, int n, const double* particles, double* out) {
+  int rc = check_component(pb, theta, n);
+  if (rc) return rc;
+  if (!particles || !out) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  if ((rc = slot0_geometry(P, theta, s))) return rc;
+  if ((rc = upload_columns(P, particles, n, P.bt.S, s))) return rc;
+  CHECK_LAUNCH(launch_prior(P.m, P.bt, 1, s));
+  SMCF_HIP(hipMemcpyAsync(out, P.bt.prior, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+static int hmc_component_state(Problem& P, double gamma, int n, const double* cov, hipStream_t s) {
+  if (P.mover == SMCF_MOVER_GIBBS) {
+    set_error("component call needs a problem with an HMC mover (its trajectory buffers)");
+    return SMCF_EUNSUPPORTED;
+  }
+  int one = 1;
+  SMCF_HIP(hipMemcpyAsync(P.bt.gamma, &gamma, sizeof(double), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(P.bt.active, &one, sizeof(int), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(P.hm.ncol, &n, sizeof(int), hipMemcpyHostToDevice, s));
+  if (cov) SMCF_HIP(hipMemcpyAsync(P.bt.cov, cov, sizeof(double) * P.m.ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipStreamSynchronize(s));  // the host values above live on this stack frame
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_grad(smcf_problem* pb, const double* theta, double gamma, int n, const double* particles,
+                        double* grad) {
+  int rc = check_component(pb, theta, n);
+  if (rc) return rc;
+  if (!particles || !grad) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  if ((rc = slot0_geometry(P, theta, s))) return rc;
+  if ((rc = hmc_component_state(P, gamma, n, nullptr, s))) return rc;
+  if ((rc = upload_columns(P, particles, n, P.hm.Xt, s))) return rc;
+  TmpBufs tb;
+  double* dg = nullptr;
+  SMCF_HIP(tb.alloc(&dg, (size_t)P.m.ndim * n));
+  CHECK_LAUNCH(launch_grad_cols(P.m, P.bt, P.hm, n, gamma, dg, s));
+  SMCF_HIP(hipMemcpyAsync(grad, dg, sizeof(double) * P.m.ndim * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_leapfrog(smcf_problem* pb, const double* theta, double gamma, int n, const double* cur,
+                       const double* p0, const double* cov_diag, double dtau, int ntau, double* cand, double* p_out,
+                       double* ham_cur, double* ham_cand) {
+  int rc = check_component(pb, theta, n);
+  if (rc) return rc;
+  if (!cur || !p0 || !cov_diag || ntau < 0) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  if ((rc = slot0_geometry(P, theta, s))) return rc;
+  if ((rc = hmc_component_state(P, gamma, n, cov_diag, s))) return rc;
+  if ((rc = upload_columns(P, cur, n, P.hm.Xt, s))) return rc;
+  if ((rc = upload_columns(P, p0, n, P.hm.Pm, s))) return rc;
+  TmpBufs tb;
+  double *dhc = nullptr, *dhn = nullptr;
+  SMCF_HIP(tb.alloc(&dhc, (size_t)n));
+  SMCF_HIP(tb.alloc(&dhn, (size_t)n));
+  CHECK_LAUNCH(launch_leapfrog_cols(P.m, P.bt, P.hm, n, gamma, dtau, ntau, P.max_slip, dhc, dhn, s));
+  if (cand && (rc = download_columns(P, P.hm.Xt, n, cand, s))) return rc;
+  if (p_out && (rc = download_columns(P, P.hm.Pm, n, p_out, s))) return rc;
+  if (ham_cur) SMCF_HIP(hipMemcpyAsync(ham_cur, dhc, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (ham_cand) SMCF_HIP(hipMemcpyAsync(ham_cand, dhn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t key, double* particles,
+                                 double* likelihood, double* prior) {
+  if (!pb || !theta) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * P.m.ndim_fault, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(P.bt.key, &key, sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  int rc = setup_slots(P, 1, nullptr, s, 0);
+  if (rc) return rc;
+  if (particles && (rc = download_columns(P, P.bt.S, P.P, particles, s))) return rc;
+  if (likelihood) SMCF_HIP(hipMemcpyAsync(likelihood, P.bt.like, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));
+  if (prior) SMCF_HIP(hipMemcpyAsync(prior, P.bt.prior, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_hmc_accept_state(const smcf_problem* pb, double* next_dtau, double* mean_accept) {
+  if (!pb) return SMCF_EINVAL;
+  const Problem& P = pb->p;
+  if (P.mover != SMCF_MOVER_HMC_ACCEPT) {
+    set_error("smcf_slip_hmc_accept_state: the problem's mover is not SMCF_MOVER_HMC_ACCEPT");
+    return SMCF_EUNSUPPORTED;
+  }
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  double ld = 0.0, ab = 0.0;
+  SMCF_HIP(hipMemcpy(&ld, P.hm.ldlo, sizeof(double), hipMemcpyDeviceToHost));
+  SMCF_HIP(hipMemcpy(&ab, P.hm.ldhi, sizeof(double), hipMemcpyDeviceToHost));
+  if (next_dtau) *next_dtau = std::exp(ld);
+  if (mean_accept) *mean_accept = ab;
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_moments(smcf_ctx* ctx, int ndim, int P, const double* particles, const double* weights,
+                           double* mean, double* cov_diag) {
+  if (!ctx || ndim < 1 || P < 1 || !particles || !weights) return SMCF_EINVAL;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  std::vector<double> t((size_t)ndim * P);
+  for (int p = 0; p < P; ++p)
+    for (int i = 0; i < ndim; ++i) t[(size_t)i * P + p] = particles[(size_t)p * ndim + i];
+  TmpBufs tb;
+  double *dS, *dw, *dm, *dc;
+  SMCF_HIP(tb.alloc(&dS, t.size()));
+  SMCF_HIP(tb.alloc(&dw, (size_t)P));
+  SMCF_HIP(tb.alloc(&dm, (size_t)ndim));
+  SMCF_HIP(tb.alloc(&dc, (size_t)ndim));
+  SMCF_HIP(hipMemcpyAsync(dS, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dw, weights, sizeof(double) * P, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_moments_raw(ndim, P, dS, dw, dm, dc, s));
+  if (mean) SMCF_HIP(hipMemcpyAsync(mean, dm, sizeof(double) * ndim, hipMemcpyDeviceToHost, s));
+  if (cov_diag) SMCF_HIP(hipMemcpyAsync(cov_diag, dc, sizeof(double) * ndim, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
   return SMCF_OK;
 }
 

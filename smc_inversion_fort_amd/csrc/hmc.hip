@@ -5,20 +5,27 @@
 // parent with more than k children (parents in ascending order, a compact column list per
 // instance).  For each wave:
 //   k_hmc_wave_init   candidate <- chain state, momenta p = z / sqrt(cov_diag), H(cur)
-//   (ntau + 1) x [k_hs_cols: Y = H X over the wave's columns ; k_hmc_step: gradient
+//   (ntau + 1) x [k_hs_mfma: Y = H X over the wave's columns ; k_hmc_step: gradient
 //                 gamma (H x - b) + L^T (L x / alpha^2), kick, drift with escape and
 //                 reflection]                                         (slip_leapfrog, 732-917)
-//   k_hs_cols + k_hmc_finish  likelihood, prior, Hamiltonian, ESJD score (tuning), MH test
+//   k_hs_mfma + k_hmc_finish  likelihood, prior, Hamiltonian, ESJD score (tuning), MH test
 // Tuning (parents 1, 6, 11, ... in 1-based numbering) runs first with log-uniform dtau per
 // child, then k_tune_resample turns the scores into the dtau of every child and the new
 // log-dtau range (slip_hmc_tuning, 919-1214); sampling then runs every parent (591-730).
+//
+// Mover SMCF_MOVER_HMC_ACCEPT (no reference counterpart) runs the same trajectories with ntau
+// steps and one step size per instance: no tuning pass; after sampling the step size is adapted
+// toward the target acceptance from the stage's mean acceptance probability (k_hmc_adapt).
+// Y = H X runs on the FP64 matrix cores (k_hs_mfma).
 #include "smcf_device.h"
 #include "smcf_internal.h"
 
 namespace smcf {
 
 constexpr int kTuneFactor = 5;
-constexpr int kNtau = 50;
+constexpr int kNtauRef = 50;  // smc_slip.f90:669 (tuned ntau values are drawn but not used)
+constexpr double kAdaptGain = 2.0;                       // mover 2: log dtau += gain * (abar - target)
+constexpr double kLogDtau0 = -2.302585092994045684;      // mover 2: first stage's log dtau = log(0.1)
 
 // per-instance wave lists: waves over all parents (mode 0) and over tuning parents (mode 1)
 __global__ void __launch_bounds__(256) k_hmc_prep(int P, BatchDev bt, HmcDev h) {
@@ -145,57 +152,74 @@ __global__ void k_idstart(int P, BatchDev bt, HmcDev h, int B) {
   }
 }
 
-// Y(:, m) = H X(:, m) over the first ncol[b] columns
-constexpr int HM = 64, HN = 64, HK = 16;
-__global__ void __launch_bounds__(256) k_hs_cols(int nd, int P, const double* __restrict__ Hall,
+// Y(:, m) = H X(:, m) over the first ncol[b] columns, on the FP64 matrix cores.  One wavefront computes a
+// 64-row x 64-column block (4 x 4 tiles of v_mfma_f64_16x16x4f64, operands streamed by mfma_panel_t:
+// A(r, k) = H(i0 + r, k) column-major, B(k, c) = X(k, p0 + c) dimension-major); a workgroup's 4 wavefronts take
+// 4 consecutive row blocks of the same 64 columns.  Lane l holds D(4t + l/16, l%16) in register t.
+constexpr int kHsMT = 4;
+__global__ void __launch_bounds__(256) k_hs_mfma(int nd, int P, const double* __restrict__ Hall,
                                                  const double* __restrict__ Xall, const int* __restrict__ ncol,
                                                  const int* __restrict__ active, double* __restrict__ Yall) {
-  __shared__ double As[HK][HM + 1];
-  __shared__ double Bs[HK][HN + 1];
+  constexpr int MT = kHsMT, KB = 16 * MT;
   int b = blockIdx.z;
   if (!active[b]) return;
   int n = ncol[b];
-  int i0 = blockIdx.y * HM, p0 = blockIdx.x * HN;
-  if (p0 >= n) return;
+  int p0 = blockIdx.x * 64;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int i0 = (blockIdx.y * 4 + wave) * KB;
+  if (p0 >= n || i0 >= nd) return;
   const double* H = Hall + (size_t)b * nd * nd;
   const double* X = Xall + (size_t)b * nd * P;
   double* Y = Yall + (size_t)b * nd * P;
-  int t = threadIdx.x, ty = t / 16, tx = t % 16;
-  double acc[4][4];
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int q = 0; q < 4; ++q) pcol[q] = min(p0 + 16 * q + fi, n - 1);
+  uint32_t bo[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int k0 = 0; k0 < nd; k0 += HK) {
+  for (int q = 0; q < 4; ++q) bo[q] = (uint32_t)(fk * P + pcol[q]);
+  uint32_t ao[MT];
 #pragma unroll
-    for (int rep = 0; rep < 4; ++rep) {
-      int ii = t % 64, kk = t / 64 + rep * 4;
-      int i = i0 + ii, k = k0 + kk, p = p0 + ii;
-      As[kk][ii] = (i < nd && k < nd) ? H[(size_t)k * nd + i] : 0.0;
-      Bs[kk][ii] = (p < n && k < nd) ? X[(size_t)k * P + p] : 0.0;
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * nd + min(i0 + 16 * m + fi, nd - 1));
+  dbl4 acc[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[m][q] = dbl4{0.0, 0.0, 0.0, 0.0};
+  int nd4 = nd & ~3;
+  mfma_panel_t<MT, kPanelBuffers>(H, ao, (uint32_t)(4 * nd), X, bo, (uint32_t)(4 * P), nd4 / 4, acc);
+  if (nd4 < nd) {  // last contraction rows (nd = 2 mod 4): rows past nd contribute zero
+    bool in = nd4 + fk < nd;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double bn = in ? X[(size_t)(nd4 + fk) * P + pcol[q]] : 0.0;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        int r = i0 + 16 * m + fi;
+        double av = (in && r < nd) ? H[(size_t)(nd4 + fk) * nd + r] : 0.0;
+        acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bn, acc[m][q], 0, 0, 0);
+      }
     }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < HK; ++kk) {
-      double a4[4], b4[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) a4[a] = As[kk][ty * 4 + a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b4[c] = Bs[kk][tx * 4 + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] += a4[a] * b4[c];
-    }
-    __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      int i = i0 + ty * 4 + a, p = p0 + tx * 4 + c;
-      if (i < nd && p < n) Y[(size_t)i * P + p] = acc[a][c];
+    for (int t = 0; t < 4; ++t) {
+      int row = i0 + 16 * m + 4 * t + fk;
+      if (row >= nd) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int col = p0 + 16 * q + fi;
+        if (col < n) Y[(size_t)row * P + col] = acc[m][q][t];
+      }
     }
+}
+
+static void launch_hs(int nd, int P, int B, const double* H, const double* X, const int* ncol, const int* active,
+                      double* Y, hipStream_t s) {
+  int nblk = (nd + 16 * kHsMT - 1) / (16 * kHsMT);
+  dim3 g((P + 63) / 64, (nblk + 3) / 4, B);
+  hipLaunchKernelGGL(k_hs_mfma, g, dim3(256), 0, s, nd, P, H, X, ncol, active, Y);
 }
 
 // one leapfrog gradient + kick + drift (slip_leapfrog, smc_slip.f90:756-904)
@@ -257,7 +281,7 @@ __global__ void k_hmc_step(MeshDev m, int P, int s, int ntau, double max_slip, B
 }
 
 // final Hamiltonian, ESJD score (tuning), Metropolis test, chain / child update
-__global__ void k_hmc_finish(MeshDev m, int P, BatchDev bt, HmcDev h, int tune) {
+__global__ void k_hmc_finish(MeshDev m, int P, BatchDev bt, HmcDev h, int tune, int record_acc) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!bt.active[b] || col >= h.ncol[b]) return;
@@ -300,6 +324,7 @@ __global__ void k_hmc_finish(MeshDev m, int P, BatchDev bt, HmcDev h, int tune) 
     }
   }
   uint32_t stage = (uint32_t)(bt.iter[b] - 1);
+  if (record_acc) h.score[(size_t)b * P + j] = exp(fmin(0.0, hc - hn));  // acceptance probability (mover 2)
   if (tune) {
     double score = 0.0;
     for (int d = 0; d < nd; ++d) {
@@ -376,40 +401,62 @@ __global__ void k_tune_resample(int P, BatchDev bt, HmcDev h, int B) {
   h.ldlo[b] = dd_add_d(lm, -l5).hi;
 }
 
-__global__ void k_hmc_tune_init(HmcDev h, int B, int P, const int* __restrict__ sel) {
+// reference mover: log-dtau range [-10, 0] and dtau = exp(-5) before the first tuning (uninitialised in the
+// reference, smc_slip.f90:993-995); mover 2: ldlo = log dtau of the first stage, ldhi = last mean acceptance
+__global__ void k_hmc_tune_init(HmcDev h, int B, int P, const int* __restrict__ sel, int accept_mover) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   if (sel) b = sel[b];
   h.ldhi[b] = 0.0;
-  h.ldlo[b] = -1e1;
+  h.ldlo[b] = accept_mover ? kLogDtau0 : -1e1;
   h.flag[b] = 0;
   for (int q = 0; q < P; ++q) h.dtau[(size_t)b * P + q] = exp(-5.0);
 }
 
-hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel) {
-  hipLaunchKernelGGL(k_hmc_tune_init, dim3((B + 63) / 64), dim3(64), 0, s, h, B, bt.P, sel);
+hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel, int mover) {
+  hipLaunchKernelGGL(k_hmc_tune_init, dim3((B + 63) / 64), dim3(64), 0, s, h, B, bt.P, sel,
+                     mover == SMCF_MOVER_HMC_ACCEPT ? 1 : 0);
   return hipGetLastError();
 }
 
+// mover 2: every child of the stage moves with dtau = exp(log dtau) of its instance
+__global__ void k_hmc_dtau_fill(int P, BatchDev bt, HmcDev h) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!bt.active[b] || j >= P) return;
+  h.dtau[(size_t)b * P + j] = exp(h.ldlo[b]);
+}
+
+// mover 2: mean acceptance probability of the stage's P children (summed in child order) and the step update
+__global__ void k_hmc_adapt(int P, BatchDev bt, HmcDev h, int B, double target) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || !bt.active[b]) return;
+  const double* a = h.score + (size_t)b * P;
+  double sa = 0.0;
+  for (int j = 0; j < P; ++j) sa = sa + a[j];
+  double abar = sa / (double)P;
+  h.ldhi[b] = abar;
+  h.ldlo[b] = h.ldlo[b] + kAdaptGain * (abar - target);
+}
+
 static hipError_t run_waves(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, int tune, int amax, double max_slip,
-                            hipStream_t s) {
+                            int ntau, int record_acc, hipStream_t s) {
   int P = bt.P, nd = m.ndim;
   dim3 gc((P + 127) / 128, B);
-  dim3 gg((P + HN - 1) / HN, (nd + HM - 1) / HM, B);
   for (int k = 0; k < amax; ++k) {
     hipLaunchKernelGGL(k_hmc_wave_init, gc, dim3(128), 0, s, nd, P, k, bt, h, tune);
-    for (int st = 0; st <= kNtau; ++st) {
-      hipLaunchKernelGGL(k_hs_cols, gg, dim3(256), 0, s, nd, P, bt.H, h.Xt, h.ncol, bt.active, bt.Y);
-      hipLaunchKernelGGL(k_hmc_step, gc, dim3(128), 0, s, m, P, st, kNtau, max_slip, bt, h, tune);
+    for (int st = 0; st <= ntau; ++st) {
+      launch_hs(nd, P, B, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+      hipLaunchKernelGGL(k_hmc_step, gc, dim3(128), 0, s, m, P, st, ntau, max_slip, bt, h, tune);
     }
-    hipLaunchKernelGGL(k_hs_cols, gg, dim3(256), 0, s, nd, P, bt.H, h.Xt, h.ncol, bt.active, bt.Y);
-    hipLaunchKernelGGL(k_hmc_finish, gc, dim3(128), 0, s, m, P, bt, h, tune);
+    launch_hs(nd, P, B, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+    hipLaunchKernelGGL(k_hmc_finish, gc, dim3(128), 0, s, m, P, bt, h, tune, record_acc);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, hipStream_t s,
-                            int* host_amax) {
+hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, const HmcCfg& cfg,
+                            hipStream_t s, int* host_amax) {
   int P = bt.P;
   hipLaunchKernelGGL(k_idstart, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   hipLaunchKernelGGL(k_hmc_prep, dim3(B), dim3(256), 0, s, P, bt, h);
@@ -425,14 +472,141 @@ hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, do
     wmax = max(wmax, host_amax[b]);
     tmax = max(tmax, host_amax[B + b]);
   }
+  if (cfg.mover == SMCF_MOVER_HMC_ACCEPT) {
+    hipLaunchKernelGGL(k_hmc_dtau_fill, dim3((P + 127) / 128, B), dim3(128), 0, s, P, bt, h);
+    hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
+    e = run_waves(m, bt, h, B, 0, wmax, max_slip, cfg.ntau, 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_hmc_adapt, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B, cfg.accept);
+    return hipGetLastError();
+  }
   // tuning
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 1);
-  e = run_waves(m, bt, h, B, 1, tmax, max_slip, s);
+  e = run_waves(m, bt, h, B, 1, tmax, max_slip, kNtauRef, 0, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tune_resample, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   // sampling
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
-  return run_waves(m, bt, h, B, 0, wmax, max_slip, s);
+  return run_waves(m, bt, h, B, 0, wmax, max_slip, kNtauRef, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+// Component kernels on one instance (slot 0) for the C-ABI counterparts of slip_calc_grad and slip_leapfrog
+// ---------------------------------------------------------------------------
+// gradient gamma (H x - b) + L^T (L x / alpha^2) of columns 0 .. n-1 of Xt (Y = H Xt already formed)
+__global__ void k_grad_cols(MeshDev m, int P, int n, double gamma, BatchDev bt, HmcDev h, double* __restrict__ G) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  int nd = m.ndim, nn2 = 2 * m.nnode;
+  double a2 = bt.scal[0];
+  for (int r = 0; r < nn2; ++r) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + bt.lv[9 * r + q] * h.Xt[(size_t)(bt.li[9 * r + q] - 1) * P + col];
+    h.LS[(size_t)r * P + col] = tt / a2;
+  }
+  for (int d = 0; d < nd; ++d) {
+    double g1 = bt.Y[(size_t)d * P + col];
+    g1 = g1 - bt.bvec[d];
+    g1 = g1 * gamma;
+    double g2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g2 = g2 + bt.ltv[9 * d + q] * h.LS[(size_t)(bt.lti[9 * d + q] - 1) * P + col];
+    G[(size_t)col * nd + d] = g1 + g2;
+  }
+}
+
+hipError_t launch_grad_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double* G, hipStream_t s) {
+  launch_hs(m.ndim, bt.P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+  hipLaunchKernelGGL(k_grad_cols, dim3((n + 127) / 128), dim3(128), 0, s, m, bt.P, n, gamma, bt, h, G);
+  return hipGetLastError();
+}
+
+// trajectory set-up of n independent columns (column m is its own parent and child): H(cur) from the
+// likelihood and prior of Xt (Y = H Xt formed) and the momenta in Pm
+__global__ void k_leapfrog_begin(MeshDev m, int P, int n, double gamma, double dtau, BatchDev bt, HmcDev h) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  int nd = m.ndim, nn2 = 2 * m.nnode;
+  const double* sc = bt.scal;
+  double v = 0.0, bs = 0.0;
+  for (int d = 0; d < nd; ++d) {
+    double x = h.Xt[(size_t)d * P + col];
+    v = v + x * bt.Y[(size_t)d * P + col];
+    bs = bs + bt.bvec[d] * x;
+  }
+  double lk = (v - 2.0 * bs + bt.dsd[0]) / 2.0 + sc[3] + sc[4];
+  double pr = 0.0, two_a2 = 2.0 * sc[0];
+  for (int r = 0; r < nn2; ++r) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + bt.lv[9 * r + q] * h.Xt[(size_t)(bt.li[9 * r + q] - 1) * P + col];
+    pr = pr + (tt * tt) / two_a2;
+  }
+  double ham = gamma * lk + pr;
+  for (int d = 0; d < nd; ++d) {
+    double p = h.Pm[(size_t)d * P + col];
+    ham = ham + p * bt.cov[d] * p / 2.0;
+    h.Xc[(size_t)d * P + col] = h.Xt[(size_t)d * P + col];
+  }
+  h.hcur[col] = ham;
+  h.cpar[col] = col;
+  h.cchild[col] = col;
+  h.dead[col] = 0;
+  h.dtau[col] = dtau;
+  h.Lc[col] = lk;
+  h.Rc[col] = pr;
+}
+
+// final Hamiltonian of the n trajectories (Y = H Xt formed): ham_cand (1e20 after an escape), the candidate and
+// the momenta
+__global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev bt, HmcDev h,
+                               double* __restrict__ hn_out) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  int nd = m.ndim, nn2 = 2 * m.nnode;
+  if (h.dead[col]) {
+    hn_out[col] = 1e20;
+    return;
+  }
+  const double* sc = bt.scal;
+  double v = 0.0, bs = 0.0;
+  for (int d = 0; d < nd; ++d) {
+    double x = h.Xt[(size_t)d * P + col];
+    v = v + x * bt.Y[(size_t)d * P + col];
+    bs = bs + bt.bvec[d] * x;
+  }
+  double lk = (v - 2.0 * bs + bt.dsd[0]) / 2.0 + sc[3] + sc[4];
+  double pr = 0.0, two_a2 = 2.0 * sc[0];
+  for (int r = 0; r < nn2; ++r) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + bt.lv[9 * r + q] * h.Xt[(size_t)(bt.li[9 * r + q] - 1) * P + col];
+    pr = pr + (tt * tt) / two_a2;
+  }
+  double hn = gamma * lk + pr;
+  for (int d = 0; d < nd; ++d) {
+    double p = h.Pm[(size_t)d * P + col];
+    hn = hn + p * bt.cov[d] * p / 2.0;
+  }
+  hn_out[col] = hn;
+}
+
+hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double dtau, int ntau,
+                                double max_slip, double* hc_out, double* hn_out, hipStream_t s) {
+  int P = bt.P, nd = m.ndim;
+  dim3 gc((n + 127) / 128);
+  launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+  hipLaunchKernelGGL(k_leapfrog_begin, gc, dim3(128), 0, s, m, P, n, gamma, dtau, bt, h);
+  for (int st = 0; st <= ntau; ++st) {
+    launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+    hipLaunchKernelGGL(k_hmc_step, dim3((P + 127) / 128, 1), dim3(128), 0, s, m, P, st, ntau, max_slip, bt, h, 0);
+  }
+  launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
+  hipLaunchKernelGGL(k_leapfrog_end, gc, dim3(128), 0, s, m, P, n, gamma, bt, h, hn_out);
+  hipError_t e = hipMemcpyAsync(hc_out, h.hcur, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
 }
 
 }  // namespace smcf

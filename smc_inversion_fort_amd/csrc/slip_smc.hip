@@ -274,25 +274,46 @@ hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStrea
 }
 
 // ---------------------------------------------------------------------------
+// weighted mean and variance diagonal of coordinate i over P particles (S dimension-major, w normalised), in
+// double-double for the reference's real(16) sums (slip_calc_mean_particles / slip_calc_cov_particles,
+// smc_slip.f90:485-570; variances below 1e-3 are floored at 1e-3)
+SMCF_DEV void moments_one(int i, int P, const double* __restrict__ S, const double* __restrict__ w, double& mean,
+                          double& cv) {
+  dd m = {0.0, 0.0};
+  for (int p = 0; p < P; ++p) m = dd_add_d(m, w[p] * S[(size_t)i * P + p]);
+  mean = m.hi + m.lo;
+  dd c = {0.0, 0.0};
+  for (int p = 0; p < P; ++p) {
+    double di = S[(size_t)i * P + p] - mean;
+    c = dd_add_d(c, w[p] * di * di);
+  }
+  cv = c.hi + c.lo;
+  if (fabs(cv) < 1e-3) cv = 1e-3;
+}
+
 // weighted mean and variance diagonal of the pre-resampling particles (source buffer)
 __global__ void k_moments(int nd, int P, BatchDev bt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!bt.active[b] || i >= nd) return;
   const double* S = (bt.par[b] ? bt.S : bt.S2) + (size_t)b * nd * P;  // source = previous buffer
-  const double* w = bt.w + (size_t)b * P;
-  dd m = {0.0, 0.0};
-  for (int p = 0; p < P; ++p) m = dd_add_d(m, w[p] * S[(size_t)i * P + p]);
-  double mean = m.hi + m.lo;
-  dd c = {0.0, 0.0};
-  for (int p = 0; p < P; ++p) {
-    double di = S[(size_t)i * P + p] - mean;
-    c = dd_add_d(c, w[p] * di * di);
-  }
-  double cv = c.hi + c.lo;
-  if (fabs(cv) < 1e-3) cv = 1e-3;
+  double mean, cv;
+  moments_one(i, P, S, bt.w + (size_t)b * P, mean, cv);
   bt.mean[(size_t)b * nd + i] = mean;
   bt.cov[(size_t)b * nd + i] = cv;
+}
+
+__global__ void k_moments_raw(int nd, int P, const double* __restrict__ S, const double* __restrict__ w,
+                              double* __restrict__ mean, double* __restrict__ cov) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  moments_one(i, P, S, w, mean[i], cov[i]);
+}
+
+hipError_t launch_moments_raw(int nd, int P, const double* S, const double* w, double* mean, double* cov,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_moments_raw, dim3((nd + 63) / 64), dim3(64), 0, s, nd, P, S, w, mean, cov);
+  return hipGetLastError();
 }
 
 hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
@@ -380,57 +401,6 @@ __device__ __noinline__ double gibbs_draw_fast(double mu, double s, double is, d
   z = fmin(fmax(z, a), b);
   double x = mu + z * is;
   return fmin(fmax(x, 0.0), max_slip);
-}
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-#ifndef SMCF_PANEL_BUFFERS
-#define SMCF_PANEL_BUFFERS 2
-#endif
-constexpr int kPanelBuffers = SMCF_PANEL_BUFFERS;
-
-// One block of MT 16-row tiles of a product Y = A B on the matrix cores (v_mfma_f64_16x16x4f64), 64 columns
-// per wavefront:  acc[m][n] += sum_j A(16m + r, j) B(j, c_n) over j = 0 .. 4 nsteps - 1, where lane
-// (r, k) = (lane % 16, lane / 16) reads A(16m + r, 4st + k) at abase[ao[m] + st lda4] and B(4st + k, c_n) at
-// bbase[bo[n] + st ldb4].  Bases are wave-uniform and offsets 32-bit, so the loads take the scalar-base +
-// 32-bit-offset form.  The loop is software-pipelined over NB register buffers: the operands of step st + NB
-// are requested as soon as the products of step st are issued.  The main loop runs whole groups of NB steps and
-// has no branches, so the wait before each group's products covers only that buffer's loads; the last
-// nsteps % NB steps follow it.  (The compiler does not unroll a loop holding MFMA instructions when that needs a
-// remainder loop, so a plain loop would wait for its own loads on every step.)
-template <int MT, int NB>
-__device__ __forceinline__ void mfma_panel_t(const double* __restrict__ abase, const uint32_t (&ao)[MT], uint32_t lda4,
-                                           const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
-                                           int nsteps, dbl4 (&acc)[MT][4]) {
-  if (nsteps <= 0) return;
-  double x[NB][MT], y[NB][4];
-  // loads of step st (clamped to the last step: a group's prefetch past the end re-reads it, unused)
-  auto load = [&](int k, int st) {
-    uint32_t q = (uint32_t)min(st, nsteps - 1);
-#pragma unroll
-    for (int m = 0; m < MT; ++m) x[k][m] = abase[ao[m] + q * lda4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
-  };
-  auto prod = [&](int k) {
-#pragma unroll
-    for (int n = 0; n < 4; ++n)
-#pragma unroll
-      for (int m = 0; m < MT; ++m)
-        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k][m], y[k][n], acc[m][n], 0, 0, 0);
-  };
-#pragma unroll
-  for (int k = 0; k < NB; ++k) load(k, k);
-  int full = nsteps - nsteps % NB;
-  for (int st = 0; st < full; st += NB) {
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      prod(k);
-      load(k, st + k + NB);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < NB - 1; ++k)
-    if (k < nsteps - full) prod(k);
 }
 
 // Likelihood of slip_calc_likelihood (smc_slip.f90:56-104) in Gram form,

@@ -369,4 +369,58 @@ SMCF_DEV double block_min(double v, double* red) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 matrix-core panel shared by the Gibbs, likelihood and HMC kernels
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+#ifndef SMCF_PANEL_BUFFERS
+#define SMCF_PANEL_BUFFERS 2
+#endif
+constexpr int kPanelBuffers = SMCF_PANEL_BUFFERS;
+
+// One block of MT 16-row tiles of a product Y = A B on the matrix cores (v_mfma_f64_16x16x4f64), 64 columns
+// per wavefront:  acc[m][n] += sum_j A(16m + r, j) B(j, c_n) over j = 0 .. 4 nsteps - 1, where lane
+// (r, k) = (lane % 16, lane / 16) reads A(16m + r, 4st + k) at abase[ao[m] + st lda4] and B(4st + k, c_n) at
+// bbase[bo[n] + st ldb4].  Bases are wave-uniform and offsets 32-bit, so the loads take the scalar-base +
+// 32-bit-offset form.  The loop is software-pipelined over NB register buffers: the operands of step st + NB
+// are requested as soon as the products of step st are issued.  The main loop runs whole groups of NB steps and
+// has no branches, so the wait before each group's products covers only that buffer's loads; the last
+// nsteps % NB steps follow it.  (The compiler does not unroll a loop holding MFMA instructions when that needs a
+// remainder loop, so a plain loop would wait for its own loads on every step.)
+template <int MT, int NB>
+__device__ __forceinline__ void mfma_panel_t(const double* __restrict__ abase, const uint32_t (&ao)[MT], uint32_t lda4,
+                                           const double* __restrict__ bbase, const uint32_t (&bo)[4], uint32_t ldb4,
+                                           int nsteps, dbl4 (&acc)[MT][4]) {
+  if (nsteps <= 0) return;
+  double x[NB][MT], y[NB][4];
+  // loads of step st (clamped to the last step: a group's prefetch past the end re-reads it, unused)
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nsteps - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) x[k][m] = abase[ao[m] + q * lda4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k][m], y[k][n], acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nsteps - nsteps % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nsteps - full) prod(k);
+}
+
 }  // namespace smcf

@@ -142,6 +142,8 @@ struct Problem {
   HmcDev hm;
   double max_slip;
   int P, mover, sweeps, precision;
+  int hmc_ntau;       // leapfrog steps (50 for SMCF_MOVER_HMC, smc_slip.f90:669)
+  double hmc_accept;  // target acceptance of SMCF_MOVER_HMC_ACCEPT
   std::vector<int> sar_r0, sar_n;  // host copies of the InSAR path layout (launch geometry)
   std::vector<long> sar_off;
   std::vector<void*> allocs;
@@ -173,6 +175,9 @@ hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hi
 hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
+// mean and variance diagonal of P dimension-major particles S with normalised weights w
+hipError_t launch_moments_raw(int nd, int P, const double* S, const double* w, double* mean, double* cov,
+                              hipStream_t s);
 hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
@@ -195,9 +200,21 @@ __host__ __device__ inline int bf_rows(int nd) { return (nd + 31) & ~31; }
 __host__ __device__ inline size_t bf_q_elems(int nd, int kb) { return (size_t)((nd + kb - 1) / kb) * kb * bf_rows(nd) * 2; }
 hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 
-hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel = nullptr);
-hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, hipStream_t s,
-                            int* host_amax);
+// HMC movers: SMCF_MOVER_HMC (ESJD tuning, ntau 50) or SMCF_MOVER_HMC_ACCEPT (ntau steps, step size adapted
+// toward `accept`)
+struct HmcCfg {
+  int mover, ntau;
+  double accept;
+};
+hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel, int mover);
+hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, const HmcCfg& cfg,
+                            hipStream_t s, int* host_amax);
+// one instance in slot 0: the tempered gradient of columns 0 .. n-1 of h.Xt into G (ndim x n, column per
+// particle), and n independent leapfrog trajectories from (h.Xt, h.Pm) with bt.cov as the mass diagonal
+// (H(cur) into hc_out, H(cand) into hn_out, both device arrays)
+hipError_t launch_grad_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double* G, hipStream_t s);
+hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double dtau, int ntau,
+                                double max_slip, double* hc_out, double* hn_out, hipStream_t s);
 
 hipError_t launch_dc3d0_batch(int n, const double* in, double* u, int* iret, hipStream_t s);
 hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,

@@ -34,7 +34,8 @@ module smcf_c
         integer(c_int) :: nobs, nsar_total, ngnss, npath
         type(c_ptr) :: nsar_index, obs_points, obs_unitvec, obs_sigma, dvec, sar_sinv
         real(c_double) :: max_slip
-        integer(c_int) :: nparticle_slip, mover, gibbs_sweeps, precision
+        integer(c_int) :: nparticle_slip, mover, gibbs_sweeps, precision, hmc_ntau
+        real(c_double) :: hmc_accept
     end type
 
     interface
@@ -93,7 +94,9 @@ module smcf_c
     double precision, save :: g_sum = -1d300
     integer(c_int64_t), save :: g_seed = 0_c_int64_t
     integer(c_int32_t), save :: g_counter = 0
-    integer, save :: g_mover = 1, g_sweeps = 5, g_max_batch = 64, g_precision = 0
+    ! default: the reference mover (HMC, ntau 50, ESJD tuning, smc_slip.f90:591-1214)
+    integer, save :: g_mover = 0, g_sweeps = 5, g_max_batch = 64, g_precision = 0, g_hmc_ntau = 10
+    double precision, save :: g_hmc_accept = 0.65d0
     ! arrays kept alive for the C side
     integer(c_int), allocatable, target, save :: t_nxi(:), t_neta(:), t_nsi(:)
     real(c_double), allocatable, target, save :: t_xb(:), t_pts(:), t_uv(:), t_sig(:), t_d(:), t_sar(:)
@@ -214,6 +217,8 @@ contains
         d%mover = g_mover
         d%gibbs_sweeps = g_sweeps
         d%precision = g_precision
+        d%hmc_ntau = g_hmc_ntau
+        d%hmc_accept = g_hmc_accept
         call smcf_check(smcf_problem_create(g_ctx, d, int(g_max_batch, c_int), g_pb), "smcf_problem_create")
         g_sig = sig
         g_sum = s

@@ -72,7 +72,8 @@ def test_fortran_fault_calc_likelihood_matches_c_abi(golden):
     ctx = S.Context(0)
     obs = dict(obs_points=g[name + "__obs_points"], obs_unitvec=g[name + "__obs_unitvec"], obs_sigma=sigma,
                dvec=dvec, nsar_total=0, ngnss=int(g[name + "__ngnss"]))
-    pb = S.Problem(ctx, nxi, neta, xmin, xmax, zmin, obs, 10.0, P, max_batch=64)
+    # the drop-in's default mover is the reference's (HMC, ntau 50, ESJD tuning)
+    pb = S.Problem(ctx, nxi, neta, xmin, xmax, zmin, obs, 10.0, P, mover=S.MOVER_HMC, max_batch=64)
     ref, _ = pb.fault_calc_likelihood(theta, [S.inner_key(seed, 0, 2, 0)])
     assert neglog == ref[0]
     pb.close()

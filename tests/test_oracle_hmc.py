@@ -63,3 +63,27 @@ def test_leapfrog_reflections_stay_in_box_and_escape_sentinel():
             assert np.array_equal(c, cur)
         else:
             assert c.min() >= 0.0 and c.max() <= syn.MAX_SLIP
+
+
+def test_accept_mover_settles_near_its_target():
+    """Mover 2 (ntau leapfrog steps, step size adapted toward a target mean acceptance): past the first stages
+    the stage's mean acceptance probability stays near the target, and the step size follows the adaptation
+    rule from stage to stage."""
+    xmin, xmax, zmin, _ = syn.fault_frame()
+
+    def greens(th, pts, uv, nx, ne):
+        return O.Fault([nx], [ne], xmin, xmax, zmin).greens(th, pts, uv, 0, pts.shape[1] // 3)
+
+    obs = syn.make_observations("c1", 0, greens)  # 8x8 patches, 300 observations: ~27 stages
+    th = syn.theta_true()
+    G = greens(th, obs["obs_points"], obs["obs_unitvec"], 8, 8)
+    for target in (0.65, 0.8):
+        po = O.ProblemHolder(O.Fault([8], [8], xmin, xmax, zmin), obs, syn.MAX_SLIP, 400, mover=2, hmc_ntau=10,
+                             hmc_accept=target)
+        r = po.slip_smc(th, G, O.inner_key(7, 0, 0, 0))
+        acc, dt = r["accept"], r["dtau"]
+        assert r["nstage"] > 15 and 1.0 - r["gamma"][-1] <= 1e-7
+        assert abs(dt[0] - 0.1) < 1e-15
+        assert abs(np.median(acc[6:]) - target) < 0.08, acc
+        # log dtau(s + 1) - log dtau(s) = 2 (acc(s) - target)
+        np.testing.assert_allclose(np.diff(np.log(dt)), 2.0 * (acc[:-1] - target), atol=1e-12)
