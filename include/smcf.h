@@ -98,6 +98,18 @@ int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_s
  * value when depth < 0, as the reference does). */
 int smcf_dc3d0(smcf_ctx* ctx, int n, const double* in, double* u, int* iret);
 
+/* call_dc3d0 (gfunc.f90:65-100) for n point sources: in 10 x n = (xsource, ysource, zsource, xobs, yobs, uxi, ueta,
+ * dip, area, strike), uret 3 x n (cm, geographic frame). */
+int smcf_call_dc3d0(smcf_ctx* ctx, int n, const double* in, double* uret);
+/* calc_responce (gfunc.f90:102-194) at npts observation points (pts 2 x npts): displacement uobs (3 x npts) of the
+ * slip distribution slip_dist (2 x nnode, (u_xi, u_eta) per node) over the target patches target_id (1-based,
+ * ntarget of them) of the plane whose frame is (xf, yf, zf, strike, dip); cny 4 x npatch and coor 2 x nnode as
+ * discretize_fault builds them.  calc_responce_dist (gfunc.f90:196-240) is this over the SAR points and one
+ * point per GNSS station. */
+int smcf_calc_responce(smcf_ctx* ctx, int npts, const double* pts, int npatch, const int* cny, int nnode,
+                       const double* coor, const double* slip_dist, double xf, double yf, double zf, double strike,
+                       double dip, int ntarget, const int* target_id, double* uobs);
+
 /* Green's matrices for B geometries (replaces calc_greens_func, gfunc.f90:242-348).
  * theta: ndim_fault x B; gmat: nobs x ndim_slip x B. */
 int smcf_calc_greens_func(smcf_problem* pb, int B, const double* theta, double* gmat);
@@ -107,6 +119,12 @@ int smcf_calc_greens_func(smcf_problem* pb, int B, const double* theta, double* 
  * llmat: ndim x ndim. */
 int smcf_laplacian(smcf_problem* pb, const double* theta, int* lmat_index, double* lmat_val,
                    int* ltmat_index, double* ltmat_val, double* llmat);
+
+/* discretize_fault (init.f90:198-293) and the dense operators of gen_laplacian (init.f90:295-474) for one theta:
+ * cny 4 x npatch, coor 2 x nnode, node_to_elem_val 4 x nnode (-1 padded), node_to_elem_size nnode, id_dof ndof
+ * (1-based ids), luni nnode x nnode, lmat 2 nnode x 2 ndof.  Any output may be NULL. */
+int smcf_discretize_fault(smcf_problem* pb, const double* theta, int* cny, double* coor, int* node_to_elem_val,
+                          int* node_to_elem_size, int* id_dof, double* luni, double* lmat);
 
 /* Gram quantities for B geometries (replaces slip_calc_gsd_gsg, smc_slip.f90:1347-1443).
  * gsgmat: ndim x ndim x B; gsdvec: ndim x B; dsd: B.  NULL outputs are skipped. */
@@ -146,7 +164,8 @@ int smcf_slip_mean_batch(smcf_problem* pb, int B, const double* theta, const uin
  *   smcf_slip_calc_grad         slip_calc_grad         (smc_slip.f90:139-182), tempered by gamma; grad ndim x n
  *   smcf_slip_leapfrog          slip_leapfrog          (smc_slip.f90:732-917) for n independent trajectories from
  *                               (cur, p0) with mass diagonal cov_diag; cand / p_out ndim x n, ham_cur / ham_cand n
- *                               (1e20 where the trajectory escaped |x| > 1e3; the candidate is then cur)
+ *                               (1e20 where the trajectory escaped |x| > 1e3; the candidate is then cur), and the
+ *                               candidates' likelihood and prior (n each; any output may be NULL)
  *   smcf_slip_gen_init_particles slip_gen_init_particles (smc_slip.f90:184-287): the sequential prior Gibbs chain
  *                               of all nparticle_slip particles in the keyed stream of `key`, with likelihood
  *                               and prior
@@ -157,9 +176,27 @@ int smcf_slip_calc_grad(smcf_problem* pb, const double* theta, double gamma, int
                         double* grad);
 int smcf_slip_leapfrog(smcf_problem* pb, const double* theta, double gamma, int n, const double* cur,
                        const double* p0, const double* cov_diag, double dtau, int ntau, double* cand, double* p_out,
-                       double* ham_cur, double* ham_cand);
+                       double* ham_cur, double* ham_cand, double* likelihood_cand, double* prior_cand);
 int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t key, double* particles,
                                  double* likelihood, double* prior);
+/* slip_calc_gsd_gsg (smc_slip.f90:1347-1443) from a given Green's matrix (nobs x ndim) and sigma^2 values, with the
+ * problem's observation set (data, sigmas, InSAR path covariance) */
+int smcf_slip_calc_gsd_gsg_gmat(smcf_problem* pb, const double* gmat, double sigma_sar2, double sigma_gnss2,
+                                double* gsgmat, double* gsdvec, double* dsd);
+/* The same procedures from the arrays of the reference's argument lists (no problem needed): gsgmat ndim x ndim
+ * and gsdvec ndim of slip_calc_gsd_gsg, alpha2_full 2 nnode, the ELL of L (lmat_index / lmat_val, 9 x 2 nnode,
+ * 1-based columns) and of L^T (ltmat_index / ltmat_val, 9 x ndim, 1-based rows) of gen_sparse_lmat.
+ *   smcf_slip_likelihood_gram  slip_calc_likelihood (smc_slip.f90:56-104)
+ *   smcf_slip_prior_ell        slip_calc_prior (106-137)
+ *   smcf_slip_grad_ell         slip_calc_grad (139-182), grad ndim x n */
+int smcf_slip_likelihood_gram(smcf_ctx* ctx, int ndim, int n, const double* particles, const double* gsgmat,
+                              const double* gsdvec, double dsd, int nsar_total, int ngnss, double log_sigma_sar2,
+                              double log_sigma_gnss2, double* out);
+int smcf_slip_prior_ell(smcf_ctx* ctx, int nnode, int ndim, int n, const double* particles, const double* alpha2_full,
+                        const int* lmat_index, const double* lmat_val, double* out);
+int smcf_slip_grad_ell(smcf_ctx* ctx, int nnode, int ndim, int n, const double* particles, double gamma,
+                       const double* gsgmat, const double* gsdvec, const double* alpha2_full, const int* lmat_index,
+                       const double* lmat_val, const int* ltmat_index, const double* ltmat_val, double* grad);
 /* SMCF_MOVER_HMC_ACCEPT (no reference counterpart): after smcf_slip_smc_exec, the step size the next stage of
  * that inversion would use and the mean acceptance probability of its last stage */
 int smcf_slip_hmc_accept_state(const smcf_problem* pb, double* next_dtau, double* mean_accept);
@@ -216,6 +253,42 @@ int smcf_fault_smc_exec_gsmc(smcf_problem* pb, const double* range, int N, uint6
                              int max_batch, const double* qmean, const double* qcov_chol, const double* qcov_inv,
                              double log_qcov_det, double* particles, double* likelihood, int* nstage,
                              double* evidence, double* gamma_trace);
+/* Stage components of the outer SMC (smc_fault.f90, one rank), host code in the reference's arithmetic order;
+ * smcf_fault_smc_exec is their composition.  Particles ndim x N column-major.  Random numbers are keyed by the
+ * outer seed and stage, as in smcf_fault_smc_exec.
+ *   smcf_fault_sample_init_particles  fault_sample_init_particles (29-64): uniform in range (2 x ndim), or, with
+ *                                     qmean != NULL, qmean + qcov_chol z (the gsmc branch)
+ *   smcf_fault_mean_std               calc_mean_std_vector (372-390): sequential sums, population std
+ *   smcf_fault_find_next_gamma        fault_find_next_gamma (392-485): bisection to CV 0.2, tol 1e-8, at most 1000
+ *                                     steps; with q != NULL the weights temper L - q (gsmc); neglog_evidence is the
+ *                                     stage's increment (the reference adds it to fault_evidence)
+ *   smcf_fault_normalize_weights      fault_normalize_weights (487-501)
+ *   smcf_fault_calc_mean_particles    fault_calc_mean_particles (503-520)
+ *   smcf_fault_calc_cov_particles     fault_calc_cov_particles (522-570): lower factor of 0.04 x the weighted
+ *                                     covariance (dpotrf 'L'); info = first non-positive pivot, 0 if none
+ *   smcf_fault_resample_particles     fault_resample_particles (572-590): real(16) running u
+ *   smcf_fault_reorder_to_send        fault_reorder_to_send (592-637) with numprocs = 1 (q may be NULL)
+ *   smcf_work_mcmc_sampling           work_mcmc_sampling (639-783): re-evaluates the parents, proposes
+ *                                     cur + cov_chol z (reflected into range) for every child, Metropolis test
+ *                                     gamma (L_cur - L_cand) [+ (1 - gamma)(q_cur - q_cand) when q != NULL];
+ *                                     children at 0-based id_start(i) + k; q, qmean, qcov_inv, q_new may be NULL */
+int smcf_fault_sample_init_particles(int ndim, int N, uint64_t seed, const double* range, const double* qmean,
+                                     const double* qcov_chol, double* particles);
+int smcf_fault_mean_std(int n, const double* vec, double* mean, double* std_out);
+int smcf_fault_find_next_gamma(int N, double gamma_prev, const double* likelihood, const double* q, double* weights,
+                               double* gamma, double* neglog_evidence);
+int smcf_fault_normalize_weights(int N, double* weights);
+int smcf_fault_calc_mean_particles(int ndim, int N, const double* particles, const double* weights, double* mean);
+int smcf_fault_calc_cov_particles(int ndim, int N, const double* particles, const double* weights, const double* mean,
+                                  double* cov, int* info);
+int smcf_fault_resample_particles(int N, const double* weights, uint64_t seed, int stage, int* assigned_num);
+int smcf_fault_reorder_to_send(int ndim, int N, int* assigned_num, double* particles, double* likelihood, double* q,
+                               int* sorted_idx);
+int smcf_work_mcmc_sampling(smcf_problem* pb, uint64_t seed, int stage, double gamma, const double* range,
+                            const double* cov_chol, int N, const int* assigned_num, const double* particles,
+                            const double* q, const double* qmean, const double* qcov_inv, double log_qcov_det,
+                            int max_batch, double* particles_new, double* likelihood_new, double* q_new,
+                            int* acc_count);
 /* RNG key of the inner SMC of one outer evaluation: (seed, outer stage, role 0 parent / 1 child, slot) */
 uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint32_t slot);
 

@@ -63,6 +63,11 @@ EXPORTS = [
     "smcf_slip_mean_batch", "smcf_batch_times", "smcf_batch_admit_times", "smcf_find_next_gamma_noalloc",
     "smcf_slip_calc_likelihood", "smcf_slip_calc_prior", "smcf_slip_calc_grad", "smcf_slip_leapfrog",
     "smcf_slip_gen_init_particles", "smcf_slip_calc_moments", "smcf_slip_hmc_accept_state",
+    "smcf_fault_sample_init_particles", "smcf_fault_mean_std", "smcf_fault_find_next_gamma",
+    "smcf_fault_normalize_weights", "smcf_fault_calc_mean_particles", "smcf_fault_calc_cov_particles",
+    "smcf_fault_resample_particles", "smcf_fault_reorder_to_send", "smcf_work_mcmc_sampling", "smcf_call_dc3d0",
+    "smcf_calc_responce", "smcf_slip_likelihood_gram", "smcf_slip_prior_ell", "smcf_slip_grad_ell",
+    "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault",
 ]
 
 
@@ -123,10 +128,30 @@ def _declare(L):
     L.smcf_slip_calc_prior.argtypes = [C.c_void_p, _D, C.c_int, _D, _D]
     L.smcf_slip_calc_grad.argtypes = [C.c_void_p, _D, C.c_double, C.c_int, _D, _D]
     L.smcf_slip_leapfrog.argtypes = [C.c_void_p, _D, C.c_double, C.c_int, _D, _D, _D, C.c_double, C.c_int, _D, _D,
-                                     _D, _D]
+                                     _D, _D, _D, _D]
+    L.smcf_slip_calc_gsd_gsg_gmat.argtypes = [C.c_void_p, _D, C.c_double, C.c_double, _D, _D, _D]
+    L.smcf_discretize_fault.argtypes = [C.c_void_p, _D, _I, _D, _I, _I, _I, _D, _D]
     L.smcf_slip_gen_init_particles.argtypes = [C.c_void_p, _D, C.c_uint64, _D, _D, _D]
     L.smcf_slip_calc_moments.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D, _D, _D]
     L.smcf_slip_hmc_accept_state.argtypes = [C.c_void_p, _D, _D]
+    L.smcf_call_dc3d0.argtypes = [C.c_void_p, C.c_int, _D, _D]
+    L.smcf_slip_likelihood_gram.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D, _D, C.c_double, C.c_int, C.c_int,
+                                            C.c_double, C.c_double, _D]
+    L.smcf_slip_prior_ell.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D, _D, _I, _D, _D]
+    L.smcf_slip_grad_ell.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D, C.c_double, _D, _D, _D, _I, _D, _I,
+                                     _D, _D]
+    L.smcf_calc_responce.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, _I, C.c_int, _D, _D, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_int, _I, _D]
+    L.smcf_fault_sample_init_particles.argtypes = [C.c_int, C.c_int, C.c_uint64, _D, _D, _D, _D]
+    L.smcf_fault_mean_std.argtypes = [C.c_int, _D, _D, _D]
+    L.smcf_fault_find_next_gamma.argtypes = [C.c_int, C.c_double, _D, _D, _D, _D, _D]
+    L.smcf_fault_normalize_weights.argtypes = [C.c_int, _D]
+    L.smcf_fault_calc_mean_particles.argtypes = [C.c_int, C.c_int, _D, _D, _D]
+    L.smcf_fault_calc_cov_particles.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, _I]
+    L.smcf_fault_resample_particles.argtypes = [C.c_int, _D, C.c_uint64, C.c_int, _I]
+    L.smcf_fault_reorder_to_send.argtypes = [C.c_int, C.c_int, _I, _D, _D, _D, _I]
+    L.smcf_work_mcmc_sampling.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_double, _D, _D, C.c_int, _I, _D, _D,
+                                          _D, _D, C.c_double, C.c_int, _D, _D, _D, _I]
 
 
 def _check(rc):
@@ -162,6 +187,82 @@ def qsort(arr, index):
     n = C.c_int(index.shape[0])
     lib().qsort_(_ip(arr), _ip(index), C.byref(n))
     return index
+
+
+# ------------------------------------------------------------------ outer-SMC stage components (host code)
+def _opt(a):
+    return None if a is None else _dp(a)
+
+
+def fault_sample_init_particles(ndim, N, seed, range_=None, qmean=None, qcov_chol=None):
+    """fault_sample_init_particles (smc_fault.f90:29-64): ndim x N, uniform in range_ or qmean + chol z."""
+    X = np.zeros((ndim, N), order="F")
+    r = None if range_ is None else _f64(range_)
+    qm = None if qmean is None else _f64(qmean)
+    qc = None if qcov_chol is None else _f64(qcov_chol)
+    _check(lib().smcf_fault_sample_init_particles(ndim, N, C.c_uint64(seed), _opt(r), _opt(qm), _opt(qc), _dp(X)))
+    return X
+
+
+def fault_mean_std(vec):
+    """calc_mean_std_vector (smc_fault.f90:372-390)."""
+    v = _f64(vec)
+    m, sd = C.c_double(), C.c_double()
+    _check(lib().smcf_fault_mean_std(v.shape[0], _dp(v), C.byref(m), C.byref(sd)))
+    return m.value, sd.value
+
+
+def fault_find_next_gamma(gamma_prev, likelihood, q=None):
+    """fault_find_next_gamma (smc_fault.f90:392-485): (gamma, weights, -log evidence increment)."""
+    L = _f64(likelihood)
+    qq = None if q is None else _f64(q)
+    w = np.zeros(L.shape[0])
+    g, ev = C.c_double(), C.c_double()
+    _check(lib().smcf_fault_find_next_gamma(L.shape[0], float(gamma_prev), _dp(L), _opt(qq), _dp(w), C.byref(g),
+                                            C.byref(ev)))
+    return g.value, w, ev.value
+
+
+def fault_normalize_weights(w):
+    w = _f64(w).copy()
+    _check(lib().smcf_fault_normalize_weights(w.shape[0], _dp(w)))
+    return w
+
+
+def fault_calc_mean_particles(X, w):
+    X = _f64(X)
+    m = np.zeros(X.shape[0])
+    _check(lib().smcf_fault_calc_mean_particles(X.shape[0], X.shape[1], _dp(X), _dp(_f64(w)), _dp(m)))
+    return m
+
+
+def fault_calc_cov_particles(X, w, mean):
+    """Lower Cholesky factor of 0.04 x the weighted covariance and dpotrf's info."""
+    X = _f64(X)
+    nd = X.shape[0]
+    cov = np.zeros((nd, nd), order="F")
+    info = C.c_int()
+    _check(lib().smcf_fault_calc_cov_particles(nd, X.shape[1], _dp(X), _dp(_f64(w)), _dp(_f64(mean)), _dp(cov),
+                                               C.byref(info)))
+    return cov, info.value
+
+
+def fault_resample_particles(w, seed, stage):
+    w = _f64(w)
+    a = np.zeros(w.shape[0], np.int32)
+    _check(lib().smcf_fault_resample_particles(w.shape[0], _dp(w), C.c_uint64(seed), int(stage), _ip(a)))
+    return a
+
+
+def fault_reorder_to_send(assigned, X, L, q=None):
+    """Descending reorder through qsort_; returns (assigned, X, L, q, sorted_idx) (copies)."""
+    a = np.array(assigned, dtype=np.int32)
+    Xc = np.array(_f64(X), order="F")
+    Lc = np.array(_f64(L))
+    qc = None if q is None else np.array(_f64(q))
+    sidx = np.zeros(a.shape[0], np.int32)
+    _check(lib().smcf_fault_reorder_to_send(Xc.shape[0], a.shape[0], _ip(a), _dp(Xc), _dp(Lc), _opt(qc), _ip(sidx)))
+    return a, Xc, Lc, qc, sidx
 
 
 class Context:
@@ -253,6 +354,55 @@ class Context:
         _check(lib().smcf_trunc_normal_f32(self.h, mu.shape[0], _dp(mu), _dp(sq), lo, hi,
                                            u.ctypes.data_as(C.POINTER(C.c_float)), _dp(out)))
         return out
+
+    def call_dc3d0(self, inputs):
+        """call_dc3d0 (gfunc.f90:65-100) for rows (xsource, ysource, zsource, xobs, yobs, uxi, ueta, dip, area,
+        strike) of inputs (n x 10); returns uret (n x 3)."""
+        a = np.ascontiguousarray(inputs, dtype=np.float64)
+        u = np.zeros((a.shape[0], 3))
+        _check(lib().smcf_call_dc3d0(self.h, a.shape[0], _dp(a), _dp(u)))
+        return u
+
+    def calc_responce(self, pts, cny, coor, slip_dist, xf, yf, zf, strike, dip, target_id):
+        """calc_responce (gfunc.f90:102-194) at each column of pts (2 x npts); returns uobs (3 x npts)."""
+        p = _f64(pts)
+        cn, co, sl, t = _i32(cny), _f64(coor), _f64(slip_dist), _i32(target_id)
+        u = np.zeros((3, p.shape[1]), order="F")
+        _check(lib().smcf_calc_responce(self.h, p.shape[1], _dp(p), cn.shape[1], _ip(cn), co.shape[1], _dp(co),
+                                        _dp(sl), xf, yf, zf, strike, dip, t.shape[0], _ip(t), _dp(u)))
+        return u
+
+    def slip_likelihood_gram(self, particles, H, b, dsd, nsar_total, ngnss, lss2, lsg2):
+        """slip_calc_likelihood from the Gram quantities, for each column of particles."""
+        X = _f64(particles)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        out = np.zeros(X.shape[1])
+        _check(lib().smcf_slip_likelihood_gram(self.h, X.shape[0], X.shape[1], _dp(X), _dp(_f64(H)), _dp(_f64(b)),
+                                               float(dsd), int(nsar_total), int(ngnss), float(lss2), float(lsg2),
+                                               _dp(out)))
+        return out
+
+    def slip_prior_ell(self, particles, alpha2_full, li, lv):
+        X = _f64(particles)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        a2 = _f64(alpha2_full)
+        out = np.zeros(X.shape[1])
+        _check(lib().smcf_slip_prior_ell(self.h, a2.shape[0] // 2, X.shape[0], X.shape[1], _dp(X), _dp(a2),
+                                         _ip(_i32(li)), _dp(_f64(lv)), _dp(out)))
+        return out
+
+    def slip_grad_ell(self, particles, gamma, H, b, alpha2_full, li, lv, lti, ltv):
+        X = _f64(particles)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        a2 = _f64(alpha2_full)
+        g = np.zeros(X.shape, order="F")
+        _check(lib().smcf_slip_grad_ell(self.h, a2.shape[0] // 2, X.shape[0], X.shape[1], _dp(X), float(gamma),
+                                        _dp(_f64(H)), _dp(_f64(b)), _dp(a2), _ip(_i32(li)), _dp(_f64(lv)),
+                                        _ip(_i32(lti)), _dp(_f64(ltv)), _dp(g)))
+        return g
 
     def philox(self, ctr, key):
         c = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
@@ -385,6 +535,21 @@ class Problem:
         _check(lib().smcf_calc_greens_func(self.h, B, _dp(th), _dp(g)))
         return g[:, :, 0] if B == 1 else g
 
+    def discretize_fault(self, theta):
+        """discretize_fault + the dense luni / lmat of gen_laplacian for one theta."""
+        th = self._theta(theta)
+        npatch = int(self._keep["nxi"].astype(int) @ self._keep["neta"].astype(int))
+        cny = np.zeros((4, npatch), np.int32, order="F")
+        coor = np.zeros((2, self.nnode), order="F")
+        n2e = np.zeros((4, self.nnode), np.int32, order="F")
+        n2es = np.zeros(self.nnode, np.int32)
+        idd = np.zeros(self.ndof, np.int32)
+        luni = np.zeros((self.nnode, self.nnode), order="F")
+        lmat = np.zeros((2 * self.nnode, 2 * self.ndof), order="F")
+        _check(lib().smcf_discretize_fault(self.h, _dp(th), _ip(cny), _dp(coor), _ip(n2e), _ip(n2es), _ip(idd),
+                                           _dp(luni), _dp(lmat)))
+        return dict(cny=cny, coor=coor, n2e=n2e, n2es=n2es, id_dof=idd, luni=luni, lmat=lmat)
+
     def laplacian(self, theta):
         th = self._theta(theta)
         li = np.zeros((9, 2 * self.nnode), np.int32, order="F")
@@ -456,10 +621,22 @@ class Problem:
         pout = np.zeros(X.shape, order="F")
         hc = np.zeros(n)
         hn = np.zeros(n)
+        lk = np.zeros(n)
+        pr = np.zeros(n)
         _check(lib().smcf_slip_leapfrog(self.h, _dp(self._theta(theta)), float(gamma), n, _dp(X), _dp(Pm),
                                         _dp(_f64(cov_diag)), float(dtau), int(ntau), _dp(cand), _dp(pout), _dp(hc),
-                                        _dp(hn)))
+                                        _dp(hn), _dp(lk), _dp(pr)))
+        self.last_leapfrog_like_prior = (lk, pr)
         return cand, pout, hc, hn
+
+    def slip_calc_gsd_gsg_gmat(self, gmat, sigma_sar2, sigma_gnss2):
+        """slip_calc_gsd_gsg from a given Green's matrix: (H, b, dsd)."""
+        H = np.zeros((self.ndim, self.ndim), order="F")
+        b = np.zeros(self.ndim)
+        dsd = C.c_double()
+        _check(lib().smcf_slip_calc_gsd_gsg_gmat(self.h, _dp(_f64(gmat)), float(sigma_sar2), float(sigma_gnss2),
+                                                 _dp(H), _dp(b), C.byref(dsd)))
+        return H, b, dsd.value
 
     def slip_gen_init_particles(self, theta, key):
         """The prior Gibbs chain of slip_gen_init_particles: (particles ndim x P, likelihood, prior)."""
@@ -475,6 +652,27 @@ class Problem:
         dt, ab = C.c_double(), C.c_double()
         _check(lib().smcf_slip_hmc_accept_state(self.h, C.byref(dt), C.byref(ab)))
         return dt.value, ab.value
+
+    def work_mcmc_sampling(self, seed, stage, gamma, range_, cov_chol, assigned, particles, max_batch=None,
+                           gsmc=None, q=None):
+        """work_mcmc_sampling (smc_fault.f90:639-783) of one outer stage: (particles_new, likelihood_new, q_new,
+        accepted).  gsmc = (qmean, qcov_inv, log_qcov_det) with q the parents' -log q."""
+        X = _f64(particles)
+        nd, N = X.shape
+        a = _i32(assigned)
+        Xn = np.zeros((nd, N), order="F")
+        Ln = np.zeros(N)
+        qn = np.zeros(N)
+        acc = C.c_int()
+        if gsmc is not None:
+            qm, qi = _f64(gsmc[0]), _f64(gsmc[1])
+            args = (_dp(_f64(q)), _dp(qm), _dp(qi), float(gsmc[2]))
+        else:
+            args = (None, None, None, 0.0)
+        _check(lib().smcf_work_mcmc_sampling(self.h, C.c_uint64(seed), int(stage), float(gamma), _dp(_f64(range_)),
+                                             _dp(_f64(cov_chol)), N, _ip(a), _dp(X), *args, int(max_batch or N),
+                                             _dp(Xn), _dp(Ln), _dp(qn), C.byref(acc)))
+        return Xn, Ln, qn, acc.value
 
     def set_output_dir(self, output_dir):
         """Per-stage dumps <output_dir><iter>.csv of fault_smc_exec (None stops them)."""

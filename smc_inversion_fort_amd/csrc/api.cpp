@@ -797,6 +797,61 @@ int smcf_dc3d0(smcf_ctx* ctx, int n, const double* in, double* u, int* iret) {
   return SMCF_OK;
 }
 
+int smcf_call_dc3d0(smcf_ctx* ctx, int n, const double* in, double* uret) {
+  if (!ctx || n < 0 || (n > 0 && (!in || !uret))) return SMCF_EINVAL;
+  if (n == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  TmpBufs tb;
+  double *din, *dout;
+  SMCF_HIP(tb.alloc(&din, 10 * (size_t)n));
+  SMCF_HIP(tb.alloc(&dout, 3 * (size_t)n));
+  SMCF_HIP(hipMemcpyAsync(din, in, sizeof(double) * 10 * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_call_dc3d0(n, din, dout, s));
+  SMCF_HIP(hipMemcpyAsync(uret, dout, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_calc_responce(smcf_ctx* ctx, int npts, const double* pts, int npatch, const int* cny, int nnode,
+                       const double* coor, const double* slip_dist, double xf, double yf, double zf, double strike,
+                       double dip, int ntarget, const int* target_id, double* uobs) {
+  if (!ctx || npts < 0 || npatch < 1 || nnode < 1 || ntarget < 0 || !cny || !coor || !slip_dist ||
+      (ntarget > 0 && !target_id) || (npts > 0 && (!pts || !uobs)))
+    return SMCF_EINVAL;
+  for (int t = 0; t < ntarget; ++t)
+    if (target_id[t] < 1 || target_id[t] > npatch) {
+      set_error("smcf_calc_responce: target patch id out of range");
+      return SMCF_EINVAL;
+    }
+  for (int k = 0; k < 4 * npatch; ++k)
+    if (cny[k] < 1 || cny[k] > nnode) {
+      set_error("smcf_calc_responce: node id out of range in cny");
+      return SMCF_EINVAL;
+    }
+  if (npts == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  TmpBufs tb;
+  double *dp, *dcoor, *dslip, *du;
+  int *dcny, *dt;
+  SMCF_HIP(tb.alloc(&dp, 2 * (size_t)npts));
+  SMCF_HIP(tb.alloc(&dcoor, 2 * (size_t)nnode));
+  SMCF_HIP(tb.alloc(&dslip, 2 * (size_t)nnode));
+  SMCF_HIP(tb.alloc(&du, 3 * (size_t)npts));
+  SMCF_HIP(tb.alloc(&dcny, 4 * (size_t)npatch));
+  SMCF_HIP(tb.alloc(&dt, (size_t)std::max(ntarget, 1)));
+  SMCF_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 2 * npts, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dcoor, coor, sizeof(double) * 2 * nnode, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dslip, slip_dist, sizeof(double) * 2 * nnode, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dcny, cny, sizeof(int) * 4 * npatch, hipMemcpyHostToDevice, s));
+  if (ntarget > 0) SMCF_HIP(hipMemcpyAsync(dt, target_id, sizeof(int) * ntarget, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_calc_responce(npts, dp, dcny, dcoor, dslip, xf, yf, zf, strike, dip, ntarget, dt, du, s));
+  SMCF_HIP(hipMemcpyAsync(uobs, du, sizeof(double) * 3 * npts, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
 int smcf_calc_greens_func(smcf_problem* pb, int B, const double* theta, double* gmat) {
   if (!pb || !theta || !gmat) return SMCF_EINVAL;
   Problem& P = pb->p;
@@ -840,6 +895,66 @@ int smcf_laplacian(smcf_problem* pb, const double* theta, int* li, double* lv, i
         if (r < 0) break;
         ll[i * nd + r] = vx[i * 25 + w];
       }
+  }
+  return SMCF_OK;
+}
+
+int smcf_discretize_fault(smcf_problem* pb, const double* theta, int* cny, double* coor, int* n2e, int* n2es,
+                          int* id_dof, double* luni, double* lmat) {
+  if (!pb || !theta) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  MeshDev& m = P.m;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  const int nn = m.nnode, np = m.nplane;
+  std::vector<int> nk(nn), nl(nn), npl(nn), pn0(np), pnx(np), pne(np), idd(m.ndof);
+  std::vector<double> geo(16 * (size_t)np), slots(9 * (size_t)nn);
+  SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * m.ndim_fault, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_geometry(m, P.bt, 1, s));
+  CHECK_LAUNCH(launch_laplacian(m, P.bt, 1, s));
+  SMCF_HIP(hipMemcpyAsync(geo.data(), P.bt.geo, sizeof(double) * geo.size(), hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(slots.data(), P.bt.luni, sizeof(double) * slots.size(), hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(nk.data(), m.node_k, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(nl.data(), m.node_l, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(npl.data(), m.node_plane, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(pn0.data(), m.plane_node0, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(pnx.data(), m.plane_nxi, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(pne.data(), m.plane_neta, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(idd.data(), m.id_dof, sizeof(int) * m.ndof, hipMemcpyDeviceToHost, s));
+  if (cny) SMCF_HIP(hipMemcpyAsync(cny, m.cny, sizeof(int) * 4 * m.npatch, hipMemcpyDeviceToHost, s));
+  if (n2e) SMCF_HIP(hipMemcpyAsync(n2e, m.n2e, sizeof(int) * 4 * nn, hipMemcpyDeviceToHost, s));
+  if (n2es) SMCF_HIP(hipMemcpyAsync(n2es, m.n2es, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  if (id_dof) std::copy(idd.begin(), idd.end(), id_dof);
+  if (coor)  // init.f90:240-244: xi = (i - 1) dxi - lxi / 2, eta = (j - 1) deta - leta / 2
+    for (int n = 0; n < nn; ++n) {
+      const double* g = &geo[16 * (size_t)npl[n]];
+      coor[2 * (size_t)n] = (nk[n] - 1) * g[9] - g[11] / 2.0;
+      coor[2 * (size_t)n + 1] = (nl[n] - 1) * g[10] - g[12] / 2.0;
+    }
+  if (luni || lmat) {
+    std::vector<double> lu((size_t)nn * nn, 0.0);  // dense luni, column-major
+    for (int n = 0; n < nn; ++n) {
+      int ip = npl[n];
+      for (int dl = -1; dl <= 1; ++dl)
+        for (int dk = -1; dk <= 1; ++dk) {
+          int k = nk[n] + dk, l = nl[n] + dl;
+          if (k < 1 || k > pnx[ip] + 1 || l < 1 || l > pne[ip] + 1) continue;
+          int j = pn0[ip] + (k - 1) + (l - 1) * (pnx[ip] + 1);
+          lu[(size_t)j * nn + n] = slots[9 * (size_t)n + (dk + 1) + 3 * (dl + 1)];
+        }
+    }
+    if (luni) std::copy(lu.begin(), lu.end(), luni);
+    if (lmat) {  // init.f90:461-473: the rows of both slip components over the DOF columns
+      size_t rows = 2 * (size_t)nn;
+      std::fill(lmat, lmat + rows * 2 * m.ndof, 0.0);
+      for (int n = 0; n < nn; ++n)
+        for (int q = 0; q < m.ndof; ++q) {
+          int j = idd[q] - 1;
+          lmat[(size_t)(2 * q) * rows + 2 * n] = lu[(size_t)j * nn + n];
+          lmat[(size_t)(2 * q + 1) * rows + 2 * n + 1] = lu[(size_t)j * nn + n];
+        }
+    }
   }
   return SMCF_OK;
 }
@@ -1053,7 +1168,7 @@ int smcf_slip_calc_grad(smcf_problem* pb, const double* theta, double gamma, int
 
 int smcf_slip_leapfrog(smcf_problem* pb, const double* theta, double gamma, int n, const double* cur,
                        const double* p0, const double* cov_diag, double dtau, int ntau, double* cand, double* p_out,
-                       double* ham_cur, double* ham_cand) {
+                       double* ham_cur, double* ham_cand, double* likelihood_cand, double* prior_cand) {
   int rc = check_component(pb, theta, n);
   if (rc) return rc;
   if (!cur || !p0 || !cov_diag || ntau < 0) return SMCF_EINVAL;
@@ -1065,10 +1180,14 @@ int smcf_slip_leapfrog(smcf_problem* pb, const double* theta, double gamma, int 
   if ((rc = upload_columns(P, cur, n, P.hm.Xt, s))) return rc;
   if ((rc = upload_columns(P, p0, n, P.hm.Pm, s))) return rc;
   TmpBufs tb;
-  double *dhc = nullptr, *dhn = nullptr;
+  double *dhc = nullptr, *dhn = nullptr, *dlk = nullptr, *dpr = nullptr;
   SMCF_HIP(tb.alloc(&dhc, (size_t)n));
   SMCF_HIP(tb.alloc(&dhn, (size_t)n));
-  CHECK_LAUNCH(launch_leapfrog_cols(P.m, P.bt, P.hm, n, gamma, dtau, ntau, P.max_slip, dhc, dhn, s));
+  SMCF_HIP(tb.alloc(&dlk, (size_t)n));
+  SMCF_HIP(tb.alloc(&dpr, (size_t)n));
+  CHECK_LAUNCH(launch_leapfrog_cols(P.m, P.bt, P.hm, n, gamma, dtau, ntau, P.max_slip, dhc, dhn, dlk, dpr, s));
+  if (likelihood_cand) SMCF_HIP(hipMemcpyAsync(likelihood_cand, dlk, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (prior_cand) SMCF_HIP(hipMemcpyAsync(prior_cand, dpr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   if (cand && (rc = download_columns(P, P.hm.Xt, n, cand, s))) return rc;
   if (p_out && (rc = download_columns(P, P.hm.Pm, n, p_out, s))) return rc;
   if (ham_cur) SMCF_HIP(hipMemcpyAsync(ham_cur, dhc, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1090,6 +1209,146 @@ int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t
   if (particles && (rc = download_columns(P, P.bt.S, P.P, particles, s))) return rc;
   if (likelihood) SMCF_HIP(hipMemcpyAsync(likelihood, P.bt.like, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));
   if (prior) SMCF_HIP(hipMemcpyAsync(prior, P.bt.prior, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+// particles (ndim x n, column per particle) -> dimension-major, leading dimension ld
+static std::vector<double> to_dim_major(const double* cols, int nd, int n, int ld) {
+  std::vector<double> t((size_t)nd * ld, 0.0);
+  for (int p = 0; p < n; ++p)
+    for (int i = 0; i < nd; ++i) t[(size_t)i * ld + p] = cols[(size_t)p * nd + i];
+  return t;
+}
+
+static int check_ell(const int* idx, int rows, int maxcol, const char* what) {
+  for (int k = 0; k < 9 * rows; ++k)
+    if (idx[k] < 1 || idx[k] > maxcol) {
+      set_error(std::string(what) + ": ELL index out of range");
+      return SMCF_EINVAL;
+    }
+  return SMCF_OK;
+}
+
+int smcf_slip_calc_gsd_gsg_gmat(smcf_problem* pb, const double* gmat, double sigma_sar2, double sigma_gnss2,
+                                double* gsgmat, double* gsdvec, double* dsd) {
+  if (!pb || !gmat) return SMCF_EINVAL;
+  Problem& P = pb->p;
+  SMCF_HIP(hipSetDevice(P.ctx->device));
+  hipStream_t s = P.ctx->stream;
+  size_t nd = P.m.ndim, nobs = P.m.nobs;
+  double scal[8] = {0, 0, 0, 0, 0, sigma_gnss2, sigma_sar2, 0};
+  SMCF_HIP(hipMemcpyAsync(P.bt.G, gmat, sizeof(double) * nobs * nd, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(P.bt.scal, scal, sizeof(scal), hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_gram(P, P.m, P.bt, 1, s, nullptr, true));
+  if (gsgmat) SMCF_HIP(hipMemcpyAsync(gsgmat, P.bt.H, sizeof(double) * nd * nd, hipMemcpyDeviceToHost, s));
+  if (gsdvec) SMCF_HIP(hipMemcpyAsync(gsdvec, P.bt.bvec, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+  if (dsd) SMCF_HIP(hipMemcpyAsync(dsd, P.bt.dsd, sizeof(double), hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_likelihood_gram(smcf_ctx* ctx, int ndim, int n, const double* particles, const double* gsgmat,
+                              const double* gsdvec, double dsd, int nsar_total, int ngnss, double log_sigma_sar2,
+                              double log_sigma_gnss2, double* out) {
+  if (!ctx || ndim < 1 || n < 1 || !particles || !gsgmat || !gsdvec || !out) return SMCF_EINVAL;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  int ld = (n + 63) / 64 * 64;
+  std::vector<double> t = to_dim_major(particles, ndim, n, ld);
+  double scal[8] = {0, 0, 0, nsar_total * log_sigma_sar2 / 2.0, 3.0 * ngnss * log_sigma_gnss2 / 2.0, 0, 0, 0};
+  int zero = 0;
+  TmpBufs tb;
+  double *dS, *dH, *db, *dd, *dsc, *dout;
+  int* dz;
+  SMCF_HIP(tb.alloc(&dS, t.size()));
+  SMCF_HIP(tb.alloc(&dH, (size_t)ndim * ndim));
+  SMCF_HIP(tb.alloc(&db, (size_t)ndim));
+  SMCF_HIP(tb.alloc(&dd, 1));
+  SMCF_HIP(tb.alloc(&dsc, 8));
+  SMCF_HIP(tb.alloc(&dout, (size_t)ld));
+  SMCF_HIP(tb.alloc(&dz, 1));
+  SMCF_HIP(hipMemcpyAsync(dS, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dH, gsgmat, sizeof(double) * ndim * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(db, gsdvec, sizeof(double) * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dd, &dsd, sizeof(double), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dsc, scal, sizeof(scal), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dz, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_like_raw(ndim, ld, dH, db, dd, dsc, dS, dz, dout, s));
+  SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_prior_ell(smcf_ctx* ctx, int nnode, int ndim, int n, const double* particles, const double* alpha2_full,
+                        const int* lmat_index, const double* lmat_val, double* out) {
+  if (!ctx || nnode < 1 || ndim < 1 || n < 1 || !particles || !alpha2_full || !lmat_index || !lmat_val || !out)
+    return SMCF_EINVAL;
+  int rc = check_ell(lmat_index, 2 * nnode, ndim, "smcf_slip_prior_ell");
+  if (rc) return rc;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  std::vector<double> t = to_dim_major(particles, ndim, n, n);
+  int nn2 = 2 * nnode;
+  TmpBufs tb;
+  double *dS, *da, *dv, *dout;
+  int* di;
+  SMCF_HIP(tb.alloc(&dS, t.size()));
+  SMCF_HIP(tb.alloc(&da, (size_t)nn2));
+  SMCF_HIP(tb.alloc(&di, 9 * (size_t)nn2));
+  SMCF_HIP(tb.alloc(&dv, 9 * (size_t)nn2));
+  SMCF_HIP(tb.alloc(&dout, (size_t)n));
+  SMCF_HIP(hipMemcpyAsync(dS, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(da, alpha2_full, sizeof(double) * nn2, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(di, lmat_index, sizeof(int) * 9 * nn2, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dv, lmat_val, sizeof(double) * 9 * nn2, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_prior_raw(nn2, n, n, dS, da, di, dv, dout, s));
+  SMCF_HIP(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipStreamSynchronize(s));
+  return SMCF_OK;
+}
+
+int smcf_slip_grad_ell(smcf_ctx* ctx, int nnode, int ndim, int n, const double* particles, double gamma,
+                       const double* gsgmat, const double* gsdvec, const double* alpha2_full, const int* lmat_index,
+                       const double* lmat_val, const int* ltmat_index, const double* ltmat_val, double* grad) {
+  if (!ctx || nnode < 1 || ndim < 1 || n < 1 || !particles || !gsgmat || !gsdvec || !alpha2_full || !lmat_index ||
+      !lmat_val || !ltmat_index || !ltmat_val || !grad)
+    return SMCF_EINVAL;
+  int nn2 = 2 * nnode;
+  int rc = check_ell(lmat_index, nn2, ndim, "smcf_slip_grad_ell");
+  if (!rc) rc = check_ell(ltmat_index, ndim, nn2, "smcf_slip_grad_ell");
+  if (rc) return rc;
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  int ld = (n + 63) / 64 * 64;
+  std::vector<double> t = to_dim_major(particles, ndim, n, ld);
+  int na[2] = {n, 1};
+  TmpBufs tb;
+  double *dS, *dH, *db, *da, *dv, *dtv, *dY, *dLS, *dG;
+  int *di, *dti, *dna;
+  SMCF_HIP(tb.alloc(&dS, t.size()));
+  SMCF_HIP(tb.alloc(&dH, (size_t)ndim * ndim));
+  SMCF_HIP(tb.alloc(&db, (size_t)ndim));
+  SMCF_HIP(tb.alloc(&da, (size_t)nn2));
+  SMCF_HIP(tb.alloc(&di, 9 * (size_t)nn2));
+  SMCF_HIP(tb.alloc(&dv, 9 * (size_t)nn2));
+  SMCF_HIP(tb.alloc(&dti, 9 * (size_t)ndim));
+  SMCF_HIP(tb.alloc(&dtv, 9 * (size_t)ndim));
+  SMCF_HIP(tb.alloc(&dY, t.size()));
+  SMCF_HIP(tb.alloc(&dLS, (size_t)nn2 * ld));
+  SMCF_HIP(tb.alloc(&dG, (size_t)ndim * n));
+  SMCF_HIP(tb.alloc(&dna, 2));
+  SMCF_HIP(hipMemcpyAsync(dS, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dH, gsgmat, sizeof(double) * ndim * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(db, gsdvec, sizeof(double) * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(da, alpha2_full, sizeof(double) * nn2, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(di, lmat_index, sizeof(int) * 9 * nn2, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dv, lmat_val, sizeof(double) * 9 * nn2, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dti, ltmat_index, sizeof(int) * 9 * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dtv, ltmat_val, sizeof(double) * 9 * ndim, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dna, na, sizeof(na), hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_grad_raw(ndim, nn2, n, ld, gamma, dH, dS, db, da, di, dv, dti, dtv, dna, dY, dLS, dG, s));
+  SMCF_HIP(hipMemcpyAsync(grad, dG, sizeof(double) * ndim * n, hipMemcpyDeviceToHost, s));
   SMCF_HIP(hipStreamSynchronize(s));
   return SMCF_OK;
 }

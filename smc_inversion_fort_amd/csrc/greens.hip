@@ -501,32 +501,22 @@ SMCF_DEV void uc0_full(double x, double y, double d, double z, const double* pot
   }
 }
 
-__global__ void k_dc3d0_full(int n, const double* __restrict__ in, double* __restrict__ uo,
-                             int* __restrict__ iret) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* a = in + 10 * (size_t)i;
-  double alpha = a[0], x = a[1], y = a[2], z = a[3], depth = a[4], dip = a[5];
-  double pot[4] = {a[6], a[7], a[8], a[9]};
-  double u[12], dua[12], dub[12], duc[12];
+// DC3D0 (DC3Dfortran.f90:2-118): all 12 outputs into u; *iret as the reference sets it (left unchanged when
+// depth < 0, where every output is zero)
+SMCF_DEV void dc3d0_dev(double alpha, double x, double y, double z, double depth, double dip, const double* pot,
+                        double* u, int* iret) {
+  double dua[12], dub[12], duc[12];
   for (int c = 0; c < 12; ++c) u[c] = 0.0;
-  double* o = uo + 12 * (size_t)i;
   double xx = x, yy = y, zz = z, dd = depth + z;
-  if (depth < 0.0) {
-    for (int c = 0; c < 12; ++c) o[c] = 0.0;
-    return;
-  }
+  if (depth < 0.0) return;
   if (fabs(xx) < 1e-6) xx = 0.0;
   if (fabs(yy) < 1e-6) yy = 0.0;
   if (fabs(dd) < 1e-6) dd = 0.0;
   int ir = 0;
   if (z > 0.0) ir = 2;
   else if (sqrt(xx * xx + yy * yy + dd * dd) == 0.0) ir = 1;
-  iret[i] = ir;
-  if (ir != 0) {
-    for (int c = 0; c < 12; ++c) o[c] = 0.0;
-    return;
-  }
+  *iret = ir;
+  if (ir != 0) return;
   double al[5] = {(1.0 - alpha) / 2.0, alpha / 2.0, (1.0 - alpha) / alpha, 1.0 - alpha, alpha};
   double sd = sin(dip * (kPI2 / 360.0)), cd = cos(dip * (kPI2 / 360.0));
   if (fabs(cd) < 1e-6) {
@@ -545,7 +535,112 @@ __global__ void k_dc3d0_full(int n, const double* __restrict__ in, double* __res
     if (c >= 9) du = du + duc[c - 9];
     u[c] = u[c] + du;
   }
+}
+
+__global__ void k_dc3d0_full(int n, const double* __restrict__ in, double* __restrict__ uo,
+                             int* __restrict__ iret) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* a = in + 10 * (size_t)i;
+  double pot[4] = {a[6], a[7], a[8], a[9]};
+  double u[12];
+  int ir = iret[i];
+  dc3d0_dev(a[0], a[1], a[2], a[3], a[4], a[5], pot, u, &ir);
+  iret[i] = ir;
+  double* o = uo + 12 * (size_t)i;
   for (int c = 0; c < 12; ++c) o[c] = u[c];
+}
+
+// call_dc3d0 (gfunc.f90:65-100): station rotated into the strike frame, alpha = 2/3, z = 0, depth = -zsource,
+// POT1 = uxi area, POT2 = ueta area; displacement rotated back and scaled by 100 (cm)
+SMCF_DEV void call_dc3d0_dev(double xs, double ys, double zs, double xo, double yo, double uxi, double ueta,
+                             double dip, double area, double strike, double* uret) {
+  const double pi = 3.141592653589793;
+  double sr = strike / 180.0 * pi;
+  double x = sin(sr) * (xo - xs) + cos(sr) * (yo - ys);
+  double y = -cos(sr) * (xo - xs) + sin(sr) * (yo - ys);
+  double pot[4] = {uxi * area, ueta * area, 0.0, 0.0};
+  double u[12];
+  int ir = 0;
+  dc3d0_dev(2.0 / 3.0, x, y, 0.0, -zs, dip, pot, u, &ir);
+  double uxr = sin(sr) * u[0] - cos(sr) * u[1];
+  double uyr = cos(sr) * u[0] + sin(sr) * u[1];
+  uret[0] = 100.0 * uxr;
+  uret[1] = 100.0 * uyr;
+  uret[2] = 100.0 * u[2];
+}
+
+__global__ void k_call_dc3d0(int n, const double* __restrict__ in, double* __restrict__ uret) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* a = in + 10 * (size_t)i;
+  call_dc3d0_dev(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], uret + 3 * (size_t)i);
+}
+
+// calc_responce (gfunc.f90:102-194) for one observation point per thread: over the target patches, the 2 x 2
+// Gauss points r = +-1/sqrt(3) with bilinearly interpolated (xi, eta, uxi, ueta), each a point source of area / 4
+__global__ void k_calc_responce(int npts, const double* __restrict__ pts, const int* __restrict__ cny,
+                                const double* __restrict__ coor, const double* __restrict__ slip, double xf, double yf,
+                                double zf, double strike, double dip, int ntarget, const int* __restrict__ target,
+                                double* __restrict__ uobs) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= npts) return;
+  const double pi = 3.141592653589793;
+  double dr = dip / 180.0 * pi, sr = strike / 180.0 * pi;
+  double rv[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
+  double xo = pts[2 * (size_t)ip], yo = pts[2 * (size_t)ip + 1];
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  for (int it = 0; it < ntarget; ++it) {
+    int ie = target[it] - 1;
+    double xn[4], en[4], uxn[4], uen[4];
+    for (int k = 0; k < 4; ++k) {
+      int jn = cny[4 * (size_t)ie + k] - 1;
+      xn[k] = coor[2 * (size_t)jn];
+      en[k] = coor[2 * (size_t)jn + 1];
+      uxn[k] = slip[2 * (size_t)jn];
+      uen[k] = slip[2 * (size_t)jn + 1];
+    }
+    double area = (xn[1] - xn[0]) * (en[2] - en[1]);
+    for (int a1 = 0; a1 < 2; ++a1) {
+      double r1 = rv[a1];
+      for (int a2 = 0; a2 < 2; ++a2) {
+        double r2 = rv[a2];
+        double nv[4] = {(1.0 - r1) * (1.0 - r2) / 4.0, (1.0 + r1) * (1.0 - r2) / 4.0, (1.0 + r1) * (1.0 + r2) / 4.0,
+                        (1.0 - r1) * (1.0 + r2) / 4.0};
+        double xi = 0.0, eta = 0.0, uxi = 0.0, ueta = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          xi = xi + nv[k] * xn[k];
+          eta = eta + nv[k] * en[k];
+          uxi = uxi + nv[k] * uxn[k];
+          ueta = ueta + nv[k] * uen[k];
+        }
+        double xs = xf - eta * cos(dr) * cos(sr) + xi * sin(sr);
+        double ys = yf + eta * cos(dr) * sin(sr) + xi * cos(sr);
+        double zs = zf + eta * sin(dr);
+        double ur[3];
+        call_dc3d0_dev(xs, ys, zs, xo, yo, uxi, ueta, dip, area / 4.0, strike, ur);
+        u0 = u0 + ur[0];
+        u1 = u1 + ur[1];
+        u2 = u2 + ur[2];
+      }
+    }
+  }
+  uobs[3 * (size_t)ip] = u0;
+  uobs[3 * (size_t)ip + 1] = u1;
+  uobs[3 * (size_t)ip + 2] = u2;
+}
+
+hipError_t launch_call_dc3d0(int n, const double* in, double* uret, hipStream_t s) {
+  hipLaunchKernelGGL(k_call_dc3d0, dim3((n + 127) / 128), dim3(128), 0, s, n, in, uret);
+  return hipGetLastError();
+}
+
+hipError_t launch_calc_responce(int npts, const double* pts, const int* cny, const double* coor, const double* slip,
+                                double xf, double yf, double zf, double strike, double dip, int ntarget,
+                                const int* target, double* uobs, hipStream_t s) {
+  hipLaunchKernelGGL(k_calc_responce, dim3((npts + 63) / 64), dim3(64), 0, s, npts, pts, cny, coor, slip, xf, yf, zf,
+                     strike, dip, ntarget, target, uobs);
+  return hipGetLastError();
 }
 
 hipError_t launch_dc3d0_batch(int n, const double* in, double* u, int* iret, hipStream_t s) {

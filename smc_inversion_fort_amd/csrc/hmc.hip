@@ -522,6 +522,64 @@ hipError_t launch_grad_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, do
   return hipGetLastError();
 }
 
+// Raw-array forms (the reference's argument lists): particles dimension-major with leading dimension ld.
+// prior  sum_r (L x)_r^2 / (2 alpha2_r)  (slip_calc_prior, smc_slip.f90:106-137)
+__global__ void k_prior_raw(int nn2, int n, int ld, const double* __restrict__ X, const double* __restrict__ a2,
+                            const int* __restrict__ li, const double* __restrict__ lv, double* __restrict__ out) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  double pr = 0.0;
+  for (int r = 0; r < nn2; ++r) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + lv[9 * r + q] * X[(size_t)(li[9 * r + q] - 1) * ld + col];
+    pr = pr + (tt * tt) / (2.0 * a2[r]);
+  }
+  out[col] = pr;
+}
+
+// gradient gamma (Y - b) + L^T (L x / alpha2) with Y = H x formed (slip_calc_grad, smc_slip.f90:139-182); LS is
+// nn2 x ld scratch; G ndim x n, column per particle
+__global__ void k_grad_raw(int nd, int nn2, int n, int ld, double gamma, const double* __restrict__ X,
+                           const double* __restrict__ Y, const double* __restrict__ bv, const double* __restrict__ a2,
+                           const int* __restrict__ li, const double* __restrict__ lv, const int* __restrict__ lti,
+                           const double* __restrict__ ltv, double* __restrict__ LS, double* __restrict__ G) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  for (int r = 0; r < nn2; ++r) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + lv[9 * r + q] * X[(size_t)(li[9 * r + q] - 1) * ld + col];
+    LS[(size_t)r * ld + col] = tt / a2[r];
+  }
+  for (int d = 0; d < nd; ++d) {
+    double g1 = Y[(size_t)d * ld + col];
+    g1 = g1 - bv[d];
+    g1 = g1 * gamma;
+    double g2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g2 = g2 + ltv[9 * d + q] * LS[(size_t)(lti[9 * d + q] - 1) * ld + col];
+    G[(size_t)col * nd + d] = g1 + g2;
+  }
+}
+
+hipError_t launch_prior_raw(int nn2, int n, int ld, const double* X, const double* a2, const int* li, const double* lv,
+                            double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_prior_raw, dim3((n + 127) / 128), dim3(128), 0, s, nn2, n, ld, X, a2, li, lv, out);
+  return hipGetLastError();
+}
+
+// ncol_active: device ints {n, 1} (column count and active flag of the single instance)
+hipError_t launch_grad_raw(int nd, int nn2, int n, int ld, double gamma, const double* H, const double* X,
+                           const double* bv, const double* a2, const int* li, const double* lv, const int* lti,
+                           const double* ltv, const int* ncol_active, double* Y, double* LS, double* G,
+                           hipStream_t s) {
+  launch_hs(nd, ld, 1, H, X, ncol_active, ncol_active + 1, Y, s);
+  hipLaunchKernelGGL(k_grad_raw, dim3((n + 127) / 128), dim3(128), 0, s, nd, nn2, n, ld, gamma, X, Y, bv, a2, li, lv,
+                     lti, ltv, LS, G);
+  return hipGetLastError();
+}
+
 // trajectory set-up of n independent columns (column m is its own parent and child): H(cur) from the
 // likelihood and prior of Xt (Y = H Xt formed) and the momenta in Pm
 __global__ void k_leapfrog_begin(MeshDev m, int P, int n, double gamma, double dtau, BatchDev bt, HmcDev h) {
@@ -561,12 +619,14 @@ __global__ void k_leapfrog_begin(MeshDev m, int P, int n, double gamma, double d
 // final Hamiltonian of the n trajectories (Y = H Xt formed): ham_cand (1e20 after an escape), the candidate and
 // the momenta
 __global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev bt, HmcDev h,
-                               double* __restrict__ hn_out) {
+                               double* __restrict__ hn_out, double* __restrict__ lk_out, double* __restrict__ pr_out) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= n) return;
   int nd = m.ndim, nn2 = 2 * m.nnode;
-  if (h.dead[col]) {
+  if (h.dead[col]) {  // escaped: the candidate is the start point (smc_slip.f90:776-782)
     hn_out[col] = 1e20;
+    lk_out[col] = h.Lc[col];
+    pr_out[col] = h.Rc[col];
     return;
   }
   const double* sc = bt.scal;
@@ -590,10 +650,13 @@ __global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev b
     hn = hn + p * bt.cov[d] * p / 2.0;
   }
   hn_out[col] = hn;
+  lk_out[col] = lk;
+  pr_out[col] = pr;
 }
 
 hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double dtau, int ntau,
-                                double max_slip, double* hc_out, double* hn_out, hipStream_t s) {
+                                double max_slip, double* hc_out, double* hn_out, double* lk_out, double* pr_out,
+                                hipStream_t s) {
   int P = bt.P, nd = m.ndim;
   dim3 gc((n + 127) / 128);
   launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
@@ -603,7 +666,7 @@ hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n
     hipLaunchKernelGGL(k_hmc_step, dim3((P + 127) / 128, 1), dim3(128), 0, s, m, P, st, ntau, max_slip, bt, h, 0);
   }
   launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
-  hipLaunchKernelGGL(k_leapfrog_end, gc, dim3(128), 0, s, m, P, n, gamma, bt, h, hn_out);
+  hipLaunchKernelGGL(k_leapfrog_end, gc, dim3(128), 0, s, m, P, n, gamma, bt, h, hn_out, lk_out, pr_out);
   hipError_t e = hipMemcpyAsync(hc_out, h.hcur, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return e;
   return hipGetLastError();

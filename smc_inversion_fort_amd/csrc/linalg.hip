@@ -202,8 +202,9 @@ __global__ void k_gram_vec(MeshDev m, const double* __restrict__ Gall, const dou
   }
 }
 
-hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
-  hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B, sel);
+hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel,
+                       bool scal_given) {
+  if (!scal_given) hipLaunchKernelGGL(k_scal, dim3((B + 63) / 64), dim3(64), 0, s, m, bt.theta, bt.scal, B, sel);
   for (int p = 0; p < m.npath && m.nsar > 0; ++p) {
     int n = P.sar_n[p];
     if (n == 0) continue;

@@ -499,6 +499,14 @@ __global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
             (pb ? L1 : L0) + (size_t)b * P);
 }
 
+hipError_t launch_like_raw(int nd, int P, const double* H, const double* bv, const double* dsd, const double* scal,
+                           const double* S, const int* zero, double* out, hipStream_t s) {
+  int chunks = (P + 255) / 256;
+  hipLaunchKernelGGL(k_like_mfma, dim3(8 * chunks), dim3(256), 0, s, nd, P, 1, H, bv, dsd, scal, S, S, zero, zero, 0,
+                     (const int*)nullptr, out, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel) {
   int chunks = (bt.P + 255) / 256;
   int groups = (B + 7) / 8;

@@ -164,9 +164,10 @@ struct StageCfg {
 hipError_t launch_geometry(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_greens(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
-// Gram quantities H, b, dsd (InSAR paths need the host copies of the path layout in P)
+// Gram quantities H, b, dsd (InSAR paths need the host copies of the path layout in P); with scal_given the
+// sigmas already in bt.scal are used instead of those of theta
 hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s,
-                       const int* sel = nullptr);
+                       const int* sel = nullptr, bool scal_given = false);
 hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s,
                              const int* sel = nullptr);
 // likelihood of the current particles (matrix cores; Gram form)
@@ -212,11 +213,25 @@ hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, do
 // one instance in slot 0: the tempered gradient of columns 0 .. n-1 of h.Xt into G (ndim x n, column per
 // particle), and n independent leapfrog trajectories from (h.Xt, h.Pm) with bt.cov as the mass diagonal
 // (H(cur) into hc_out, H(cand) into hn_out, both device arrays)
+hipError_t launch_prior_raw(int nn2, int n, int ld, const double* X, const double* a2, const int* li, const double* lv,
+                            double* out, hipStream_t s);
+hipError_t launch_grad_raw(int nd, int nn2, int n, int ld, double gamma, const double* H, const double* X,
+                           const double* bv, const double* a2, const int* li, const double* lv, const int* lti,
+                           const double* ltv, const int* ncol_active, double* Y, double* LS, double* G,
+                           hipStream_t s);
+// Gram-form likelihood of one instance's particles from raw arrays (k_like_mfma)
+hipError_t launch_like_raw(int nd, int P, const double* H, const double* bv, const double* dsd, const double* scal,
+                           const double* S, const int* zero, double* out, hipStream_t s);
 hipError_t launch_grad_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double* G, hipStream_t s);
 hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n, double gamma, double dtau, int ntau,
-                                double max_slip, double* hc_out, double* hn_out, hipStream_t s);
+                                double max_slip, double* hc_out, double* hn_out, double* lk_out, double* pr_out,
+                                hipStream_t s);
 
 hipError_t launch_dc3d0_batch(int n, const double* in, double* u, int* iret, hipStream_t s);
+hipError_t launch_call_dc3d0(int n, const double* in, double* uret, hipStream_t s);
+hipError_t launch_calc_responce(int npts, const double* pts, const int* cny, const double* coor, const double* slip,
+                                double xf, double yf, double zf, double strike, double dip, int ntarget,
+                                const int* target, double* uobs, hipStream_t s);
 hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,
                                   double* out3, double* w, hipStream_t s);
 hipError_t launch_normalize_resample(int P, const double* w, double uh, double ul, double* wn,

@@ -128,3 +128,79 @@ def test_moments_vs_oracle(setup):
     np.testing.assert_allclose(m, mo, rtol=1e-15, atol=0)
     np.testing.assert_allclose(c, co, rtol=1e-14, atol=0)
     assert c[7] == 1e-3
+
+
+def _plane_frame(theta, xmin, xmax, zmin):
+    """get_geometry + calc_greens_func's frame of a single plane (init.f90:121-173, gfunc.f90:283-296)."""
+    xl, yl, xr, yr, dip = xmin, theta[0], xmax, theta[1], theta[2]
+    dr = dip * math.pi / 180.0
+    sr = math.atan2(xr - xl, yr - yl)
+    xf = (xl + xr) / 2.0 - zmin * math.cos(sr) / (2.0 * math.tan(dr))
+    yf = (yl + yr) / 2.0 + zmin * math.sin(sr) / (2.0 * math.tan(dr))
+    return xf, yf, zmin / 2.0, sr / math.pi * 180.0, dip
+
+
+def test_call_dc3d0_vs_oracle(setup):
+    say("case: call_dc3d0 vs oracle")
+    import ctypes as C
+    L = O.lib()
+    L.orc_call_dc3d0.argtypes = [C.c_double] * 10 + [C.POINTER(C.c_double)]
+    rng = np.random.default_rng(8)
+    rows = np.column_stack([rng.uniform(-20, 20, 40), rng.uniform(-20, 20, 40), rng.uniform(-15, -0.5, 40),
+                            rng.uniform(-50, 50, 40), rng.uniform(-50, 50, 40), rng.normal(size=40),
+                            rng.normal(size=40), rng.uniform(10, 89, 40), rng.uniform(1, 20, 40),
+                            rng.uniform(0, 360, 40)])
+    u = setup["ctx"].call_dc3d0(rows)
+    for k in range(rows.shape[0]):
+        out = (C.c_double * 3)()
+        L.orc_call_dc3d0(*[float(v) for v in rows[k]], out)
+        assert np.abs(u[k] - np.array(out[:])).max() <= 1e-12 * max(1e-30, np.abs(out[:]).max())
+
+
+def test_calc_responce_reproduces_greens_columns(setup):
+    """calc_responce_dist of the unit slip at each DOF (gen_unit_slip: strike -1, dip +1) projected on the
+    line-of-sight vectors is that DOF's column of calc_greens_func (gfunc.f90:300-330)."""
+    say("case: calc_responce vs calc_greens_func columns")
+    d = setup
+    pb, th, obs = d["pb"], d["theta"], d["obs"]
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    mesh = pb.discretize_fault(th)
+    xf, yf, zf, strike, dip = _plane_frame(th, xmin, xmax, zmin)
+    G = pb.calc_greens_func(th)
+    pts = obs["obs_points"][:, ::3]  # one point per GNSS station
+    for col in range(0, G.shape[1], 3):
+        idof, direction = col // 2, col % 2
+        node = mesh["id_dof"][idof]
+        slip = np.zeros((2, pb.nnode), order="F")
+        slip[direction, node - 1] = -1.0 if direction == 0 else 1.0
+        targets = mesh["n2e"][:mesh["n2es"][node - 1], node - 1]
+        u = d["ctx"].calc_responce(pts, mesh["cny"], mesh["coor"], slip, xf, yf, zf, strike, dip, targets)
+        g = np.zeros(G.shape[0])
+        for i in range(pts.shape[1]):
+            for c in range(3):
+                r = 3 * i + c
+                g[r] = np.dot(u[:, i], obs["obs_unitvec"][:, r])
+        assert np.abs(g - G[:, col]).max() <= 1e-10 * np.abs(G[:, col]).max(), col
+
+
+def test_raw_array_forms_vs_oracle(setup):
+    say("case: likelihood / prior / grad from raw arrays, Gram from a given G")
+    d = setup
+    f, th, nd = d["f"], d["theta"], d["f"].ndim
+    ctx, lap = d["ctx"], d["lap"]
+    X = np.asfortranarray(np.random.default_rng(2).uniform(0, 3.0, size=(nd, 9)))
+    Ld = ctx.slip_likelihood_gram(X, d["H"], d["b"], d["dsd"], 0, d["obs"]["ngnss"], th[4], th[5])
+    Rd = ctx.slip_prior_ell(X, d["alpha2"], lap["li"], lap["lv"])
+    Gd = ctx.slip_grad_ell(X, 0.7, d["H"], d["b"], d["alpha2"], lap["li"], lap["lv"], lap["lti"], lap["ltv"])
+    for k in range(X.shape[1]):
+        Lo = O.likelihood(X[:, k], d["H"], d["b"], d["dsd"], 0, d["obs"]["ngnss"], th[4], th[5])
+        Ro = O.prior(X[:, k], d["alpha2"], lap["li"], lap["lv"], f.nnode)
+        go = O.grad(X[:, k], f.nnode, f.ndof, lap["li"], lap["lv"], lap["lti"], lap["ltv"], 0.7, d["alpha2"], d["b"],
+                    d["H"])
+        assert Ld[k] == pytest.approx(Lo, rel=1e-12)
+        assert Rd[k] == pytest.approx(Ro, rel=1e-13)
+        assert np.abs(Gd[:, k] - go).max() <= 1e-12 * np.abs(go).max()
+    H, b, dsd = d["pb"].slip_calc_gsd_gsg_gmat(d["G"], math.exp(th[4]), math.exp(th[5]))
+    assert np.abs(H - d["H"]).max() <= 1e-12 * np.abs(d["H"]).max()
+    assert np.abs(b - d["b"]).max() <= 1e-12 * np.abs(d["b"]).max()
+    assert dsd == pytest.approx(d["dsd"], rel=1e-13)
