@@ -309,6 +309,10 @@ int smcf_server_submit(smcf_server* s, const smcf_problem_desc* desc, int max_ba
 int smcf_server_wait(smcf_server* s, int64_t id, double* particles, double* likelihood, int* nstage,
                      double* evidence, double* gamma_trace, double* times);
 int smcf_server_status(smcf_server* s, int* queued, int* running, size_t* bytes_in_use);
+/* At most max_running requests run at once (1 .. 16; default GPU_MAX_HW_QUEUES / 2, each request using two HIP
+ * streams); the others wait in arrival order.  Admission also counts a fixed 64 MiB per request beside its
+ * smcf_problem_bytes. */
+int smcf_server_set_max_running(smcf_server* s, int max_running);
 int smcf_server_destroy(smcf_server* s);
 
 /* sort.cpp:3-7 - 1-based index sorted by descending arr(index) (std::sort, same tie order). */

@@ -32,6 +32,8 @@ PRECISION_FP64 = 0   # every operation in FP64
 PRECISION_MIXED = 1  # Gibbs Q.S products in FP32 on the matrix cores, single-precision draws; means/weights FP64
 PRECISION_MIXED_BF16 = 2  # as PRECISION_MIXED with the Q.S products from split-BF16 operands (hi.hi + hi.lo + lo.hi)
 
+REQUEST_OVERHEAD_BYTES = 64 << 20  # serving admission: fixed device bytes counted per request beside its problem
+
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 _U64 = C.POINTER(C.c_uint64)
@@ -67,7 +69,7 @@ EXPORTS = [
     "smcf_fault_normalize_weights", "smcf_fault_calc_mean_particles", "smcf_fault_calc_cov_particles",
     "smcf_fault_resample_particles", "smcf_fault_reorder_to_send", "smcf_work_mcmc_sampling", "smcf_call_dc3d0",
     "smcf_calc_responce", "smcf_slip_likelihood_gram", "smcf_slip_prior_ell", "smcf_slip_grad_ell",
-    "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault",
+    "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault", "smcf_server_set_max_running",
 ]
 
 
@@ -121,6 +123,7 @@ def _declare(L):
     L.smcf_server_wait.argtypes = [C.c_void_p, C.c_int64, _D, _D, _I, C.POINTER(C.c_double), _D, _D]
     L.smcf_server_status.argtypes = [C.c_void_p, _I, _I, C.POINTER(C.c_size_t)]
     L.smcf_server_destroy.argtypes = [C.c_void_p]
+    L.smcf_server_set_max_running.argtypes = [C.c_void_p, C.c_int]
     L.smcf_slip_mean_batch.argtypes = [C.c_void_p, C.c_int, _D, _U64, _D, _D]
     L.smcf_batch_times.argtypes = [C.c_void_p, C.c_int, _D]
     L.smcf_batch_admit_times.argtypes = [C.c_void_p, C.c_int, _D]
@@ -473,6 +476,10 @@ class Server:
                                       _dp(times)))
         return dict(particles=X, likelihood=L, nstage=ns.value, evidence=ev.value, gamma=gt[:ns.value].copy(),
                     t_submit=times[0], t_start=times[1], t_done=times[2])
+
+    def set_max_running(self, n):
+        """At most n requests (1 .. 16) run at once."""
+        _check(lib().smcf_server_set_max_running(self.h, int(n)))
 
     def status(self):
         q, r, b = C.c_int(), C.c_int(), C.c_size_t()

@@ -8,10 +8,17 @@
 // arrival order), and their kernels overlap on the device with those of the requests already running.
 // Every random number is keyed by the request's seed, so a request's result does not depend on what else
 // runs beside it.
+//
+// Admission needs both the device bytes (smcf_problem_bytes plus a fixed per-request overhead for the context,
+// its streams and the runtime's per-stream state) and a free running place: at most max_running requests run at
+// once.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default, at most 32 accepted
+// here) and each request uses two streams, so the default is GPU_MAX_HW_QUEUES / 2, never more than 16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -27,6 +34,18 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+constexpr size_t kRequestOverheadBytes = (size_t)64 << 20;  // context, two streams, runtime state per request
+constexpr int kMaxRunningCap = 16;
+
+int default_max_running() {
+  int q = 4;  // HIP's default number of hardware queues per process
+  if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) {
+    int v = std::atoi(e);
+    if (v > 0) q = std::min(v, 32);
+  }
+  return std::max(1, std::min(kMaxRunningCap, q / 2));
+}
 
 // A request owns copies of everything it was submitted with (the caller may free its arrays at once).
 struct Request {
@@ -64,6 +83,7 @@ struct smcf_server {
   std::map<int64_t, std::shared_ptr<Request>> all;
   size_t in_use = 0;
   int running = 0;
+  int max_running = 1;
   int64_t next_id = 1;
   bool stopping = false;
   std::thread admitter;
@@ -104,7 +124,7 @@ struct smcf_server {
       bool admitted = false;
       for (auto it = queue.begin(); it != queue.end();) {
         auto r = *it;
-        if (in_use + r->bytes <= budget) {
+        if (running < max_running && in_use + r->bytes <= budget) {
           in_use += r->bytes;
           ++running;
           r->t_start = now();
@@ -134,6 +154,7 @@ int smcf_server_create(int device, size_t hbm_budget_bytes, smcf_server** out) {
   auto* s = new smcf_server();
   s->device = device;
   s->budget = hbm_budget_bytes;
+  s->max_running = default_max_running();
   s->t0 = Clock::now();
   s->admitter = std::thread(&smcf_server::admit_loop, s);
   *out = s;
@@ -181,6 +202,7 @@ int smcf_server_submit(smcf_server* s, const smcf_problem_desc* d, int max_batch
   r->max_batch = max_batch;
   int rc = smcf_problem_bytes(&r->desc, max_batch, &r->bytes);
   if (rc) return rc;
+  r->bytes += kRequestOverheadBytes;
   std::lock_guard<std::mutex> lk(s->mu);
   if (r->bytes > s->budget) {
     smcf::set_error("request needs " + std::to_string(r->bytes) + " device bytes, more than the budget " +
@@ -225,6 +247,17 @@ int smcf_server_wait(smcf_server* s, int64_t id, double* particles, double* like
   if (nstage) *nstage = r->nstage;
   if (evidence) *evidence = r->evidence;
   if (gamma_trace) std::memcpy(gamma_trace, r->gamma_trace.data(), sizeof(double) * r->gamma_trace.size());
+  return SMCF_OK;
+}
+
+int smcf_server_set_max_running(smcf_server* s, int max_running) {
+  if (!s || max_running < 1 || max_running > kMaxRunningCap) {
+    smcf::set_error("max_running must be 1 .. 16");
+    return SMCF_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->max_running = max_running;
+  s->cv.notify_all();
   return SMCF_OK;
 }
 
