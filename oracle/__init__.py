@@ -44,7 +44,7 @@ class Problem(C.Structure):
                 ("nsar_index", I), ("obs_points", D), ("obs_unitvec", D), ("obs_sigma", D),
                 ("dvec", D), ("sar_sinv", D), ("fault", FaultDesc), ("max_slip", C.c_double),
                 ("nparticle_slip", C.c_int), ("mover", C.c_int), ("gibbs_sweeps", C.c_int),
-                ("hmc_ntau", C.c_int), ("hmc_accept", C.c_double), ("stage_cap", C.c_int)]
+                ("hmc_ntau", C.c_int), ("hmc_accept", C.c_double), ("stage_cap", C.c_int), ("test_like", D)]
 
 
 class SlipResult(C.Structure):
@@ -240,8 +240,11 @@ class ProblemHolder:
     """Problem description for the end-to-end oracle (GNSS rows and optional SAR paths)."""
 
     def __init__(self, fault, obs, max_slip, nparticle_slip, mover, gibbs_sweeps=5, hmc_ntau=10, hmc_accept=0.65,
-                 stage_cap=0):
+                 stage_cap=0, test_like=None):
+        """test_like = (m, s) replaces the fault likelihood by 0.5 sum ((theta - m) / s)^2 (a test hook for
+        the outer SMC's composition; no inner SMC runs)."""
         self.fault = fault
+        self.test_like = None if test_like is None else f64(np.concatenate([test_like[0], test_like[1]]))
         self.obs_points = f64(obs["obs_points"])
         self.obs_unitvec = f64(obs["obs_unitvec"])
         self.obs_sigma = f64(obs["obs_sigma"])
@@ -253,7 +256,8 @@ class ProblemHolder:
                           len(self.nsar_index) - 1, ip(self.nsar_index), dp(self.obs_points),
                           dp(self.obs_unitvec), dp(self.obs_sigma), dp(self.dvec), dp(self.sar_sinv),
                           fault.desc, float(max_slip), int(nparticle_slip), int(mover), int(gibbs_sweeps),
-                          int(hmc_ntau), float(hmc_accept), int(stage_cap))
+                          int(hmc_ntau), float(hmc_accept), int(stage_cap),
+                          None if self.test_like is None else dp(self.test_like))
 
     def slip_smc(self, theta, gmat, key):
         f = self.fault

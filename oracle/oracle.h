@@ -128,6 +128,9 @@ typedef struct {
     int hmc_ntau;              /* leapfrog steps of mover 2 */
     double hmc_accept;         /* target mean acceptance probability of mover 2 */
     int stage_cap;             /* test hook: stop after this many stages (0: run to gamma = 1) */
+    const double* test_like;   /* test hook: when set, the fault likelihood is the Gaussian
+                                  0.5 sum_d ((theta_d - m_d) / s_d)^2 with m = test_like[0..nd) and
+                                  s = test_like[nd..2 nd), and no inner SMC runs */
 } orc_problem;
 
 /* Step-size adaptation of mover 2 (no reference counterpart): the stage's children are moved with
