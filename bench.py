@@ -75,6 +75,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--budget-s", type=float, default=480.0,
                     help="wall-clock budget of the whole run: a leg whose estimate does not fit is skipped")
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+                    help="c2: the headline (16x16 patches, Gibbs x5, FP64 and mixed forms); c3: 32x16 patches, "
+                         "3000 observations, HMC with 10 leapfrog steps tuned to 0.65 acceptance, FP64")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
@@ -85,15 +88,16 @@ def elapsed():
     return time.perf_counter() - T0
 
 
-def traffic_from_profiles(form):
-    """HBM bytes per launch of the form's Gibbs kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary committed for it (null when there is none)."""
+def traffic_from_profiles(form, instances_per_launch):
+    """HBM bytes per launch of the form's Gibbs kernel: the per-instance bytes of the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed for it (total bytes over the profiled run's instance-launches) times this run's
+    mean instances per launch; null when there is no summary."""
     path = os.path.join(ROOT, "profiles", "gibbs_c2_traffic_%s.txt" % form)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
+        if line.startswith("hbm_bytes_per_instance_launch:"):
+            return float("%.4g" % (float(line.split(":", 1)[1]) * instances_per_launch))
     return None
 
 
@@ -114,19 +118,26 @@ def stream_plan(a):
     return W, K, E, Ew, tail, W * Ew + K * E + tail
 
 
+C3_NTAU, C3_ACCEPT = 10, 0.65
+
+
 def run_form(a, S, syn, obs, precision):
     """One arithmetic form: one streamed batch call, cut into warm-up and timed steps by completion order."""
-    cfg = syn.CONFIGS["c2"]
+    cfg = syn.CONFIGS[a.workload]
     xmin, xmax, zmin, _ = syn.fault_frame()
     name = FORM_NAMES[precision]
     W, K, E, Ew, tail, total = stream_plan(a)
     ctx = S.Context(0)
-    pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
-                   mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots, precision=precision)
+    if a.workload == "c2":
+        pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
+                       mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots, precision=precision)
+    else:
+        pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
+                       mover=S.MOVER_HMC_ACCEPT, hmc_ntau=C3_NTAU, hmc_accept=C3_ACCEPT, max_batch=a.slots)
     ndim, P, sweeps = pb.ndim, cfg["P"], cfg["sweeps"]
-    print("bench[%s]: c2 16x16 patches ndim=%d nobs=%d P=%d; streaming %d geometries through %d slots "
-          "(warm-up %d x %d, timed %d x %d, tail %d)" % (name, ndim, pb.nobs, P, total, a.slots, W, Ew, K, E, tail),
-          flush=True)
+    print("bench[%s]: %s %dx%d patches ndim=%d nobs=%d P=%d; streaming %d geometries through %d slots "
+          "(warm-up %d x %d, timed %d x %d, tail %d)" % (name, a.workload, cfg["nxi"], cfg["neta"], ndim, pb.nobs, P,
+                                                          total, a.slots, W, Ew, K, E, tail), flush=True)
     th = syn.sample_thetas(total, a.seed)
     keys = keys_for(a.seed, total)
     ctx.set_profiling(True)
@@ -135,7 +146,7 @@ def run_form(a, S, syn, obs, precision):
     wall = time.perf_counter() - t0
     t_done = pb.batch_times(total)
     t_adm = pb.batch_admit_times(total)
-    n_gibbs, ms_gibbs = ctx.kernel_stats("gibbs")
+    n_gibbs, ms_gibbs = ctx.kernel_stats("gibbs" if a.workload == "c2" else "hmc")
     order = np.argsort(t_done, kind="stable")
     c = t_done[order]
     n_warm = W * Ew
@@ -148,9 +159,19 @@ def run_form(a, S, syn, obs, precision):
     timed_s = float(ends[-1] - t_begin)
     timed = order[n_warm:n_warm + K * E]
     lat = t_done[timed] - t_adm[timed]
+    # moves per geometry: one per stage, except the 201st stage of a geometry that reaches the reference's
+    # 200-stage limit (it returns the sentinel without moving)
+    moves = float((ns - (nl >= 1e10)).sum())
     # algorithmic flops of the Gibbs kernel over the whole call: per particle-sweep 2*ndim^2 (the products Q S)
-    flops = float(ns.sum()) * P * sweeps * 2.0 * ndim * ndim
-    if name == "fp64":
+    flops = moves * P * sweeps * 2.0 * ndim * ndim
+    if a.workload == "c3":
+        # the HMC stage (every kernel of one stage's trajectories): per child ntau + 2 products Y = H x
+        # (initial gradient, ntau kicks, final likelihood) of 2*ndim^2 flop each
+        flops = moves * P * (C3_NTAU + 2) * 2.0 * ndim * ndim
+        peak, kernel = (FP64_PEAK_TFLOPS, "HMC stage (k_hmc_leap: Q_g X on the FP64 matrix cores with the kick and "
+                                          "drift in its epilogue, k_hs_work: H X, + FP64 VALU wave kernels; timed "
+                                          "per stage)")
+    elif name == "fp64":
         peak, kernel = FP64_PEAK_TFLOPS, "k_gibbs (FP64 matrix cores + FP64 VALU draws)"
     elif name == "mixed":
         peak, kernel = (FP32_MFMA_PEAK_TFLOPS,
@@ -162,12 +183,13 @@ def run_form(a, S, syn, obs, precision):
                         "k_gibbs_bf (split-BF16 matrix cores, 3 products per term, + FP32 VALU draws; "
                         "achieved counts the 3 BF16 products; peak = dense BF16 matrix)")
     achieved = flops / (ms_gibbs * 1e-3) / 1e12 if ms_gibbs > 0 else 0.0
-    traffic = traffic_from_profiles(name)
+    inst_per_launch = moves / max(n_gibbs, 1)
+    traffic = traffic_from_profiles(name, inst_per_launch) if a.workload == "c2" else None
     avg_ms = ms_gibbs / max(n_gibbs, 1)
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": kernel,
             "launches": n_gibbs, "avg_launch_ms": round(avg_ms, 4),
-            "mean_instances_per_launch": round(float(ns.sum()) / max(n_gibbs, 1), 1),
+            "mean_instances_per_launch": round(inst_per_launch, 1),
             # the measured HBM bytes per launch (rocprofv3, profiles/) over this run's average launch time
             "hbm_tb_per_s": round(traffic / (avg_ms * 1e-3) / 1e12, 3) if traffic and avg_ms > 0 else None}
     out = dict(value=K * E / timed_s, ms_per_step=1e3 * timed_s / K, step_s=step_s, stages=ns, neglog=nl,
@@ -312,7 +334,6 @@ def main():
         del x, y
         print("copy probe: filled and copied 1 GiB", flush=True)
 
-    cfg = syn.CONFIGS["c2"]
     xmin, xmax, zmin, _ = syn.fault_frame()
     gctx = S.Context(0)
 
@@ -324,8 +345,11 @@ def main():
         g = p.calc_greens_func(th)
         p.close()
         return g
-    obs = syn.make_observations("c2", a.seed, greens)
+    obs = syn.make_observations(a.workload, a.seed, greens)
     gctx.close()
+    if a.workload == "c3":
+        # the HMC mover has one (FP64) form; the CPU baseline and the serving leg belong to the headline
+        a.precision, a.no_cpu, a.serve_seconds = ["fp64"], True, 0.0
 
     forms = []
     for f in a.precision:
@@ -358,15 +382,21 @@ def main():
             skipped.append("serving")
 
     W, K, E, Ew, tail, total = stream_plan(a)
-    out = {"metric": METRIC, "value": round(head["value"], 4), "unit": "outer particles/s", "n_gpus": 1,
+    metric = METRIC if a.workload == "c2" else ("outer-SMC particles/s at 32x16 patches, HMC (10 leapfrog steps, "
+                                                "0.65 acceptance), FP64")
+    out = {"metric": metric, "value": round(head["value"], 4), "unit": "outer particles/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(head["ms_per_step"], 2),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": {"fp64": "f64", "mixed": "mixed (fp32 matrix + fp32 draws; fp64 means/likelihood)",
                      "bf16": "mixed (split-bf16 matrix + fp32 draws; fp64 means/likelihood)"}[forms[0]],
-           "data": "synthetic (seeded GNSS stations, Gaussian slip bump, 5 mm noise; thetas from the prior)",
-           "config": {"workload": "c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, truncated-MVN "
-                                  "Gibbs x5 per stage; one outer-SMC wave stream of %d geometries through %d "
-                                  "device slots, step = %d completed evaluations" % (total, a.slots, E),
+           "data": "synthetic (seeded GNSS stations, Gaussian slip bump, %d mm noise; thetas from the prior)"
+                   % (5 if a.workload == "c2" else 3),
+           "config": {"workload": ("c2: 16x16 patches, 400 GNSS stations (1200 obs), inner P=2000, truncated-MVN "
+                                   "Gibbs x5 per stage" if a.workload == "c2" else
+                                   "c3: 32x16 patches, 1000 GNSS stations (3000 obs, 3 mm), inner P=2000, HMC with "
+                                   "%d leapfrog steps, step size adapted toward %.2f acceptance" % (C3_NTAU, C3_ACCEPT))
+                                  + "; one outer-SMC wave stream of %d geometries through %d device slots, step = %d "
+                                    "completed evaluations" % (total, a.slots, E),
                       "form": forms[0], "evals_per_step": E, "warmup_evals_per_step": Ew, "tail_evals": tail,
                       "slots": a.slots, "ndim_slip": head["ndim"], "nobs": head["nobs"],
                       "inner_particles": head["P"],

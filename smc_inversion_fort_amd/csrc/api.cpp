@@ -262,7 +262,6 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
     return code;
   };
   StageCfg cfg{P.max_slip, bt.P, P.sweeps};
-  std::vector<int> amax(2 * (size_t)W);
   while (busy > 0 || adm.live) {
     if (busy > 0) {
       {
@@ -272,7 +271,7 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
       if ((want_moments || hmc) && launch_moments(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
       if (hmc) {
         ProfScope ps(ctx, "hmc");
-        if (launch_hmc_stage(m, bt, P.hm, W, P.max_slip, hcfg, s, amax.data()) != hipSuccess)
+        if (launch_hmc_stage(m, bt, P.hm, P.hh, W, P.max_slip, hcfg, s) != hipSuccess)
           return fail(SMCF_EHIP);
       } else {
         if (launch_gather(m, bt, W, s) != hipSuccess || launch_form_q(m, bt, W, s) != hipSuccess)
@@ -694,6 +693,13 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
     size_t nn2 = 2 * (size_t)nnode;
     dalloc(P, &h.Xc, nd * Pn * C);
     dalloc(P, &h.Xt, nd * Pn * C);
+    dalloc(P, &h.Xt2, nd * Pn * C);
+    dalloc(P, &h.Qg, nd * nd * C);
+    dalloc(P, &h.accf, Pn * C);
+    dalloc(P, &h.wla, 2 * hmc_work_entries((int)C, (int)Pn));
+    dalloc(P, &h.wlw, 2 * hmc_work_entries((int)C, (int)Pn));
+    dalloc(P, &h.wln, 2 * hmc_work_entries((int)C, (int)Pn));
+    dalloc(P, &h.part, (size_t)32 * 6 * Pn * C);
     dalloc(P, &h.Pm, nd * Pn * C);
     dalloc(P, &h.LS, nn2 * Pn * C);
     dalloc(P, &h.Lc, Pn * C);

@@ -3,12 +3,16 @@
 // The children of one parent form a sequential Metropolis chain; different parents are
 // independent.  Work is therefore organised in waves: wave k holds the k-th child of every
 // parent with more than k children (parents in ascending order, a compact column list per
-// instance).  For each wave:
-//   k_hmc_wave_init   candidate <- chain state, momenta p = z / sqrt(cov_diag), H(cur)
-//   (ntau + 1) x [k_hs_mfma: Y = H X over the wave's columns ; k_hmc_step: gradient
-//                 gamma (H x - b) + L^T (L x / alpha^2), kick, drift with escape and
-//                 reflection]                                         (slip_leapfrog, 732-917)
-//   k_hs_mfma + k_hmc_finish  likelihood, prior, Hamiltonian, ESJD score (tuning), MH test
+// instance).  Once per stage Q_g = gamma H + L^T L / alpha2 is formed (the Hessian of the tempered
+// potential), so a leapfrog gradient is one matrix product.  For each wave:
+//   k_hmc_wave_cols, k_hmc_wave_init, k_hmc_hcur   columns; candidate <- chain state, momenta
+//                 p = z / sqrt(cov_diag) (element-parallel); H(cur)
+//   (ntau + 1) x k_hmc_leap   G = Q_g X - gamma b on the FP64 matrix cores, kick and drift with
+//                 escape and reflection fused into the product's epilogue        (slip_leapfrog, 732-917)
+//   k_hs_work (Y = H X), k_hmc_sums (per-column partial sums), k_hmc_finish, k_hmc_commit   likelihood,
+//                 prior, Hamiltonian, ESJD score (tuning) or acceptance probability, MH test, chain / child update
+// Every per-(dimension, column) step is spread over threads: a thread per column looping over the
+// dimensions waits one memory round trip per dimension.
 // Tuning (parents 1, 6, 11, ... in 1-based numbering) runs first with log-uniform dtau per
 // child, then k_tune_resample turns the scores into the dtau of every child and the new
 // log-dtau range (slip_hmc_tuning, 919-1214); sampling then runs every parent (591-730).
@@ -16,7 +20,6 @@
 // Mover SMCF_MOVER_HMC_ACCEPT (no reference counterpart) runs the same trajectories with ntau
 // steps and one step size per instance: no tuning pass; after sampling the step size is adapted
 // toward the target acceptance from the stage's mean acceptance probability (k_hmc_adapt).
-// Y = H X runs on the FP64 matrix cores (k_hs_mfma).
 #include "smcf_device.h"
 #include "smcf_internal.h"
 
@@ -32,7 +35,13 @@ __global__ void __launch_bounds__(256) k_hmc_prep(int P, BatchDev bt, HmcDev h) 
   __shared__ int red[256];
   __shared__ int base;
   int b = blockIdx.x, t = threadIdx.x;
-  if (!bt.active[b]) return;
+  if (!bt.active[b]) {  // a finished instance has no waves (the host lists its work from these counts)
+    if (t == 0) {
+      h.wamax[b] = 0;
+      h.tamax[b] = 0;
+    }
+    return;
+  }
   const int* a = bt.assigned + (size_t)b * P;
   for (int mode = 0; mode < 2; ++mode) {
     int* ptr = (mode ? h.tptr : h.wptr) + (size_t)b * (P + 1);
@@ -109,7 +118,8 @@ __global__ void k_hmc_chain_init(int nd, int P, BatchDev bt, HmcDev h, int tune)
   }
 }
 
-__global__ void k_hmc_wave_init(int nd, int P, int k, BatchDev bt, HmcDev h, int tune) {
+// the wave's columns: column m of wave k is the k-th child of the m-th listed parent
+__global__ void k_hmc_wave_cols(int P, int k, BatchDev bt, HmcDev h, int tune) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!bt.active[b]) return;
@@ -120,24 +130,64 @@ __global__ void k_hmc_wave_init(int nd, int P, int k, BatchDev bt, HmcDev h, int
   if (m == 0) h.ncol[b] = n;
   if (m >= n) return;
   int i = cols[ptr[k] + m];
-  int j = h.idstart[(size_t)b * P + i] + k;  // child slot j = id_start(i) + k
-  h.cpar[(size_t)b * P + m] = i;
-  h.cchild[(size_t)b * P + m] = j;
-  h.dead[(size_t)b * P + m] = 0;
+  size_t cb = (size_t)b * P + m;
+  h.cpar[cb] = i;
+  h.cchild[cb] = h.idstart[(size_t)b * P + i] + k;  // child slot j = id_start(i) + k
+  h.dead[cb] = 0;
+}
+
+// Element-parallel kernels over (dimension, column) of a wave's listed 64-column tiles (items (instance, tile)
+// built per stage on the host): blockIdx.x = item, threadIdx.x / 64 + 4 blockIdx.y strides the dimensions.  Sums
+// over a column's dimensions are formed there as S = 4 gridDim.y partials (thread s adds dimensions s, s + S, ...
+// in order) and the partials are added in order by the per-column kernel: fixed order, so a result does not depend
+// on the batch it ran in.
+constexpr int kRowGroups = 8;
+constexpr int kSums = 6;  // partial sums per (column, s): x.Hx, b.x, prior rows, kinetic, tuning score, H(cur) kinetic
+
+SMCF_DEV double* part_sum(const HmcDev& h, int P, int b, int s, int which, int col) {
+  return h.part + (((size_t)b * 4 * kRowGroups + s) * kSums + which) * P + col;
+}
+
+SMCF_DEV bool wave_element(const int* __restrict__ work, const HmcDev& h, int& b, int& col) {
+  b = work[2 * blockIdx.x];
+  col = work[2 * blockIdx.x + 1] * 64 + threadIdx.x % 64;
+  return col < h.ncol[b];
+}
+
+// candidate <- chain state and momenta p = z / sqrt(cov) (slip_hmc_sampling / slip_hmc_tuning before
+// slip_leapfrog, smc_slip.f90:680-700, 1000-1020)
+__global__ void __launch_bounds__(256) k_hmc_wave_init(int nd, int P, const int* __restrict__ work, BatchDev bt,
+                                                       HmcDev h, int tune) {
+  int b, m;
+  if (!wave_element(work, h, b, m)) return;
+  size_t cb = (size_t)b * P + m;
+  int i = h.cpar[cb], j = h.cchild[cb];
   const double* Xc = h.Xc + (size_t)b * nd * P;
   double* Xt = h.Xt + (size_t)b * nd * P;
   double* Pm = h.Pm + (size_t)b * nd * P;
   const double* cov = bt.cov + (size_t)b * nd;
   uint32_t stage = (uint32_t)(bt.iter[b] - 1);
   uint32_t site = tune ? kSiteTuneMom : kSiteHmcMom;
-  double ham = h.postc[(size_t)b * P + i];
-  for (int d = 0; d < nd; ++d) {
+  int s0 = blockIdx.y * 4 + threadIdx.x / 64;
+  double kin = 0.0;
+  for (int d = s0; d < nd; d += gridDim.y * 4) {
     Xt[(size_t)d * P + m] = Xc[(size_t)d * P + i];
     double p = draw_normal(bt.key[b], site, stage, (uint32_t)j, (uint32_t)d) / sqrt(cov[d]);
     Pm[(size_t)d * P + m] = p;
-    ham = ham + p * cov[d] * p / 2.0;
+    kin = kin + p * cov[d] * p / 2.0;
   }
-  h.hcur[(size_t)b * P + m] = ham;
+  *part_sum(h, P, b, s0, 5, m) = kin;
+}
+
+// H(cur) = chain state's gamma L + prior, plus the S kinetic partials of k_hmc_wave_init in order
+__global__ void k_hmc_hcur(int P, int S, BatchDev bt, HmcDev h) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!bt.active[b] || m >= h.ncol[b]) return;
+  size_t cb = (size_t)b * P + m;
+  double kin = 0.0;
+  for (int q = 0; q < S; ++q) kin = kin + *part_sum(h, P, b, q, 5, m);
+  h.hcur[cb] = h.postc[(size_t)b * P + h.cpar[cb]] + kin;
 }
 
 __global__ void k_idstart(int P, BatchDev bt, HmcDev h, int B) {
@@ -222,134 +272,423 @@ static void launch_hs(int nd, int P, int B, const double* H, const double* X, co
   hipLaunchKernelGGL(k_hs_mfma, g, dim3(256), 0, s, nd, P, H, X, ncol, active, Y);
 }
 
-// one leapfrog gradient + kick + drift (slip_leapfrog, smc_slip.f90:756-904)
-//   s = 0: half kick, full drift; 0 < s < ntau: full kick, full drift; s = ntau: full kick, half drift
-__global__ void k_hmc_step(MeshDev m, int P, int s, int ntau, double max_slip, BatchDev bt, HmcDev h, int tune) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
+// ---------------------------------------------------------------------------
+// Products over the stage's work lists.  A wave's columns are listed per stage on the host as items (instance,
+// column tile); a launch covers exactly the listed tiles.  (Grids over every (column tile, row group, instance)
+// with early exits place the few live workgroups of a small wave on the same few compute units.)
+//  * wide kernels (an instance with more than kNarrowCols columns in the wave): blockIdx.x = item * nrg + row
+//    group, the 4 wavefronts of a workgroup take 4 consecutive 64-row blocks of one 64-column tile;
+//  * narrow kernels (at most kNarrowCols columns): a workgroup forms one 64-row x 16-column tile with the
+//    contraction split over its 4 wavefronts (a quarter of the per-wavefront chain of loads), the partials added
+//    in LDS in wavefront order.
+// The choice depends on the instance's own column count, so an instance's arithmetic does not depend on its batch.
+// ---------------------------------------------------------------------------
+constexpr int kNarrowCols = 48;  // an instance whose wave has at most this many columns takes the narrow kernels
+
+SMCF_DEV int row_groups(int nd) { return ((nd + 16 * kHsMT - 1) / (16 * kHsMT) + 3) / 4; }
+
+// acc[m][q] += A(16m + r, k) B(k, c_q) over the contraction, for MT row tiles and NT column tiles (the operand
+// addressing and register pipeline of mfma_panel_t, for any NT)
+template <int MT, int NT, int NB>
+__device__ __forceinline__ void mfma_panel_nt(const double* __restrict__ abase, const uint32_t (&ao)[MT],
+                                              uint32_t lda4, const double* __restrict__ bbase,
+                                              const uint32_t (&bo)[NT], uint32_t ldb4, int nsteps,
+                                              dbl4 (&acc)[MT][NT]) {
+  if (nsteps <= 0) return;
+  double x[NB][MT], y[NB][NT];
+  auto load = [&](int k, int st) {
+    uint32_t q = (uint32_t)min(st, nsteps - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) x[k][m] = abase[ao[m] + q * lda4];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) y[k][n] = bbase[bo[n] + q * ldb4];
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k][m], y[k][n], acc[m][n], 0, 0, 0);
+  };
+#pragma unroll
+  for (int k = 0; k < NB; ++k) load(k, k);
+  int full = nsteps - nsteps % NB;
+  for (int st = 0; st < full; st += NB) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      prod(k);
+      load(k, st + k + NB);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB - 1; ++k)
+    if (k < nsteps - full) prod(k);
+}
+
+// rows i0 .. i0 + 63 x columns p0 .. p0 + 16 NT - 1 of A X (A symmetric nd x nd, X dimension-major with leading
+// dimension P, columns past n - 1 read column n - 1); lane l holds D(4t + l/16, l%16) of tile (m, q) in acc[m][q][t]
+// register buffers of the operand pipeline: deep for the narrow tile (a small wave's launch waits on one
+// wavefront's chain of loads), the shared default for the wide one (its registers go to accumulators)
+template <int NT>
+struct TileDepth {
+  static constexpr int value = NT == 1 ? 8 : kPanelBuffers;
+};
+
+template <int NT>
+__device__ __forceinline__ void ax_tile(int nd, int P, int n, const double* __restrict__ A,
+                                        const double* __restrict__ X, int p0, int i0, int lane,
+                                        dbl4 (&acc)[kHsMT][NT]) {
+  constexpr int MT = kHsMT;
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[NT];  // n >= 1: the callers return for tiles at or past column n
+#pragma unroll
+  for (int q = 0; q < NT; ++q) pcol[q] = min(p0 + 16 * q + fi, n - 1);
+  uint32_t bo[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) bo[q] = (uint32_t)(fk * P + pcol[q]);
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * nd + min(i0 + 16 * m + fi, nd - 1));
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int q = 0; q < NT; ++q) acc[m][q] = dbl4{0.0, 0.0, 0.0, 0.0};
+  int nd4 = nd & ~3;
+  mfma_panel_nt<MT, NT, TileDepth<NT>::value>(A, ao, (uint32_t)(4 * nd), X, bo, (uint32_t)(4 * P), nd4 / 4, acc);
+  if (nd4 < nd) {  // last contraction rows (nd = 2 mod 4): rows past nd contribute zero
+    bool in = nd4 + fk < nd;
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+      double bn = in ? X[(size_t)(nd4 + fk) * P + pcol[q]] : 0.0;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        int r = i0 + 16 * m + fi;
+        double av = (in && r < nd) ? A[(size_t)(nd4 + fk) * nd + r] : 0.0;
+        acc[m][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bn, acc[m][q], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Y = H X over the listed tiles of one wave (the final positions' likelihood products)
+template <int NT>
+__global__ void __launch_bounds__(256) k_hs_work(int nd, int P, const double* __restrict__ Hall,
+                                                 const double* __restrict__ Xall, const int* __restrict__ work,
+                                                 HmcDev h, double* __restrict__ Yall) {
+  int nrg = row_groups(nd);
+  int item = blockIdx.x / nrg, rg = blockIdx.x % nrg;
+  int b = work[2 * item], p0 = work[2 * item + 1] * 16 * NT;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int i0 = (rg * 4 + wave) * 16 * kHsMT;
+  int n = h.ncol[b];
+  if (i0 >= nd || p0 >= n) return;
+  dbl4 acc[kHsMT][NT];
+  ax_tile<NT>(nd, P, n, Hall + (size_t)b * nd * nd, Xall + (size_t)b * nd * P, p0, i0, lane, acc);
+  double* Y = Yall + (size_t)b * nd * P;
+  int fi = lane % 16, fk = lane / 16;
+#pragma unroll
+  for (int m = 0; m < kHsMT; ++m)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      int row = i0 + 16 * m + 4 * t + fk;
+      if (row >= nd) continue;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) {
+        int col = p0 + 16 * q + fi;
+        if (col < n) Y[(size_t)row * P + col] = acc[m][q][t];
+      }
+    }
+}
+
+// Q_g = gamma H + (L^T L) / alpha2 of every active instance, column-major: the Hessian of gamma * likelihood +
+// prior, so that Q_g x - gamma b is the gradient of slip_calc_grad (smc_slip.f90:139-182).  The dense gamma H,
+// then the <= 25 entries of each column of L^T L, in the expression of the Gibbs precision.
+__global__ void k_hmc_q_dense(int nd, BatchDev bt, HmcDev h) {
   int b = blockIdx.y;
-  if (!bt.active[b] || col >= h.ncol[b]) return;
-  size_t cb = (size_t)b * P + col;
-  if (h.dead[cb]) return;
-  int nd = m.ndim, nn2 = 2 * m.nnode;
-  const double* Y = bt.Y + (size_t)b * nd * P;
-  double* Xt = h.Xt + (size_t)b * nd * P;
+  if (!bt.active[b]) return;
+  size_t n = (size_t)nd * nd;
+  double g = bt.gamma[b];
+  const double* H = bt.H + (size_t)b * n;
+  double* Q = h.Qg + (size_t)b * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    Q[e] = g * H[e];
+}
+
+__global__ void k_hmc_q_sparse(int nd, BatchDev bt, HmcDev h) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (!bt.active[b] || i >= nd) return;
+  double g = bt.gamma[b], a2 = bt.scal[8 * (size_t)b];
+  const double* H = bt.H + (size_t)b * nd * nd;
+  double* Q = h.Qg + (size_t)b * nd * nd;
+  const int* ix = bt.llc_idx + ((size_t)b * nd + i) * 25;
+  const double* vx = bt.llc_val + ((size_t)b * nd + i) * 25;
+  for (int w = 0; w < 25; ++w) {
+    int r = ix[w];
+    if (r < 0) break;
+    Q[(size_t)i * nd + r] = g * H[(size_t)i * nd + r] + vx[w] / a2;
+  }
+}
+
+static void launch_form_qg(int nd, BatchDev& bt, HmcDev& h, int B, hipStream_t s) {
+  size_t n = (size_t)nd * nd;
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_hmc_q_dense, dim3(blocks, B), dim3(256), 0, s, nd, bt, h);
+  hipLaunchKernelGGL(k_hmc_q_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, bt, h);
+}
+
+// One leapfrog step of the wave's columns (slip_leapfrog, smc_slip.f90:756-904) fused with its gradient:
+//   G = Q_g X_in - gamma b on the FP64 matrix cores (a listed tile per workgroup, as k_hs_work), then each lane
+//   kicks and drifts the elements of G it holds:
+//   s = 0: half kick, full drift; 0 < s < ntau: full kick, full drift; s = ntau: full kick, half drift.
+// Momenta are updated in place (each element has one owner); positions go from X_in to X_out, since other
+// workgroups still read X_in for their products.  A position past |x| > 1e3 ends its column's trajectory (dead;
+// the candidate is then the start point, smc_slip.f90:776-782); dead columns are not updated.
+template <int NT>
+__global__ void __launch_bounds__(256) k_hmc_leap(int nd, int P, int s, int ntau, double max_slip,
+                                                  const double* __restrict__ Xin_all, double* __restrict__ Xout_all,
+                                                  const int* __restrict__ work, BatchDev bt, HmcDev h) {
+  int nrg = row_groups(nd);
+  int item = blockIdx.x / nrg, rg = blockIdx.x % nrg;
+  int b = work[2 * item], p0 = work[2 * item + 1] * 16 * NT;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int i0 = (rg * 4 + wave) * 16 * kHsMT;
+  int n = h.ncol[b];
+  if (i0 >= nd || p0 >= n) return;
+  const double* X = Xin_all + (size_t)b * nd * P;
+  dbl4 acc[kHsMT][NT];
+  ax_tile<NT>(nd, P, n, h.Qg + (size_t)b * nd * nd, X, p0, i0, lane, acc);
+  double* Xo = Xout_all + (size_t)b * nd * P;
   double* Pm = h.Pm + (size_t)b * nd * P;
-  double* LS = h.LS + (size_t)b * nn2 * P;
   const double* bv = bt.bvec + (size_t)b * nd;
   const double* cov = bt.cov + (size_t)b * nd;
-  const int* li = bt.li + (size_t)b * nn2 * 9;
-  const double* lv = bt.lv + (size_t)b * nn2 * 9;
-  const int* lti = bt.lti + (size_t)b * nd * 9;
-  const double* ltv = bt.ltv + (size_t)b * nd * 9;
-  double gamma = bt.gamma[b], a2 = bt.scal[8 * (size_t)b];
-  double dtau = h.dtau[(size_t)b * P + h.cchild[cb]];
-  for (int r = 0; r < nn2; ++r) {
-    double tt = 0.0;
+  double gam = bt.gamma[b];
+  int fi = lane % 16, fk = lane / 16;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) tt = tt + lv[9 * r + q] * Xt[(size_t)(li[9 * r + q] - 1) * P + col];
-    LS[(size_t)r * P + col] = tt / a2;
-  }
-  for (int d = 0; d < nd; ++d) {
-    double g1 = Y[(size_t)d * P + col];
-    g1 = g1 - bv[d];
-    g1 = g1 * gamma;
-    double g2 = 0.0;
+  for (int q = 0; q < NT; ++q) {
+    int col = p0 + 16 * q + fi;
+    if (col >= n) continue;
+    size_t cb = (size_t)b * P + col;
+    if (h.dead[cb]) continue;
+    double dt = h.dtau[(size_t)b * P + h.cchild[cb]];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) g2 = g2 + ltv[9 * d + q] * LS[(size_t)(lti[9 * d + q] - 1) * P + col];
-    double g = g1 + g2;
-    double p = Pm[(size_t)d * P + col];
-    p = (s == 0) ? p - g * 5e-1 * dtau : p - g * dtau;
-    Pm[(size_t)d * P + col] = p;
+    for (int m = 0; m < kHsMT; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int row = i0 + 16 * m + 4 * t + fk;
+        if (row >= nd) continue;
+        size_t e = (size_t)row * P + col;
+        double g = acc[m][q][t] - gam * bv[row];
+        double p = Pm[e];
+        p = (s == 0) ? p - g * 5e-1 * dt : p - g * dt;
+        double x = X[e];
+        x = (s < ntau) ? x + cov[row] * p * dt : x + cov[row] * p * 5e-1 * dt;
+        if (fabs(x) > 1e3) {
+          h.dead[cb] = 1;
+          continue;
+        }
+        while (x < 0.0 || x > max_slip) {
+          if (x > max_slip) { x = 2.0 * max_slip - x; p = -p; }
+          if (x < 0.0) { x = -x; p = -p; }
+        }
+        Xo[e] = x;
+        Pm[e] = p;
+      }
   }
-  const double* Xc = h.Xc + (size_t)b * nd * P;
-  int ip = h.cpar[cb];
-  for (int d = 0; d < nd; ++d) {
-    double x = Xt[(size_t)d * P + col], p = Pm[(size_t)d * P + col];
-    x = (s < ntau) ? x + cov[d] * p * dtau : x + cov[d] * p * 5e-1 * dtau;
+}
+
+// The narrow tile: rows i0 .. i0 + 63, columns p0 .. p0 + 15 of A X with the contraction split over the 4
+// wavefronts, then either Y (LEAP = false) or the leapfrog kick and drift of k_hmc_leap (LEAP = true).
+template <bool LEAP>
+__global__ void __launch_bounds__(256) k_narrow(int nd, int P, int s, int ntau, double max_slip,
+                                                const double* __restrict__ Aall, const double* __restrict__ Xin_all,
+                                                double* __restrict__ Out_all, const int* __restrict__ work,
+                                                BatchDev bt, HmcDev h) {
+  constexpr int MT = kHsMT;
+  __shared__ double red[4][MT][4][64];
+  int nrb = (nd + 16 * MT - 1) / (16 * MT);
+  int item = blockIdx.x / nrb, i0 = (blockIdx.x % nrb) * 16 * MT;
+  int b = work[2 * item], p0 = work[2 * item + 1] * 16;
+  int n = h.ncol[b];
+  if (p0 >= n) return;  // the whole workgroup (before its barrier)
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int fi = lane % 16, fk = lane / 16;
+  const double* A = Aall + (size_t)b * nd * nd;
+  const double* X = Xin_all + (size_t)b * nd * P;
+  int pcol = min(p0 + fi, n - 1);
+  uint32_t bo[1] = {(uint32_t)(fk * P + pcol)};
+  uint32_t ao[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)(fk * nd + min(i0 + 16 * m + fi, nd - 1));
+  dbl4 acc[MT][1];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) acc[m][0] = dbl4{0.0, 0.0, 0.0, 0.0};
+  int nd4 = nd & ~3, nsteps = nd4 / 4, q = (nsteps + 3) / 4;
+  int st0 = min(nsteps, wave * q), st1 = min(nsteps, st0 + q);
+  mfma_panel_nt<MT, 1, TileDepth<1>::value>(A + (size_t)st0 * 4 * nd, ao, (uint32_t)(4 * nd), X + (size_t)st0 * 4 * P,
+                                            bo, (uint32_t)(4 * P), st1 - st0, acc);
+  if (wave == 3 && nd4 < nd) {  // last contraction rows (nd = 2 mod 4): rows past nd contribute zero
+    bool in = nd4 + fk < nd;
+    double bn = in ? X[(size_t)(nd4 + fk) * P + pcol] : 0.0;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      int r = i0 + 16 * m + fi;
+      double av = (in && r < nd) ? A[(size_t)(nd4 + fk) * nd + r] : 0.0;
+      acc[m][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bn, acc[m][0], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) red[wave][m][t][lane] = acc[m][0][t];
+  __syncthreads();
+  // wavefront w finishes row tile m = w
+  int m = wave, col = p0 + fi;
+  if (col >= n) return;
+  size_t cb = (size_t)b * P + col;
+  if (LEAP && h.dead[cb]) return;
+  double* Out = Out_all + (size_t)b * nd * P;
+  double* Pm = h.Pm + (size_t)b * nd * P;
+  const double* bv = bt.bvec + (size_t)b * nd;
+  const double* cov = bt.cov + (size_t)b * nd;
+  double gam = bt.gamma[b];
+  double dt = LEAP ? h.dtau[(size_t)b * P + h.cchild[cb]] : 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int row = i0 + 16 * m + 4 * t + fk;
+    if (row >= nd) continue;
+    double y = ((red[0][m][t][lane] + red[1][m][t][lane]) + red[2][m][t][lane]) + red[3][m][t][lane];
+    size_t e = (size_t)row * P + col;
+    if (!LEAP) {
+      Out[e] = y;
+      continue;
+    }
+    double g = y - gam * bv[row];
+    double p = Pm[e];
+    p = (s == 0) ? p - g * 5e-1 * dt : p - g * dt;
+    double x = X[e];
+    x = (s < ntau) ? x + cov[row] * p * dt : x + cov[row] * p * 5e-1 * dt;
     if (fabs(x) > 1e3) {
-      for (int e = 0; e < nd; ++e) Xt[(size_t)e * P + col] = Xc[(size_t)e * P + ip];
       h.dead[cb] = 1;
-      return;
+      continue;
     }
     while (x < 0.0 || x > max_slip) {
       if (x > max_slip) { x = 2.0 * max_slip - x; p = -p; }
       if (x < 0.0) { x = -x; p = -p; }
     }
-    Xt[(size_t)d * P + col] = x;
-    Pm[(size_t)d * P + col] = p;
+    Out[e] = x;
+    Pm[e] = p;
   }
 }
 
-// final Hamiltonian, ESJD score (tuning), Metropolis test, chain / child update
-__global__ void k_hmc_finish(MeshDev m, int P, BatchDev bt, HmcDev h, int tune, int record_acc) {
+// The per-column sums of the final positions as S partials (Y = H x formed): x.Y and b.x (the likelihood,
+// slip_calc_likelihood, smc_slip.f90:56-104), the prior rows (L x)_r^2 / (2 alpha2) (slip_calc_prior, 106-137),
+// the kinetic energy and, in tuning, the ESJD displacement sum (x - x_cur)^2 / cov (slip_hmc_tuning, 1020-1040).
+__global__ void __launch_bounds__(256) k_hmc_sums(int nd, int nn2, int P, const double* __restrict__ Xall,
+                                                  const int* __restrict__ work, BatchDev bt, HmcDev h, int tune) {
+  int b, col;
+  if (!wave_element(work, h, b, col)) return;
+  size_t cb = (size_t)b * P + col;
+  int i = h.cpar[cb];
+  const double* X = Xall + (size_t)b * nd * P;
+  const double* Y = bt.Y + (size_t)b * nd * P;
+  const double* Pm = h.Pm + (size_t)b * nd * P;
+  const double* Xc = h.Xc + (size_t)b * nd * P;
+  const double* bv = bt.bvec + (size_t)b * nd;
+  const double* cov = bt.cov + (size_t)b * nd;
+  const int* li = bt.li + (size_t)b * nn2 * 9;
+  const double* lv = bt.lv + (size_t)b * nn2 * 9;
+  double two_a2 = 2.0 * bt.scal[8 * (size_t)b];
+  int s0 = blockIdx.y * 4 + threadIdx.x / 64, S = gridDim.y * 4;
+  double v = 0.0, bs = 0.0, kin = 0.0, sc = 0.0, pr = 0.0;
+  for (int d = s0; d < nd; d += S) {
+    double x = X[(size_t)d * P + col];
+    v = v + x * Y[(size_t)d * P + col];
+    bs = bs + bv[d] * x;
+    double p = Pm[(size_t)d * P + col];
+    kin = kin + p * cov[d] * p / 2.0;
+    if (tune) {
+      double df = x - Xc[(size_t)d * P + i];
+      sc = sc + (df * df) / cov[d];
+    }
+  }
+  for (int r = s0; r < nn2; r += S) {
+    double tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tt = tt + lv[9 * r + q] * X[(size_t)(li[9 * r + q] - 1) * P + col];
+    pr = pr + (tt * tt) / two_a2;
+  }
+  *part_sum(h, P, b, s0, 0, col) = v;
+  *part_sum(h, P, b, s0, 1, col) = bs;
+  *part_sum(h, P, b, s0, 2, col) = pr;
+  *part_sum(h, P, b, s0, 3, col) = kin;
+  *part_sum(h, P, b, s0, 4, col) = sc;
+}
+
+// final Hamiltonian from the partials of k_hmc_sums (added in order), tuning score or acceptance probability,
+// Metropolis test and chain-state values (slip_hmc_sampling / slip_hmc_tuning after slip_leapfrog,
+// smc_slip.f90:700-730, 1020-1060); k_hmc_commit then moves the positions
+__global__ void k_hmc_finish(int P, int S, BatchDev bt, HmcDev h, int tune, int record_acc) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (!bt.active[b] || col >= h.ncol[b]) return;
   size_t cb = (size_t)b * P + col;
-  int nd = m.ndim, nn2 = 2 * m.nnode;
   int i = h.cpar[cb], j = h.cchild[cb];
-  double* Xc = h.Xc + (size_t)b * nd * P;
-  const double* Xt = h.Xt + (size_t)b * nd * P;
-  const double* Pm = h.Pm + (size_t)b * nd * P;
-  const double* Y = bt.Y + (size_t)b * nd * P;
-  const double* cov = bt.cov + (size_t)b * nd;
-  const double* bv = bt.bvec + (size_t)b * nd;
   const double* sc = bt.scal + 8 * (size_t)b;
   double gamma = bt.gamma[b];
-  double hc = h.hcur[cb], hn, lk = 0.0, pr = 0.0, post = 0.0;
-  if (h.dead[cb]) {
-    hn = 1e20;
+  bool dead = h.dead[cb] != 0;
+  double hc = h.hcur[cb], hn, lk = 0.0, pr = 0.0, post = 0.0, disp = 0.0;
+  if (dead) {
+    hn = 1e20;  // an escaped trajectory's candidate is its start point: no displacement either
   } else {
-    double v = 0.0, bs = 0.0;
-    for (int d = 0; d < nd; ++d) {
-      double x = Xt[(size_t)d * P + col];
-      v = v + x * Y[(size_t)d * P + col];
-      bs = bs + bv[d] * x;
+    double v = 0.0, bs = 0.0, kin = 0.0;
+    for (int q = 0; q < S; ++q) {
+      v = v + *part_sum(h, P, b, q, 0, col);
+      bs = bs + *part_sum(h, P, b, q, 1, col);
+      pr = pr + *part_sum(h, P, b, q, 2, col);
+      kin = kin + *part_sum(h, P, b, q, 3, col);
+      if (tune) disp = disp + *part_sum(h, P, b, q, 4, col);
     }
     lk = (v - 2.0 * bs + bt.dsd[b]) / 2.0 + sc[3] + sc[4];
-    const int* li = bt.li + (size_t)b * nn2 * 9;
-    const double* lv = bt.lv + (size_t)b * nn2 * 9;
-    double two_a2 = 2.0 * sc[0];
-    for (int r = 0; r < nn2; ++r) {
-      double tt = 0.0;
-#pragma unroll
-      for (int q = 0; q < 9; ++q) tt = tt + lv[9 * r + q] * Xt[(size_t)(li[9 * r + q] - 1) * P + col];
-      pr = pr + (tt * tt) / two_a2;
-    }
     post = gamma * lk + pr;
-    hn = post;
-    for (int d = 0; d < nd; ++d) {
-      double p = Pm[(size_t)d * P + col];
-      hn = hn + p * cov[d] * p / 2.0;
-    }
+    hn = post + kin;
   }
   uint32_t stage = (uint32_t)(bt.iter[b] - 1);
   if (record_acc) h.score[(size_t)b * P + j] = exp(fmin(0.0, hc - hn));  // acceptance probability (mover 2)
-  if (tune) {
-    double score = 0.0;
-    for (int d = 0; d < nd; ++d) {
-      double df = Xt[(size_t)d * P + col] - Xc[(size_t)d * P + i];
-      score = score + (df * df) / cov[d];
-    }
-    score = score * exp(fmin(0.0, hc - hn));
-    h.score[(size_t)b * P + j] = score;
-  }
+  if (tune) h.score[(size_t)b * P + j] = disp * exp(fmin(0.0, hc - hn));
   uint64_t w0, w1;
   philox_draw(bt.key[b], tune ? kSiteTuneMH : kSiteHmcMH, stage, (uint32_t)j, 0, 0, w0, w1);
   double mh = clamp_u(u01(w0));
   size_t ib = (size_t)b * P + i;
-  if (hc - hn > log(mh)) {
-    for (int d = 0; d < nd; ++d) Xc[(size_t)d * P + i] = Xt[(size_t)d * P + col];
+  int acc = hc - hn > log(mh) ? 1 : 0;
+  h.accf[cb] = acc;
+  if (acc) {
     h.Lc[ib] = lk;
     h.Rc[ib] = pr;
     h.postc[ib] = post;
   }
   if (!tune) {
     int cur = bt.par[b];
-    double* Sd = (cur ? bt.S2 : bt.S) + (size_t)b * nd * P;
-    for (int d = 0; d < nd; ++d) Sd[(size_t)d * P + j] = Xc[(size_t)d * P + i];
     (cur ? bt.like2 : bt.like)[(size_t)b * P + j] = h.Lc[ib];
     (cur ? bt.prior2 : bt.prior)[(size_t)b * P + j] = h.Rc[ib];
+  }
+}
+
+// the chain state of each column's parent takes the accepted candidate; in sampling, child j <- chain state
+__global__ void __launch_bounds__(256) k_hmc_commit(int nd, int P, const double* __restrict__ Xall,
+                                                    const int* __restrict__ work, BatchDev bt, HmcDev h, int tune) {
+  int b, col;
+  if (!wave_element(work, h, b, col)) return;
+  size_t cb = (size_t)b * P + col;
+  int i = h.cpar[cb], j = h.cchild[cb];
+  bool acc = h.accf[cb] != 0;
+  const double* Xt = Xall + (size_t)b * nd * P;
+  double* Xc = h.Xc + (size_t)b * nd * P;
+  double* Sd = (bt.par[b] ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  for (int d = blockIdx.y * 4 + threadIdx.x / 64; d < nd; d += gridDim.y * 4) {
+    double x = acc ? Xt[(size_t)d * P + col] : Xc[(size_t)d * P + i];
+    if (acc) Xc[(size_t)d * P + i] = x;
+    if (!tune) Sd[(size_t)d * P + j] = x;
   }
 }
 
@@ -439,55 +778,146 @@ __global__ void k_hmc_adapt(int P, BatchDev bt, HmcDev h, int B, double target) 
   h.ldlo[b] = h.ldlo[b] + kAdaptGain * (abar - target);
 }
 
-static hipError_t run_waves(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, int tune, int amax, double max_slip,
-                            int ntau, int record_acc, hipStream_t s) {
-  int P = bt.P, nd = m.ndim;
+// one pass (tuning or sampling) of a stage: the waves of the pass's work lists (device copies at wa: every
+// instance's 64-column tiles; ww: the wide instances' 64-column tiles; wn: the narrow instances' 16-column tiles)
+static hipError_t run_waves(const MeshDev& m, BatchDev& bt, HmcDev& h, const HmcPassHost& ph, const int* wa,
+                            const int* ww, const int* wn, int B, int tune, double max_slip, int ntau, int record_acc,
+                            hipStream_t s) {
+  int P = bt.P, nd = m.ndim, nn2 = 2 * m.nnode;
+  int nrg = (((nd + 16 * kHsMT - 1) / (16 * kHsMT)) + 3) / 4, nrb = (nd + 16 * kHsMT - 1) / (16 * kHsMT);
   dim3 gc((P + 127) / 128, B);
-  for (int k = 0; k < amax; ++k) {
-    hipLaunchKernelGGL(k_hmc_wave_init, gc, dim3(128), 0, s, nd, P, k, bt, h, tune);
+  int ry = std::min((nd + 3) / 4, kRowGroups), rl = std::min((std::max(nd, nn2) + 3) / 4, kRowGroups);
+  int kmax = (int)ph.offa.size() - 1;
+  for (int k = 0; k < kmax; ++k) {
+    int na = ph.offa[k + 1] - ph.offa[k], nw = ph.offw[k + 1] - ph.offw[k], nn = ph.offn[k + 1] - ph.offn[k];
+    if (na == 0) continue;  // no instance has a k-th child left
+    const int* la = wa + 2 * (size_t)ph.offa[k];
+    const int* lw = ww + 2 * (size_t)ph.offw[k];
+    const int* ln = wn + 2 * (size_t)ph.offn[k];
+    hipLaunchKernelGGL(k_hmc_wave_cols, gc, dim3(128), 0, s, P, k, bt, h, tune);
+    hipLaunchKernelGGL(k_hmc_wave_init, dim3(na, ry), dim3(256), 0, s, nd, P, la, bt, h, tune);
+    hipLaunchKernelGGL(k_hmc_hcur, gc, dim3(128), 0, s, P, 4 * ry, bt, h);
+    const double* xin = h.Xt;
+    double* xout = h.Xt2;
     for (int st = 0; st <= ntau; ++st) {
-      launch_hs(nd, P, B, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
-      hipLaunchKernelGGL(k_hmc_step, gc, dim3(128), 0, s, m, P, st, ntau, max_slip, bt, h, tune);
+      if (nw)
+        hipLaunchKernelGGL(k_hmc_leap<4>, dim3(nw * nrg), dim3(256), 0, s, nd, P, st, ntau, max_slip, xin, xout, lw,
+                           bt, h);
+      if (nn)
+        hipLaunchKernelGGL(k_narrow<true>, dim3(nn * nrb), dim3(256), 0, s, nd, P, st, ntau, max_slip, h.Qg, xin,
+                           xout, ln, bt, h);
+      double* t = const_cast<double*>(xin);
+      xin = xout;
+      xout = t;
     }
-    launch_hs(nd, P, B, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
-    hipLaunchKernelGGL(k_hmc_finish, gc, dim3(128), 0, s, m, P, bt, h, tune, record_acc);
+    if (nw) hipLaunchKernelGGL(k_hs_work<4>, dim3(nw * nrg), dim3(256), 0, s, nd, P, bt.H, xin, lw, h, bt.Y);
+    if (nn)
+      hipLaunchKernelGGL(k_narrow<false>, dim3(nn * nrb), dim3(256), 0, s, nd, P, 0, ntau, max_slip, bt.H, xin, bt.Y,
+                         ln, bt, h);
+    hipLaunchKernelGGL(k_hmc_sums, dim3(na, rl), dim3(256), 0, s, nd, nn2, P, xin, la, bt, h, tune);
+    hipLaunchKernelGGL(k_hmc_finish, gc, dim3(128), 0, s, P, 4 * rl, bt, h, tune, record_acc);
+    hipLaunchKernelGGL(k_hmc_commit, dim3(na, ry), dim3(256), 0, s, nd, P, xin, la, bt, h, tune);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, const HmcCfg& cfg,
-                            hipStream_t s, int* host_amax) {
+// reads the pass's wave pointers (B x (kmax + 1)) back
+static hipError_t build_pass(HmcPassHost& ph, const int* dptr, int B, int P, int kmax, hipStream_t s) {
+  ph.ptr.assign((size_t)B * (kmax + 1), 0);
+  return hipMemcpy2DAsync(ph.ptr.data(), sizeof(int) * (kmax + 1), dptr, sizeof(int) * (P + 1),
+                          sizeof(int) * (kmax + 1), B, hipMemcpyDeviceToHost, s);
+}
+
+// the work items of every wave of the pass: for every instance with a k-th child left, its 64-column tiles (all,
+// and as wide items) or its 16-column tiles (narrow items)
+static void list_pass(HmcPassHost& ph, const int* amax, int B, int kmax) {
+  ph.la.clear();
+  ph.lw.clear();
+  ph.ln.clear();
+  ph.offa.assign(kmax + 1, 0);
+  ph.offw.assign(kmax + 1, 0);
+  ph.offn.assign(kmax + 1, 0);
+  auto push = [](std::vector<int>& l, int b, int t) {
+    l.push_back(b);
+    l.push_back(t);
+  };
+  for (int k = 0; k < kmax; ++k) {
+    ph.offa[k] = (int)ph.la.size() / 2;
+    ph.offw[k] = (int)ph.lw.size() / 2;
+    ph.offn[k] = (int)ph.ln.size() / 2;
+    for (int b = 0; b < B; ++b) {
+      if (k >= amax[b]) continue;
+      const int* pp = ph.ptr.data() + (size_t)b * (kmax + 1);
+      int n = pp[k + 1] - pp[k];
+      for (int t = 0; t < (n + 63) / 64; ++t) push(ph.la, b, t);
+      if (n > kNarrowCols)
+        for (int t = 0; t < (n + 63) / 64; ++t) push(ph.lw, b, t);
+      else
+        for (int t = 0; t < (n + 15) / 16; ++t) push(ph.ln, b, t);
+    }
+  }
+  ph.offa[kmax] = (int)ph.la.size() / 2;
+  ph.offw[kmax] = (int)ph.lw.size() / 2;
+  ph.offn[kmax] = (int)ph.ln.size() / 2;
+}
+
+static hipError_t upload_pass(const HmcPassHost& ph, int* wa, int* ww, int* wn, hipStream_t s) {
+  const std::vector<int>* src[3] = {&ph.la, &ph.lw, &ph.ln};
+  int* dst[3] = {wa, ww, wn};
+  for (int q = 0; q < 3; ++q)
+    if (!src[q]->empty()) {
+      hipError_t e = hipMemcpyAsync(dst[q], src[q]->data(), sizeof(int) * src[q]->size(), hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return e;
+    }
+  return hipSuccess;
+}
+
+hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, HmcHost& hh, int B, double max_slip,
+                            const HmcCfg& cfg, hipStream_t s) {
   int P = bt.P;
+  bool accept = cfg.mover == SMCF_MOVER_HMC_ACCEPT;
   hipLaunchKernelGGL(k_idstart, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   hipLaunchKernelGGL(k_hmc_prep, dim3(B), dim3(256), 0, s, P, bt, h);
-  // wave counts for the host loop bounds
-  hipError_t e = hipMemcpyAsync(host_amax, h.wamax, sizeof(int) * B, hipMemcpyDeviceToHost, s);
+  launch_form_qg(m.ndim, bt, h, B, s);  // the stage's gamma is set
+  // wave counts, then the wave pointers, for the work lists of the passes
+  hh.amax.resize(2 * (size_t)B);
+  hipError_t e = hipMemcpyAsync(hh.amax.data(), h.wamax, sizeof(int) * B, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(host_amax + B, h.tamax, sizeof(int) * B, hipMemcpyDeviceToHost, s);
+  e = hipMemcpyAsync(hh.amax.data() + B, h.tamax, sizeof(int) * B, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
   int wmax = 0, tmax = 0;
   for (int b = 0; b < B; ++b) {
-    wmax = max(wmax, host_amax[b]);
-    tmax = max(tmax, host_amax[B + b]);
+    wmax = max(wmax, hh.amax[b]);
+    tmax = max(tmax, hh.amax[B + b]);
   }
-  if (cfg.mover == SMCF_MOVER_HMC_ACCEPT) {
+  size_t wl = hmc_work_entries(B, P);
+  if ((e = build_pass(hh.pass[0], h.wptr, B, P, wmax, s)) != hipSuccess) return e;
+  if (!accept && (e = build_pass(hh.pass[1], h.tptr, B, P, tmax, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  list_pass(hh.pass[0], hh.amax.data(), B, wmax);
+  if ((e = upload_pass(hh.pass[0], h.wla, h.wlw, h.wln, s)) != hipSuccess) return e;
+  if (!accept) {
+    list_pass(hh.pass[1], hh.amax.data() + B, B, tmax);
+    if ((e = upload_pass(hh.pass[1], h.wla + wl, h.wlw + wl, h.wln + wl, s)) != hipSuccess) return e;
+  }
+  if (accept) {
     hipLaunchKernelGGL(k_hmc_dtau_fill, dim3((P + 127) / 128, B), dim3(128), 0, s, P, bt, h);
     hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
-    e = run_waves(m, bt, h, B, 0, wmax, max_slip, cfg.ntau, 1, s);
+    e = run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, cfg.ntau, 1, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_hmc_adapt, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B, cfg.accept);
     return hipGetLastError();
   }
   // tuning
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 1);
-  e = run_waves(m, bt, h, B, 1, tmax, max_slip, kNtauRef, 0, s);
+  e = run_waves(m, bt, h, hh.pass[1], h.wla + wl, h.wlw + wl, h.wln + wl, B, 1, max_slip, kNtauRef, 0, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tune_resample, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   // sampling
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
-  return run_waves(m, bt, h, B, 0, wmax, max_slip, kNtauRef, 0, s);
+  return run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, kNtauRef, 0, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -616,14 +1046,16 @@ __global__ void k_leapfrog_begin(MeshDev m, int P, int n, double gamma, double d
   h.Rc[col] = pr;
 }
 
-// final Hamiltonian of the n trajectories (Y = H Xt formed): ham_cand (1e20 after an escape), the candidate and
-// the momenta
-__global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev bt, HmcDev h,
-                               double* __restrict__ hn_out, double* __restrict__ lk_out, double* __restrict__ pr_out) {
+// final Hamiltonian of the n trajectories (Y = H Xf formed, Xf the final positions): ham_cand (1e20 after an
+// escape), and the candidate into Xt (the start point after an escape, smc_slip.f90:776-782)
+__global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, const double* __restrict__ Xf, BatchDev bt,
+                               HmcDev h, double* __restrict__ hn_out, double* __restrict__ lk_out,
+                               double* __restrict__ pr_out) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= n) return;
   int nd = m.ndim, nn2 = 2 * m.nnode;
-  if (h.dead[col]) {  // escaped: the candidate is the start point (smc_slip.f90:776-782)
+  if (h.dead[col]) {
+    for (int d = 0; d < nd; ++d) h.Xt[(size_t)d * P + col] = h.Xc[(size_t)d * P + col];
     hn_out[col] = 1e20;
     lk_out[col] = h.Lc[col];
     pr_out[col] = h.Rc[col];
@@ -632,7 +1064,7 @@ __global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev b
   const double* sc = bt.scal;
   double v = 0.0, bs = 0.0;
   for (int d = 0; d < nd; ++d) {
-    double x = h.Xt[(size_t)d * P + col];
+    double x = Xf[(size_t)d * P + col];
     v = v + x * bt.Y[(size_t)d * P + col];
     bs = bs + bt.bvec[d] * x;
   }
@@ -641,9 +1073,11 @@ __global__ void k_leapfrog_end(MeshDev m, int P, int n, double gamma, BatchDev b
   for (int r = 0; r < nn2; ++r) {
     double tt = 0.0;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) tt = tt + bt.lv[9 * r + q] * h.Xt[(size_t)(bt.li[9 * r + q] - 1) * P + col];
+    for (int q = 0; q < 9; ++q) tt = tt + bt.lv[9 * r + q] * Xf[(size_t)(bt.li[9 * r + q] - 1) * P + col];
     pr = pr + (tt * tt) / two_a2;
   }
+  if (Xf != h.Xt)
+    for (int d = 0; d < nd; ++d) h.Xt[(size_t)d * P + col] = Xf[(size_t)d * P + col];
   double hn = gamma * lk + pr;
   for (int d = 0; d < nd; ++d) {
     double p = h.Pm[(size_t)d * P + col];
@@ -661,14 +1095,27 @@ hipError_t launch_leapfrog_cols(const MeshDev& m, BatchDev& bt, HmcDev& h, int n
   dim3 gc((n + 127) / 128);
   launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
   hipLaunchKernelGGL(k_leapfrog_begin, gc, dim3(128), 0, s, m, P, n, gamma, dtau, bt, h);
-  for (int st = 0; st <= ntau; ++st) {
-    launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
-    hipLaunchKernelGGL(k_hmc_step, dim3((P + 127) / 128, 1), dim3(128), 0, s, m, P, st, ntau, max_slip, bt, h, 0);
-  }
-  launch_hs(nd, P, 1, bt.H, h.Xt, h.ncol, bt.active, bt.Y, s);
-  hipLaunchKernelGGL(k_leapfrog_end, gc, dim3(128), 0, s, m, P, n, gamma, bt, h, hn_out, lk_out, pr_out);
-  hipError_t e = hipMemcpyAsync(hc_out, h.hcur, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+  launch_form_qg(nd, bt, h, 1, s);
+  int nrg = (((nd + 16 * kHsMT - 1) / (16 * kHsMT)) + 3) / 4, nt = (n + 63) / 64;
+  std::vector<int> list(2 * (size_t)nt, 0);
+  for (int t = 0; t < nt; ++t) list[2 * t + 1] = t;  // instance 0, tile t
+  hipError_t e = hipMemcpyAsync(h.wla, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
+  const double* xin = h.Xt;
+  double* xout = h.Xt2;
+  for (int st = 0; st <= ntau; ++st) {
+    hipLaunchKernelGGL(k_hmc_leap<4>, dim3(nt * nrg), dim3(256), 0, s, nd, P, st, ntau, max_slip, xin, xout, h.wla,
+                       bt, h);
+    double* t = const_cast<double*>(xin);
+    xin = xout;
+    xout = t;
+  }
+  launch_hs(nd, P, 1, bt.H, xin, h.ncol, bt.active, bt.Y, s);
+  hipLaunchKernelGGL(k_leapfrog_end, gc, dim3(128), 0, s, m, P, n, gamma, xin, bt, h, hn_out, lk_out, pr_out);
+  e = hipMemcpyAsync(hc_out, h.hcur, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  // the list lives on this frame: the copy must have been consumed before return
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   return hipGetLastError();
 }
 

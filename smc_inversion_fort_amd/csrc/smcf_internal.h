@@ -128,18 +128,41 @@ struct BatchDev {
 // state of the HMC mover (per instance: P columns / parents / children)
 struct HmcDev {
   double *Xc, *Lc, *Rc, *postc;  // chain state per parent
-  double *Xt, *Pm, *LS, *hcur;   // trajectory per wave column (candidate, momentum, L x / alpha^2, H_cur)
+  double *Xt, *Xt2, *Pm, *LS, *hcur;  // trajectory per wave column (positions, ping-pong; momentum; L x; H_cur)
+  double* Qg;                         // gamma H + L^T L / alpha2 of the stage (ndim x ndim per instance)
+  int* accf;                          // Metropolis outcome per wave column
+  int *wla, *wlw, *wln;               // work lists (instance, column tile) of the sampling and tuning passes
+  double* part;                       // per-column partial sums of the wave kernels (32 x 6 x P per instance)
   double *dtau, *dtmp, *score;   // per child
   double *ldlo, *ldhi;           // per instance log-dtau range
   int *wptr, *wcols, *tptr, *tcols, *wamax, *tamax;  // wave lists (sampling / tuning)
   int *ncol, *cpar, *cchild, *dead, *idstart, *flag;
 };
 
+// host side of one HMC pass (tuning or sampling) of a stage: the wave pointers read back once per stage and the
+// work lists built from them (2 ints per item: instance, column tile; la: every instance's 64-column tiles, lw:
+// those of the instances the wide kernels take, ln: the 16-column tiles of the others; off*[k] = wave k's first
+// item)
+struct HmcPassHost {
+  std::vector<int> ptr;
+  std::vector<int> la, lw, ln;
+  std::vector<int> offa, offw, offn;
+};
+
+struct HmcHost {
+  std::vector<int> amax;  // wave counts per instance: sampling (B), then tuning (B)
+  HmcPassHost pass[2];    // sampling, tuning
+};
+
+// entries (ints) of one pass's work list: per instance at most P / 16 + P + 1 items of 2 ints
+inline size_t hmc_work_entries(int cap, int P) { return (size_t)cap * 2 * ((size_t)P + P / 16 + 1); }
+
 struct Problem {
   smcf_ctx* ctx;
   MeshDev m;
   BatchDev bt;
   HmcDev hm;
+  HmcHost hh;
   double max_slip;
   int P, mover, sweeps, precision;
   int hmc_ntau;       // leapfrog steps (50 for SMCF_MOVER_HMC, smc_slip.f90:669)
@@ -208,8 +231,8 @@ struct HmcCfg {
   double accept;
 };
 hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel, int mover);
-hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, int B, double max_slip, const HmcCfg& cfg,
-                            hipStream_t s, int* host_amax);
+hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, HmcHost& hh, int B, double max_slip,
+                            const HmcCfg& cfg, hipStream_t s);
 // one instance in slot 0: the tempered gradient of columns 0 .. n-1 of h.Xt into G (ndim x n, column per
 // particle), and n independent leapfrog trajectories from (h.Xt, h.Pm) with bt.cov as the mass diagonal
 // (H(cur) into hc_out, H(cand) into hn_out, both device arrays)

@@ -142,3 +142,26 @@ def test_hmc_accept_small_vs_oracle(ctx):
     sem = np.sqrt(dm.var(axis=0, ddof=1) / 8 + om.var(axis=0, ddof=1) / 8)
     assert np.all(np.abs(dm.mean(axis=0) - om.mean(axis=0)) < 4 * sem + 0.02)
     pb.close()
+
+
+@pytest.mark.parametrize("mover", [S.MOVER_HMC, S.MOVER_HMC_ACCEPT])
+def test_hmc_batch_matches_solo(ctx, mover):
+    """Instances of one batch finish at different stages; the finished ones must drop out of the stage's wave
+    work, and every instance's result must equal its run alone (bit for bit)."""
+    say("case: HMC batch of 4 vs solo, mover", mover)
+    c = syn.CONFIGS["c1"]
+    obs = device_obs(ctx, "c1", 2)
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    P = 400
+    kw = dict(mover=mover, hmc_ntau=10, hmc_accept=0.65) if mover == S.MOVER_HMC_ACCEPT else dict(mover=mover)
+    th = syn.sample_thetas(4, 21)
+    keys = [O.inner_key(5, 1, 0, k) for k in range(4)]
+    pb = S.Problem(ctx, [c["nxi"]], [c["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, P, max_batch=4, **kw)
+    nl, ns = pb.fault_calc_likelihood(th, keys)
+    say("batch: stages", ns.tolist(), "-log Z", nl.tolist())
+    assert len(set(ns.tolist())) > 1, "the geometries should finish at different stages"
+    for k in range(4):
+        n1, s1 = pb.fault_calc_likelihood(th[:, k:k + 1], keys[k:k + 1])
+        assert s1[0] == ns[k] and n1[0] == nl[k], (k, s1[0], ns[k], n1[0], nl[k])
+    pb.close()
+
