@@ -346,7 +346,9 @@ __global__ void k_gather(int nd, int P, BatchDev bt) {
       Sd[(size_t)i * P + j] = v;
       Td[(size_t)i * P + j] = (float)v;
     }
-  } else {
+  } else if (!bt.SX) {
+    // (the split-BF16 sweep reads only the split copy and writes the FP64 particles in its last sweep, and the
+    // centred likelihood takes its reference particle from the parents: the FP64 children are not gathered)
     for (int i = blockIdx.y; i < nd; i += gridDim.y) Sd[(size_t)i * P + j] = Ss[(size_t)i * P + pa];
   }
   if (blockIdx.y == 0) {
@@ -1312,12 +1314,13 @@ hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
 // ---------------------------------------------------------------------------
 constexpr int kLikeBfMT = 4;  // 16-row tiles per block of H (64 rows)
 
-// m = particle 0 of the current buffer; g = H m - b; L(m) (one workgroup of GT threads per instance)
+// m = child 0 after the resampling (its parent's FP64 particle: the children's FP64 copy is written by the last
+// sweep); g = H m - b; L(m) (one workgroup of GT threads per instance)
 __global__ void __launch_bounds__(GT) k_like_ref(int nd, int P, BatchDev bt) {
   __shared__ double red[GT];
   int b = blockIdx.x;
   if (!bt.active[b]) return;
-  const double* S = (bt.par[b] ? bt.S2 : bt.S) + (size_t)b * nd * P;
+  const double* S = (bt.par[b] ? bt.S : bt.S2) + (size_t)b * nd * P + bt.parent[(size_t)b * P];
   const double* H = bt.H + (size_t)b * nd * nd;
   const double* bv = bt.bvec + (size_t)b * nd;
   double* mref = bt.mref + (size_t)b * nd;
