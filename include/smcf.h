@@ -89,6 +89,8 @@ int smcf_problem_create(smcf_ctx* ctx, const smcf_problem_desc* desc, int max_ba
  * reference, used for admission by memory budget), and the bytes a created problem holds. */
 int smcf_problem_bytes(const smcf_problem_desc* desc, int max_batch, size_t* bytes);
 int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes);
+/* The context a problem was created on. */
+smcf_ctx* smcf_problem_context(const smcf_problem* pb);
 int smcf_problem_destroy(smcf_problem* pb);
 int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_slip, int* ndim_fault,
                       int* nobs);
@@ -291,6 +293,25 @@ int smcf_work_mcmc_sampling(smcf_problem* pb, uint64_t seed, int stage, double g
                             int* acc_count);
 /* RNG key of the inner SMC of one outer evaluation: (seed, outer stage, role 0 parent / 1 child, slot) */
 uint64_t smcf_inner_key(uint64_t seed, uint32_t outer_stage, uint32_t role, uint32_t slot);
+
+/* The outer reweight and resample of one stage on the device, in one call: fault_find_next_gamma,
+ * fault_normalize_weights, fault_calc_mean_particles, fault_calc_cov_particles, fault_resample_particles and
+ * fault_reorder_to_send (smc_fault.f90:372-637) - the components above, for callers that keep the stage's O(N)
+ * work on the GPU.  Host arrays: particles (ndim x N), likelihood and q (q may be NULL: plain tempering) are
+ * reordered in place as fault_reorder_to_send does; weights receives the normalised weights in the order before
+ * the reorder, cov_chol the lower factor of 0.04 x the weighted covariance, assigned_num the counts (reordered),
+ * sorted_idx the 1-based permutation.  The stage's sums are block reductions in a fixed order: gamma, the evidence
+ * increment, the weights and the factor agree with the host components to rounding; the counts are those of
+ * smcf_fault_resample_particles for the same weights.  device_sort = 0 orders the parents with qsort_ on the host
+ * (the reference's order); 1 with a stable radix sort on the device (descending counts, tied parents in ascending
+ * index, where std::sort leaves them in its own order).  ndim <= 64. */
+int smcf_outer_reweight_resample(smcf_ctx* ctx, int ndim, int N, double gamma_prev, uint64_t seed, int stage,
+                                 int device_sort, double* particles, double* likelihood, double* q, double* gamma,
+                                 double* neglog_evidence, double* weights, double* cov_chol, int* info,
+                                 int* assigned_num, int* sorted_idx);
+/* Where smcf_fault_smc_exec / _gsmc run each stage's reweight and resample: 0 the host components (default, the
+ * reference's arithmetic order), 1 smcf_outer_reweight_resample with the host sort, 2 with the device sort. */
+int smcf_fault_smc_set_outer_stage(smcf_problem* pb, int mode);
 
 /*
  * Serving (no counterpart in the reference, whose main.f90 runs one inversion per job).  A server admits

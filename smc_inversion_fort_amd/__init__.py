@@ -70,6 +70,7 @@ EXPORTS = [
     "smcf_fault_resample_particles", "smcf_fault_reorder_to_send", "smcf_work_mcmc_sampling", "smcf_call_dc3d0",
     "smcf_calc_responce", "smcf_slip_likelihood_gram", "smcf_slip_prior_ell", "smcf_slip_grad_ell",
     "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault", "smcf_server_set_max_running",
+    "smcf_outer_reweight_resample", "smcf_fault_smc_set_outer_stage", "smcf_problem_context",
 ]
 
 
@@ -153,6 +154,11 @@ def _declare(L):
     L.smcf_fault_calc_cov_particles.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, _I]
     L.smcf_fault_resample_particles.argtypes = [C.c_int, _D, C.c_uint64, C.c_int, _I]
     L.smcf_fault_reorder_to_send.argtypes = [C.c_int, C.c_int, _I, _D, _D, _D, _I]
+    L.smcf_outer_reweight_resample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
+                                               C.c_int, _D, _D, _D, _D, _D, _D, _D, _I, _I, _I]
+    L.smcf_fault_smc_set_outer_stage.argtypes = [C.c_void_p, C.c_int]
+    L.smcf_problem_context.argtypes = [C.c_void_p]
+    L.smcf_problem_context.restype = C.c_void_p
     L.smcf_work_mcmc_sampling.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_double, _D, _D, C.c_int, _I, _D, _D,
                                           _D, _D, C.c_double, C.c_int, _D, _D, _D, _I]
 
@@ -266,6 +272,27 @@ def fault_reorder_to_send(assigned, X, L, q=None):
     sidx = np.zeros(a.shape[0], np.int32)
     _check(lib().smcf_fault_reorder_to_send(Xc.shape[0], a.shape[0], _ip(a), _dp(Xc), _dp(Lc), _opt(qc), _ip(sidx)))
     return a, Xc, Lc, qc, sidx
+
+
+def outer_reweight_resample(ctx, X, L, gamma_prev, seed, stage, q=None, device_sort=False):
+    """The outer reweight and resample of one stage on the device (smcf_outer_reweight_resample): returns a dict with
+    gamma, neglog_evidence, weights (before the reorder), cov_chol, info, and the reordered assigned, X, L, q and
+    sorted_idx (1-based)."""
+    Xc = np.array(_f64(X), order="F")
+    nd, N = Xc.shape
+    Lc = np.array(_f64(L))
+    qc = None if q is None else np.array(_f64(q))
+    g, ev = C.c_double(), C.c_double()
+    w = np.zeros(N)
+    cov = np.zeros((nd, nd), order="F")
+    info = np.zeros(1, np.int32)
+    a = np.zeros(N, np.int32)
+    sidx = np.zeros(N, np.int32)
+    _check(lib().smcf_outer_reweight_resample(ctx.h, nd, N, float(gamma_prev), C.c_uint64(seed), int(stage),
+                                              int(bool(device_sort)), _dp(Xc), _dp(Lc), _opt(qc), C.byref(g),
+                                              C.byref(ev), _dp(w), _dp(cov), _ip(info), _ip(a), _ip(sidx)))
+    return dict(gamma=g.value, neglog_evidence=ev.value, weights=w, cov_chol=cov, info=int(info[0]), assigned=a,
+                particles=Xc, likelihood=Lc, q=qc, sorted_idx=sidx)
 
 
 class Context:
@@ -680,6 +707,11 @@ class Problem:
                                              _dp(_f64(cov_chol)), N, _ip(a), _dp(X), *args, int(max_batch or N),
                                              _dp(Xn), _dp(Ln), _dp(qn), C.byref(acc)))
         return Xn, Ln, qn, acc.value
+
+    def set_outer_stage(self, mode):
+        """Where fault_smc_exec runs each stage's reweight and resample: 0 host (default), 1 device with the host
+        sort, 2 device with the device radix sort."""
+        _check(lib().smcf_fault_smc_set_outer_stage(self.h, int(mode)))
 
     def set_output_dir(self, output_dir):
         """Per-stage dumps <output_dir><iter>.csv of fault_smc_exec (None stops them)."""

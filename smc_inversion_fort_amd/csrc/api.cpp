@@ -763,12 +763,15 @@ int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes) {
   return SMCF_OK;
 }
 
+smcf_ctx* smcf_problem_context(const smcf_problem* pb) { return pb ? pb->p.ctx : nullptr; }
+
 int smcf_problem_destroy(smcf_problem* pb) {
   if (!pb) return SMCF_OK;
   (void)hipSetDevice(pb->p.ctx->device);
   (void)hipStreamSynchronize(pb->p.ctx->stream);
   for (void* p : pb->p.allocs) (void)hipFree(p);
   (void)smcf_fault_smc_set_output_dir(pb, nullptr);  // a later problem at the same address starts without dumps
+  (void)smcf_fault_smc_set_outer_stage(pb, 0);         // ... and with the host outer stage
   delete pb;
   return SMCF_OK;
 }
@@ -1425,6 +1428,71 @@ int smcf_find_next_gamma(smcf_ctx* ctx, int P, double gprev, const double* L, do
   SMCF_HIP(hipStreamSynchronize(s));
   if (gamma) *gamma = out3[0];
   if (nev) *nev = out3[1];
+  return SMCF_OK;
+}
+
+int smcf_outer_reweight_resample(smcf_ctx* ctx, int ndim, int N, double gamma_prev, uint64_t seed, int stage,
+                                 int device_sort, double* particles, double* likelihood, double* q, double* gamma,
+                                 double* neglog_evidence, double* weights, double* cov_chol, int* info,
+                                 int* assigned_num, int* sorted_idx) {
+  if (!ctx || ndim < 1 || ndim > kMaxOuterDim || N < 1 || !particles || !likelihood || !gamma || !neglog_evidence ||
+      !weights || !cov_chol || !assigned_num || !sorted_idx) {
+    set_error("outer_reweight_resample: missing argument, or ndim outside 1..64, or N < 1");
+    return SMCF_EINVAL;
+  }
+  SMCF_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  auto mix = [](uint64_t x) {  // the outer key derivation of the host stage functions
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  uint64_t okey = mix(seed ^ 0x5EEDF00Dull);
+  size_t n = (size_t)N, nx = (size_t)N * ndim;
+  TmpBufs tb;
+  double *dX = nullptr, *dL = nullptr, *dq = nullptr, *dw = nullptr, *dscal = nullptr, *dcov = nullptr;
+  double *dX2 = nullptr, *dL2 = nullptr, *dq2 = nullptr;
+  int *da = nullptr, *da2 = nullptr, *didx = nullptr, *dtmp = nullptr, *dsi = nullptr;
+  SMCF_HIP(tb.alloc(&dX, nx));
+  SMCF_HIP(tb.alloc(&dL, n));
+  SMCF_HIP(tb.alloc(&dw, n));
+  SMCF_HIP(tb.alloc(&dscal, 4));
+  SMCF_HIP(tb.alloc(&dcov, (size_t)ndim * ndim));
+  SMCF_HIP(tb.alloc(&da, n));
+  if (q) SMCF_HIP(tb.alloc(&dq, n));
+  SMCF_HIP(hipMemcpyAsync(dX, particles, sizeof(double) * nx, hipMemcpyHostToDevice, s));
+  SMCF_HIP(hipMemcpyAsync(dL, likelihood, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  if (q) SMCF_HIP(hipMemcpyAsync(dq, q, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  CHECK_LAUNCH(launch_outer_stage(N, ndim, gamma_prev, dL, dq, dX, okey, stage, dw, dscal, dcov, da, s));
+  double sc[4];
+  SMCF_HIP(hipMemcpyAsync(sc, dscal, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(weights, dw, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  SMCF_HIP(hipMemcpyAsync(cov_chol, dcov, sizeof(double) * ndim * ndim, hipMemcpyDeviceToHost, s));
+  if (device_sort) {
+    SMCF_HIP(tb.alloc(&dX2, nx));
+    SMCF_HIP(tb.alloc(&dL2, n));
+    SMCF_HIP(tb.alloc(&da2, n));
+    SMCF_HIP(tb.alloc(&didx, n));
+    SMCF_HIP(tb.alloc(&dtmp, n));
+    SMCF_HIP(tb.alloc(&dsi, n));
+    if (q) SMCF_HIP(tb.alloc(&dq2, n));
+    CHECK_LAUNCH(launch_outer_sort(N, ndim, da, didx, dtmp, dX, dL, dq, da2, dX2, dL2, dq2, dsi, s));
+    SMCF_HIP(hipMemcpyAsync(assigned_num, da2, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(sorted_idx, dsi, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(particles, dX2, sizeof(double) * nx, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipMemcpyAsync(likelihood, dL2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (q) SMCF_HIP(hipMemcpyAsync(q, dq2, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipStreamSynchronize(s));
+  } else {  // the reference's order: qsort_ on the host (fault_reorder_to_send)
+    SMCF_HIP(hipMemcpyAsync(assigned_num, da, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipStreamSynchronize(s));
+    int rc = smcf_fault_reorder_to_send(ndim, N, assigned_num, particles, likelihood, q, sorted_idx);
+    if (rc) return rc;
+  }
+  *gamma = sc[0];
+  *neglog_evidence = sc[1];
+  if (info) *info = (int)sc[2];
   return SMCF_OK;
 }
 

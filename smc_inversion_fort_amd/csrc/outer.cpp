@@ -147,6 +147,15 @@ std::string output_dir_of(const smcf_problem* pb) {
   return it == g_out_dir.end() ? std::string() : it->second;
 }
 
+// where each stage's reweight and resample runs (smcf_fault_smc_set_outer_stage): 0 host, 1 / 2 device
+std::map<const smcf_problem*, int> g_outer_mode;
+
+int outer_mode_of(const smcf_problem* pb) {
+  std::lock_guard<std::mutex> lk(g_out_mu);
+  auto it = g_outer_mode.find(pb);
+  return it == g_outer_mode.end() ? 0 : it->second;
+}
+
 int write_stage(const std::string& dir, int iter, int N, int nd, const std::vector<double>& X,
                 const std::vector<double>& L) {
   if (dir.empty()) return SMCF_OK;
@@ -200,21 +209,29 @@ int fault_smc(smcf_problem* pb, const double* range, int N, uint64_t seed, int m
   if (g)
     for (int i = 0; i < N; ++i) Q[i] = neglog_q(&X[(size_t)i * nd], *g, nd);
   const std::string out_dir = output_dir_of(pb);
+  const int omode = outer_mode_of(pb);
   if ((rc = write_stage(out_dir, 0, N, nd, X, L))) return rc;
   double gamma = 0.0, ev_total = 0.0;
   int iter = 1;
   while (1.0 - gamma > 1e-8 && iter <= max_stages) {
     double gprev = gamma, ev;
-    if ((rc = smcf_fault_find_next_gamma(N, gprev, L.data(), g ? Q.data() : nullptr, w.data(), &gamma, &ev)))
-      return rc;
+    int info = 0;  // ignored, as in the reference
+    if (omode) {
+      if ((rc = smcf_outer_reweight_resample(smcf_problem_context(pb), nd, N, gprev, seed, iter, omode == 2, X.data(),
+                                             L.data(), g ? Q.data() : nullptr, &gamma, &ev, w.data(), cov.data(),
+                                             &info, assigned.data(), sidx.data())))
+        return rc;
+    } else {
+      if ((rc = smcf_fault_find_next_gamma(N, gprev, L.data(), g ? Q.data() : nullptr, w.data(), &gamma, &ev)))
+        return rc;
+      smcf_fault_normalize_weights(N, w.data());
+      smcf_fault_calc_mean_particles(nd, N, X.data(), w.data(), mean.data());
+      smcf_fault_calc_cov_particles(nd, N, X.data(), w.data(), mean.data(), cov.data(), &info);
+      smcf_fault_resample_particles(N, w.data(), seed, iter, assigned.data());
+      smcf_fault_reorder_to_send(nd, N, assigned.data(), X.data(), L.data(), Q.data(), sidx.data());
+    }
     ev_total = ev_total + ev;
     if (gamma_trace) gamma_trace[iter - 1] = gamma;
-    int info = 0;
-    smcf_fault_normalize_weights(N, w.data());
-    smcf_fault_calc_mean_particles(nd, N, X.data(), w.data(), mean.data());
-    smcf_fault_calc_cov_particles(nd, N, X.data(), w.data(), mean.data(), cov.data(), &info);  // info ignored
-    smcf_fault_resample_particles(N, w.data(), seed, iter, assigned.data());
-    smcf_fault_reorder_to_send(nd, N, assigned.data(), X.data(), L.data(), Q.data(), sidx.data());
     int acc = 0;
     if ((rc = smcf_work_mcmc_sampling(pb, seed, iter, gamma, range, cov.data(), N, assigned.data(), X.data(),
                                       g ? Q.data() : nullptr, g ? g->qmean : nullptr, g ? g->qcov_inv : nullptr,
@@ -481,6 +498,16 @@ int smcf_work_mcmc_sampling(smcf_problem* pb, uint64_t seed, int stage, double g
     }
   }
   if (acc_count) *acc_count = acc_n;
+  return SMCF_OK;
+}
+
+int smcf_fault_smc_set_outer_stage(smcf_problem* pb, int mode) {
+  if (!pb || mode < 0 || mode > 2) return SMCF_EINVAL;
+  std::lock_guard<std::mutex> lk(g_out_mu);
+  if (mode)
+    g_outer_mode[pb] = mode;
+  else
+    g_outer_mode.erase(pb);
   return SMCF_OK;
 }
 

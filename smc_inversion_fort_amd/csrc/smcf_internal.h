@@ -257,6 +257,13 @@ hipError_t launch_calc_responce(int npts, const double* pts, const int* cny, con
                                 const int* target, double* uobs, hipStream_t s);
 hipError_t launch_find_next_gamma(int P, double gprev, const double* L, double cv, double tol, int cap,
                                   double* out3, double* w, hipStream_t s);
+constexpr int kMaxOuterDim = 64;  // geometry parameters of the outer SMC (6 per plane at most, 3 nplane + 3)
+hipError_t launch_outer_stage(int N, int nd, double gprev, const double* L, const double* q, const double* X,
+                              uint64_t okey, int stage, double* w, double* scal, double* cov, int* assigned,
+                              hipStream_t s);
+hipError_t launch_outer_sort(int N, int nd, const int* assigned, int* idx, int* tmp, const double* X_in,
+                             const double* L_in, const double* q_in, int* a_out, double* X_out, double* L_out,
+                             double* q_out, int* sorted_idx, hipStream_t s);
 hipError_t launch_normalize_resample(int P, const double* w, double uh, double ul, double* wn,
                                      int* assigned, hipStream_t s);
 hipError_t launch_trunc_normal_f32(int n, const double* mu, const double* sq, double lo, double hi, const float* u,
