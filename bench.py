@@ -133,7 +133,8 @@ def run_form(a, S, syn, obs, precision):
                        mover=S.MOVER_GIBBS, gibbs_sweeps=cfg["sweeps"], max_batch=a.slots, precision=precision)
     else:
         pb = S.Problem(ctx, [cfg["nxi"]], [cfg["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, cfg["P"],
-                       mover=S.MOVER_HMC_ACCEPT, hmc_ntau=C3_NTAU, hmc_accept=C3_ACCEPT, max_batch=a.slots)
+                       mover=S.MOVER_HMC_ACCEPT, hmc_ntau=C3_NTAU, hmc_accept=C3_ACCEPT, max_batch=a.slots,
+                       precision=precision)
     ndim, P, sweeps = pb.ndim, cfg["P"], cfg["sweeps"]
     print("bench[%s]: %s %dx%d patches ndim=%d nobs=%d P=%d; streaming %d geometries through %d slots "
           "(warm-up %d x %d, timed %d x %d, tail %d)" % (name, a.workload, cfg["nxi"], cfg["neta"], ndim, pb.nobs, P,
@@ -168,9 +169,10 @@ def run_form(a, S, syn, obs, precision):
         # the HMC stage (every kernel of one stage's trajectories): per child ntau + 2 products Y = H x
         # (initial gradient, ntau kicks, final likelihood) of 2*ndim^2 flop each
         flops = moves * P * (C3_NTAU + 2) * 2.0 * ndim * ndim
-        peak, kernel = (FP64_PEAK_TFLOPS, "HMC stage (k_hmc_leap: Q_g X on the FP64 matrix cores with the kick and "
-                                          "drift in its epilogue, k_hs_work: H X, + FP64 VALU wave kernels; timed "
-                                          "per stage)")
+        peak, kernel = (FP64_PEAK_TFLOPS, "HMC stage (k_hmc_leap: Q_g X on the %s matrix cores with the kick and "
+                                          "drift in its epilogue, k_hs_work: H X in FP64, + FP64 VALU wave kernels; "
+                                          "timed per stage; FP64-equivalent flops against the FP64 peak)"
+                                          % ("FP64" if name == "fp64" else "split-BF16 (wide waves)"))
     elif name == "fp64":
         peak, kernel = FP64_PEAK_TFLOPS, "k_gibbs (FP64 matrix cores + FP64 VALU draws)"
     elif name == "mixed":
@@ -348,8 +350,10 @@ def main():
     obs = syn.make_observations(a.workload, a.seed, greens)
     gctx.close()
     if a.workload == "c3":
-        # the HMC mover has one (FP64) form; the CPU baseline and the serving leg belong to the headline
-        a.precision, a.no_cpu, a.serve_seconds = ["fp64"], True, 0.0
+        # the HMC mover's forms: FP64 and split-BF16 leapfrog gradients; the CPU baseline and the serving leg belong
+        # to the headline
+        a.precision = [f for f in a.precision if f in ("fp64", "bf16")] or ["fp64"]
+        a.no_cpu, a.serve_seconds = True, 0.0
 
     forms = []
     for f in a.precision:

@@ -35,7 +35,10 @@ extern "C" {
                                    hmc_accept (log dtau += 2.0 * (mean acceptance - hmc_accept), first dtau 0.1);
                                    no reference counterpart: it replaces the ESJD tuning pass */
 
-/* arithmetic of the Gibbs mover (no counterpart in the reference, which is FP64 throughout) */
+/* arithmetic of the inner movers (no counterpart in the reference, which is FP64 throughout).  Gibbs: all three.
+ * HMC movers: FP64 or MIXED_BF16, where the leapfrog gradients of the waves with many columns come from split-BF16
+ * products of Q_g = gamma H + L^T L / alpha2 and the trajectory's final likelihood, the Hamiltonian and the
+ * Metropolis test stay FP64 (the sampler stays exact; trajectories follow a force within ~2^-16 of the gradient). */
 #define SMCF_PRECISION_FP64 0  /* every operation in FP64 (default; parity with the reference's arithmetic) */
 #define SMCF_PRECISION_MIXED 1 /* Q.S products in FP32 on the matrix cores and single-precision inverse-CDF
                                   draws; conditional means, corrections, likelihood, weights and evidence FP64 */
@@ -78,7 +81,7 @@ typedef struct {
   int nparticle_slip;
   int mover;                  /* SMCF_MOVER_* */
   int gibbs_sweeps;
-  int precision;              /* SMCF_PRECISION_* (MIXED needs the Gibbs mover) */
+  int precision;              /* SMCF_PRECISION_* (MIXED needs the Gibbs mover; HMC movers take FP64 or MIXED_BF16) */
   int hmc_ntau;               /* SMCF_MOVER_HMC_ACCEPT: leapfrog steps per trajectory (0: 10) */
   double hmc_accept;          /* SMCF_MOVER_HMC_ACCEPT: target mean acceptance probability (0: 0.65) */
 } smcf_problem_desc;

@@ -231,7 +231,7 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
     return SMCF_EINVAL;
   }
   bool hmc = P.mover != SMCF_MOVER_GIBBS;
-  const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept};
+  const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept, P.precision == SMCF_PRECISION_MIXED_BF16 ? 1 : 0};
   const int nf = m.ndim_fault;
   const auto t_call = std::chrono::steady_clock::now();
   P.t_done.assign(N, 0.0);
@@ -465,8 +465,8 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
     set_error("unknown precision " + std::to_string(P.precision));
     return SMCF_EINVAL;
   }
-  if (P.precision != SMCF_PRECISION_FP64 && P.mover != SMCF_MOVER_GIBBS) {
-    set_error("mixed precision is implemented for the Gibbs mover only");
+  if (P.precision == SMCF_PRECISION_MIXED && P.mover != SMCF_MOVER_GIBBS) {
+    set_error("the FP32 form is implemented for the Gibbs mover only (HMC movers take SMCF_PRECISION_MIXED_BF16)");
     return SMCF_EUNSUPPORTED;
   }
   MeshDev& m = P.m;
@@ -649,7 +649,7 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   }
   bt.HX = bt.DX = nullptr;
   bt.mref = bt.gref = bt.lref = nullptr;
-  if (P.precision == SMCF_PRECISION_MIXED_BF16) {
+  if (P.precision == SMCF_PRECISION_MIXED_BF16 && P.mover == SMCF_MOVER_GIBBS) {
     dalloc(P, &bt.QX, bf_q_elems((int)nd, bt.kb) * C);
     dalloc(P, &bt.SX, (size_t)bf_rows((int)nd) * Pn * 2 * C);
     dalloc(P, &bt.SXb, (size_t)bf_rows((int)nd) * Pn * 2 * C);
@@ -700,6 +700,23 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
     dalloc(P, &h.wlw, 2 * hmc_work_entries((int)C, (int)Pn));
     dalloc(P, &h.wln, 2 * hmc_work_entries((int)C, (int)Pn));
     dalloc(P, &h.part, (size_t)32 * 6 * Pn * C);
+    h.QgX = h.XXa = h.XXb = nullptr;
+    if (P.precision == SMCF_PRECISION_MIXED_BF16) {
+      uint4 *q = nullptr, *xa = nullptr, *xb = nullptr;  // 16-byte fragments
+      size_t xf = hmc_xx_rows((int)nd) / 8 * Pn * 2 * C;
+      dalloc(P, &q, hmc_qgx_frags((int)nd) * C);
+      dalloc(P, &xa, xf);
+      dalloc(P, &xb, xf);
+      h.QgX = q;
+      h.XXa = xa;
+      h.XXb = xb;
+      // rows past ndim are read by the products and never written
+      if (!P.alloc_rc && !sizing &&
+          (hipMemset(xa, 0, sizeof(uint4) * xf) != hipSuccess || hipMemset(xb, 0, sizeof(uint4) * xf) != hipSuccess)) {
+        set_error("hipMemset failed");
+        P.alloc_rc = SMCF_EHIP;
+      }
+    }
     dalloc(P, &h.Pm, nd * Pn * C);
     dalloc(P, &h.LS, nn2 * Pn * C);
     dalloc(P, &h.Lc, Pn * C);

@@ -169,9 +169,18 @@ __global__ void __launch_bounds__(256) k_hmc_wave_init(int nd, int P, const int*
   uint32_t stage = (uint32_t)(bt.iter[b] - 1);
   uint32_t site = tune ? kSiteTuneMom : kSiteHmcMom;
   int s0 = blockIdx.y * 4 + threadIdx.x / 64;
+  __bf16* XX = h.XXa ? (__bf16*)h.XXa + (size_t)b * hmc_xx_rows(nd) * P * 2 : nullptr;
   double kin = 0.0;
   for (int d = s0; d < nd; d += gridDim.y * 4) {
-    Xt[(size_t)d * P + m] = Xc[(size_t)d * P + i];
+    double x = Xc[(size_t)d * P + i];
+    Xt[(size_t)d * P + m] = x;
+    if (XX) {  // the split copy read by the split-BF16 products: [d/8][column][hi 8 | lo 8]
+      __bf16 hi, lo;
+      split_bf16((float)x, hi, lo);
+      size_t o = (((size_t)(d / 8) * P + m) * 2) * 8 + d % 8;
+      XX[o] = hi;
+      XX[o + 8] = lo;
+    }
     double p = draw_normal(bt.key[b], site, stage, (uint32_t)j, (uint32_t)d) / sqrt(cov[d]);
     Pm[(size_t)d * P + m] = p;
     kin = kin + p * cov[d] * p / 2.0;
@@ -497,6 +506,155 @@ __global__ void __launch_bounds__(256) k_hmc_leap(int nd, int P, int s, int ntau
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 leapfrog gradients (SMCF_PRECISION_MIXED_BF16 with an HMC mover; no reference counterpart).  The wide
+// leapfrog product G = Q_g X is formed from split operands on the BF16 matrix cores, Q_g X ~ Qhi Xhi + Qhi Xlo +
+// Qlo Xhi with FP32 accumulation (about 2^-16 relative error per term), and the kick and drift follow in FP64.
+// The trajectory then follows a slightly different force, but every leapfrog map is still reversible and volume
+// preserving (kicks depend on position only), and the Metropolis test uses the FP64 Hamiltonian (H x of the final
+// position in FP64, k_hs_work): the sampler stays exact, only its acceptance can change.  Small waves (the narrow
+// kernel) stay FP64.
+// ---------------------------------------------------------------------------
+// Q_g in fragment order for v_mfma_f32_16x16x32_bf16: fragment f = (rt nks + st) 64 + l of lane l = r + 16g holds
+// Q_g(16 rt + r, 32 st + 8g + e), e = 0..7 (zero past ndim), hi at 2f and lo at 2f + 1
+__global__ void k_qg_to_bf(int nd, BatchDev bt, HmcDev h) {
+  int b = blockIdx.y;
+  if (!bt.active[b]) return;
+  int nrt = (nd + 15) / 16, nks = (nd + 31) / 32;
+  size_t nf = (size_t)nrt * nks * 64;
+  const double* Q = h.Qg + (size_t)b * nd * nd;
+  bf16x8* X = (bf16x8*)h.QgX + (size_t)b * nf * 2;
+  for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += (size_t)gridDim.x * blockDim.x) {
+    int l = (int)(f % 64);
+    size_t rest = f / 64;
+    int st = (int)(rest % nks), rt = (int)(rest / nks);
+    int r = 16 * rt + l % 16, k0 = 32 * st + 8 * (l / 16);
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      double v = (r < nd && k0 + e < nd) ? Q[(size_t)(k0 + e) * nd + r] : 0.0;
+      __bf16 a, c;
+      split_bf16((float)v, a, c);
+      hi[e] = a;
+      lo[e] = c;
+    }
+    X[2 * f] = hi;
+    X[2 * f + 1] = lo;
+  }
+}
+
+// the wide leapfrog step with split-BF16 products: tiles, items and epilogue as k_hmc_leap<4>; the positions' split
+// copy XX_in gives the B fragments (one 16-byte load per part), the epilogue writes the new positions to X_out and
+// their split copy to XX_out; lane l holds D(4 (l/16) + t, l%16) of a tile in register t (the FP32-accumulating
+// instruction's layout)
+__global__ void __launch_bounds__(256) k_hmc_leap_bf(int nd, int P, int s, int ntau, double max_slip,
+                                                     const double* __restrict__ Xin_all, double* __restrict__ Xout_all,
+                                                     const bf16x8* __restrict__ XXin_all, __bf16* __restrict__ XXout_all,
+                                                     const int* __restrict__ work, BatchDev bt, HmcDev h) {
+  constexpr int MT = kHsMT, NT = 4;
+  int nrg = row_groups(nd);
+  int item = blockIdx.x / nrg, rg = blockIdx.x % nrg;
+  int b = work[2 * item], p0 = work[2 * item + 1] * 16 * NT;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int i0 = (rg * 4 + wave) * 16 * MT;
+  int n = h.ncol[b];
+  if (i0 >= nd || p0 >= n) return;
+  int nrt = (nd + 15) / 16, nks = (nd + 31) / 32, rt0 = i0 / 16;
+  size_t xrows = hmc_xx_rows(nd);
+  const bf16x8* QX = (const bf16x8*)h.QgX + (size_t)b * nrt * nks * 64 * 2;
+  const bf16x8* XXin = XXin_all + (size_t)b * (xrows / 8) * P * 2;
+  const double* X = Xin_all + (size_t)b * nd * P;
+  int fi = lane % 16, fk = lane / 16;
+  int pcol[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) pcol[q] = min(p0 + 16 * q + fi, n - 1);
+  flt4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int q = 0; q < NT; ++q) acc[m][q] = flt4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+  bf16x8 ah[2][MT], al[2][MT], bh[2][NT], bl[2][NT];
+  auto load = [&](int k, int st) {
+    st = min(st, nks - 1);  // a prefetch past the end re-reads the last step, unused
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      bool in = rt0 + m < nrt;  // row tiles past the last one (wave-uniform)
+      size_t f = ((size_t)(rt0 + m) * nks + st) * 64 + lane;
+      ah[k][m] = in ? QX[2 * f] : zero8;
+      al[k][m] = in ? QX[2 * f + 1] : zero8;
+    }
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+      size_t o = ((size_t)(4 * st + fk) * P + pcol[q]) * 2;
+      bh[k][q] = XXin[o];
+      bl[k][q] = XXin[o + 1];
+    }
+  };
+  auto prod = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[k][m], bh[k][q], acc[m][q], 0, 0, 0);
+        acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[k][m], bl[k][q], acc[m][q], 0, 0, 0);
+        acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[k][m], bh[k][q], acc[m][q], 0, 0, 0);
+      }
+  };
+  load(0, 0);
+  load(1, 1);
+  int full = nks - nks % 2;
+  for (int st = 0; st < full; st += 2) {
+    prod(0);
+    load(0, st + 2);
+    prod(1);
+    load(1, st + 3);
+  }
+  if (nks % 2) prod(0);
+  double* Xo = Xout_all + (size_t)b * nd * P;
+  __bf16* XXo = XXout_all + (size_t)b * xrows * P * 2;
+  double* Pm = h.Pm + (size_t)b * nd * P;
+  const double* bv = bt.bvec + (size_t)b * nd;
+  const double* cov = bt.cov + (size_t)b * nd;
+  double gam = bt.gamma[b];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    int col = p0 + 16 * q + fi;
+    if (col >= n) continue;
+    size_t cb = (size_t)b * P + col;
+    if (h.dead[cb]) continue;
+    double dt = h.dtau[(size_t)b * P + h.cchild[cb]];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int row = i0 + 16 * m + 4 * fk + t;
+        if (row >= nd) continue;
+        size_t e = (size_t)row * P + col;
+        double g = (double)acc[m][q][t] - gam * bv[row];
+        double p = Pm[e];
+        p = (s == 0) ? p - g * 5e-1 * dt : p - g * dt;
+        double x = X[e];
+        x = (s < ntau) ? x + cov[row] * p * dt : x + cov[row] * p * 5e-1 * dt;
+        if (fabs(x) > 1e3) {
+          h.dead[cb] = 1;
+          continue;
+        }
+        while (x < 0.0 || x > max_slip) {
+          if (x > max_slip) { x = 2.0 * max_slip - x; p = -p; }
+          if (x < 0.0) { x = -x; p = -p; }
+        }
+        Xo[e] = x;
+        Pm[e] = p;
+        __bf16 hi, lo;
+        split_bf16((float)x, hi, lo);
+        size_t o = (((size_t)(row / 8) * P + col) * 2) * 8 + row % 8;
+        XXo[o] = hi;
+        XXo[o + 8] = lo;
+      }
+  }
+}
+
 // The narrow tile: rows i0 .. i0 + 63, columns p0 .. p0 + 15 of A X with the contraction split over the 4
 // wavefronts, then either Y (LEAP = false) or the leapfrog kick and drift of k_hmc_leap (LEAP = true).
 template <bool LEAP>
@@ -782,7 +940,7 @@ __global__ void k_hmc_adapt(int P, BatchDev bt, HmcDev h, int B, double target) 
 // instance's 64-column tiles; ww: the wide instances' 64-column tiles; wn: the narrow instances' 16-column tiles)
 static hipError_t run_waves(const MeshDev& m, BatchDev& bt, HmcDev& h, const HmcPassHost& ph, const int* wa,
                             const int* ww, const int* wn, int B, int tune, double max_slip, int ntau, int record_acc,
-                            hipStream_t s) {
+                            int bf, hipStream_t s) {
   int P = bt.P, nd = m.ndim, nn2 = 2 * m.nnode;
   int nrg = (((nd + 16 * kHsMT - 1) / (16 * kHsMT)) + 3) / 4, nrb = (nd + 16 * kHsMT - 1) / (16 * kHsMT);
   dim3 gc((P + 127) / 128, B);
@@ -800,7 +958,10 @@ static hipError_t run_waves(const MeshDev& m, BatchDev& bt, HmcDev& h, const Hmc
     const double* xin = h.Xt;
     double* xout = h.Xt2;
     for (int st = 0; st <= ntau; ++st) {
-      if (nw)
+      if (nw && bf)
+        hipLaunchKernelGGL(k_hmc_leap_bf, dim3(nw * nrg), dim3(256), 0, s, nd, P, st, ntau, max_slip, xin, xout,
+                           (const bf16x8*)(st % 2 ? h.XXb : h.XXa), (__bf16*)(st % 2 ? h.XXa : h.XXb), lw, bt, h);
+      else if (nw)
         hipLaunchKernelGGL(k_hmc_leap<4>, dim3(nw * nrg), dim3(256), 0, s, nd, P, st, ntau, max_slip, xin, xout, lw,
                            bt, h);
       if (nn)
@@ -879,6 +1040,11 @@ hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, HmcHost& 
   hipLaunchKernelGGL(k_idstart, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   hipLaunchKernelGGL(k_hmc_prep, dim3(B), dim3(256), 0, s, P, bt, h);
   launch_form_qg(m.ndim, bt, h, B, s);  // the stage's gamma is set
+  if (cfg.bf) {
+    size_t nf = (size_t)((m.ndim + 15) / 16) * ((m.ndim + 31) / 32) * 64;
+    hipLaunchKernelGGL(k_qg_to_bf, dim3((unsigned)std::min<size_t>((nf + 255) / 256, 1024), B), dim3(256), 0, s,
+                       m.ndim, bt, h);
+  }
   // wave counts, then the wave pointers, for the work lists of the passes
   hh.amax.resize(2 * (size_t)B);
   hipError_t e = hipMemcpyAsync(hh.amax.data(), h.wamax, sizeof(int) * B, hipMemcpyDeviceToHost, s);
@@ -905,19 +1071,19 @@ hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, HmcHost& 
   if (accept) {
     hipLaunchKernelGGL(k_hmc_dtau_fill, dim3((P + 127) / 128, B), dim3(128), 0, s, P, bt, h);
     hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
-    e = run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, cfg.ntau, 1, s);
+    e = run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, cfg.ntau, 1, cfg.bf, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_hmc_adapt, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B, cfg.accept);
     return hipGetLastError();
   }
   // tuning
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 1);
-  e = run_waves(m, bt, h, hh.pass[1], h.wla + wl, h.wlw + wl, h.wln + wl, B, 1, max_slip, kNtauRef, 0, s);
+  e = run_waves(m, bt, h, hh.pass[1], h.wla + wl, h.wlw + wl, h.wln + wl, B, 1, max_slip, kNtauRef, 0, cfg.bf, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_tune_resample, dim3((B + 63) / 64), dim3(64), 0, s, P, bt, h, B);
   // sampling
   hipLaunchKernelGGL(k_hmc_chain_init, dim3((P + 127) / 128, B), dim3(128), 0, s, m.ndim, P, bt, h, 0);
-  return run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, kNtauRef, 0, s);
+  return run_waves(m, bt, h, hh.pass[0], h.wla, h.wlw, h.wln, B, 0, max_slip, kNtauRef, 0, cfg.bf, s);
 }
 
 // ---------------------------------------------------------------------------

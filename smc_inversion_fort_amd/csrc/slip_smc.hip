@@ -721,8 +721,6 @@ constexpr int kMixedMT = SMCF_MIXED_MT;    // 16-row tiles per block
 constexpr int kMixedNB = SMCF_MIXED_NB;    // register buffers of the matrix-phase pipeline
 constexpr int kMixedOcc = SMCF_MIXED_OCC;  // waves per SIMD the register budget is sized for
 
-typedef float flt4 __attribute__((ext_vector_type(4)));
-
 // mfma_panel_t in FP32: acc[m][n] += sum_j A(16m + r, j) B(j, c_n), same operand addressing and pipeline
 template <int MT, int NB>
 __device__ __forceinline__ void mfma_panel_f32(const float* __restrict__ abase, const uint32_t (&ao)[MT],
@@ -939,13 +937,6 @@ __global__ void __launch_bounds__(256, kMixedOcc)
 constexpr int kBfMT = SMCF_BF_MT;    // 16-row tiles per block
 constexpr int kBfNB = SMCF_BF_NB;    // register buffers of the matrix-phase pipeline
 constexpr int kBfOcc = SMCF_BF_OCC;  // waves per SIMD the register budget is sized for
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-SMCF_DEV void split_bf16(float x, __bf16& h, __bf16& l) {
-  h = (__bf16)x;
-  l = (__bf16)(x - (float)h);
-}
 
 // acc[m][n] += sum over nst 32-wide steps of A_m B_n in split BF16.  A fragment (st, m) at
 // qa[ao[m] + st astep] (hi) and +1 (lo); B fragment (st, n) at sb[bo[n] + st bstep] (hi) and +1 (lo);

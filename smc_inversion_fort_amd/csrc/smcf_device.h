@@ -373,6 +373,15 @@ SMCF_DEV double block_min(double v, double* red) {
 // FP64 matrix-core panel shared by the Gibbs, likelihood and HMC kernels
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi): the operands of the split-BF16 products
+SMCF_DEV void split_bf16(float x, __bf16& h, __bf16& l) {
+  h = (__bf16)x;
+  l = (__bf16)(x - (float)h);
+}
+
 #ifndef SMCF_PANEL_BUFFERS
 #define SMCF_PANEL_BUFFERS 2
 #endif

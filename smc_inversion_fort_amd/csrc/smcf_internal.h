@@ -133,6 +133,8 @@ struct HmcDev {
   int* accf;                          // Metropolis outcome per wave column
   int *wla, *wlw, *wln;               // work lists (instance, column tile) of the sampling and tuning passes
   double* part;                       // per-column partial sums of the wave kernels (32 x 6 x P per instance)
+  void* QgX;                          // split-BF16 Q_g in fragment order (SMCF_PRECISION_MIXED_BF16)
+  void *XXa, *XXb;                    // split-BF16 positions (ping-pong with Xt / Xt2), [d/8][column][hi 8 | lo 8]
   double *dtau, *dtmp, *score;   // per child
   double *ldlo, *ldhi;           // per instance log-dtau range
   int *wptr, *wcols, *tptr, *tcols, *wamax, *tamax;  // wave lists (sampling / tuning)
@@ -153,6 +155,12 @@ struct HmcHost {
   std::vector<int> amax;  // wave counts per instance: sampling (B), then tuning (B)
   HmcPassHost pass[2];    // sampling, tuning
 };
+
+// rows of the split-BF16 position copies (whole 32-row contraction steps; rows past ndim stay zero)
+__host__ __device__ inline size_t hmc_xx_rows(int nd) { return (size_t)((nd + 31) / 32) * 32; }
+
+// 16-byte fragments (hi and lo) of the split-BF16 Q_g of one instance
+inline size_t hmc_qgx_frags(int nd) { return (size_t)((nd + 15) / 16) * ((nd + 31) / 32) * 64 * 2; }
 
 // entries (ints) of one pass's work list: per instance at most P / 16 + P + 1 items of 2 ints
 inline size_t hmc_work_entries(int cap, int P) { return (size_t)cap * 2 * ((size_t)P + P / 16 + 1); }
@@ -229,6 +237,7 @@ hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 struct HmcCfg {
   int mover, ntau;
   double accept;
+  int bf;  // leapfrog gradients from split-BF16 products (SMCF_PRECISION_MIXED_BF16)
 };
 hipError_t launch_hmc_init(BatchDev& bt, HmcDev& h, int B, hipStream_t s, const int* sel, int mover);
 hipError_t launch_hmc_stage(const MeshDev& m, BatchDev& bt, HmcDev& h, HmcHost& hh, int B, double max_slip,

@@ -165,3 +165,35 @@ def test_hmc_batch_matches_solo(ctx, mover):
         assert s1[0] == ns[k] and n1[0] == nl[k], (k, s1[0], ns[k], n1[0], nl[k])
     pb.close()
 
+
+@pytest.mark.parametrize("mover", [S.MOVER_HMC, S.MOVER_HMC_ACCEPT])
+def test_hmc_split_bf16_gradients_vs_fp64(ctx, mover):
+    """The split-BF16 form of the HMC movers (leapfrog gradients from split-BF16 products, FP64 Hamiltonian in the
+    Metropolis test): at c1 sizes over 8 keys, -log Z and the posterior means agree with the FP64 form within Monte
+    Carlo error, and the first tempering step (before any move) is identical."""
+    say("case: HMC split-BF16 gradients vs FP64, mover", mover)
+    c = syn.CONFIGS["c1"]
+    obs = device_obs(ctx, "c1", 3)
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    P = 1000
+    kw = dict(mover=mover, hmc_ntau=10, hmc_accept=0.65) if mover == S.MOVER_HMC_ACCEPT else dict(mover=mover)
+    res = {}
+    for prec in (S.PRECISION_FP64, S.PRECISION_MIXED_BF16):
+        pb = S.Problem(ctx, [c["nxi"]], [c["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, P, precision=prec, **kw)
+        theta = syn.theta_true()
+        out = [pb.slip_smc_exec(theta, O.inner_key(40 + k, 0, 0, 0)) for k in range(8)]
+        res[prec] = out
+        pb.close()
+    f, m = res[S.PRECISION_FP64], res[S.PRECISION_MIXED_BF16]
+    for a, b in zip(f, m):
+        assert a["gamma"][0] == b["gamma"][0]
+    nf, nm = np.array([r["neglog"] for r in f]), np.array([r["neglog"] for r in m])
+    se = math.sqrt(nf.var(ddof=1) / 8 + nm.var(ddof=1) / 8)
+    say("-log Z fp64 %.4f split-bf16 %.4f se %.4f; stages" % (nf.mean(), nm.mean(), se),
+        [r["nstage"] for r in f], [r["nstage"] for r in m])
+    assert abs(nf.mean() - nm.mean()) < 4 * se + 0.05
+    mf = np.array([r["particles"].mean(axis=1) for r in f])
+    mm = np.array([r["particles"].mean(axis=1) for r in m])
+    sem = np.sqrt(mf.var(axis=0, ddof=1) / 8 + mm.var(axis=0, ddof=1) / 8)
+    assert np.all(np.abs(mf.mean(axis=0) - mm.mean(axis=0)) < 4 * sem + 0.02 * np.abs(mf.mean(axis=0)).max())
+
