@@ -638,7 +638,8 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   dalloc(P, &bt.bvec, nd * C);
   dalloc(P, &bt.dsd, C);
   bt.kb = gibbs_kb(P.precision);
-  dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked Gibbs layout, padded to whole blocks
+  bt.Q = nullptr;
+  if (P.mover == SMCF_MOVER_GIBBS) dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked layout, whole blocks
   bt.Q32 = nullptr;
   bt.S32 = bt.S32b = nullptr;
   bt.SX = bt.SXb = bt.QX = nullptr;
