@@ -78,6 +78,9 @@ def parse():
     ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
                     help="c2: the headline (16x16 patches, Gibbs x5, FP64 and mixed forms); c3: 32x16 patches, "
                          "3000 observations, HMC with 10 leapfrog steps tuned to 0.65 acceptance, FP64")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the Gibbs sweep kernels' block-row variants per mesh size and form, print the table "
+                         "(the library's tuning table is this output) and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
@@ -314,6 +317,51 @@ def run_serving(a, S, syn):
                         "N 64..512 outer particles, 3 outer stages, inner P=512 Gibbs x3"}
 
 
+TUNE_MESHES = [(4, 4), (8, 8), (12, 12), (16, 16), (24, 16), (24, 24)]
+
+
+def run_autotune(a, S, syn):
+    """For each mesh and form, the Gibbs kernel time per instance-stage of every block-row variant on the same
+    inner SMCs (32 geometries through 32 slots, P = 2000, Gibbs x5), printed as one JSON line per case and a final
+    table of the fastest variant per ndim."""
+    xmin, xmax, zmin, _ = syn.fault_frame()
+    ctx = S.Context(0)
+
+    def greens(th, pts, uv, nx, ne):
+        obs0 = dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.ones(pts.shape[1]), dvec=np.zeros(pts.shape[1]),
+                    nsar_total=0, ngnss=pts.shape[1] // 3)
+        p = S.Problem(ctx, [nx], [ne], xmin, xmax, zmin, obs0, syn.MAX_SLIP, 8)
+        g = p.calc_greens_func(th)
+        p.close()
+        return g
+    table = []
+    for nxi, neta in TUNE_MESHES:
+        obs = syn.make_observations("c2", a.seed, greens, nxi=nxi, neta=neta)
+        th = syn.sample_thetas(32, a.seed)
+        keys = keys_for(a.seed, 32)
+        for form, rows_list in (("fp64", (16, 32)), ("bf16", (16, 32, 48))):
+            prec = S.PRECISION_FP64 if form == "fp64" else S.PRECISION_MIXED_BF16
+            best = None
+            for rows in rows_list:
+                S.set_gibbs_block_rows(prec, 1, 1 << 30, rows)
+                pb = S.Problem(ctx, [nxi], [neta], xmin, xmax, zmin, obs, syn.MAX_SLIP, 2000, mover=S.MOVER_GIBBS,
+                               gibbs_sweeps=5, max_batch=32, precision=prec)
+                ctx.set_profiling(True)
+                nl, ns = pb.fault_calc_likelihood(th, keys)
+                n, ms = ctx.kernel_stats("gibbs")
+                moves = float((ns - (nl >= 1e10)).sum())
+                us = 1e3 * ms / moves
+                print(json.dumps({"autotune": form, "nxi": nxi, "neta": neta, "ndim": pb.ndim, "rows": rows,
+                                  "us_per_instance_stage": round(us, 2), "launches": n}), flush=True)
+                if best is None or us < best[1]:
+                    best = (rows, us)
+                ndim = pb.ndim
+                pb.close()
+            table.append({"form": form, "ndim": ndim, "rows": best[0], "us_per_instance_stage": round(best[1], 2)})
+    ctx.close()
+    print(json.dumps({"autotune_table": table}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -326,6 +374,9 @@ def main():
     import smc_inversion_fort_amd as S
     from smc_inversion_fort_amd import synthetic as syn
 
+    if a.autotune:
+        run_autotune(a, S, syn)
+        return
     if a.copy_probe:
         import torch
         x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")

@@ -94,6 +94,13 @@ int smcf_problem_bytes(const smcf_problem_desc* desc, int max_batch, size_t* byt
 int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes);
 /* The context a problem was created on. */
 smcf_ctx* smcf_problem_context(const smcf_problem* pb);
+/* Rows per coordinate block of the Gibbs sweep kernels (no reference counterpart): the value a problem of this form
+ * and ndim created now would use, from the library's tuning table (measured on the MI355X by the bench's
+ * --autotune mode).  The rows set the order in which a conditional mean's terms are added, so results are
+ * reproducible for a given value; smcf_set_gibbs_block_rows overrides the table for ndim_lo..ndim_hi (FP64: 16 or
+ * 32; split BF16: 16, 32 or 48) for problems created afterwards. */
+int smcf_gibbs_block_rows(int precision, int ndim);
+int smcf_set_gibbs_block_rows(int precision, int ndim_lo, int ndim_hi, int rows);
 int smcf_problem_destroy(smcf_problem* pb);
 int smcf_problem_dims(const smcf_problem* pb, int* nnode, int* ndof, int* ndim_slip, int* ndim_fault,
                       int* nobs);

@@ -71,6 +71,7 @@ EXPORTS = [
     "smcf_calc_responce", "smcf_slip_likelihood_gram", "smcf_slip_prior_ell", "smcf_slip_grad_ell",
     "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault", "smcf_server_set_max_running",
     "smcf_outer_reweight_resample", "smcf_fault_smc_set_outer_stage", "smcf_problem_context",
+    "smcf_gibbs_block_rows", "smcf_set_gibbs_block_rows",
 ]
 
 
@@ -157,6 +158,8 @@ def _declare(L):
     L.smcf_outer_reweight_resample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int,
                                                C.c_int, _D, _D, _D, _D, _D, _D, _D, _I, _I, _I]
     L.smcf_fault_smc_set_outer_stage.argtypes = [C.c_void_p, C.c_int]
+    L.smcf_gibbs_block_rows.argtypes = [C.c_int, C.c_int]
+    L.smcf_set_gibbs_block_rows.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     L.smcf_problem_context.argtypes = [C.c_void_p]
     L.smcf_problem_context.restype = C.c_void_p
     L.smcf_work_mcmc_sampling.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_double, _D, _D, C.c_int, _I, _D, _D,
@@ -272,6 +275,19 @@ def fault_reorder_to_send(assigned, X, L, q=None):
     sidx = np.zeros(a.shape[0], np.int32)
     _check(lib().smcf_fault_reorder_to_send(Xc.shape[0], a.shape[0], _ip(a), _dp(Xc), _dp(Lc), _opt(qc), _ip(sidx)))
     return a, Xc, Lc, qc, sidx
+
+
+def gibbs_block_rows(precision, ndim):
+    """Rows per coordinate block the Gibbs sweep kernels use for a problem of this form and ndim (tuning table)."""
+    r = lib().smcf_gibbs_block_rows(int(precision), int(ndim))
+    if r < 0:
+        raise SmcfError("libsmcf error %d: %s" % (r, lib().smcf_last_error().decode()))
+    return r
+
+
+def set_gibbs_block_rows(precision, ndim_lo, ndim_hi, rows):
+    """Override the tuning table for ndim_lo..ndim_hi (problems created afterwards)."""
+    _check(lib().smcf_set_gibbs_block_rows(int(precision), int(ndim_lo), int(ndim_hi), int(rows)))
 
 
 def outer_reweight_resample(ctx, X, L, gamma_prev, seed, stage, q=None, device_sort=False):

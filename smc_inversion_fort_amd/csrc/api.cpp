@@ -637,7 +637,7 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   dalloc(P, &bt.H, nd * nd * C);
   dalloc(P, &bt.bvec, nd * C);
   dalloc(P, &bt.dsd, C);
-  bt.kb = gibbs_kb(P.precision);
+  bt.kb = gibbs_kb(P.precision, (int)nd);
   bt.Q = nullptr;
   if (P.mover == SMCF_MOVER_GIBBS) dalloc(P, &bt.Q, (nd + bt.kb) * nd * C);  // blocked layout, whole blocks
   bt.Q32 = nullptr;
@@ -782,6 +782,20 @@ int smcf_problem_device_bytes(const smcf_problem* pb, size_t* bytes) {
 }
 
 smcf_ctx* smcf_problem_context(const smcf_problem* pb) { return pb ? pb->p.ctx : nullptr; }
+
+int smcf_gibbs_block_rows(int precision, int ndim) {
+  if (ndim < 1) return SMCF_EINVAL;
+  return gibbs_kb(precision, ndim);
+}
+
+int smcf_set_gibbs_block_rows(int precision, int ndim_lo, int ndim_hi, int rows) {
+  if (ndim_lo < 1 || ndim_hi < ndim_lo || !gibbs_rows_valid(precision, rows)) {
+    set_error("set_gibbs_block_rows: rows 16 or 32 (FP64), 16, 32 or 48 (split BF16), and 1 <= ndim_lo <= ndim_hi");
+    return SMCF_EINVAL;
+  }
+  set_gibbs_rows(precision, ndim_lo, ndim_hi, rows);
+  return SMCF_OK;
+}
 
 int smcf_problem_destroy(smcf_problem* pb) {
   if (!pb) return SMCF_OK;

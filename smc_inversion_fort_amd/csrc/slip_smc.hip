@@ -8,6 +8,9 @@
 //  k_moments         weighted mean / variance diagonal in double-double (smc_slip.f90:485-570)
 //  k_gather          children <- parents (index map from the resampling)
 //  k_gibbs           truncated-MVN Gibbs sweeps on the tempered posterior (benchmark mover)
+#include <mutex>
+#include <vector>
+
 #include "smcf_device.h"
 #include "smcf_internal.h"
 
@@ -546,13 +549,14 @@ constexpr int kGibbsKB = 16 * kGibbsMT;
 #endif
 constexpr int kGibbsOcc = SMCF_GIBBS_OCC;  // waves per SIMD the register budget of k_gibbs is sized for
 
+template <int MT>
 __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
                                                const double* __restrict__ Qball, double* __restrict__ S0,
                                                double* __restrict__ S1, const double* __restrict__ cdall,
                                                const uint64_t* __restrict__ keys,
                                                const int* __restrict__ iters, const int* __restrict__ active,
                                                const int* __restrict__ par) {
-  constexpr int MT = kGibbsMT, KB = kGibbsKB;
+  constexpr int KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
   //   CQ[kk (kk - 1) / 2 + k] = Q(i0 + 16m + k, i0 + 16m + kk), k < kk: the strictly lower part of the tile's
@@ -1001,6 +1005,7 @@ __device__ __forceinline__ void mfma_panel_bf3(const bf16x8* __restrict__ qa, co
     if (k < nst - full) prod(k);
 }
 
+template <int MT>
 __global__ void __launch_bounds__(256, kBfOcc)
     k_gibbs_bf(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
                const bf16x8* __restrict__ QXall, double* __restrict__ S0, double* __restrict__ S1,
@@ -1008,7 +1013,7 @@ __global__ void __launch_bounds__(256, kBfOcc)
                const double* __restrict__ mrefall, const double* __restrict__ cdall,
                const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
                const int* __restrict__ par) {
-  constexpr int MT = kBfMT, KB = 16 * MT;
+  constexpr int KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
   //   CQ[kk][k] = Q(i0 + 16m + k, i0 + 16m + kk), the tile's diagonal block (in-tile corrections)
@@ -1447,9 +1452,43 @@ hipError_t launch_like_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) 
   return hipGetLastError();
 }
 
-int gibbs_kb(int precision) {
-  return precision == SMCF_PRECISION_MIXED ? 16 * kMixedMT : precision == SMCF_PRECISION_MIXED_BF16 ? 16 * kBfMT
-                                                                                                     : kGibbsKB;
+// Block rows of the Gibbs sweep kernels per (form, ndim): the FP64 and split-BF16 kernels come in 16-, 32- and (split
+// BF16) 48-row variants.  The rows change the order in which a conditional mean's terms are added (the block's own
+// earlier coordinates enter as corrections), so the choice comes from a fixed table, not from timing at run time:
+// a problem's results are reproducible.  The table is the output of the bench's --autotune mode on the MI355X
+// (fastest variant per ndim range); smcf_set_gibbs_block_rows replaces entries for problems created afterwards.
+namespace {
+struct BlockRule {
+  int precision, ndim_lo, ndim_hi, rows;
+};
+std::mutex g_tile_mu;
+// measured by the bench's --autotune mode: 16 rows win up to ndim 112 in both forms, 32 (FP64) and 48 (split
+// BF16) from ndim 264 on; the boundary between the measured sizes is put at 160
+std::vector<BlockRule> g_tile_rules = {
+    {SMCF_PRECISION_FP64, 1, 160, 16},
+    {SMCF_PRECISION_FP64, 161, 1 << 30, kGibbsKB},
+    {SMCF_PRECISION_MIXED_BF16, 1, 160, 16},
+    {SMCF_PRECISION_MIXED_BF16, 161, 1 << 30, 16 * kBfMT},
+};
+}  // namespace
+
+int gibbs_kb(int precision, int nd) {
+  if (precision == SMCF_PRECISION_MIXED) return 16 * kMixedMT;
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  for (auto it = g_tile_rules.rbegin(); it != g_tile_rules.rend(); ++it)  // the latest matching rule
+    if (it->precision == precision && nd >= it->ndim_lo && nd <= it->ndim_hi) return it->rows;
+  return precision == SMCF_PRECISION_MIXED_BF16 ? 16 * kBfMT : kGibbsKB;
+}
+
+bool gibbs_rows_valid(int precision, int rows) {
+  if (precision == SMCF_PRECISION_FP64) return rows == 16 || rows == 32;
+  if (precision == SMCF_PRECISION_MIXED_BF16) return rows == 16 || rows == 32 || rows == 48;
+  return false;
+}
+
+void set_gibbs_rows(int precision, int lo, int hi, int rows) {
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  g_tile_rules.push_back({precision, lo, hi, rows});
 }
 
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
@@ -1458,18 +1497,20 @@ hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& c
   int groups = (B + kXcd - 1) / kXcd;
   dim3 g(groups * kXcd * chunks);
   if (bt.QX) {
-    if (bt.kb != 16 * kBfMT) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gibbs_bf, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q,
-                       (const bf16x8*)bt.QX, bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, (bf16x8*)bt.DX, bt.mref,
-                       bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+    auto k = bt.kb == 16 ? k_gibbs_bf<1> : bt.kb == 32 ? k_gibbs_bf<2> : bt.kb == 48 ? k_gibbs_bf<3> : nullptr;
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, (const bf16x8*)bt.QX,
+                       bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, (bf16x8*)bt.DX, bt.mref, bt.qdiag, bt.key,
+                       bt.iter, bt.active, bt.par);
   } else if (bt.Q32) {
     if (bt.kb != 16 * kMixedMT) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.Q32, bt.S,
                        bt.S2, bt.S32, bt.S32b, bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
   } else {
-    if (bt.kb != kGibbsKB) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gibbs, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2,
-                       bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+    auto k = bt.kb == 16 ? k_gibbs<1> : bt.kb == 32 ? k_gibbs<2> : nullptr;
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2, bt.qdiag,
+                       bt.key, bt.iter, bt.active, bt.par);
   }
   return hipGetLastError();
 }

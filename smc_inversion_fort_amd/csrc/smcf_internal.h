@@ -214,7 +214,9 @@ hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
 // columns per block of the blocked Gibbs precision layout (SMCF_PRECISION_*)
-int gibbs_kb(int precision);
+int gibbs_kb(int precision, int nd);
+bool gibbs_rows_valid(int precision, int rows);
+void set_gibbs_rows(int precision, int lo, int hi, int rows);
 // FP32 copies of the current particles of the given slots (mixed precision)
 hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 // split-BF16 copies of the current particles of the given slots (SMCF_PRECISION_MIXED_BF16)
