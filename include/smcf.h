@@ -324,6 +324,32 @@ int smcf_outer_reweight_resample(smcf_ctx* ctx, int ndim, int N, double gamma_pr
 int smcf_fault_smc_set_outer_stage(smcf_problem* pb, int mode);
 
 /*
+ * Groups: many independent inversions in one stage loop (no counterpart in the reference, whose only batching is
+ * the MPI scatter of one inversion's outer particles, smc_fault.f90:896-1069).  The members are problems of
+ * different meshes, observation sets, particle counts and forms (Gibbs mover), created on the group's context;
+ * every stage kernel runs once over the instances of all members (the Gibbs sweeps once per kernel class: form
+ * and block rows of the tuning table), and with graphs on (the default) a stage is one HIP graph launch.  Each
+ * member's results equal those of the same calls on the member alone.  A problem belongs to one group at a time.
+ *   smcf_group_calc_likelihood_batch  smcf_fault_calc_likelihood_batch for every member at once: B[i] geometries
+ *                                     (theta[i] ndim_fault x B[i], keys[i]) stream through member i's slots;
+ *                                     nstage may be NULL, B[i] may be 0
+ *   smcf_group_fault_smc_exec         smcf_fault_smc_exec for every member at once (range[i], N[i], seed[i];
+ *                                     particles[i] ndim_fault x N[i], likelihood[i] N[i]; nstage and evidence
+ *                                     N-less arrays of one entry per member, may be NULL): each member's outer SMC
+ *                                     runs on its own host thread, its waves served by the shared stage loop
+ *   smcf_group_set_graph              1: capture each call's stage launches into a HIP graph (default); 0: launch
+ *                                     them one by one (identical results) */
+typedef struct smcf_group smcf_group;
+int smcf_group_create(smcf_ctx* ctx, int n, smcf_problem* const* members, smcf_group** out);
+int smcf_group_set_graph(smcf_group* g, int on);
+int smcf_group_calc_likelihood_batch(smcf_group* g, const int* B, const double* const* theta,
+                                     const uint64_t* const* keys, double* const* neglog, int* const* nstage);
+int smcf_group_fault_smc_exec(smcf_group* g, const double* const* range, const int* N, const uint64_t* seed,
+                              int max_stages, double* const* particles, double* const* likelihood, int* nstage,
+                              double* evidence);
+int smcf_group_destroy(smcf_group* g);
+
+/*
  * Serving (no counterpart in the reference, whose main.f90 runs one inversion per job).  A server admits
  * inversion requests - a problem description plus the outer-SMC settings of smcf_fault_smc_exec - against a
  * device-memory budget as they arrive (first fit in arrival order, sized by smcf_problem_bytes), and runs each

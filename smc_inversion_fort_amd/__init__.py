@@ -71,7 +71,8 @@ EXPORTS = [
     "smcf_calc_responce", "smcf_slip_likelihood_gram", "smcf_slip_prior_ell", "smcf_slip_grad_ell",
     "smcf_slip_calc_gsd_gsg_gmat", "smcf_discretize_fault", "smcf_server_set_max_running",
     "smcf_outer_reweight_resample", "smcf_fault_smc_set_outer_stage", "smcf_problem_context",
-    "smcf_gibbs_block_rows", "smcf_set_gibbs_block_rows",
+    "smcf_gibbs_block_rows", "smcf_set_gibbs_block_rows", "smcf_group_create", "smcf_group_set_graph",
+    "smcf_group_calc_likelihood_batch", "smcf_group_fault_smc_exec", "smcf_group_destroy",
 ]
 
 
@@ -162,6 +163,12 @@ def _declare(L):
     L.smcf_set_gibbs_block_rows.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     L.smcf_problem_context.argtypes = [C.c_void_p]
     L.smcf_problem_context.restype = C.c_void_p
+    _PP = C.POINTER(C.c_void_p)
+    L.smcf_group_create.argtypes = [C.c_void_p, C.c_int, _PP, _PP]
+    L.smcf_group_set_graph.argtypes = [C.c_void_p, C.c_int]
+    L.smcf_group_calc_likelihood_batch.argtypes = [C.c_void_p, _I, _PP, _PP, _PP, _PP]
+    L.smcf_group_fault_smc_exec.argtypes = [C.c_void_p, _PP, _I, _U64, C.c_int, _PP, _PP, _I, _D]
+    L.smcf_group_destroy.argtypes = [C.c_void_p]
     L.smcf_work_mcmc_sampling.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_double, _D, _D, C.c_int, _I, _D, _D,
                                           _D, _D, C.c_double, C.c_int, _D, _D, _D, _I]
 
@@ -794,3 +801,67 @@ class Problem:
         _check(lib().smcf_fault_calc_likelihood_batch(self.h, B, _dp(np.asfortranarray(th)),
                                                       k.ctypes.data_as(_U64), _dp(out), _ip(ns)))
         return out, ns
+
+
+def _ptrs(arrays):
+    """C array of the data pointers of numpy arrays (None -> NULL)."""
+    out = (C.c_void_p * len(arrays))()
+    for i, a in enumerate(arrays):
+        out[i] = None if a is None else a.ctypes.data
+    return out
+
+
+class Group:
+    """Problems (Gibbs mover, one context) whose inner SMCs run in one stage loop: every stage kernel covers the
+    instances of all members, whatever their meshes, observation sets and particle counts, and each member's
+    results equal those of the same call on the member alone."""
+
+    def __init__(self, ctx, problems, graph=True):
+        self.ctx = ctx
+        self.problems = list(problems)
+        arr = (C.c_void_p * len(self.problems))(*[p.h.value for p in self.problems])
+        h = C.c_void_p()
+        _check(lib().smcf_group_create(ctx.h, len(self.problems), arr, C.byref(h)))
+        self.h = h
+        self.set_graph(graph)
+
+    def set_graph(self, on):
+        """Capture each call's stage launches into a HIP graph (on) or launch them one by one."""
+        _check(lib().smcf_group_set_graph(self.h, int(bool(on))))
+
+    def fault_calc_likelihood(self, thetas, keys):
+        """-log Z and stage counts of each member's geometries: thetas[i] (ndim_fault x B_i), keys[i] (B_i)."""
+        n = len(self.problems)
+        th = [self.problems[i]._theta(thetas[i]) if len(keys[i]) else None for i in range(n)]
+        ks = [np.ascontiguousarray(k, dtype=np.uint64) for k in keys]
+        B = np.array([len(k) for k in ks], dtype=np.int32)
+        out = [np.zeros(b) for b in B]
+        ns = [np.zeros(b, dtype=np.int32) for b in B]
+        _check(lib().smcf_group_calc_likelihood_batch(self.h, _ip(B), _ptrs(th), _ptrs(ks), _ptrs(out), _ptrs(ns)))
+        return out, ns
+
+    def fault_smc_exec(self, ranges, Ns, seeds, max_stages=100):
+        """Each member's outer SMC (smcf_fault_smc_exec) at once: a list of dicts as Problem.fault_smc_exec returns
+        (without the gamma trace)."""
+        n = len(self.problems)
+        rg = [_f64(r) for r in ranges]
+        Nv = np.array(Ns, dtype=np.int32)
+        sd = np.array(seeds, dtype=np.uint64)
+        X = [np.zeros((p.ndim_fault, int(N)), order="F") for p, N in zip(self.problems, Nv)]
+        L = [np.zeros(int(N)) for N in Nv]
+        ns = np.zeros(n, dtype=np.int32)
+        ev = np.zeros(n)
+        _check(lib().smcf_group_fault_smc_exec(self.h, _ptrs(rg), _ip(Nv), sd.ctypes.data_as(_U64), int(max_stages),
+                                               _ptrs(X), _ptrs(L), _ip(ns), _dp(ev)))
+        return [dict(particles=X[i], likelihood=L[i], nstage=int(ns[i]), evidence=float(ev[i])) for i in range(n)]
+
+    def close(self):
+        if self.h:
+            lib().smcf_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -52,10 +52,6 @@ static int scratch_resolve(int device, double** out) {
   return SMCF_OK;
 }
 
-struct smcf_problem {
-  Problem p;
-};
-
 
 #define CHECK_LAUNCH(expr)                                                           \
   do {                                                                               \
@@ -66,29 +62,7 @@ struct smcf_problem {
     }                                                                                \
   } while (0)
 
-namespace {
-
-struct ProfScope {
-  smcf_ctx* ctx;
-  const char* name;
-  hipStream_t st;
-  hipEvent_t a{}, b{};
-  ProfScope(smcf_ctx* c, const char* n, hipStream_t s = nullptr) : ctx(c), name(n), st(s ? s : c->stream) {
-    if (ctx->profiling) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, st);
-    }
-  }
-  ~ProfScope() {
-    if (ctx->profiling) {
-      (void)hipEventRecord(b, st);
-      ctx->pending.push_back({name, {a, b}});
-    }
-  }
-};
-
-void resolve_profiles(smcf_ctx* ctx) {
+void smcf::resolve_profiles(smcf_ctx* ctx) {
   for (auto& pe : ctx->pending) {
     float ms = 0.f;
     (void)hipEventSynchronize(pe.second.second);
@@ -102,6 +76,8 @@ void resolve_profiles(smcf_ctx* ctx) {
   }
   ctx->pending.clear();
 }
+
+namespace {
 
 // Allocations of a problem.  After the first failure every later call returns that failure's code without
 // allocating, so problem_build can issue its allocations in sequence and check P.alloc_rc once.
@@ -177,19 +153,15 @@ __global__ void k_state_init(BatchDev bt, int B, const int* __restrict__ sel, in
   bt.par[b] = 0;
 }
 
-__global__ void k_activate(BatchDev bt, int B, const int* __restrict__ sel) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  bt.done[sel[b]] = 0;
-}
+}  // namespace
 
-// Set up n instances on stream s: slots 0 .. n-1 (sel == nullptr) or the slots listed in sel.
-// Thetas and keys must already be in their slots.  hold = 1 leaves them marked done.
-int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
+namespace smcf {
+
+int setup_pre(Problem& P, int n, const int* sel, hipStream_t s) {
   smcf_ctx* ctx = P.ctx;
   MeshDev& m = P.m;
   BatchDev& bt = P.bt;
-  hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel, hold);
+  hipLaunchKernelGGL(k_state_init, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, sel, 1);
   if (P.mover != SMCF_MOVER_GIBBS) CHECK_LAUNCH(launch_hmc_init(bt, P.hm, n, s, sel, P.mover));
   {
     ProfScope ps(ctx, "geometry_greens", s);
@@ -202,10 +174,12 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
     CHECK_LAUNCH(launch_gram(P, m, bt, n, s, sel));
   }
   if (bt.HX) CHECK_LAUNCH(launch_h_bf(m, bt, n, s, sel));
-  {
-    ProfScope ps(ctx, "init_chain", s);
-    CHECK_LAUNCH(launch_init_chain(m, bt, n, P.max_slip, s, sel));
-  }
+  return SMCF_OK;
+}
+
+int setup_post(Problem& P, int n, const int* sel, hipStream_t s) {
+  MeshDev& m = P.m;
+  BatchDev& bt = P.bt;
   if (bt.S32) CHECK_LAUNCH(launch_s32(m, bt, n, s, sel));
   if (bt.SX) CHECK_LAUNCH(launch_sbf(m, bt, n, s, sel));
   CHECK_LAUNCH(launch_like(m, bt, n, 0, s, sel));
@@ -213,152 +187,7 @@ int setup_slots(Problem& P, int n, const int* sel, hipStream_t s, int hold) {
   return SMCF_OK;
 }
 
-// Inner SMC for N geometries streamed through the problem's slots.  The first min(N, cap)
-// geometries start together.  After every stage the slots whose instance has finished are
-// harvested; once enough are free, the next geometries are set up in them on the set-up stream
-// (overlapping the stage loop, with the slots held as done so that the stage kernels skip them)
-// and released into the stage loop at the first stage boundary after their set-up completed.
-// Every instance keeps its own stage counter and RNG key, so a result does not depend on which
-// slot, batch or moment it ran in.  With N <= cap, slot b holds geometry b at the end.
-int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
-               int* nstage) {
-  smcf_ctx* ctx = P.ctx;
-  hipStream_t s = ctx->stream;
-  MeshDev& m = P.m;
-  BatchDev& bt = P.bt;
-  if (N < 1) {
-    set_error("no geometries");
-    return SMCF_EINVAL;
-  }
-  bool hmc = P.mover != SMCF_MOVER_GIBBS;
-  const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept, P.precision == SMCF_PRECISION_MIXED_BF16 ? 1 : 0};
-  const int nf = m.ndim_fault;
-  const auto t_call = std::chrono::steady_clock::now();
-  P.t_done.assign(N, 0.0);
-  P.t_admit.assign(N, 0.0);
-  const int W = std::min(N, bt.cap);  // slots in use
-  SMCF_HIP(hipMemcpyAsync(bt.theta, theta, sizeof(double) * nf * W, hipMemcpyHostToDevice, s));
-  SMCF_HIP(hipMemcpyAsync(bt.key, keys, sizeof(uint64_t) * W, hipMemcpyHostToDevice, s));
-  int rc = setup_slots(P, W, nullptr, s, 0);
-  if (rc) return rc;
-  std::vector<int> job(W), done(W), iter(W);
-  std::vector<double> nl(W);
-  for (int b = 0; b < W; ++b) job[b] = b;
-  int next = W, busy = W;
-  const int admit_min = std::max(1, W / 16);  // refill once this many slots are free
-  std::vector<int> freed;
-  struct Admission {
-    bool live = false;
-    int half = 0;
-    std::vector<int> slots, jobs;
-    std::vector<double> th;
-    std::vector<uint64_t> key;
-    hipEvent_t ready{};
-  } adm;
-  SMCF_HIP(hipEventCreateWithFlags(&adm.ready, hipEventDisableTiming));
-  auto fail = [&](int code) {
-    (void)hipStreamSynchronize(ctx->setup_stream);
-    (void)hipEventDestroy(adm.ready);
-    return code;
-  };
-  StageCfg cfg{P.max_slip, bt.P, P.sweeps};
-  while (busy > 0 || adm.live) {
-    if (busy > 0) {
-      {
-        ProfScope ps(ctx, "gamma_resample");
-        if (launch_gamma_resample(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
-      }
-      if ((want_moments || hmc) && launch_moments(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
-      if (hmc) {
-        ProfScope ps(ctx, "hmc");
-        if (launch_hmc_stage(m, bt, P.hm, P.hh, W, P.max_slip, hcfg, s) != hipSuccess)
-          return fail(SMCF_EHIP);
-      } else {
-        if (launch_gather(m, bt, W, s) != hipSuccess || launch_form_q(m, bt, W, s) != hipSuccess)
-          return fail(SMCF_EHIP);
-        if (bt.HX && launch_like_ref(m, bt, W, s) != hipSuccess) return fail(SMCF_EHIP);
-        {
-          ProfScope ps(ctx, "gibbs");
-          if (launch_gibbs(m, bt, W, cfg, s) != hipSuccess) return fail(SMCF_EHIP);
-        }
-        {
-          ProfScope ps(ctx, "likelihood");
-          // the Gibbs mover reads no prior values; they are formed once for the final particles
-          hipError_t e = bt.HX ? launch_like_bf(m, bt, W, s) : launch_like(m, bt, W, 1, s);
-          if (e != hipSuccess) return fail(SMCF_EHIP);
-        }
-      }
-      if (hipMemcpyAsync(done.data(), bt.done, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipMemcpyAsync(iter.data(), bt.iter, sizeof(int) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipMemcpyAsync(nl.data(), bt.neglog, sizeof(double) * W, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
-        return fail(SMCF_EHIP);
-      for (int b = 0; b < W; ++b)
-        if (job[b] >= 0 && done[b]) {
-          if (neglog) neglog[job[b]] = nl[b];
-          P.t_done[job[b]] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_call).count();
-          if (nstage) nstage[job[b]] = std::min(iter[b] - 1, 201);
-          job[b] = -1;
-          --busy;
-          freed.push_back(b);
-        }
-    }
-    // release a completed admission at this stage boundary
-    if (adm.live) {
-      hipError_t q = busy > 0 ? hipEventQuery(adm.ready) : hipEventSynchronize(adm.ready);
-      if (q == hipSuccess) {
-        int n = (int)adm.slots.size();
-        hipLaunchKernelGGL(k_activate, dim3((n + 63) / 64), dim3(64), 0, s, bt, n, bt.sel + adm.half * bt.cap);
-        if (hipGetLastError() != hipSuccess) return fail(SMCF_EHIP);
-        for (int q2 = 0; q2 < n; ++q2) job[adm.slots[q2]] = adm.jobs[q2];
-        busy += n;
-        adm.live = false;
-      } else if (q != hipErrorNotReady) {
-        return fail(SMCF_EHIP);
-      }
-    }
-    // start the next admission
-    if (!adm.live && next < N && !freed.empty() && ((int)freed.size() >= admit_min || busy == 0)) {
-      int n = std::min((int)freed.size(), N - next);
-      adm.half = 1 - adm.half;
-      adm.slots.assign(freed.begin(), freed.begin() + n);
-      freed.erase(freed.begin(), freed.begin() + n);
-      adm.jobs.resize(n);
-      adm.th.resize((size_t)nf * n);
-      adm.key.resize(n);
-      const double t_adm = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_call).count();
-      for (int q = 0; q < n; ++q) {
-        adm.jobs[q] = next + q;
-        P.t_admit[next + q] = t_adm;
-        std::copy(theta + (size_t)(next + q) * nf, theta + (size_t)(next + q + 1) * nf,
-                  adm.th.begin() + (size_t)q * nf);
-        adm.key[q] = keys[next + q];
-      }
-      hipStream_t s2 = ctx->setup_stream;
-      for (int q = 0; q < n; ++q) {
-        if (hipMemcpyAsync(bt.theta + (size_t)adm.slots[q] * nf, adm.th.data() + (size_t)q * nf, sizeof(double) * nf,
-                           hipMemcpyHostToDevice, s2) != hipSuccess ||
-            hipMemcpyAsync(bt.key + adm.slots[q], &adm.key[q], sizeof(uint64_t), hipMemcpyHostToDevice, s2) !=
-                hipSuccess)
-          return fail(SMCF_EHIP);
-      }
-      int* sel = bt.sel + adm.half * bt.cap;
-      if (hipMemcpyAsync(sel, adm.slots.data(), sizeof(int) * n, hipMemcpyHostToDevice, s2) != hipSuccess)
-        return fail(SMCF_EHIP);
-      rc = setup_slots(P, n, sel, s2, 1);
-      if (rc) return fail(rc);
-      if (hipEventRecord(adm.ready, s2) != hipSuccess) return fail(SMCF_EHIP);
-      next += n;
-      adm.live = true;
-    }
-  }
-  (void)hipEventDestroy(adm.ready);
-  SMCF_HIP(hipStreamSynchronize(s));
-  resolve_profiles(ctx);
-  return SMCF_OK;
-}
-
-}  // namespace
+}  // namespace smcf
 
 extern "C" {
 
@@ -689,6 +518,7 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   dalloc(P, &bt.sel, 2 * C);
   dalloc(P, &bt.par, C);
   dalloc(P, &bt.gtrace, 202 * C);
+  dalloc(P, &P.mdev, 1);
   if (P.mover != SMCF_MOVER_GIBBS) {
     HmcDev& h = P.hm;
     size_t nn2 = 2 * (size_t)nnode;
@@ -749,6 +579,11 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
   }
   SMCF_HIP(hipMemset(bt.parent, 0, sizeof(int) * Pn * C));
   SMCF_HIP(hipMemset(bt.gtrace, 0, sizeof(double) * 202 * C));
+  {
+    // the problem as a one-entry member table of the stage kernels
+    MemberDev md = member_of(P);
+    SMCF_HIP(hipMemcpy(P.mdev, &md, sizeof(md), hipMemcpyHostToDevice));
+  }
   guard.p = nullptr;
   *out = holder.release();
   return SMCF_OK;
@@ -1245,8 +1080,10 @@ int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t
   hipStream_t s = P.ctx->stream;
   SMCF_HIP(hipMemcpyAsync(P.bt.theta, theta, sizeof(double) * P.m.ndim_fault, hipMemcpyHostToDevice, s));
   SMCF_HIP(hipMemcpyAsync(P.bt.key, &key, sizeof(uint64_t), hipMemcpyHostToDevice, s));
-  int rc = setup_slots(P, 1, nullptr, s, 0);
+  int rc = setup_pre(P, 1, nullptr, s);
   if (rc) return rc;
+  CHECK_LAUNCH(launch_init_chain(P.mdev, nullptr, nullptr, 1, P.m.chain_w, s));
+  if ((rc = setup_post(P, 1, nullptr, s))) return rc;
   if (particles && (rc = download_columns(P, P.bt.S, P.P, particles, s))) return rc;
   if (likelihood) SMCF_HIP(hipMemcpyAsync(likelihood, P.bt.like, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));
   if (prior) SMCF_HIP(hipMemcpyAsync(prior, P.bt.prior, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));

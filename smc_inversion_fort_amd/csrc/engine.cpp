@@ -1,0 +1,760 @@
+// Stage engine: the inner-SMC stage loop of slip_smc_exec (smc_slip.f90:1445-1663) over the slots of one or more
+// problems at once, and the problem groups built on it.
+//
+// Every stage kernel runs over a *stage set*: a device table of members (problems: per-slot buffers, dimensions,
+// sampler settings) and the list of (member, slot) instances, so one launch covers instances whose meshes,
+// observation sets and particle counts differ.  The chunked kernels (Gibbs sweeps, likelihoods) take item lists
+// (member, slot, 256-particle chunk) in XCD-aware order; the Gibbs sweeps launch once per kernel class (form and
+// block rows from the tuning table), so every instance keeps the block rows - and therefore the arithmetic - it has
+// when it runs alone.  Instances that have finished are masked, not compacted.
+//
+// After every stage the host reads the instances' status back in one copy and harvests the finished ones; once
+// enough slots of a member are free its next geometries are set up in them on the set-up stream (each member's
+// Green's, Laplacian and Gram kernels, then one prior-chain launch over every instance admitted together),
+// overlapping the stage loop, and released into it at the first stage boundary after their set-up completed.
+// Every instance keeps its own stage counter and RNG key, so results do not depend on the slot, the batch, the
+// other members or the moment an instance ran.  With graphs on, the stage's launches are captured once per engine
+// run into a HIP graph and each stage is one graph launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "smcf_internal.h"
+
+namespace smcf {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// one batch call's geometries (thetas nf x n, keys) and where their results go
+struct Request {
+  int n = 0;
+  const double* theta = nullptr;
+  const uint64_t* keys = nullptr;
+  double* neglog = nullptr;    // may be null
+  int* nstage = nullptr;       // may be null
+  double* t_done = nullptr;    // seconds since t0 to each geometry's result (may be null)
+  double* t_admit = nullptr;   // seconds since t0 to each geometry's set-up launch (may be null)
+  Clock::time_point t0;
+  int remaining = 0;
+  bool finished = false;
+  int rc = SMCF_OK;
+  std::string err;
+};
+
+struct Geo {
+  Request* r = nullptr;
+  int i = 0;
+};
+
+// one admitted instance as k_admit reads it
+struct AdmitEntry {
+  int member, slot, k, pad;
+  uint64_t key;
+  double theta[kMaxOuterDim];
+};
+
+// every instance of the set held as done (skipped by the stage kernels) until an admission releases it
+__global__ void k_idle(StageSet ss) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ss.ninst) return;
+  int2 is = ss.inst[g];
+  const BatchDev& bt = ss.mem[is.x].bt;
+  bt.done[is.y] = 1;
+  bt.active[is.y] = 0;
+}
+
+// thetas and keys of the admitted instances into their slots, and each member's slot list (entry k of the member
+// at sel[half cap + k]) for its set-up kernels
+__global__ void k_admit(const MemberDev* __restrict__ mem, const AdmitEntry* __restrict__ e, int n, int half) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const AdmitEntry& a = e[k];
+  const MemberDev& M = mem[a.member];
+  for (int d = 0; d < M.nf; ++d) M.bt.theta[(size_t)a.slot * M.nf + d] = a.theta[d];
+  M.bt.key[a.slot] = a.key;
+  M.bt.sel[(size_t)half * M.bt.cap + a.k] = a.slot;
+}
+
+// the admitted instances join the stage loop
+__global__ void k_release(const MemberDev* __restrict__ mem, const int2* __restrict__ list, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  mem[list[k].x].bt.done[list[k].y] = 0;
+}
+
+// status of every instance after the stage: -log Z, done, stage counter
+__global__ void k_status(StageSet ss, double* __restrict__ out) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ss.ninst) return;
+  int2 is = ss.inst[g];
+  const BatchDev& bt = ss.mem[is.x].bt;
+  out[3 * (size_t)g] = bt.neglog[is.y];
+  out[3 * (size_t)g + 1] = (double)bt.done[is.y];
+  out[3 * (size_t)g + 2] = (double)bt.iter[is.y];
+}
+
+struct MemberRun {
+  Problem* P = nullptr;
+  int W = 0;      // slots in use: 0 .. W-1
+  int first = 0;  // its first instance in the stage set
+  int admit_min = 1;
+  std::deque<Geo> pending;
+  std::vector<Geo> job;  // per slot (r == nullptr: free)
+  std::vector<int> freed;
+  int busy = 0;
+};
+
+struct Admission {
+  bool live = false;
+  int half = 0;
+  std::vector<AdmitEntry> ent;
+  std::vector<int2> list;
+  std::vector<Geo> geo;
+  hipEvent_t ready{};
+};
+
+// items of the chunked kernels for the given instances (member, slot, chunks): instance g of the list runs on
+// XCD g % 8, its chunks at consecutive positions of that XCD's queue; queues are padded to equal length with empty
+// items
+void xcd_items(const std::vector<int4>& insts, std::vector<int4>& out) {
+  std::vector<std::vector<int4>> q(8);
+  for (size_t g = 0; g < insts.size(); ++g)
+    for (int c = 0; c < insts[g].z; ++c) q[g % 8].push_back(make_int4(insts[g].x, insts[g].y, c, 0));
+  size_t L = 0;
+  for (auto& v : q) L = std::max(L, v.size());
+  out.assign(8 * L, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < 8; ++x)
+    for (size_t r = 0; r < q[x].size(); ++r) out[8 * r + x] = q[x][r];
+}
+
+}  // namespace
+
+struct Engine {
+  smcf_ctx* ctx = nullptr;
+  std::vector<MemberRun> mr;
+  bool want_moments = false, hmc = false, any_f32 = false, any_bf = false, use_graph = false, async = false;
+  std::vector<void*> dbufs;
+  MemberDev* dmem = nullptr;
+  double* dstat = nullptr;
+  AdmitEntry* dadm[2] = {nullptr, nullptr};
+  int2* dlist[2] = {nullptr, nullptr};
+  StageSet ss{};
+  struct GibbsLaunch {
+    int precision, kb;
+    GibbsArgs args;
+    ItemList it;
+  };
+  std::vector<GibbsLaunch> glaunch;
+  ItemList like_fp{}, like_bf{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  Admission adm;
+  std::vector<double> stat;
+  int busy_total = 0;
+  Clock::time_point t_start;
+  // asynchronous submissions (group outer SMCs): requests from client threads, served by the thread in run()
+  std::mutex mu;
+  std::condition_variable cv_engine, cv_done;
+  std::deque<std::pair<int, Request*>> inbox;
+  int clients = 0;
+  int rc = SMCF_OK;
+  std::string err;
+
+  ~Engine() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (adm.ready) (void)hipEventDestroy(adm.ready);
+    for (void* p : dbufs) (void)hipFree(p);
+  }
+
+  template <class T>
+  int dalloc(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) {
+      set_error(std::string("stage engine: hipMalloc: ") + hipGetErrorString(e));
+      return SMCF_ENOMEM;
+    }
+    dbufs.push_back(q);
+    *p = (T*)q;
+    return SMCF_OK;
+  }
+
+  // members with their slots in use; tables, lists and (optionally) the stage graph
+  int init(smcf_ctx* c, const std::vector<Problem*>& ps, const std::vector<int>& W, bool moments, bool graph_on) {
+    ctx = c;
+    want_moments = moments;
+    int n = (int)ps.size(), ninst = 0;
+    mr.resize(n);
+    std::vector<MemberDev> md(n);
+    std::vector<int2> inst;
+    int nd_max = 1, P_max = 1, kb_max = 16;
+    for (int i = 0; i < n; ++i) {
+      Problem& P = *ps[i];
+      MemberRun& m = mr[i];
+      m.P = &P;
+      m.W = W[i];
+      m.first = ninst;
+      m.admit_min = std::max(1, m.W / 16);
+      m.job.assign(m.W, Geo{});
+      for (int b = 0; b < m.W; ++b) m.freed.push_back(b);
+      md[i] = member_of(P);
+      for (int b = 0; b < m.W; ++b) inst.push_back(make_int2(i, b));
+      ninst += m.W;
+      nd_max = std::max(nd_max, P.m.ndim);
+      P_max = std::max(P_max, P.P);
+      kb_max = std::max(kb_max, P.bt.kb);
+      if (P.mover != SMCF_MOVER_GIBBS) hmc = true;
+      if (P.precision == SMCF_PRECISION_MIXED) any_f32 = true;
+      if (P.bt.QX) any_bf = true;
+    }
+    if (hmc && n != 1) {
+      set_error("stage engine: HMC movers run one problem at a time");
+      return SMCF_EUNSUPPORTED;
+    }
+    // item lists: likelihoods (FP64 Gram form / centred split BF16; member = index in the device table) and the
+    // Gibbs launches: per kernel class (form, block rows) groups of at most kGibbsArgMembers members whose table
+    // travels as a kernel argument (member = index in that table)
+    std::vector<int4> fp, bf;
+    struct Cls {
+      int precision, kb;
+      std::vector<int> members;
+    };
+    std::vector<Cls> cls;
+    for (int i = 0; i < n; ++i) {
+      Problem& P = *ps[i];
+      int chunks = (P.P + 255) / 256;
+      for (int b = 0; b < mr[i].W; ++b) (P.bt.QX ? bf : fp).push_back(make_int4(i, b, chunks, 0));
+      if (P.mover != SMCF_MOVER_GIBBS || mr[i].W == 0) continue;
+      auto it = std::find_if(cls.begin(), cls.end(),
+                             [&](const Cls& c) { return c.precision == P.precision && c.kb == P.bt.kb; });
+      if (it == cls.end()) {
+        cls.push_back({P.precision, P.bt.kb, {}});
+        it = cls.end() - 1;
+      }
+      it->members.push_back(i);
+    }
+    std::vector<int4> all, tmp;
+    auto add = [&](const std::vector<int4>& v, size_t& off, int& cnt) {
+      xcd_items(v, tmp);
+      off = all.size();
+      cnt = (int)tmp.size();
+      all.insert(all.end(), tmp.begin(), tmp.end());
+    };
+    size_t off_fp = 0, off_bf = 0;
+    int n_fp = 0, n_bf = 0;
+    add(fp, off_fp, n_fp);
+    add(bf, off_bf, n_bf);
+    std::vector<std::pair<size_t, int>> goff;
+    for (auto& c : cls)
+      for (size_t g0 = 0; g0 < c.members.size(); g0 += kGibbsArgMembers) {
+        GibbsLaunch gl{};
+        gl.precision = c.precision;
+        gl.kb = c.kb;
+        std::vector<int4> v;
+        for (size_t k = g0; k < std::min(c.members.size(), g0 + kGibbsArgMembers); ++k) {
+          int i = c.members[k];
+          gl.args.m[k - g0] = gibbs_member_of(*ps[i]);
+          int chunks = (ps[i]->P + 255) / 256;
+          for (int b = 0; b < mr[i].W; ++b) v.push_back(make_int4((int)(k - g0), b, chunks, 0));
+        }
+        goff.push_back({0, 0});
+        add(v, goff.back().first, goff.back().second);
+        glaunch.push_back(gl);
+      }
+    int2* dinst = nullptr;
+    int4* ditems = nullptr;
+    int r;
+    if ((r = dalloc(&dmem, n)) || (r = dalloc(&dinst, inst.size())) || (r = dalloc(&ditems, all.size())) ||
+        (r = dalloc(&dstat, 3 * inst.size())) || (r = dalloc(&dadm[0], inst.size())) ||
+        (r = dalloc(&dadm[1], inst.size())) || (r = dalloc(&dlist[0], inst.size())) ||
+        (r = dalloc(&dlist[1], inst.size())))
+      return r;
+    SMCF_HIP(hipMemcpy(dmem, md.data(), sizeof(MemberDev) * n, hipMemcpyHostToDevice));
+    if (!inst.empty()) SMCF_HIP(hipMemcpy(dinst, inst.data(), sizeof(int2) * inst.size(), hipMemcpyHostToDevice));
+    if (!all.empty()) SMCF_HIP(hipMemcpy(ditems, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+    ss = StageSet{dmem, dinst, ninst, nd_max, P_max, kb_max};
+    like_fp = ItemList{ditems + off_fp, n_fp};
+    like_bf = ItemList{ditems + off_bf, n_bf};
+    for (size_t k = 0; k < glaunch.size(); ++k) glaunch[k].it = ItemList{ditems + goff[k].first, goff[k].second};
+    stat.assign(3 * (size_t)ninst, 0.0);
+    SMCF_HIP(hipEventCreateWithFlags(&adm.ready, hipEventDisableTiming));
+    hipStream_t s = ctx->stream;
+    if (ninst > 0) {
+      hipLaunchKernelGGL(k_idle, dim3((ninst + 255) / 256), dim3(256), 0, s, ss);
+      SMCF_HIP(hipGetLastError());
+    }
+    use_graph = graph_on && !hmc && !ctx->profiling && ninst > 0;
+    if (use_graph) {
+      SMCF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      int rl = stage_launch(s);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rl) {
+        if (g) (void)hipGraphDestroy(g);
+        return rl;
+      }
+      if (e != hipSuccess) {
+        set_error(std::string("stage engine: hipStreamEndCapture: ") + hipGetErrorString(e));
+        return SMCF_EHIP;
+      }
+      graph = g;
+      SMCF_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
+    return SMCF_OK;
+  }
+
+  // the launches of one stage over the whole set, ending with the status copy kernel
+  int stage_launch(hipStream_t s) {
+    {
+      ProfScope ps(ctx, "gamma_resample", s);
+      if (launch_gamma_resample(ss, s) != hipSuccess) return launch_failed("gamma_resample");
+    }
+    if ((want_moments || hmc) && launch_moments(ss, s) != hipSuccess) return launch_failed("moments");
+    if (hmc) {
+      ProfScope ps(ctx, "hmc", s);
+      Problem& P = *mr[0].P;
+      const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept, P.precision == SMCF_PRECISION_MIXED_BF16 ? 1 : 0};
+      if (launch_hmc_stage(P.m, P.bt, P.hm, P.hh, mr[0].W, P.max_slip, hcfg, s) != hipSuccess)
+        return launch_failed("hmc_stage");
+    } else {
+      if (launch_gather(ss, s) != hipSuccess) return launch_failed("gather");
+      if (launch_form_q(ss, any_f32, any_bf, s) != hipSuccess) return launch_failed("form_q");
+      if (any_bf && launch_like_ref(ss, s) != hipSuccess) return launch_failed("like_ref");
+      {
+        ProfScope ps(ctx, "gibbs", s);
+        for (auto& g : glaunch)
+          if (launch_gibbs(g.args, g.it, g.precision, g.kb, s) != hipSuccess) return launch_failed("gibbs");
+      }
+      {
+        // the Gibbs mover reads no prior values; they are formed once for the final particles
+        ProfScope ps(ctx, "likelihood", s);
+        if (launch_like_bf(ss, like_bf, s) != hipSuccess || launch_like_stage(ss, like_fp, s) != hipSuccess)
+          return launch_failed("likelihood");
+      }
+    }
+    hipLaunchKernelGGL(k_status, dim3((ss.ninst + 255) / 256), dim3(256), 0, s, ss, dstat);
+    if (hipGetLastError() != hipSuccess) return launch_failed("status");
+    return SMCF_OK;
+  }
+
+  int launch_failed(const char* what) {
+    set_error(std::string("stage engine: launch of ") + what + ": " + hipGetErrorString(hipGetLastError()));
+    return SMCF_EHIP;
+  }
+
+  double since_start() const { return std::chrono::duration<double>(Clock::now() - t_start).count(); }
+  static double since(const Clock::time_point& t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  // one stage, its status read back, finished instances harvested
+  int step() {
+    hipStream_t s = ctx->stream;
+    if (gexec) {
+      SMCF_HIP(hipGraphLaunch(gexec, s));
+    } else {
+      int r = stage_launch(s);
+      if (r) return r;
+    }
+    SMCF_HIP(hipMemcpyAsync(stat.data(), dstat, sizeof(double) * stat.size(), hipMemcpyDeviceToHost, s));
+    SMCF_HIP(hipStreamSynchronize(s));
+    for (auto& m : mr)
+      for (int b = 0; b < m.W; ++b) {
+        Geo& g = m.job[b];
+        size_t k = 3 * (size_t)(m.first + b);
+        if (!g.r || stat[k + 1] == 0.0) continue;
+        Request& r = *g.r;
+        if (r.neglog) r.neglog[g.i] = stat[k];
+        if (r.nstage) r.nstage[g.i] = std::min((int)stat[k + 2] - 1, 201);
+        if (r.t_done) r.t_done[g.i] = since(r.t0);
+        g.r = nullptr;
+        --m.busy;
+        --busy_total;
+        m.freed.push_back(b);
+        if (--r.remaining == 0) finish(r, SMCF_OK, "");
+      }
+    return SMCF_OK;
+  }
+
+  void finish(Request& r, int code, const std::string& msg) {
+    if (async) {
+      std::lock_guard<std::mutex> lk(mu);
+      r.rc = code;
+      r.err = msg;
+      r.finished = true;
+      cv_done.notify_all();
+    } else {
+      r.rc = code;
+      r.err = msg;
+      r.finished = true;
+    }
+  }
+
+  // set up the next geometries of every member with enough free slots, in one admission on the set-up stream
+  int start_admission() {
+    adm.ent.clear();
+    adm.list.clear();
+    adm.geo.clear();
+    std::vector<std::pair<int, int>> who;  // member, count
+    for (int i = 0; i < (int)mr.size(); ++i) {
+      MemberRun& m = mr[i];
+      if (m.pending.empty() || m.freed.empty()) continue;
+      if ((int)m.freed.size() < m.admit_min && m.busy > 0) continue;
+      int n = (int)std::min(m.freed.size(), m.pending.size());
+      const int nf = m.P->m.ndim_fault;
+      for (int k = 0; k < n; ++k) {
+        Geo g = m.pending.front();
+        m.pending.pop_front();
+        AdmitEntry e{};
+        e.member = i;
+        e.slot = m.freed[k];
+        e.k = k;
+        e.key = g.r->keys[g.i];
+        std::copy(g.r->theta + (size_t)g.i * nf, g.r->theta + (size_t)(g.i + 1) * nf, e.theta);
+        if (g.r->t_admit) g.r->t_admit[g.i] = since(g.r->t0);
+        adm.ent.push_back(e);
+        adm.list.push_back(make_int2(i, e.slot));
+        adm.geo.push_back(g);
+      }
+      m.freed.erase(m.freed.begin(), m.freed.begin() + n);
+      who.push_back({i, n});
+    }
+    if (who.empty()) return SMCF_OK;
+    adm.half = 1 - adm.half;
+    const int h = adm.half, n = (int)adm.list.size();
+    hipStream_t s2 = ctx->setup_stream;
+    SMCF_HIP(hipMemcpyAsync(dadm[h], adm.ent.data(), sizeof(AdmitEntry) * n, hipMemcpyHostToDevice, s2));
+    SMCF_HIP(hipMemcpyAsync(dlist[h], adm.list.data(), sizeof(int2) * n, hipMemcpyHostToDevice, s2));
+    hipLaunchKernelGGL(k_admit, dim3((n + 63) / 64), dim3(64), 0, s2, dmem, dadm[h], n, h);
+    SMCF_HIP(hipGetLastError());
+    int chain_w = 1, r;
+    for (auto& w : who) {
+      Problem& P = *mr[w.first].P;
+      if ((r = setup_pre(P, w.second, P.bt.sel + (size_t)h * P.bt.cap, s2))) return r;
+      chain_w = std::max(chain_w, P.m.chain_w);
+    }
+    {
+      ProfScope ps(ctx, "init_chain", s2);
+      hipError_t e = launch_init_chain(dmem, dlist[h], nullptr, n, chain_w, s2);
+      if (e != hipSuccess) {
+        set_error(std::string("stage engine: launch of init_chain: ") + hipGetErrorString(e));
+        return SMCF_EHIP;
+      }
+    }
+    for (auto& w : who) {
+      Problem& P = *mr[w.first].P;
+      if ((r = setup_post(P, w.second, P.bt.sel + (size_t)h * P.bt.cap, s2))) return r;
+    }
+    SMCF_HIP(hipEventRecord(adm.ready, s2));
+    adm.live = true;
+    return SMCF_OK;
+  }
+
+  // the completed admission's instances join the stage loop
+  int release_admission() {
+    int n = (int)adm.list.size();
+    hipLaunchKernelGGL(k_release, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, dmem, dlist[adm.half], n);
+    SMCF_HIP(hipGetLastError());
+    for (int k = 0; k < n; ++k) {
+      MemberRun& m = mr[adm.list[k].x];
+      m.job[adm.list[k].y] = adm.geo[k];
+      ++m.busy;
+      ++busy_total;
+    }
+    adm.live = false;
+    return SMCF_OK;
+  }
+
+  bool nothing_pending() const {
+    for (auto& m : mr)
+      if (!m.pending.empty()) return false;
+    return true;
+  }
+
+  void queue(int member, Request* r) {
+    for (int i = 0; i < r->n; ++i) mr[member].pending.push_back(Geo{r, i});
+    if (r->n == 0) r->finished = true;
+  }
+
+  // serve until every request has finished (and, asynchronously, every client has left)
+  int run() {
+    t_start = Clock::now();
+    int r = loop();
+    if (r) {
+      // the error ends every request still queued or in flight
+      std::string msg = smcf_error_text();
+      (void)hipStreamSynchronize(ctx->setup_stream);
+      (void)hipStreamSynchronize(ctx->stream);
+      std::vector<Request*> open;
+      for (auto& m : mr) {
+        for (auto& g : m.pending) open.push_back(g.r);
+        for (auto& g : m.job)
+          if (g.r) open.push_back(g.r);
+      }
+      for (auto& g : adm.geo) open.push_back(g.r);
+      std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+      if (async) lk.lock();
+      rc = r;
+      err = msg;
+      for (auto& p : inbox) open.push_back(p.second);
+      inbox.clear();
+      for (Request* q : open) {
+        q->rc = r;
+        q->err = msg;
+        q->finished = true;
+      }
+      if (async) {
+        cv_done.notify_all();
+        // clients still running their outer SMC submit again and get the error at once; wait until they left
+        cv_engine.wait(lk, [&] { return clients == 0; });
+      }
+      return r;
+    }
+    SMCF_HIP(hipStreamSynchronize(ctx->stream));
+    resolve_profiles(ctx);
+    return SMCF_OK;
+  }
+
+  int loop() {
+    for (;;) {
+      if (async) {
+        std::unique_lock<std::mutex> lk(mu);
+        while (!inbox.empty()) {
+          queue(inbox.front().first, inbox.front().second);
+          inbox.pop_front();
+        }
+        if (busy_total == 0 && !adm.live && nothing_pending()) {
+          if (clients == 0) return SMCF_OK;
+          cv_engine.wait(lk, [&] { return !inbox.empty() || clients == 0; });
+          continue;
+        }
+      } else if (busy_total == 0 && !adm.live && nothing_pending()) {
+        return SMCF_OK;
+      }
+      int r;
+      if (busy_total > 0 && (r = step())) return r;
+      if (adm.live) {
+        hipError_t q = busy_total > 0 ? hipEventQuery(adm.ready) : hipEventSynchronize(adm.ready);
+        if (q == hipSuccess) {
+          if ((r = release_admission())) return r;
+        } else if (q != hipErrorNotReady) {
+          set_error(std::string("stage engine: set-up: ") + hipGetErrorString(q));
+          return SMCF_EHIP;
+        }
+      }
+      if (!adm.live && (r = start_admission())) return r;
+    }
+  }
+
+  static std::string smcf_error_text() { return std::string(smcf_last_error()); }
+};
+
+int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
+               int* nstage) {
+  if (N < 1) {
+    set_error("no geometries");
+    return SMCF_EINVAL;
+  }
+  if (P.engine) return engine_submit_wait(P, N, theta, keys, neglog, nstage);
+  Engine E;
+  int r = E.init(P.ctx, {&P}, {std::min(N, P.bt.cap)}, want_moments, false);
+  if (r) return r;
+  P.t_done.assign(N, 0.0);
+  P.t_admit.assign(N, 0.0);
+  Request q;
+  q.n = q.remaining = N;
+  q.theta = theta;
+  q.keys = keys;
+  q.neglog = neglog;
+  q.nstage = nstage;
+  q.t_done = P.t_done.data();
+  q.t_admit = P.t_admit.data();
+  q.t0 = Clock::now();
+  E.queue(0, &q);
+  return E.run();
+}
+
+int engine_submit_wait(Problem& P, int N, const double* theta, const uint64_t* keys, double* neglog, int* nstage) {
+  Engine* E = P.engine;
+  P.t_done.assign(N, 0.0);
+  P.t_admit.assign(N, 0.0);
+  Request q;
+  q.n = q.remaining = N;
+  q.theta = theta;
+  q.keys = keys;
+  q.neglog = neglog;
+  q.nstage = nstage;
+  q.t_done = P.t_done.data();
+  q.t_admit = P.t_admit.data();
+  q.t0 = Clock::now();
+  std::unique_lock<std::mutex> lk(E->mu);
+  if (E->rc) {
+    set_error(E->err);
+    return E->rc;
+  }
+  E->inbox.push_back({P.member, &q});
+  E->cv_engine.notify_all();
+  E->cv_done.wait(lk, [&] { return q.finished; });
+  if (q.rc) set_error(q.err);
+  return q.rc;
+}
+
+}  // namespace smcf
+
+using namespace smcf;
+
+struct smcf_group {
+  smcf_ctx* ctx = nullptr;
+  std::vector<smcf_problem*> members;
+  bool graph = true;
+};
+
+extern "C" {
+
+int smcf_group_create(smcf_ctx* ctx, int n, smcf_problem* const* members, smcf_group** out) {
+  if (!ctx || n < 1 || !members || !out) return SMCF_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    if (!members[i]) return SMCF_EINVAL;
+    Problem& P = members[i]->p;
+    if (P.ctx != ctx) {
+      set_error("smcf_group_create: every member must be created on the group's context");
+      return SMCF_EINVAL;
+    }
+    if (P.mover != SMCF_MOVER_GIBBS) {
+      set_error("smcf_group_create: group members take the Gibbs mover");
+      return SMCF_EUNSUPPORTED;
+    }
+    if (P.in_group) {
+      set_error("smcf_group_create: a problem belongs to one group at a time");
+      return SMCF_ESTATE;
+    }
+    for (int j = 0; j < i; ++j)
+      if (members[j] == members[i]) {
+        set_error("smcf_group_create: a problem appears twice");
+        return SMCF_EINVAL;
+      }
+  }
+  auto* g = new smcf_group();
+  g->ctx = ctx;
+  g->members.assign(members, members + n);
+  for (auto* m : g->members) m->p.in_group = true;
+  *out = g;
+  return SMCF_OK;
+}
+
+int smcf_group_set_graph(smcf_group* g, int on) {
+  if (!g) return SMCF_EINVAL;
+  g->graph = on != 0;
+  return SMCF_OK;
+}
+
+int smcf_group_destroy(smcf_group* g) {
+  if (!g) return SMCF_OK;
+  for (auto* m : g->members) m->p.in_group = false;
+  delete g;
+  return SMCF_OK;
+}
+
+int smcf_group_calc_likelihood_batch(smcf_group* g, const int* B, const double* const* theta,
+                                     const uint64_t* const* keys, double* const* neglog, int* const* nstage) {
+  if (!g || !B || !theta || !keys || !neglog) return SMCF_EINVAL;
+  int n = (int)g->members.size(), total = 0;
+  for (int i = 0; i < n; ++i) {
+    if (B[i] < 0 || (B[i] > 0 && (!theta[i] || !keys[i] || !neglog[i]))) return SMCF_EINVAL;
+    total += B[i];
+  }
+  if (total == 0) return SMCF_OK;
+  SMCF_HIP(hipSetDevice(g->ctx->device));
+  std::vector<Problem*> ps;
+  std::vector<int> W;
+  for (int i = 0; i < n; ++i) {
+    Problem& P = g->members[i]->p;
+    ps.push_back(&P);
+    W.push_back(std::min(B[i], P.bt.cap));
+  }
+  Engine E;
+  int r = E.init(g->ctx, ps, W, false, g->graph);
+  if (r) return r;
+  std::vector<Request> q(n);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < n; ++i) {
+    Problem& P = *ps[i];
+    P.t_done.assign(B[i], 0.0);
+    P.t_admit.assign(B[i], 0.0);
+    q[i].n = q[i].remaining = B[i];
+    q[i].theta = theta[i];
+    q[i].keys = keys[i];
+    q[i].neglog = neglog[i];
+    q[i].nstage = nstage ? nstage[i] : nullptr;
+    q[i].t_done = P.t_done.data();
+    q[i].t_admit = P.t_admit.data();
+    q[i].t0 = t0;
+    E.queue(i, &q[i]);
+  }
+  return E.run();
+}
+
+int smcf_group_fault_smc_exec(smcf_group* g, const double* const* range, const int* N, const uint64_t* seed,
+                              int max_stages, double* const* particles, double* const* likelihood, int* nstage,
+                              double* evidence) {
+  if (!g || !range || !N || !seed || !particles || !likelihood) return SMCF_EINVAL;
+  int n = (int)g->members.size();
+  for (int i = 0; i < n; ++i)
+    if (!range[i] || N[i] < 1 || !particles[i] || !likelihood[i]) return SMCF_EINVAL;
+  SMCF_HIP(hipSetDevice(g->ctx->device));
+  std::vector<Problem*> ps;
+  std::vector<int> W;
+  for (int i = 0; i < n; ++i) {
+    ps.push_back(&g->members[i]->p);
+    W.push_back(ps.back()->bt.cap);
+  }
+  Engine E;
+  E.async = true;
+  int r = E.init(g->ctx, ps, W, false, g->graph);
+  if (r) return r;
+  E.clients = n;
+  for (int i = 0; i < n; ++i) {
+    ps[i]->engine = &E;
+    ps[i]->member = i;
+  }
+  std::vector<int> rcs(n, SMCF_OK);
+  std::vector<std::string> errs(n);
+  std::vector<std::thread> th;
+  const int device = g->ctx->device;
+  for (int i = 0; i < n; ++i)
+    th.emplace_back([&, i] {
+      (void)hipSetDevice(device);
+      // every wave in one call: the engine streams it through the member's slots
+      rcs[i] = smcf_fault_smc_exec(g->members[i], range[i], N[i], seed[i], max_stages, INT_MAX / 2, particles[i],
+                                   likelihood[i], nstage ? &nstage[i] : nullptr, evidence ? &evidence[i] : nullptr,
+                                   nullptr);
+      if (rcs[i]) errs[i] = smcf_last_error();
+      std::lock_guard<std::mutex> lk(E.mu);
+      --E.clients;
+      E.cv_engine.notify_all();
+    });
+  r = E.run();
+  for (auto& t : th) t.join();
+  for (int i = 0; i < n; ++i) {
+    ps[i]->engine = nullptr;
+    ps[i]->member = -1;
+  }
+  if (r) return r;
+  for (int i = 0; i < n; ++i)
+    if (rcs[i]) {
+      set_error(errs[i]);
+      return rcs[i];
+    }
+  return SMCF_OK;
+}
+
+}  // extern "C"

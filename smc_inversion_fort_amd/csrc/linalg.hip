@@ -256,15 +256,17 @@ hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, co
 // Gibbs precision Q = gamma H + LL / alpha^2, written in the blocked layout of the Gibbs
 // kernel: Qb[(blk*nd + j)*KB + k] = Q(j, blk*KB + k) (zero past the last column).
 
-__global__ void k_form_qb(int nd, int KB, const double* __restrict__ Hall, const double* __restrict__ gamma,
-                          const int* __restrict__ active, double* __restrict__ Qball) {
-  int b = blockIdx.y;
-  if (!active[b]) return;
+__global__ void k_form_qb(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
+  if (!bt.Q || !bt.active[b]) return;
   int nblk = (nd + KB - 1) / KB;
   size_t n = (size_t)nblk * nd * KB;
-  double g = gamma[b];
-  const double* H = Hall + (size_t)b * nd * nd;
-  double* Qb = Qball + (size_t)b * n;
+  double g = bt.gamma[b];
+  const double* H = bt.H + (size_t)b * nd * nd;
+  double* Qb = bt.Q + (size_t)b * n;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     int k = (int)(e % KB);
     size_t rest = e / KB;
@@ -274,19 +276,19 @@ __global__ void k_form_qb(int nd, int KB, const double* __restrict__ Hall, const
   }
 }
 
-__global__ void k_form_qb_sparse(int nd, int KB, const double* __restrict__ Hall, const double* __restrict__ gamma,
-                                 const int* __restrict__ active, const double* __restrict__ scal,
-                                 const int* __restrict__ llidx, const double* __restrict__ llval,
-                                 double* __restrict__ Qball) {
+__global__ void k_form_qb_sparse(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
-  if (!active[b] || i >= nd) return;
+  if (!bt.Q || !bt.active[b] || i >= nd) return;
   int nblk = (nd + KB - 1) / KB;
-  double g = gamma[b], a2 = scal[8 * (size_t)b];
-  const double* H = Hall + (size_t)b * nd * nd;
-  double* Qb = Qball + (size_t)b * nblk * nd * KB;
-  const int* ix = llidx + ((size_t)b * nd + i) * 25;
-  const double* vx = llval + ((size_t)b * nd + i) * 25;
+  double g = bt.gamma[b], a2 = bt.scal[8 * (size_t)b];
+  const double* H = bt.H + (size_t)b * nd * nd;
+  double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
+  const int* ix = bt.llc_idx + ((size_t)b * nd + i) * 25;
+  const double* vx = bt.llc_val + ((size_t)b * nd + i) * 25;
   int blk = i / KB, k = i % KB;
   for (int w = 0; w < 25; ++w) {
     int r = ix[w];
@@ -296,30 +298,34 @@ __global__ void k_form_qb_sparse(int nd, int KB, const double* __restrict__ Hall
 }
 
 // per-coordinate constants of the stage for the Gibbs draws: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
-__global__ void k_form_qdiag(int nd, int KB, const double* __restrict__ Qball, const double* __restrict__ gamma,
-                             const double* __restrict__ bvec, const int* __restrict__ active,
-                             double* __restrict__ cdall) {
+__global__ void k_form_qdiag(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
-  if (!active[b] || i >= nd) return;
+  if (!bt.Q || !bt.active[b] || i >= nd) return;
   int nblk = (nd + KB - 1) / KB;
-  const double* Qb = Qball + (size_t)b * nblk * nd * KB;
+  const double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
   double qii = Qb[((size_t)(i / KB) * nd + i) * KB + i % KB];
-  double* cd = cdall + (size_t)b * 4 * nd;
+  double* cd = bt.qdiag + (size_t)b * 4 * nd;
   double sq = sqrt(qii);
-  cd[i] = gamma[b] * bvec[(size_t)b * nd + i];
+  cd[i] = bt.gamma[b] * bt.bvec[(size_t)b * nd + i];
   cd[nd + i] = 1.0 / qii;
   cd[2 * nd + i] = sq;
   cd[3 * nd + i] = 1.0 / sq;
 }
 
 // FP32 copy of the blocked precision (mixed precision)
-__global__ void k_q_to_f32(size_t n, const double* __restrict__ Qball, const int* __restrict__ active,
-                           float* __restrict__ Q32all) {
-  int b = blockIdx.y;
-  if (!active[b]) return;
-  const double* Q = Qball + (size_t)b * n;
-  float* Q32 = Q32all + (size_t)b * n;
+__global__ void k_q_to_f32(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
+  if (!bt.Q32 || !bt.active[b]) return;
+  size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
+  const double* Q = bt.Q + (size_t)b * n;
+  float* Q32 = bt.Q32 + (size_t)b * n;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
     Q32[e] = (float)Q[e];
 }
@@ -340,17 +346,15 @@ hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, cons
   return hipGetLastError();
 }
 
-hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int nd = m.ndim, KB = bt.kb;
+hipError_t launch_form_q(const StageSet& ss, bool any_f32, bool any_bf, hipStream_t s) {
+  int nd = ss.nd_max, KB = ss.kb_max;
   size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
   int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_form_qb, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.H, bt.gamma, bt.active, bt.Q);
-  hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.H, bt.gamma,
-                     bt.active, bt.scal, bt.llc_idx, bt.llc_val, bt.Q);
-  hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, B), dim3(128), 0, s, nd, KB, bt.Q, bt.gamma, bt.bvec,
-                     bt.active, bt.qdiag);
-  if (bt.Q32) hipLaunchKernelGGL(k_q_to_f32, dim3(blocks, B), dim3(256), 0, s, n, bt.Q, bt.active, bt.Q32);
-  if (bt.QX) return launch_q_bf(m, bt, B, s);
+  hipLaunchKernelGGL(k_form_qb, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
+  hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, ss.ninst), dim3(128), 0, s, ss);
+  hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, ss.ninst), dim3(128), 0, s, ss);
+  if (any_f32) hipLaunchKernelGGL(k_q_to_f32, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
+  if (any_bf) return launch_q_bf(ss, s);
   return hipGetLastError();
 }
 

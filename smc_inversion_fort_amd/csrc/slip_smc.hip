@@ -30,20 +30,20 @@ namespace smcf {
 // everything particle k reads.  Each lane keeps its particle's last W coordinates in an LDS
 // ring (W >= lag + 1); 31 * lag >= ndof guarantees that a lane's ring is not reused while the
 // next lane still reads it, and that particle 0 reads zeros from the ring of lane 31.
-__global__ void __launch_bounds__(64) k_init_chain(MeshDev m, int P, double max_slip,
-                                                   const uint64_t* __restrict__ keys,
-                                                   const double* __restrict__ scal, const int* __restrict__ llidx,
-                                                   const double* __restrict__ llval, double* __restrict__ Sall,
+__global__ void __launch_bounds__(64) k_init_chain(const MemberDev* __restrict__ mem, const int2* __restrict__ list,
                                                    const int* __restrict__ sel) {
   extern __shared__ double ring[];
-  int b = sel ? sel[blockIdx.x] : (int)blockIdx.x;
+  int2 is = list ? list[blockIdx.x] : make_int2(0, sel ? sel[blockIdx.x] : (int)blockIdx.x);
+  const MemberDev& M = mem[is.x];
+  int b = is.y;
   int lane = threadIdx.x, c = lane / kChainLanes, l = lane % kChainLanes;
-  int nd = m.ndim, ndof = nd / 2, lag = m.chain_lag, W = m.chain_w;
-  uint64_t key = keys[b];
-  double a2 = scal[8 * (size_t)b];
-  const int* IX = llidx + (size_t)b * nd * 25;
-  const double* VX = llval + (size_t)b * nd * 25;
-  double* S = Sall + (size_t)b * nd * P;
+  int nd = M.nd, ndof = nd / 2, lag = M.chain_lag, W = M.chain_w, P = M.P;
+  double max_slip = M.max_slip;
+  uint64_t key = M.bt.key[b];
+  double a2 = M.bt.scal[8 * (size_t)b];
+  const int* IX = M.bt.llc_idx + (size_t)b * nd * 25;
+  const double* VX = M.bt.llc_val + (size_t)b * nd * 25;
+  double* S = M.bt.S + (size_t)b * nd * P;
   double* mine = ring + (size_t)lane * W;
   const double* prev = ring + (size_t)(c * kChainLanes + (l + kChainLanes - 1) % kChainLanes) * W;
   for (int w = 0; w < W; ++w) mine[w] = 0.0;
@@ -105,11 +105,10 @@ __global__ void __launch_bounds__(64) k_init_chain(MeshDev m, int P, double max_
   }
 }
 
-hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s, const int* sel) {
-  size_t sh = sizeof(double) * 2 * kChainLanes * (size_t)m.chain_w;
+hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, int B, int chain_w, hipStream_t s) {
+  size_t sh = sizeof(double) * 2 * kChainLanes * (size_t)chain_w;
   if (sh > 64 * 1024) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_init_chain, dim3(B), dim3(2 * kChainLanes), sh, s, m, bt.P, max_slip, bt.key, bt.scal,
-                     bt.llc_idx, bt.llc_val, bt.S, sel);
+  hipLaunchKernelGGL(k_init_chain, dim3(B), dim3(2 * kChainLanes), sh, s, mem, list, sel);
   return hipGetLastError();
 }
 
@@ -223,14 +222,17 @@ SMCF_DEV void normalize_resample(double* w, int P, dd u, int* assigned, int* par
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(GT) k_gamma_resample(int P, const double* __restrict__ L0,
-                                                       const double* __restrict__ L1, BatchDev bt) {
+__global__ void __launch_bounds__(GT) k_gamma_resample(StageSet ss) {
   extern __shared__ double smem[];
+  int2 is = ss.inst[blockIdx.x];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int P = M.P;
   double* Ls = smem;            // P
   double* ws = Ls + P;          // P
   double* red = ws + P;         // GT
   dd* scan = (dd*)(red + GT);   // GT + 1
-  int b = blockIdx.x, t = threadIdx.x;
+  int b = is.y, t = threadIdx.x;
   if (bt.done[b]) {
     if (t == 0) bt.active[b] = 0;
     return;
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(GT) k_gamma_resample(int P, const double* __re
   int pb = bt.par[b];
   int it = bt.iter[b];
   double gprev = bt.gamma[b];
-  const double* L = (pb ? L1 : L0) + (size_t)b * P;
+  const double* L = (pb ? bt.like2 : bt.like) + (size_t)b * P;
   for (int i = t; i < P; i += GT) Ls[i] = L[i];
   __syncthreads();
   double nev;
@@ -269,10 +271,10 @@ __global__ void __launch_bounds__(GT) k_gamma_resample(int P, const double* __re
   for (int i = t; i < P; i += GT) bt.w[(size_t)b * P + i] = ws[i];
 }
 
-hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  (void)m;
-  size_t sh = sizeof(double) * (2 * (size_t)bt.P + GT) + sizeof(dd) * (GT + 1);
-  hipLaunchKernelGGL(k_gamma_resample, dim3(B), dim3(GT), sh, s, bt.P, bt.like, bt.like2, bt);
+hipError_t launch_gamma_resample(const StageSet& ss, hipStream_t s) {
+  // the workgroup's LDS is sized for the largest P: the sums run over its own P only
+  size_t sh = sizeof(double) * (2 * (size_t)ss.P_max + GT) + sizeof(dd) * (GT + 1);
+  hipLaunchKernelGGL(k_gamma_resample, dim3(ss.ninst), dim3(GT), sh, s, ss);
   return hipGetLastError();
 }
 
@@ -295,9 +297,12 @@ SMCF_DEV void moments_one(int i, int P, const double* __restrict__ S, const doub
 }
 
 // weighted mean and variance diagonal of the pre-resampling particles (source buffer)
-__global__ void k_moments(int nd, int P, BatchDev bt) {
+__global__ void k_moments(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int nd = M.nd, P = M.P, b = is.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
   if (!bt.active[b] || i >= nd) return;
   const double* S = (bt.par[b] ? bt.S : bt.S2) + (size_t)b * nd * P;  // source = previous buffer
   double mean, cv;
@@ -319,15 +324,18 @@ hipError_t launch_moments_raw(int nd, int P, const double* S, const double* w, d
   return hipGetLastError();
 }
 
-hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  hipLaunchKernelGGL(k_moments, dim3((m.ndim + 63) / 64, B), dim3(64), 0, s, m.ndim, bt.P, bt);
+hipError_t launch_moments(const StageSet& ss, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments, dim3((ss.nd_max + 63) / 64, ss.ninst), dim3(64), 0, s, ss);
   return hipGetLastError();
 }
 
 // children <- parents into the (new) current buffer; likelihood / prior follow
-__global__ void k_gather(int nd, int P, BatchDev bt) {
+__global__ void k_gather(StageSet ss) {
+  int2 is = ss.inst[blockIdx.z];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int nd = M.nd, P = M.P, b = is.y;
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int b = blockIdx.z;
   if (!bt.active[b] || j >= P) return;
   int cur = bt.par[b];
   const double* Ss = (cur ? bt.S : bt.S2) + (size_t)b * nd * P;
@@ -364,9 +372,9 @@ __global__ void k_gather(int nd, int P, BatchDev bt) {
   }
 }
 
-hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int gy = m.ndim < 64 ? m.ndim : 64;
-  hipLaunchKernelGGL(k_gather, dim3((bt.P + 255) / 256, gy, B), dim3(256), 0, s, m.ndim, bt.P, bt);
+hipError_t launch_gather(const StageSet& ss, hipStream_t s) {
+  int gy = ss.nd_max < 64 ? ss.nd_max : 64;
+  hipLaunchKernelGGL(k_gather, dim3((ss.P_max + 255) / 256, gy, ss.ninst), dim3(256), 0, s, ss);
   return hipGetLastError();
 }
 
@@ -504,6 +512,24 @@ __global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
             (pb ? L1 : L0) + (size_t)b * P);
 }
 
+// the same per item (instance, 256-particle chunk) of a stage set (moved particles of the FP64 and FP32 forms)
+__global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
+    k_like_items(const int4* __restrict__ items, const MemberDev* __restrict__ mem) {
+  int4 it = items[blockIdx.x];
+  if (it.x < 0) return;
+  const MemberDev& M = mem[it.x];
+  const BatchDev& bt = M.bt;
+  int b = it.y, nd = M.nd, P = M.P;
+  if (!bt.active[b]) return;
+  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int p0 = it.z * 256 + wave * 64;
+  if (p0 >= P) return;
+  int pb = bt.par[b];
+  const double* sc = bt.scal + 8 * (size_t)b;
+  wave_like(nd, P, bt.H + (size_t)b * nd * nd, (pb ? bt.S2 : bt.S) + (size_t)b * nd * P, bt.bvec + (size_t)b * nd,
+            bt.dsd[b], sc[3] + sc[4], p0, lane, (pb ? bt.like2 : bt.like) + (size_t)b * P);
+}
+
 hipError_t launch_like_raw(int nd, int P, const double* H, const double* bv, const double* dsd, const double* scal,
                            const double* S, const int* zero, double* out, hipStream_t s) {
   int chunks = (P + 255) / 256;
@@ -517,6 +543,11 @@ hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hi
   int groups = (B + 7) / 8;
   hipLaunchKernelGGL(k_like_mfma, dim3(groups * 8 * chunks), dim3(256), 0, s, m.ndim, bt.P, B, bt.H, bt.bvec, bt.dsd,
                      bt.scal, bt.S, bt.S2, bt.par, bt.active, use_active, sel, bt.like, bt.like2);
+  return hipGetLastError();
+}
+
+hipError_t launch_like_stage(const StageSet& ss, const ItemList& it, hipStream_t s) {
+  if (it.n > 0) hipLaunchKernelGGL(k_like_items, dim3(it.n), dim3(256), 0, s, it.items, ss.mem);
   return hipGetLastError();
 }
 
@@ -550,12 +581,11 @@ constexpr int kGibbsKB = 16 * kGibbsMT;
 constexpr int kGibbsOcc = SMCF_GIBBS_OCC;  // waves per SIMD the register budget of k_gibbs is sized for
 
 template <int MT>
-__global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, int sweeps, double max_slip,
-                                               const double* __restrict__ Qball, double* __restrict__ S0,
-                                               double* __restrict__ S1, const double* __restrict__ cdall,
-                                               const uint64_t* __restrict__ keys,
-                                               const int* __restrict__ iters, const int* __restrict__ active,
-                                               const int* __restrict__ par) {
+__device__ __forceinline__ void gibbs_body(int nd, int P, int b, int chunk, int sweeps, double max_slip,
+                                           const double* __restrict__ Qball, double* __restrict__ S0,
+                                           double* __restrict__ S1, const double* __restrict__ cdall,
+                                           const uint64_t* __restrict__ keys, const int* __restrict__ iters,
+                                           const int* __restrict__ active, const int* __restrict__ par) {
   constexpr int KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
@@ -564,10 +594,6 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
   //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
   __shared__ double CQ[4][120];
   __shared__ double CD[4][4][16];
-  int chunks = (P + 255) / 256;
-  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
-  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
-  if (b >= B || !active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   int pw0 = chunk * 256 + wave * 64;
   int p = pw0 + lane;
@@ -702,6 +728,18 @@ __global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(int nd, int P, int B, 
   }
 }
 
+// one workgroup per item (instance, 256-particle chunk) of the FP64 form's class with KB = 16 MT rows
+template <int MT>
+__global__ void __launch_bounds__(256, kGibbsOcc) k_gibbs(const int4* __restrict__ items, GibbsArgs a) {
+  int4 it = items[blockIdx.x];
+  if (it.x < 0) return;
+  const GibbsMember& M = a.m[it.x];
+  if (!M.active[it.y]) return;
+  gibbs_body<MT>(M.nd, M.P, it.y, it.z, M.sweeps, M.max_slip, M.Q, M.S, M.S2, M.qdiag, M.key, M.iter, M.active,
+                 M.par);
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Mixed-precision Gibbs sweeps (SMCF_PRECISION_MIXED).  The sweep of k_gibbs, with
@@ -767,12 +805,13 @@ __device__ __noinline__ double gibbs_draw_f32(double mu, double s, double is, do
   return trunc_normal_prec_f32(mu, s, is, 0.0, max_slip, u);
 }
 
-__global__ void __launch_bounds__(256, kMixedOcc)
-    k_gibbs_mixed(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
-                  const float* __restrict__ Q32all, double* __restrict__ S0, double* __restrict__ S1,
-                  float* __restrict__ T0, float* __restrict__ T1, const double* __restrict__ cdall,
-                  const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
-                  const int* __restrict__ par) {
+__device__ __forceinline__ void gibbs_mixed_body(int nd, int P, int b, int chunk, int sweeps, double max_slip,
+                                                 const double* __restrict__ Qball, const float* __restrict__ Q32all,
+                                                 double* __restrict__ S0, double* __restrict__ S1,
+                                                 float* __restrict__ T0, float* __restrict__ T1,
+                                                 const double* __restrict__ cdall, const uint64_t* __restrict__ keys,
+                                                 const int* __restrict__ iters, const int* __restrict__ active,
+                                                 const int* __restrict__ par) {
   constexpr int MT = kMixedMT, KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
@@ -780,10 +819,6 @@ __global__ void __launch_bounds__(256, kMixedOcc)
   //   CD[c][kk] = cd[c nd + i0 + 16m + kk]: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
   __shared__ double CQ[4][16][16];
   __shared__ double CD[4][4][16];
-  int chunks = (P + 255) / 256;
-  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
-  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
-  if (b >= B || !active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   int pw0 = chunk * 256 + wave * 64;
   int p = pw0 + lane;
@@ -909,6 +944,17 @@ __global__ void __launch_bounds__(256, kMixedOcc)
   }
 }
 
+// one workgroup per item (instance, 256-particle chunk) of the FP32 form
+__global__ void __launch_bounds__(256, kMixedOcc) k_gibbs_mixed(const int4* __restrict__ items, GibbsArgs a) {
+  int4 it = items[blockIdx.x];
+  if (it.x < 0) return;
+  const GibbsMember& M = a.m[it.x];
+  if (!M.active[it.y]) return;
+  gibbs_mixed_body(M.nd, M.P, it.y, it.z, M.sweeps, M.max_slip, M.Q, M.Q32, M.S, M.S2, M.S32, M.S32b, M.qdiag, M.key,
+                   M.iter, M.active, M.par);
+}
+
+
 // ---------------------------------------------------------------------------
 // Mixed-precision Gibbs sweeps with split-BF16 products (SMCF_PRECISION_MIXED_BF16).  The sweep of
 // k_gibbs_mixed, with the products q = Q S formed on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16):
@@ -1006,13 +1052,14 @@ __device__ __forceinline__ void mfma_panel_bf3(const bf16x8* __restrict__ qa, co
 }
 
 template <int MT>
-__global__ void __launch_bounds__(256, kBfOcc)
-    k_gibbs_bf(int nd, int P, int B, int sweeps, double max_slip, const double* __restrict__ Qball,
-               const bf16x8* __restrict__ QXall, double* __restrict__ S0, double* __restrict__ S1,
-               bf16x8* __restrict__ X0, bf16x8* __restrict__ X1, bf16x8* __restrict__ DXall,
-               const double* __restrict__ mrefall, const double* __restrict__ cdall,
-               const uint64_t* __restrict__ keys, const int* __restrict__ iters, const int* __restrict__ active,
-               const int* __restrict__ par) {
+__device__ __forceinline__ void gibbs_bf_body(int nd, int P, int b, int chunk, int sweeps, double max_slip,
+                                              const double* __restrict__ Qball, const bf16x8* __restrict__ QXall,
+                                              double* __restrict__ S0, double* __restrict__ S1,
+                                              bf16x8* __restrict__ X0, bf16x8* __restrict__ X1,
+                                              bf16x8* __restrict__ DXall, const double* __restrict__ mrefall,
+                                              const double* __restrict__ cdall, const uint64_t* __restrict__ keys,
+                                              const int* __restrict__ iters, const int* __restrict__ active,
+                                              const int* __restrict__ par) {
   constexpr int KB = 16 * MT;
   __shared__ double T[4][16][65];
   // per wave, the constants of the tile being drawn (read with wave-uniform addresses, so broadcast):
@@ -1021,10 +1068,6 @@ __global__ void __launch_bounds__(256, kBfOcc)
   //   reference particle mref(i0 + 16m + kk) of the centred likelihood
   __shared__ double CQ[4][16][16];
   __shared__ double CD[4][5][16];
-  int chunks = (P + 255) / 256;
-  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
-  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
-  if (b >= B || !active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   int pw0 = chunk * 256 + wave * 64;
   int p = pw0 + lane;
@@ -1227,16 +1270,30 @@ __global__ void __launch_bounds__(256, kBfOcc)
   }
 }
 
+// one workgroup per item (instance, 256-particle chunk) of the split-BF16 form's class with KB = 16 MT rows
+template <int MT>
+__global__ void __launch_bounds__(256, kBfOcc) k_gibbs_bf(const int4* __restrict__ items, GibbsArgs a) {
+  int4 it = items[blockIdx.x];
+  if (it.x < 0) return;
+  const GibbsMember& M = a.m[it.x];
+  if (!M.active[it.y]) return;
+  gibbs_bf_body<MT>(M.nd, M.P, it.y, it.z, M.sweeps, M.max_slip, M.Q, (const bf16x8*)M.QX, M.S, M.S2,
+                    (bf16x8*)M.SX, (bf16x8*)M.SXb, (bf16x8*)M.DX, M.mref, M.qdiag, M.key, M.iter, M.active, M.par);
+}
+
+
 // Split-BF16 copy of the blocked precision in fragment order: fragment f = ((blk nst + st) MT + m) 64 + lane
 // of lane = r + 16g holds Q(32 st + 8g + e, blk KB + 16m + r), e = 0..7, hi at QX[2f], lo at QX[2f + 1].
-__global__ void k_q_to_bf(int nd, int KB, const double* __restrict__ Qball, const int* __restrict__ active,
-                          bf16x8* __restrict__ QXall) {
-  int b = blockIdx.y;
-  if (!active[b]) return;
+__global__ void k_q_to_bf(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
+  if (!bt.QX || !bt.active[b]) return;
   int MT = KB / 16, nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
   size_t nfrag = (size_t)nblk * nst * MT * 64;
-  const double* Qb = Qball + (size_t)b * nblk * nd * KB;
-  bf16x8* QX = QXall + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  const double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
+  bf16x8* QX = (bf16x8*)bt.QX + (size_t)b * (bf_q_elems(nd, KB) / 8);
   for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nfrag; f += (size_t)gridDim.x * blockDim.x) {
     int lane = (int)(f % 64);
     size_t rest = f / 64;
@@ -1289,11 +1346,10 @@ hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, cons
   return hipGetLastError();
 }
 
-hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int nd = m.ndim, KB = bt.kb;
-  size_t nfrag = bf_q_elems(nd, KB) / 16;
+hipError_t launch_q_bf(const StageSet& ss, hipStream_t s) {
+  size_t nfrag = bf_q_elems(ss.nd_max, ss.kb_max) / 16;
   int blocks = (int)std::min<size_t>((nfrag + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_q_to_bf, dim3(blocks, B), dim3(256), 0, s, nd, KB, bt.Q, bt.active, (bf16x8*)bt.QX);
+  hipLaunchKernelGGL(k_q_to_bf, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
   return hipGetLastError();
 }
 
@@ -1312,10 +1368,13 @@ constexpr int kLikeBfMT = 4;  // 16-row tiles per block of H (64 rows)
 
 // m = child 0 after the resampling (its parent's FP64 particle: the children's FP64 copy is written by the last
 // sweep); g = H m - b; L(m) (one workgroup of GT threads per instance)
-__global__ void __launch_bounds__(GT) k_like_ref(int nd, int P, BatchDev bt) {
+__global__ void __launch_bounds__(GT) k_like_ref(StageSet ss) {
   __shared__ double red[GT];
-  int b = blockIdx.x;
-  if (!bt.active[b]) return;
+  int2 is = ss.inst[blockIdx.x];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, P = M.P;
+  if (!bt.HX || !bt.active[b]) return;
   const double* S = (bt.par[b] ? bt.S : bt.S2) + (size_t)b * nd * P + bt.parent[(size_t)b * P];
   const double* H = bt.H + (size_t)b * nd * nd;
   const double* bv = bt.bvec + (size_t)b * nd;
@@ -1371,12 +1430,14 @@ __global__ void k_h_to_bf(int nd, const double* __restrict__ Hall, bf16x8* __res
 
 // one workgroup of 4 wavefronts per 256 particles of an instance, XCD-aware as k_like_mfma
 __global__ void __launch_bounds__(256, 2)
-    k_like_bf(int nd, int P, int B, BatchDev bt) {
+    k_like_bf(const int4* __restrict__ items, const MemberDev* __restrict__ mem) {
   constexpr int MT = kLikeBfMT, KB = 16 * MT;
-  int chunks = (P + 255) / 256;
-  int w = blockIdx.x, xcd = w % kXcd, r = w / kXcd;
-  int b = (r / chunks) * kXcd + xcd, chunk = r % chunks;
-  if (b >= B || !bt.active[b]) return;
+  int4 it = items[blockIdx.x];
+  if (it.x < 0) return;
+  const MemberDev& M = mem[it.x];
+  const BatchDev& bt = M.bt;
+  int b = it.y, chunk = it.z, nd = M.nd, P = M.P;
+  if (!bt.active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   int p0 = chunk * 256 + wave * 64;
   if (p0 >= P) return;
@@ -1440,15 +1501,13 @@ hipError_t launch_h_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, con
   return hipGetLastError();
 }
 
-hipError_t launch_like_ref(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  hipLaunchKernelGGL(k_like_ref, dim3(B), dim3(GT), 0, s, m.ndim, bt.P, bt);
+hipError_t launch_like_ref(const StageSet& ss, hipStream_t s) {
+  hipLaunchKernelGGL(k_like_ref, dim3(ss.ninst), dim3(GT), 0, s, ss);
   return hipGetLastError();
 }
 
-hipError_t launch_like_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s) {
-  int chunks = (bt.P + 255) / 256;
-  int groups = (B + kXcd - 1) / kXcd;
-  hipLaunchKernelGGL(k_like_bf, dim3(groups * kXcd * chunks), dim3(256), 0, s, m.ndim, bt.P, B, bt);
+hipError_t launch_like_bf(const StageSet& ss, const ItemList& it, hipStream_t s) {
+  if (it.n > 0) hipLaunchKernelGGL(k_like_bf, dim3(it.n), dim3(256), 0, s, it.items, ss.mem);
   return hipGetLastError();
 }
 
@@ -1491,26 +1550,20 @@ void set_gibbs_rows(int precision, int lo, int hi, int rows) {
   g_tile_rules.push_back({precision, lo, hi, rows});
 }
 
-hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s) {
-  int nd = m.ndim;
-  int chunks = (bt.P + 255) / 256;
-  int groups = (B + kXcd - 1) / kXcd;
-  dim3 g(groups * kXcd * chunks);
-  if (bt.QX) {
-    auto k = bt.kb == 16 ? k_gibbs_bf<1> : bt.kb == 32 ? k_gibbs_bf<2> : bt.kb == 48 ? k_gibbs_bf<3> : nullptr;
+hipError_t launch_gibbs(const GibbsArgs& a, const ItemList& it, int precision, int kb, hipStream_t s) {
+  if (it.n < 1) return hipSuccess;
+  dim3 g(it.n);
+  if (precision == SMCF_PRECISION_MIXED_BF16) {
+    auto k = kb == 16 ? k_gibbs_bf<1> : kb == 32 ? k_gibbs_bf<2> : kb == 48 ? k_gibbs_bf<3> : nullptr;
     if (!k) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, (const bf16x8*)bt.QX,
-                       bt.S, bt.S2, (bf16x8*)bt.SX, (bf16x8*)bt.SXb, (bf16x8*)bt.DX, bt.mref, bt.qdiag, bt.key,
-                       bt.iter, bt.active, bt.par);
-  } else if (bt.Q32) {
-    if (bt.kb != 16 * kMixedMT) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.Q32, bt.S,
-                       bt.S2, bt.S32, bt.S32b, bt.qdiag, bt.key, bt.iter, bt.active, bt.par);
+    hipLaunchKernelGGL(k, g, dim3(256), 0, s, it.items, a);
+  } else if (precision == SMCF_PRECISION_MIXED) {
+    if (kb != 16 * kMixedMT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gibbs_mixed, g, dim3(256), 0, s, it.items, a);
   } else {
-    auto k = bt.kb == 16 ? k_gibbs<1> : bt.kb == 32 ? k_gibbs<2> : nullptr;
+    auto k = kb == 16 ? k_gibbs<1> : kb == 32 ? k_gibbs<2> : nullptr;
     if (!k) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, g, dim3(256), 0, s, nd, bt.P, B, cfg.sweeps, cfg.max_slip, bt.Q, bt.S, bt.S2, bt.qdiag,
-                       bt.key, bt.iter, bt.active, bt.par);
+    hipLaunchKernelGGL(k, g, dim3(256), 0, s, it.items, a);
   }
   return hipGetLastError();
 }

@@ -165,12 +165,71 @@ inline size_t hmc_qgx_frags(int nd) { return (size_t)((nd + 15) / 16) * ((nd + 3
 // entries (ints) of one pass's work list: per instance at most P / 16 + P + 1 items of 2 ints
 inline size_t hmc_work_entries(int cap, int P) { return (size_t)cap * 2 * ((size_t)P + P / 16 + 1); }
 
+// One problem's part of a stage launch (a device table of these lets one launch cover instances of problems with
+// different meshes, observation sets and particle counts): its per-slot buffers, dimensions and sampler settings.
+struct MemberDev {
+  BatchDev bt;
+  int nd, nf, P, kb, sweeps;
+  int chain_lag, chain_w;  // pipelined prior chain (MeshDev)
+  double max_slip;
+};
+
+// The instances one stage launch covers (device arrays built by the stage engine):
+//   inst[g] = (member, slot), g < ninst, for the kernels with one workgroup (or grid row) per instance;
+//   items = (member, slot, chunk, 0), one workgroup per 256 particles, for the chunked kernels, in XCD-aware
+//   order: item w runs on XCD w % 8 and all chunks of one instance share w % 8, so an instance's matrices are
+//   fetched into one L2; member -1 marks an empty item.
+struct StageSet {
+  const MemberDev* mem;
+  const int2* inst;
+  int ninst;
+  int nd_max, P_max, kb_max;
+};
+struct ItemList {
+  const int4* items;
+  int n;
+};
+
+// Member table of one Gibbs-sweep launch, passed as a kernel argument (at most kGibbsArgMembers members per launch;
+// the items' member field indexes it).  Kernel arguments are constant for the launch, so the compiler may reload
+// these values instead of keeping them in registers: read from a device table they pushed the split-BF16 sweep
+// kernel past 256 VGPRs into scratch memory.
+constexpr int kGibbsArgMembers = 16;
+struct GibbsMember {
+  int nd, P, sweeps, pad;
+  double max_slip;
+  const double* Q;
+  const void* QX;     // split BF16 (bf16x8 fragments)
+  const float* Q32;   // FP32 form
+  double* S;
+  double* S2;
+  void* SX;           // split BF16
+  void* SXb;
+  float* S32;         // FP32 form
+  float* S32b;
+  void* DX;           // split BF16
+  const double* mref;
+  const double* qdiag;
+  const uint64_t* key;
+  const int* iter;
+  const int* active;
+  const int* par;
+};
+struct GibbsArgs {
+  GibbsMember m[kGibbsArgMembers];
+};
+
+struct Engine;
+
 struct Problem {
   smcf_ctx* ctx;
   MeshDev m;
   BatchDev bt;
   HmcDev hm;
   HmcHost hh;
+  MemberDev* mdev = nullptr;  // this problem as a one-entry member table (device)
+  Engine* engine = nullptr;   // set while the problem is a member of a running group: batch calls go to its engine
+  int member = -1;            // index in that engine
   double max_slip;
   int P, mover, sweeps, precision;
   int hmc_ntau;       // leapfrog steps (50 for SMCF_MOVER_HMC, smc_slip.f90:669)
@@ -178,6 +237,7 @@ struct Problem {
   std::vector<int> sar_r0, sar_n;  // host copies of the InSAR path layout (launch geometry)
   std::vector<long> sar_off;
   std::vector<void*> allocs;
+  bool in_group = false;  // a member of a group (smcf_group_create)
   bool sizing = false;  // count the device bytes of a description without allocating (smcf_problem_bytes)
   size_t bytes = 0;     // device bytes allocated (or counted) for the problem
   int alloc_rc = 0;     // first failure of a problem allocation or upload (SMCF_* code), 0 while none
@@ -185,10 +245,45 @@ struct Problem {
   std::vector<double> t_admit; // seconds from the start of the last batch call to each geometry's set-up launch
 };
 
-struct StageCfg {
-  double max_slip;
-  int P, sweeps;
-};
+inline GibbsMember gibbs_member_of(const Problem& P) {
+  GibbsMember g{};
+  g.nd = P.m.ndim;
+  g.P = P.P;
+  g.sweeps = P.sweeps;
+  g.max_slip = P.max_slip;
+  g.Q = P.bt.Q;
+  g.QX = P.bt.QX;
+  g.Q32 = P.bt.Q32;
+  g.S = P.bt.S;
+  g.S2 = P.bt.S2;
+  g.SX = P.bt.SX;
+  g.SXb = P.bt.SXb;
+  g.S32 = P.bt.S32;
+  g.S32b = P.bt.S32b;
+  g.DX = P.bt.DX;
+  g.mref = P.bt.mref;
+  g.qdiag = P.bt.qdiag;
+  g.key = P.bt.key;
+  g.iter = P.bt.iter;
+  g.active = P.bt.active;
+  g.par = P.bt.par;
+  return g;
+}
+
+inline MemberDev member_of(const Problem& P) {
+  MemberDev d{};
+  d.bt = P.bt;
+  d.nd = P.m.ndim;
+  d.nf = P.m.ndim_fault;
+  d.P = P.P;
+  d.kb = P.bt.kb;
+  d.sweeps = P.sweeps;
+  d.chain_lag = P.m.chain_lag;
+  d.chain_w = P.m.chain_w;
+  d.max_slip = P.max_slip;
+  return d;
+}
+
 
 // kernels (launchers return hipError_t of the launch)
 // sel (optional): the B instance slots to work on, instead of slots 0 .. B-1
@@ -199,20 +294,32 @@ hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s
 // sigmas already in bt.scal are used instead of those of theta
 hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s,
                        const int* sel = nullptr, bool scal_given = false);
-hipError_t launch_init_chain(const MeshDev& m, BatchDev& bt, int B, double max_slip, hipStream_t s,
-                             const int* sel = nullptr);
-// likelihood of the current particles (matrix cores; Gram form)
+// prior chain of B instances: (member, slot) = list[k] of the member table mem, or with list == nullptr member 0 and
+// slot sel[k] (slot k when sel == nullptr); chain_w = the largest ring length of the members involved
+hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, int B, int chain_w, hipStream_t s);
+// likelihood of the current particles of the given slots of one problem (matrix cores; Gram form; set-up)
 hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel = nullptr);
 // prior of the current particles
 hipError_t launch_prior(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
-hipError_t launch_gamma_resample(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_moments(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 // mean and variance diagonal of P dimension-major particles S with normalised weights w
 hipError_t launch_moments_raw(int nd, int P, const double* S, const double* w, double* mean, double* cov,
                               hipStream_t s);
-hipError_t launch_gather(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_form_q(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_gibbs(const MeshDev& m, BatchDev& bt, int B, const StageCfg& cfg, hipStream_t s);
+// Stage kernels over a stage set (every instance of every member; instances not active this stage are skipped)
+hipError_t launch_gamma_resample(const StageSet& ss, hipStream_t s);
+hipError_t launch_moments(const StageSet& ss, hipStream_t s);
+hipError_t launch_gather(const StageSet& ss, hipStream_t s);
+// the stage's blocked Gibbs precision, its diagonal constants and (members of the mixed forms) its FP32 or
+// split-BF16 copies
+hipError_t launch_form_q(const StageSet& ss, bool any_f32, bool any_bf, hipStream_t s);
+hipError_t launch_q_bf(const StageSet& ss, hipStream_t s);
+// Gibbs sweeps of the items of one launch of a kernel class (precision, block rows); items index the launch's
+// member table
+hipError_t launch_gibbs(const GibbsArgs& a, const ItemList& it, int precision, int kb, hipStream_t s);
+// likelihood of the moved particles: Gram form on the FP64 matrix cores (FP64 and FP32 members) or the centred
+// split-BF16 form (split-BF16 members; launch_like_ref before the sweeps)
+hipError_t launch_like_stage(const StageSet& ss, const ItemList& it, hipStream_t s);
+hipError_t launch_like_ref(const StageSet& ss, hipStream_t s);
+hipError_t launch_like_bf(const StageSet& ss, const ItemList& it, hipStream_t s);
 // columns per block of the blocked Gibbs precision layout (SMCF_PRECISION_*)
 int gibbs_kb(int precision, int nd);
 bool gibbs_rows_valid(int precision, int rows);
@@ -221,18 +328,12 @@ void set_gibbs_rows(int precision, int lo, int hi, int rows);
 hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
 // split-BF16 copies of the current particles of the given slots (SMCF_PRECISION_MIXED_BF16)
 hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
-// split-BF16 fragment-ordered copy of the stage's blocked precision (SMCF_PRECISION_MIXED_BF16)
-hipError_t launch_q_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-// centred split-BF16 likelihood (SMCF_PRECISION_MIXED_BF16): H in fragment order for the given slots (set-up),
-// the stage's reference particle, H m - b and L(m) (before the Gibbs sweeps), and the likelihood (after them)
+// centred split-BF16 likelihood (SMCF_PRECISION_MIXED_BF16): H in fragment order for the given slots (set-up)
 hipError_t launch_h_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel = nullptr);
-hipError_t launch_like_ref(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
-hipError_t launch_like_bf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s);
 // coordinates of the split-BF16 particle copies (nd rounded up to whole 32-row matrix-core steps)
 __host__ __device__ inline int bf_rows(int nd) { return (nd + 31) & ~31; }
 // bf16 values of one instance's fragment-ordered precision QX (KB-row blocks x 32-column steps)
 __host__ __device__ inline size_t bf_q_elems(int nd, int kb) { return (size_t)((nd + kb - 1) / kb) * kb * bf_rows(nd) * 2; }
-hipError_t launch_stage_end(BatchDev& bt, int B, hipStream_t s);
 
 // HMC movers: SMCF_MOVER_HMC (ESJD tuning, ntau 50) or SMCF_MOVER_HMC_ACCEPT (ntau steps, step size adapted
 // toward `accept`)
@@ -285,6 +386,10 @@ hipError_t launch_philox(int n, const uint32_t* ctr, const uint32_t* key, uint32
 
 }  // namespace smcf
 
+struct smcf_problem {
+  smcf::Problem p;
+};
+
 struct smcf_ctx {
   int device;
   hipStream_t stream;
@@ -299,3 +404,46 @@ struct smcf_ctx {
   };
   std::vector<Stat> stats;
 };
+
+namespace smcf {
+
+// Device time of a named section of the library's stream (smcf_set_profiling): events around the section,
+// resolved into ctx->stats by resolve_profiles.
+struct ProfScope {
+  smcf_ctx* ctx;
+  const char* name;
+  hipStream_t st;
+  hipEvent_t a{}, b{};
+  ProfScope(smcf_ctx* c, const char* n, hipStream_t s = nullptr) : ctx(c), name(n), st(s ? s : c->stream) {
+    if (ctx->profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, st);
+    }
+  }
+  ~ProfScope() {
+    if (ctx->profiling) {
+      (void)hipEventRecord(b, st);
+      ctx->pending.push_back({name, {a, b}});
+    }
+  }
+};
+void resolve_profiles(smcf_ctx* ctx);
+
+// Set-up of n instances of one problem on stream s (slots sel[0 .. n-1], device array; thetas and keys already in
+// their slots), left held (marked done) until the stage engine releases them: setup_pre forms everything up to
+// the Gram quantities, the prior chain runs once over every instance admitted together (launch_init_chain), and
+// setup_post forms the particle copies, the likelihood and the prior of the initial particles.
+int setup_pre(Problem& P, int n, const int* sel, hipStream_t s);
+int setup_post(Problem& P, int n, const int* sel, hipStream_t s);
+
+// The inner SMC of N geometries of one problem streamed through its slots (the stage engine with one member):
+// -log Z into neglog and the stage counts into nstage (either may be null); want_moments keeps the weighted mean
+// and variance of every stage's particles.
+int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
+               int* nstage);
+// batch call of a problem that is a member of a running group: queued to the group's engine, returns when the
+// engine has finished all N geometries
+int engine_submit_wait(Problem& P, int N, const double* theta, const uint64_t* keys, double* neglog, int* nstage);
+
+}  // namespace smcf

@@ -33,28 +33,30 @@ LOG_SIGMA_GNSS2_RANGE = (-1.0, 1.0)
 LOG_ALPHA2_TRUE = -3.0
 
 
-def fault_frame():
-    """xmin, xmax, zmin and the true (ymin, ymax, dip) for the surface-breaking fault."""
-    s = math.radians(STRIKE_DEG)
-    dx, dy = LENGTH_KM * math.sin(s), LENGTH_KM * math.cos(s)
+def fault_frame(strike=STRIKE_DEG, dip=DIP_DEG, length=LENGTH_KM, width=WIDTH_KM):
+    """xmin, xmax, zmin and the true (ymin, ymax, dip) of a surface-breaking fault centred on the origin
+    (defaults: the fault of the single-inversion configurations)."""
+    s = math.radians(strike)
+    dx, dy = length * math.sin(s), length * math.cos(s)
     xmin, xmax = -dx / 2, dx / 2
     ymin, ymax = -dy / 2, dy / 2
-    zmin = -WIDTH_KM * math.sin(math.radians(DIP_DEG))
-    return xmin, xmax, zmin, (ymin, ymax, DIP_DEG)
+    zmin = -width * math.sin(math.radians(dip))
+    return xmin, xmax, zmin, (ymin, ymax, dip)
 
 
-def theta_true():
-    _, _, _, (ymin, ymax, dip) = fault_frame()
+def theta_true(frame=None):
+    _, _, _, (ymin, ymax, dip) = frame or fault_frame()
     return np.array([ymin, ymax, dip, LOG_ALPHA2_TRUE, 0.0, 0.0])
 
 
-def prior_range(frac=0.2):
+def prior_range(frac=0.2, frame=None):
     """2 x 6 range (range.dat layout): +-frac of the true geometry, fixed hyper-parameter ranges."""
-    _, _, _, (ymin, ymax, dip) = fault_frame()
+    _, _, _, (ymin, ymax, dip) = frame or fault_frame()
     r = np.zeros((2, 6))
     for k, v in enumerate((ymin, ymax, dip)):
         lo, hi = v * (1 - frac), v * (1 + frac)
         r[:, k] = (min(lo, hi), max(lo, hi))
+    r[1, 2] = min(r[1, 2], 89.0)  # a steep fault's dip range stays below vertical
     r[:, 3] = LOG_ALPHA2_RANGE
     r[:, 4] = LOG_SIGMA_SAR2_RANGE
     r[:, 5] = LOG_SIGMA_GNSS2_RANGE
@@ -73,9 +75,9 @@ def stations(nsta, rng):
     return pts, uv
 
 
-def truth_slip(nxi, neta, theta):
+def truth_slip(nxi, neta, theta, frame=None):
     """Gaussian bump on the DOF nodes (both slip components >= 0), DOF order of discretize_fault."""
-    xmin, xmax, zmin, _ = fault_frame()
+    xmin, xmax, zmin, _ = frame or fault_frame()
     ymin, ymax, dip = theta[0], theta[1], theta[2]
     lxi = math.hypot(xmax - xmin, ymax - ymin)
     leta = -zmin / math.sin(math.radians(dip))
@@ -109,11 +111,12 @@ def sar_points(npath, nper, rng):
     return pts, uv
 
 
-def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, neta=None, npath=0, nsar_per_path=0):
+def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, neta=None, npath=0, nsar_per_path=0,
+                      frame=None):
     """Build the observation dict of config `cfg_name` (sizes overridable for small tests).
 
-    greens(theta, obs_points, obs_unitvec, nxi, neta) -> G (nobs x ndim) at the true geometry.
-    npath > 0 adds npath InSAR paths of nsar_per_path pixels before the GNSS rows (the reference's row
+    greens(theta, obs_points, obs_unitvec, nxi, neta) -> G (nobs x ndim) at the true geometry; frame (fault_frame's
+    output) sets another fault than the configurations' one.  npath > 0 adds npath InSAR paths of nsar_per_path pixels before the GNSS rows (the reference's row
     order), with noise correlated as the path covariance of calc_sigma_sar_mat, and stores the inverse
     covariance blocks under "sar_sinv"; greens is then called as greens(theta, pts, uv, nxi, neta, nsar).
     """
@@ -126,9 +129,9 @@ def make_observations(cfg_name, seed, greens, nsta=None, sigma=None, nxi=None, n
     neta = neta or cfg["neta"]
     rng = np.random.default_rng(np.random.SeedSequence([seed, 0x5A17]))
     pts, uv = stations(nsta, rng)
-    th = theta_true()
+    th = theta_true(frame)
     G = greens(th, pts, uv, nxi, neta)
-    s = truth_slip(nxi, neta, th)
+    s = truth_slip(nxi, neta, th, frame)
     d = G @ s + sigma * rng.standard_normal(G.shape[0])
     return dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.full(G.shape[0], sigma), dvec=d,
                 nsar_total=0, ngnss=nsta, nsar_index=np.array([1, 1], np.int32), nxi=nxi, neta=neta,
@@ -165,8 +168,37 @@ def _make_observations_sar(cfg_name, seed, greens, nsta, sigma, nxi, neta, npath
                 nsar_index=nsar_index, sar_sinv=sinv, nxi=nxi, neta=neta, slip_true=s)
 
 
-def sample_thetas(B, seed, frac=0.2):
+def sample_thetas(B, seed, frac=0.2, frame=None):
     """B geometry particles drawn uniformly from the prior box (fault_sample_init_particles)."""
     rng = np.random.default_rng(np.random.SeedSequence([seed, 0x7E7A]))
-    r = prior_range(frac)
+    r = prior_range(frac, frame)
     return np.asfortranarray(r[0][:, None] + rng.uniform(size=(6, B)) * (r[1] - r[0])[:, None])
+
+
+# The batching configuration: independent inversions whose sizes and faults are drawn from the seed
+BATCH = dict(n=64, patches=(4, 24), nsta=(50, 800), strike=(0.0, 360.0), dip=(20.0, 80.0), length=(10.0, 60.0),
+             N=(1000, 8000), P=(500, 2000), sigma=(0.2, 1.0))
+
+
+def batch_instances(n=None, seed=0, **over):
+    """n independent inversions of the batching configuration: per instance a patch grid (nxi, neta uniform in
+    4..24), a station count (50..800), the true fault (strike 0..360 deg, dip 20..80 deg, length 10..60 km, width
+    half the length, surface-breaking, centred on the origin), the noise (2..10 mm), outer N (1000..8000) and inner
+    P (500..2000) particle counts.  Keyword arguments replace ranges of BATCH (e.g. patches=(4, 8))."""
+    cfg = dict(BATCH, **over)
+    n = n or cfg["n"]
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0xBA7C]))
+    out = []
+    for k in range(n):
+        lo, hi = cfg["patches"]
+        nxi, neta = (int(v) for v in rng.integers(lo, hi + 1, size=2))
+        nsta = int(rng.integers(cfg["nsta"][0], cfg["nsta"][1] + 1))
+        strike = float(rng.uniform(*cfg["strike"]))
+        dip = float(rng.uniform(*cfg["dip"]))
+        length = float(rng.uniform(*cfg["length"]))
+        sigma = float(rng.uniform(*cfg["sigma"]))
+        N = int(rng.integers(cfg["N"][0], cfg["N"][1] + 1))
+        P = int(rng.integers(cfg["P"][0], cfg["P"][1] + 1))
+        out.append(dict(k=k, seed=int(seed) * 1000 + k, nxi=nxi, neta=neta, nsta=nsta, sigma=sigma, N=N, P=P,
+                        frame=fault_frame(strike, dip, length, length / 2), strike=strike, dip=dip, length=length))
+    return out
