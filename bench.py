@@ -75,9 +75,13 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--budget-s", type=float, default=480.0,
                     help="wall-clock budget of the whole run: a leg whose estimate does not fit is skipped")
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+    ap.add_argument("--batch-n", type=int, default=64, help="inversions of the batching leg (0 skips it)")
+    ap.add_argument("--batch-slots", type=int, default=16, help="device slots per inversion in the batching leg")
+    ap.add_argument("--batch-evals", type=int, default=32, help="geometries per inversion in the batching leg")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
                     help="c2: the headline (16x16 patches, Gibbs x5, FP64 and mixed forms); c3: 32x16 patches, "
-                         "3000 observations, HMC with 10 leapfrog steps tuned to 0.65 acceptance, FP64")
+                         "3000 observations, HMC with 10 leapfrog steps tuned to 0.65 acceptance, FP64; c4: the "
+                         "batching leg alone (64 inversions of different sizes as one group) with a solo comparison")
     ap.add_argument("--autotune", action="store_true",
                     help="time the Gibbs sweep kernels' block-row variants per mesh size and form, print the table "
                          "(the library's tuning table is this output) and exit")
@@ -317,6 +321,78 @@ def run_serving(a, S, syn):
                         "N 64..512 outer particles, 3 outer stages, inner P=512 Gibbs x3"}
 
 
+def device_greens(S, syn, ctx, frame):
+    """G (nobs x ndim) at a geometry from the device's calc_greens_func, for generating data."""
+    xmin, xmax, zmin, _ = frame
+
+    def greens(th, pts, uv, nx, ne):
+        obs0 = dict(obs_points=pts, obs_unitvec=uv, obs_sigma=np.ones(pts.shape[1]), dvec=np.zeros(pts.shape[1]),
+                    nsar_total=0, ngnss=pts.shape[1] // 3)
+        p = S.Problem(ctx, [nx], [ne], xmin, xmax, zmin, obs0, syn.MAX_SLIP, 8)
+        g = p.calc_greens_func(th)
+        p.close()
+        return g
+    return greens
+
+
+def run_batch(a, S, syn, solo_sample=0):
+    """Batching form: a.batch_n independent inversions drawn from the seed (patch grids 4x4 .. 24x24, 50 .. 800
+    stations, faults of any strike with dip 20 .. 80 deg and length 10 .. 60 km, noise 2 .. 10 mm, inner P
+    500 .. 2000), split-BF16 form, Gibbs x5, as one group: every member streams a.batch_evals geometries (thetas from
+    its own prior) through its a.batch_slots slots, and every stage of all members is one HIP graph launch (each
+    Gibbs kernel class one launch over the instances of all members).  Value = evaluations completed per second of
+    the group call (set-up of every geometry included).  solo_sample > 0 also times that many members' calls one
+    after another (each alone on the device), for comparison."""
+    insts = syn.batch_instances(a.batch_n, a.seed)
+    ctx = S.Context(0)
+    probs, ths, keys = [], [], []
+    for it in insts:
+        frame = it["frame"]
+        obs = syn.make_observations("c2", it["seed"], device_greens(S, syn, ctx, frame), nsta=it["nsta"],
+                                    sigma=it["sigma"], nxi=it["nxi"], neta=it["neta"], frame=frame)
+        xmin, xmax, zmin, _ = frame
+        probs.append(S.Problem(ctx, [it["nxi"]], [it["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, it["P"],
+                               mover=S.MOVER_GIBBS, gibbs_sweeps=5, max_batch=a.batch_slots,
+                               precision=S.PRECISION_MIXED_BF16))
+        ths.append(syn.sample_thetas(a.batch_evals, it["seed"], frame=frame))
+        keys.append(keys_for(it["seed"], a.batch_evals))
+    nd = [p.ndim for p in probs]
+    dev_bytes = sum(S.problem_bytes(p.desc, a.batch_slots) for p in probs)
+    print("batch: %d inversions, ndim %d..%d, P %d..%d, %d slots and %d geometries each, %.1f GB"
+          % (len(probs), min(nd), max(nd), min(p.P for p in probs), max(p.P for p in probs), a.batch_slots,
+             a.batch_evals, dev_bytes / 1e9), flush=True)
+    g = S.Group(ctx, probs, graph=True)
+    t0 = time.perf_counter()
+    nl, ns = g.fault_calc_likelihood(ths, keys)
+    wall = time.perf_counter() - t0
+    evals = len(probs) * a.batch_evals
+    stages = np.concatenate(ns)
+    out = {"inversions": len(probs), "evals": evals, "wall_s": round(wall, 2), "evals_per_s": round(evals / wall, 3),
+           "slots_per_inversion": a.batch_slots, "form": "bf16", "graph": True,
+           "ndim_range": [min(nd), max(nd)], "nobs_range": [min(p.nobs for p in probs), max(p.nobs for p in probs)],
+           "inner_particles_range": [min(p.P for p in probs), max(p.P for p in probs)],
+           "mean_inner_stages": round(float(stages.mean()), 2), "device_gb": round(dev_bytes / 1e9, 2),
+           "workload": "%d inversions drawn from the seed: patch grids 4x4..24x24, 50..800 GNSS stations, strike "
+                       "0..360, dip 20..80, length 10..60 km, 2..10 mm noise, inner P 500..2000, Gibbs x5, "
+                       "split-BF16; %d geometries per inversion from its prior" % (len(probs), a.batch_evals)}
+    print("batch: %d evaluations in %.2f s (%.2f/s), mean %.1f inner stages" % (evals, wall, evals / wall,
+                                                                                stages.mean()), flush=True)
+    g.close()
+    if solo_sample > 0:
+        t0 = time.perf_counter()
+        for k in range(solo_sample):
+            probs[k].fault_calc_likelihood(ths[k], keys[k])
+        ws = time.perf_counter() - t0
+        out["solo_sample"] = {"inversions": solo_sample, "evals": solo_sample * a.batch_evals, "wall_s": round(ws, 2),
+                              "evals_per_s": round(solo_sample * a.batch_evals / ws, 3)}
+        print("batch: the first %d inversions one after another: %.2f evaluations/s"
+              % (solo_sample, solo_sample * a.batch_evals / ws), flush=True)
+    for p in probs:
+        p.close()
+    ctx.close()
+    return out
+
+
 TUNE_MESHES = [(4, 4), (8, 8), (12, 12), (16, 16), (24, 16), (24, 24)]
 
 
@@ -377,6 +453,13 @@ def main():
     if a.autotune:
         run_autotune(a, S, syn)
         return
+    if a.workload == "c4":
+        b = run_batch(a, S, syn, solo_sample=4)
+        print(json.dumps({"metric": "outer-particle evaluations/s of %d independent inversions as one group"
+                                    % a.batch_n, "value": b["evals_per_s"], "unit": "evaluations/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "mixed (split-bf16 matrix + fp32 draws; fp64 "
+                          "means/likelihood)", "data": "synthetic", "batching": b}), flush=True)
+        return
     if a.copy_probe:
         import torch
         x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
@@ -429,6 +512,12 @@ def main():
             cpu = run_cpu(a, syn, obs, head)
         else:
             skipped.append("cpu_baseline")
+    batching = None
+    if a.batch_n > 0 and a.workload == "c2":
+        if elapsed() + 90 <= a.budget_s:
+            batching = run_batch(a, S, syn)
+        else:
+            skipped.append("batching")
     serving = None
     if a.serve_seconds > 0:
         if elapsed() + 3 * a.serve_seconds + 20 <= a.budget_s:
@@ -462,7 +551,7 @@ def main():
                       # per geometry: from its set-up launch to the stage boundary that returned its -log Z
                       "p50_eval_latency_s": head["eval_lat"][0], "p99_eval_latency_s": head["eval_lat"][1],
                       "stream": head["stream"]},
-           "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving,
+           "roofline": head["roofline"], "cpu_baseline": cpu, "serving": serving, "batching": batching,
            "skipped_legs": skipped, "wall_s": round(elapsed(), 1)}
     if "fp64" in res and len(res) > 1:
         f64 = res["fp64"]

@@ -142,7 +142,8 @@ void xcd_items(const std::vector<int4>& insts, std::vector<int4>& out) {
 struct Engine {
   smcf_ctx* ctx = nullptr;
   std::vector<MemberRun> mr;
-  bool want_moments = false, hmc = false, any_f32 = false, any_bf = false, use_graph = false, async = false;
+  bool want_moments = false, hmc = false, any_full = false, any_f32 = false, any_bf = false, use_graph = false,
+       async = false;
   std::vector<void*> dbufs;
   MemberDev* dmem = nullptr;
   double* dstat = nullptr;
@@ -217,6 +218,7 @@ struct Engine {
       if (P.mover != SMCF_MOVER_GIBBS) hmc = true;
       if (P.precision == SMCF_PRECISION_MIXED) any_f32 = true;
       if (P.bt.QX) any_bf = true;
+      if (P.bt.Q && !P.bt.QX) any_full = true;
     }
     if (hmc && n != 1) {
       set_error("stage engine: HMC movers run one problem at a time");
@@ -233,7 +235,8 @@ struct Engine {
     std::vector<Cls> cls;
     for (int i = 0; i < n; ++i) {
       Problem& P = *ps[i];
-      int chunks = (P.P + 255) / 256;
+      // 256 particles per workgroup (Gram-form likelihood), 64 (centred split-BF16 likelihood)
+      int chunks = P.bt.QX ? (P.P + 63) / 64 : (P.P + 255) / 256;
       for (int b = 0; b < mr[i].W; ++b) (P.bt.QX ? bf : fp).push_back(make_int4(i, b, chunks, 0));
       if (P.mover != SMCF_MOVER_GIBBS || mr[i].W == 0) continue;
       auto it = std::find_if(cls.begin(), cls.end(),
@@ -329,7 +332,7 @@ struct Engine {
         return launch_failed("hmc_stage");
     } else {
       if (launch_gather(ss, s) != hipSuccess) return launch_failed("gather");
-      if (launch_form_q(ss, any_f32, any_bf, s) != hipSuccess) return launch_failed("form_q");
+      if (launch_form_q(ss, any_full, any_f32, any_bf, s) != hipSuccess) return launch_failed("form_q");
       if (any_bf && launch_like_ref(ss, s) != hipSuccess) return launch_failed("like_ref");
       {
         ProfScope ps(ctx, "gibbs", s);

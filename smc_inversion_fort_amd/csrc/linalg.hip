@@ -261,7 +261,7 @@ __global__ void k_form_qb(StageSet ss) {
   const MemberDev& M = ss.mem[is.x];
   const BatchDev& bt = M.bt;
   int b = is.y, nd = M.nd, KB = M.kb;
-  if (!bt.Q || !bt.active[b]) return;
+  if (!bt.Q || bt.QX || !bt.active[b]) return;  // split-BF16 members: k_form_qx
   int nblk = (nd + KB - 1) / KB;
   size_t n = (size_t)nblk * nd * KB;
   double g = bt.gamma[b];
@@ -276,25 +276,90 @@ __global__ void k_form_qb(StageSet ss) {
   }
 }
 
+// one thread per (column i, slot w) of the sparse columns of LL: the slot's entry (row r) of column i
 __global__ void k_form_qb_sparse(StageSet ss) {
   int2 is = ss.inst[blockIdx.y];
   const MemberDev& M = ss.mem[is.x];
   const BatchDev& bt = M.bt;
   int b = is.y, nd = M.nd, KB = M.kb;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!bt.Q || !bt.active[b] || i >= nd) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / 25;
+  if (!bt.Q || bt.QX || !bt.active[b] || i >= nd) return;  // split-BF16 members: k_form_qx_sparse
+  int r = bt.llc_idx[(size_t)b * nd * 25 + t];
+  if (r < 0) return;  // padding (after the column's entries)
   int nblk = (nd + KB - 1) / KB;
   double g = bt.gamma[b], a2 = bt.scal[8 * (size_t)b];
   const double* H = bt.H + (size_t)b * nd * nd;
   double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
-  const int* ix = bt.llc_idx + ((size_t)b * nd + i) * 25;
-  const double* vx = bt.llc_val + ((size_t)b * nd + i) * 25;
+  Qb[((size_t)(i / KB) * nd + r) * KB + i % KB] = g * H[(size_t)i * nd + r] + bt.llc_val[(size_t)b * nd * 25 + t] / a2;
+}
+
+// Split-BF16 members: the stage's precision straight into the fragment order of the sweep kernel's products
+// (fragment f = ((blk nst + st) MT + m) 64 + lane of lane = r + 16g holds Q(32 st + 8g + e, blk KB + 16m + r),
+// e = 0..7, hi at QX[2f], lo at QX[2f + 1]), each value the same double as in the FP64 blocked layout rounded to
+// single and split.  k_form_qx writes gamma H; k_form_qx_sparse then rewrites the entries of the sparse columns of
+// LL (gamma H + LL / alpha^2), as k_form_qb and k_form_qb_sparse do for the FP64 layout.  Of the FP64 blocked layout
+// only the diagonal KB x KB blocks, which the sweep's in-tile corrections and the draw constants read, are written.
+SMCF_DEV void qx_store(bf16x8* __restrict__ QX, int KB, int nst, int i, int j, double v) {
   int blk = i / KB, k = i % KB;
-  for (int w = 0; w < 25; ++w) {
-    int r = ix[w];
-    if (r < 0) break;
-    Qb[((size_t)blk * nd + r) * KB + k] = g * H[(size_t)i * nd + r] + vx[w] / a2;
+  int lane = k % 16 + 16 * ((j % 32) / 8);
+  size_t f = (((size_t)blk * nst + j / 32) * (KB / 16) + k / 16) * 64 + lane;
+  __bf16 h, l;
+  split_bf16((float)v, h, l);
+  ((__bf16*)(QX + 2 * f))[j % 8] = h;
+  ((__bf16*)(QX + 2 * f + 1))[j % 8] = l;
+}
+
+__global__ void k_form_qx(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
+  if (!bt.QX || !bt.active[b]) return;
+  int MT = KB / 16, nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
+  size_t nfrag = (size_t)nblk * nst * MT * 64;
+  double g = bt.gamma[b];
+  const double* H = bt.H + (size_t)b * nd * nd;
+  double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
+  bf16x8* QX = (bf16x8*)bt.QX + (size_t)b * (bf_q_elems(nd, KB) / 8);
+  for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nfrag; f += (size_t)gridDim.x * blockDim.x) {
+    int lane = (int)(f % 64);
+    size_t rest = f / 64;
+    int m = (int)(rest % MT);
+    rest /= MT;
+    int st = (int)(rest % nst), blk = (int)(rest / nst);
+    int k = 16 * m + lane % 16, i = blk * KB + k, j0 = 32 * st + 8 * (lane / 16);
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int j = j0 + e;
+      double q = (i < nd && j < nd) ? g * H[(size_t)i * nd + j] : 0.0;
+      __bf16 h, l;
+      split_bf16((float)q, h, l);
+      hi[e] = h;
+      lo[e] = l;
+      if (i < nd && j < nd && j >= blk * KB && j < blk * KB + KB) Qb[((size_t)blk * nd + j) * KB + k] = q;
+    }
+    QX[2 * f] = hi;
+    QX[2 * f + 1] = lo;
   }
+}
+
+__global__ void k_form_qx_sparse(StageSet ss) {
+  int2 is = ss.inst[blockIdx.y];
+  const MemberDev& M = ss.mem[is.x];
+  const BatchDev& bt = M.bt;
+  int b = is.y, nd = M.nd, KB = M.kb;
+  int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / 25;
+  if (!bt.QX || !bt.active[b] || i >= nd) return;
+  int r = bt.llc_idx[(size_t)b * nd * 25 + t];
+  if (r < 0) return;  // padding (after the column's entries)
+  int nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
+  double g = bt.gamma[b], a2 = bt.scal[8 * (size_t)b];
+  const double* H = bt.H + (size_t)b * nd * nd;
+  double v = g * H[(size_t)i * nd + r] + bt.llc_val[(size_t)b * nd * 25 + t] / a2;
+  qx_store((bf16x8*)bt.QX + (size_t)b * (bf_q_elems(nd, KB) / 8), KB, nst, i, r, v);
+  int blk = i / KB;
+  if (r >= blk * KB && r < blk * KB + KB) bt.Q[(size_t)b * nblk * nd * KB + ((size_t)blk * nd + r) * KB + i % KB] = v;
 }
 
 // per-coordinate constants of the stage for the Gibbs draws: gamma b_i, 1 / Q_ii, sqrt(Q_ii), 1 / sqrt(Q_ii)
@@ -346,15 +411,22 @@ hipError_t launch_s32(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, cons
   return hipGetLastError();
 }
 
-hipError_t launch_form_q(const StageSet& ss, bool any_f32, bool any_bf, hipStream_t s) {
+hipError_t launch_form_q(const StageSet& ss, bool any_full, bool any_f32, bool any_bf, hipStream_t s) {
   int nd = ss.nd_max, KB = ss.kb_max;
   size_t n = (size_t)((nd + KB - 1) / KB) * nd * KB;
   int blocks = (int)std::min<size_t>((n + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_form_qb, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
-  hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd + 127) / 128, ss.ninst), dim3(128), 0, s, ss);
+  if (any_full) {
+    hipLaunchKernelGGL(k_form_qb, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
+    hipLaunchKernelGGL(k_form_qb_sparse, dim3((nd * 25 + 255) / 256, ss.ninst), dim3(256), 0, s, ss);
+  }
+  if (any_bf) {
+    size_t nfrag = bf_q_elems(nd, KB) / 16;
+    hipLaunchKernelGGL(k_form_qx, dim3((int)std::min<size_t>((nfrag + 255) / 256, 1024), ss.ninst), dim3(256), 0, s,
+                       ss);
+    hipLaunchKernelGGL(k_form_qx_sparse, dim3((nd * 25 + 255) / 256, ss.ninst), dim3(256), 0, s, ss);
+  }
   hipLaunchKernelGGL(k_form_qdiag, dim3((nd + 127) / 128, ss.ninst), dim3(128), 0, s, ss);
   if (any_f32) hipLaunchKernelGGL(k_q_to_f32, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
-  if (any_bf) return launch_q_bf(ss, s);
   return hipGetLastError();
 }
 

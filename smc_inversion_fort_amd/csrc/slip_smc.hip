@@ -1282,40 +1282,6 @@ __global__ void __launch_bounds__(256, kBfOcc) k_gibbs_bf(const int4* __restrict
 }
 
 
-// Split-BF16 copy of the blocked precision in fragment order: fragment f = ((blk nst + st) MT + m) 64 + lane
-// of lane = r + 16g holds Q(32 st + 8g + e, blk KB + 16m + r), e = 0..7, hi at QX[2f], lo at QX[2f + 1].
-__global__ void k_q_to_bf(StageSet ss) {
-  int2 is = ss.inst[blockIdx.y];
-  const MemberDev& M = ss.mem[is.x];
-  const BatchDev& bt = M.bt;
-  int b = is.y, nd = M.nd, KB = M.kb;
-  if (!bt.QX || !bt.active[b]) return;
-  int MT = KB / 16, nblk = (nd + KB - 1) / KB, nst = bf_rows(nd) / 32;
-  size_t nfrag = (size_t)nblk * nst * MT * 64;
-  const double* Qb = bt.Q + (size_t)b * nblk * nd * KB;
-  bf16x8* QX = (bf16x8*)bt.QX + (size_t)b * (bf_q_elems(nd, KB) / 8);
-  for (size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x; f < nfrag; f += (size_t)gridDim.x * blockDim.x) {
-    int lane = (int)(f % 64);
-    size_t rest = f / 64;
-    int m = (int)(rest % MT);
-    rest /= MT;
-    int st = (int)(rest % nst), blk = (int)(rest / nst);
-    int col = 16 * m + lane % 16, g = lane / 16;
-    bf16x8 hi, lo;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int j = 32 * st + 8 * g + e;
-      float x = j < nd ? (float)Qb[((size_t)blk * nd + j) * KB + col] : 0.f;
-      __bf16 a, c;
-      split_bf16(x, a, c);
-      hi[e] = a;
-      lo[e] = c;
-    }
-    QX[2 * f] = hi;
-    QX[2 * f + 1] = lo;
-  }
-}
-
 // split-BF16 copy of the current particles of the given slots: [j/8][p][hi 8 | lo 8], zero past nd
 __global__ void k_s_to_bf(int nd, int P, BatchDev bt, const int* __restrict__ sel) {
   int b = sel ? sel[blockIdx.y] : (int)blockIdx.y;
@@ -1343,13 +1309,6 @@ __global__ void k_s_to_bf(int nd, int P, BatchDev bt, const int* __restrict__ se
 
 hipError_t launch_sbf(const MeshDev& m, BatchDev& bt, int B, hipStream_t s, const int* sel) {
   hipLaunchKernelGGL(k_s_to_bf, dim3(128, B), dim3(256), 0, s, m.ndim, bt.P, bt, sel);
-  return hipGetLastError();
-}
-
-hipError_t launch_q_bf(const StageSet& ss, hipStream_t s) {
-  size_t nfrag = bf_q_elems(ss.nd_max, ss.kb_max) / 16;
-  int blocks = (int)std::min<size_t>((nfrag + 255) / 256, 1024);
-  hipLaunchKernelGGL(k_q_to_bf, dim3(blocks, ss.ninst), dim3(256), 0, s, ss);
   return hipGetLastError();
 }
 
@@ -1428,10 +1387,15 @@ __global__ void k_h_to_bf(int nd, const double* __restrict__ Hall, bf16x8* __res
   }
 }
 
-// one workgroup of 4 wavefronts per 256 particles of an instance, XCD-aware as k_like_mfma
+// One workgroup per 64 particles of an instance (XCD-aware items as k_like_mfma): its four wavefronts take the
+// 64-row blocks of H round-robin (wave w: blocks w, w + 4, ...) for the same 64 particles, so the particles' split
+// copy D is streamed from memory by one workgroup, its four wavefronts sharing it through the caches, instead of
+// once per block.  Each wavefront sums its blocks' rows in ascending order; the four partial sums of a particle
+// are added in wavefront order.
 __global__ void __launch_bounds__(256, 2)
     k_like_bf(const int4* __restrict__ items, const MemberDev* __restrict__ mem) {
   constexpr int MT = kLikeBfMT, KB = 16 * MT;
+  __shared__ double red[4][64];
   int4 it = items[blockIdx.x];
   if (it.x < 0) return;
   const MemberDev& M = mem[it.x];
@@ -1439,7 +1403,7 @@ __global__ void __launch_bounds__(256, 2)
   int b = it.y, chunk = it.z, nd = M.nd, P = M.P;
   if (!bt.active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  int p0 = chunk * 256 + wave * 64;
+  int p0 = chunk * 64;
   if (p0 >= P) return;
   int pb = bt.par[b];
   int ndp = bf_rows(nd), nst = ndp / 32, nblk = (nd + KB - 1) / KB;
@@ -1459,7 +1423,7 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
   for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)((m * 64 + lane) * 2);
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int blk = 0; blk < nblk; ++blk) {
+  for (int blk = wave; blk < nblk; blk += 4) {
     int i0 = blk * KB;
     flt4 acc[MT][4];
 #pragma unroll
@@ -1482,15 +1446,18 @@ __global__ void __launch_bounds__(256, 2)
         }
       }
   }
-  double lm = bt.lref[b];
-  double* Lout = (pb ? bt.like2 : bt.like) + (size_t)b * P;
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     double v = part[n];
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
-    int p = p0 + 16 * n + fi;
-    if (fk == 0 && p < P) Lout[p] = lm + v;
+    if (fk == 0) red[wave][16 * n + fi] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && p0 + (int)threadIdx.x < P) {
+    int t = threadIdx.x;
+    double* Lout = (pb ? bt.like2 : bt.like) + (size_t)b * P;
+    Lout[p0 + t] = bt.lref[b] + (((red[0][t] + red[1][t]) + red[2][t]) + red[3][t]);
   }
 }
 

@@ -92,7 +92,7 @@ struct BatchDev {
   float* S32b;
   // split-BF16 copies (SMCF_PRECISION_MIXED_BF16 only, else null), bf16 bits:
   //   SX, SXb  particles of S and S2 as [ndp/8][P][hi 8 | lo 8], ndp = nd rounded up to 32 (pad rows zero)
-  //   QX       the stage's blocked precision in matrix-core fragment order (k_q_to_bf)
+  //   QX       the stage's precision in matrix-core fragment order (k_form_qx)
   uint16_t* SX;
   uint16_t* SXb;
   uint16_t* QX;
@@ -308,10 +308,9 @@ hipError_t launch_moments_raw(int nd, int P, const double* S, const double* w, d
 hipError_t launch_gamma_resample(const StageSet& ss, hipStream_t s);
 hipError_t launch_moments(const StageSet& ss, hipStream_t s);
 hipError_t launch_gather(const StageSet& ss, hipStream_t s);
-// the stage's blocked Gibbs precision, its diagonal constants and (members of the mixed forms) its FP32 or
-// split-BF16 copies
-hipError_t launch_form_q(const StageSet& ss, bool any_f32, bool any_bf, hipStream_t s);
-hipError_t launch_q_bf(const StageSet& ss, hipStream_t s);
+// the stage's Gibbs precision: the FP64 blocked layout (any_full: FP64 and FP32 members), its FP32 copy (any_f32),
+// the split-BF16 fragment layout with the FP64 diagonal blocks (any_bf), and the diagonal constants
+hipError_t launch_form_q(const StageSet& ss, bool any_full, bool any_f32, bool any_bf, hipStream_t s);
 // Gibbs sweeps of the items of one launch of a kernel class (precision, block rows); items index the launch's
 // member table
 hipError_t launch_gibbs(const GibbsArgs& a, const ItemList& it, int precision, int kb, hipStream_t s);
