@@ -351,13 +351,15 @@ int smcf_group_destroy(smcf_group* g);
 
 /*
  * Serving (no counterpart in the reference, whose main.f90 runs one inversion per job).  A server admits
- * inversion requests - a problem description plus the outer-SMC settings of smcf_fault_smc_exec - against a
- * device-memory budget as they arrive (first fit in arrival order, sized by smcf_problem_bytes), and runs each
- * admitted request on its own context and host thread, so requests overlap on the device; the problem stays
- * resident for the whole outer SMC.  A request's result does not depend on what runs beside it (keyed RNG).
- * smcf_server_submit copies its inputs; smcf_server_wait blocks until the request is done, returns its results
- * as smcf_fault_smc_exec would (gamma_trace: max_stages + 1 entries) and times[3] = seconds from server start
- * to submission, admission and completion.  Each request id is waited for once.
+ * inversion requests - a problem description (Gibbs mover, any form) plus the outer-SMC settings of
+ * smcf_fault_smc_exec - against a device-memory budget as they arrive (first fit in arrival order, sized by
+ * smcf_problem_bytes) and batches them continuously: the server holds one context and one stage engine, an admitted
+ * request's problem joins it as a member (resident for the whole outer SMC, driven by a host thread of its own), and
+ * every stage kernel runs once over the instances of all running requests.  A request's result does not depend on
+ * what runs beside it (keyed RNG, per-instance arithmetic).  smcf_server_submit copies its inputs (requests with an
+ * HMC mover are refused, SMCF_EUNSUPPORTED); smcf_server_wait blocks until the request is done, returns its results
+ * as smcf_fault_smc_exec would (gamma_trace: max_stages + 1 entries) and times[3] = seconds from server start to
+ * submission, admission and completion.  Each request id is waited for once.
  */
 typedef struct smcf_server smcf_server;
 int smcf_server_create(int device, size_t hbm_budget_bytes, smcf_server** out);
@@ -366,8 +368,8 @@ int smcf_server_submit(smcf_server* s, const smcf_problem_desc* desc, int max_ba
 int smcf_server_wait(smcf_server* s, int64_t id, double* particles, double* likelihood, int* nstage,
                      double* evidence, double* gamma_trace, double* times);
 int smcf_server_status(smcf_server* s, int* queued, int* running, size_t* bytes_in_use);
-/* At most max_running requests run at once (1 .. 16; default GPU_MAX_HW_QUEUES / 2, each request using two HIP
- * streams); the others wait in arrival order.  Admission also counts a fixed 64 MiB per request beside its
+/* At most max_running requests run at once (1 .. 256, default 64; each holds a host thread); the others wait in
+ * arrival order.  Admission also counts a fixed 16 MiB per request (engine tables and admission staging) beside its
  * smcf_problem_bytes. */
 int smcf_server_set_max_running(smcf_server* s, int max_running);
 int smcf_server_destroy(smcf_server* s);

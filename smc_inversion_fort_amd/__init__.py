@@ -32,7 +32,7 @@ PRECISION_FP64 = 0   # every operation in FP64
 PRECISION_MIXED = 1  # Gibbs Q.S products in FP32 on the matrix cores, single-precision draws; means/weights FP64
 PRECISION_MIXED_BF16 = 2  # as PRECISION_MIXED with the Q.S products from split-BF16 operands (hi.hi + hi.lo + lo.hi)
 
-REQUEST_OVERHEAD_BYTES = 64 << 20  # serving admission: fixed device bytes counted per request beside its problem
+REQUEST_OVERHEAD_BYTES = 16 << 20  # serving admission: fixed device bytes counted per request beside its problem
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
@@ -494,8 +494,8 @@ def problem_bytes(desc, max_batch):
 
 
 class Server:
-    """Serving loop of libsmcf (smcf_server_*): inversion requests admitted by a device-memory budget as
-    they arrive, each run on its own context and host thread, overlapping on the device."""
+    """Serving loop of libsmcf (smcf_server_*): inversion requests (Gibbs mover) admitted by a device-memory
+    budget as they arrive and batched continuously: every stage kernel runs over all running requests."""
 
     def __init__(self, device=0, hbm_budget_bytes=200 << 30):
         h = C.c_void_p()
@@ -528,7 +528,7 @@ class Server:
                     t_submit=times[0], t_start=times[1], t_done=times[2])
 
     def set_max_running(self, n):
-        """At most n requests (1 .. 16) run at once."""
+        """At most n requests (1 .. 256) run at once."""
         _check(lib().smcf_server_set_max_running(self.h, int(n)))
 
     def status(self):

@@ -22,6 +22,7 @@
 #include <climits>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -107,6 +108,7 @@ struct MemberRun {
   Problem* P = nullptr;
   int W = 0;      // slots in use: 0 .. W-1
   int first = 0;  // its first instance in the stage set
+  bool fresh = false;  // added since the last build: its instances are not yet marked idle
   int admit_min = 1;
   std::deque<Geo> pending;
   std::vector<Geo> job;  // per slot (r == nullptr: free)
@@ -142,9 +144,14 @@ void xcd_items(const std::vector<int4>& insts, std::vector<int4>& out) {
 struct Engine {
   smcf_ctx* ctx = nullptr;
   std::vector<MemberRun> mr;
-  bool want_moments = false, hmc = false, any_full = false, any_f32 = false, any_bf = false, use_graph = false,
-       async = false;
-  std::vector<void*> dbufs;
+  bool want_moments = false, graph_on = false, async = false;
+  bool hmc = false, any_full = false, any_f32 = false, any_bf = false, use_graph = false;
+  // device tables, grown as members join (never shrunk while the engine lives)
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  Buf b_mem, b_inst, b_items, b_stat, b_adm[2], b_list[2];
   MemberDev* dmem = nullptr;
   double* dstat = nullptr;
   AdmitEntry* dadm[2] = {nullptr, nullptr};
@@ -162,56 +169,84 @@ struct Engine {
   Admission adm;
   std::vector<double> stat;
   int busy_total = 0;
-  Clock::time_point t_start;
-  // asynchronous submissions (group outer SMCs): requests from client threads, served by the thread in run()
+  // asynchronous submissions: batch calls of client threads (a group's outer SMCs, a server's requests), served by
+  // the thread in run(); with `persistent` (serving) members join and leave while it runs
   std::mutex mu;
+  std::mutex capture_mu;  // held while the stage graph is captured and by threads that call HIP beside the engine
   std::condition_variable cv_engine, cv_done;
-  std::deque<std::pair<int, Request*>> inbox;
+  std::deque<std::pair<Problem*, Request*>> inbox;
+  std::vector<Problem*> add_q, leave_q;
+  std::function<void(Problem*)> on_added, on_removed;
+  bool persistent = false, stopping = false;
   int clients = 0;
   int rc = SMCF_OK;
   std::string err;
 
   ~Engine() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     if (adm.ready) (void)hipEventDestroy(adm.ready);
-    for (void* p : dbufs) (void)hipFree(p);
+    for (Buf* b : {&b_mem, &b_inst, &b_items, &b_stat, &b_adm[0], &b_adm[1], &b_list[0], &b_list[1]})
+      if (b->p) (void)hipFree(b->p);
   }
 
-  template <class T>
-  int dalloc(T** p, size_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  int grow(Buf& b, size_t bytes) {
+    if (bytes <= b.bytes) return SMCF_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t cap = std::max(bytes, (size_t)4096);
+    hipError_t e = hipMalloc(&b.p, cap);
     if (e != hipSuccess) {
       set_error(std::string("stage engine: hipMalloc: ") + hipGetErrorString(e));
       return SMCF_ENOMEM;
     }
-    dbufs.push_back(q);
-    *p = (T*)q;
+    b.bytes = cap;
     return SMCF_OK;
   }
 
-  // members with their slots in use; tables, lists and (optionally) the stage graph
-  int init(smcf_ctx* c, const std::vector<Problem*>& ps, const std::vector<int>& W, bool moments, bool graph_on) {
+  int init(smcf_ctx* c, bool moments, bool graph) {
     ctx = c;
     want_moments = moments;
-    int n = (int)ps.size(), ninst = 0;
-    mr.resize(n);
+    graph_on = graph;
+    SMCF_HIP(hipEventCreateWithFlags(&adm.ready, hipEventDisableTiming));
+    return SMCF_OK;
+  }
+
+  // a member with W slots (0 .. W-1) in use, all free; its instances are held idle until admissions release them
+  void add(Problem* P, int W) {
+    MemberRun m;
+    m.P = P;
+    m.W = W;
+    m.admit_min = std::max(1, W / 16);
+    m.job.assign(W, Geo{});
+    for (int b = 0; b < W; ++b) m.freed.push_back(b);
+    m.fresh = true;
+    mr.push_back(std::move(m));
+    P->engine = this;
+  }
+
+  // tables, item lists and (optionally) the stage graph of the current members; members added since the last build
+  // have their instances marked idle
+  int build() {
+    drop_graph();
+    int n = (int)mr.size(), ninst = 0;
     std::vector<MemberDev> md(n);
     std::vector<int2> inst;
     int nd_max = 1, P_max = 1, kb_max = 16;
+    hmc = any_full = any_f32 = any_bf = false;
     for (int i = 0; i < n; ++i) {
-      Problem& P = *ps[i];
-      MemberRun& m = mr[i];
-      m.P = &P;
-      m.W = W[i];
-      m.first = ninst;
-      m.admit_min = std::max(1, m.W / 16);
-      m.job.assign(m.W, Geo{});
-      for (int b = 0; b < m.W; ++b) m.freed.push_back(b);
+      Problem& P = *mr[i].P;
+      mr[i].first = ninst;
       md[i] = member_of(P);
-      for (int b = 0; b < m.W; ++b) inst.push_back(make_int2(i, b));
-      ninst += m.W;
+      for (int b = 0; b < mr[i].W; ++b) inst.push_back(make_int2(i, b));
+      ninst += mr[i].W;
       nd_max = std::max(nd_max, P.m.ndim);
       P_max = std::max(P_max, P.P);
       kb_max = std::max(kb_max, P.bt.kb);
@@ -234,7 +269,7 @@ struct Engine {
     };
     std::vector<Cls> cls;
     for (int i = 0; i < n; ++i) {
-      Problem& P = *ps[i];
+      Problem& P = *mr[i].P;
       // 256 particles per workgroup (Gram-form likelihood), 64 (centred split-BF16 likelihood)
       int chunks = P.bt.QX ? (P.P + 63) / 64 : (P.P + 255) / 256;
       for (int b = 0; b < mr[i].W; ++b) (P.bt.QX ? bf : fp).push_back(make_int4(i, b, chunks, 0));
@@ -248,7 +283,7 @@ struct Engine {
       it->members.push_back(i);
     }
     std::vector<int4> all, tmp;
-    auto add = [&](const std::vector<int4>& v, size_t& off, int& cnt) {
+    auto add_list = [&](const std::vector<int4>& v, size_t& off, int& cnt) {
       xcd_items(v, tmp);
       off = all.size();
       cnt = (int)tmp.size();
@@ -256,8 +291,9 @@ struct Engine {
     };
     size_t off_fp = 0, off_bf = 0;
     int n_fp = 0, n_bf = 0;
-    add(fp, off_fp, n_fp);
-    add(bf, off_bf, n_bf);
+    add_list(fp, off_fp, n_fp);
+    add_list(bf, off_bf, n_bf);
+    glaunch.clear();
     std::vector<std::pair<size_t, int>> goff;
     for (auto& c : cls)
       for (size_t g0 = 0; g0 < c.members.size(); g0 += kGibbsArgMembers) {
@@ -267,38 +303,55 @@ struct Engine {
         std::vector<int4> v;
         for (size_t k = g0; k < std::min(c.members.size(), g0 + kGibbsArgMembers); ++k) {
           int i = c.members[k];
-          gl.args.m[k - g0] = gibbs_member_of(*ps[i]);
-          int chunks = (ps[i]->P + 255) / 256;
+          gl.args.m[k - g0] = gibbs_member_of(*mr[i].P);
+          int chunks = (mr[i].P->P + 255) / 256;
           for (int b = 0; b < mr[i].W; ++b) v.push_back(make_int4((int)(k - g0), b, chunks, 0));
         }
         goff.push_back({0, 0});
-        add(v, goff.back().first, goff.back().second);
+        add_list(v, goff.back().first, goff.back().second);
         glaunch.push_back(gl);
       }
-    int2* dinst = nullptr;
-    int4* ditems = nullptr;
     int r;
-    if ((r = dalloc(&dmem, n)) || (r = dalloc(&dinst, inst.size())) || (r = dalloc(&ditems, all.size())) ||
-        (r = dalloc(&dstat, 3 * inst.size())) || (r = dalloc(&dadm[0], inst.size())) ||
-        (r = dalloc(&dadm[1], inst.size())) || (r = dalloc(&dlist[0], inst.size())) ||
-        (r = dalloc(&dlist[1], inst.size())))
+    if ((r = grow(b_mem, sizeof(MemberDev) * std::max(n, 1))) || (r = grow(b_inst, sizeof(int2) * std::max(ninst, 1))) ||
+        (r = grow(b_items, sizeof(int4) * std::max<size_t>(all.size(), 1))) ||
+        (r = grow(b_stat, sizeof(double) * 3 * std::max(ninst, 1))))
       return r;
-    SMCF_HIP(hipMemcpy(dmem, md.data(), sizeof(MemberDev) * n, hipMemcpyHostToDevice));
-    if (!inst.empty()) SMCF_HIP(hipMemcpy(dinst, inst.data(), sizeof(int2) * inst.size(), hipMemcpyHostToDevice));
-    if (!all.empty()) SMCF_HIP(hipMemcpy(ditems, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+    for (int h = 0; h < 2; ++h)
+      if ((r = grow(b_adm[h], sizeof(AdmitEntry) * std::max(ninst, 1))) ||
+          (r = grow(b_list[h], sizeof(int2) * std::max(ninst, 1))))
+        return r;
+    dmem = (MemberDev*)b_mem.p;
+    dstat = (double*)b_stat.p;
+    for (int h = 0; h < 2; ++h) {
+      dadm[h] = (AdmitEntry*)b_adm[h].p;
+      dlist[h] = (int2*)b_list[h].p;
+    }
+    int2* dinst = (int2*)b_inst.p;
+    int4* ditems = (int4*)b_items.p;
+    hipStream_t s = ctx->stream;
+    if (n) SMCF_HIP(hipMemcpyAsync(dmem, md.data(), sizeof(MemberDev) * n, hipMemcpyHostToDevice, s));
+    if (ninst) SMCF_HIP(hipMemcpyAsync(dinst, inst.data(), sizeof(int2) * ninst, hipMemcpyHostToDevice, s));
+    if (!all.empty()) SMCF_HIP(hipMemcpyAsync(ditems, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, s));
     ss = StageSet{dmem, dinst, ninst, nd_max, P_max, kb_max};
     like_fp = ItemList{ditems + off_fp, n_fp};
     like_bf = ItemList{ditems + off_bf, n_bf};
     for (size_t k = 0; k < glaunch.size(); ++k) glaunch[k].it = ItemList{ditems + goff[k].first, goff[k].second};
     stat.assign(3 * (size_t)ninst, 0.0);
-    SMCF_HIP(hipEventCreateWithFlags(&adm.ready, hipEventDisableTiming));
-    hipStream_t s = ctx->stream;
-    if (ninst > 0) {
-      hipLaunchKernelGGL(k_idle, dim3((ninst + 255) / 256), dim3(256), 0, s, ss);
-      SMCF_HIP(hipGetLastError());
-    }
+    for (auto& m : mr)
+      if (m.fresh && m.W > 0) {
+        StageSet sub = ss;
+        sub.inst = dinst + m.first;
+        sub.ninst = m.W;
+        hipLaunchKernelGGL(k_idle, dim3((m.W + 255) / 256), dim3(256), 0, s, sub);
+        SMCF_HIP(hipGetLastError());
+        m.fresh = false;
+      }
+    // the host copies above are read before this call returns
+    SMCF_HIP(hipStreamSynchronize(s));
     use_graph = graph_on && !hmc && !ctx->profiling && ninst > 0;
     if (use_graph) {
+      // other threads of the process make no HIP call while the stage is captured (capture_lock)
+      std::lock_guard<std::mutex> cl(capture_mu);
       SMCF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       int rl = stage_launch(s);
       hipGraph_t g = nullptr;
@@ -319,44 +372,44 @@ struct Engine {
 
   // the launches of one stage over the whole set, ending with the status copy kernel
   int stage_launch(hipStream_t s) {
+    hipError_t e;
     {
       ProfScope ps(ctx, "gamma_resample", s);
-      if (launch_gamma_resample(ss, s) != hipSuccess) return launch_failed("gamma_resample");
+      if ((e = launch_gamma_resample(ss, s)) != hipSuccess) return launch_failed("gamma_resample", e);
     }
-    if ((want_moments || hmc) && launch_moments(ss, s) != hipSuccess) return launch_failed("moments");
+    if ((want_moments || hmc) && (e = launch_moments(ss, s)) != hipSuccess) return launch_failed("moments", e);
     if (hmc) {
       ProfScope ps(ctx, "hmc", s);
       Problem& P = *mr[0].P;
       const HmcCfg hcfg{P.mover, P.hmc_ntau, P.hmc_accept, P.precision == SMCF_PRECISION_MIXED_BF16 ? 1 : 0};
-      if (launch_hmc_stage(P.m, P.bt, P.hm, P.hh, mr[0].W, P.max_slip, hcfg, s) != hipSuccess)
-        return launch_failed("hmc_stage");
+      if ((e = launch_hmc_stage(P.m, P.bt, P.hm, P.hh, mr[0].W, P.max_slip, hcfg, s)) != hipSuccess)
+        return launch_failed("hmc_stage", e);
     } else {
-      if (launch_gather(ss, s) != hipSuccess) return launch_failed("gather");
-      if (launch_form_q(ss, any_full, any_f32, any_bf, s) != hipSuccess) return launch_failed("form_q");
-      if (any_bf && launch_like_ref(ss, s) != hipSuccess) return launch_failed("like_ref");
+      if ((e = launch_gather(ss, s)) != hipSuccess) return launch_failed("gather", e);
+      if ((e = launch_form_q(ss, any_full, any_f32, any_bf, s)) != hipSuccess) return launch_failed("form_q", e);
+      if (any_bf && (e = launch_like_ref(ss, s)) != hipSuccess) return launch_failed("like_ref", e);
       {
         ProfScope ps(ctx, "gibbs", s);
         for (auto& g : glaunch)
-          if (launch_gibbs(g.args, g.it, g.precision, g.kb, s) != hipSuccess) return launch_failed("gibbs");
+          if ((e = launch_gibbs(g.args, g.it, g.precision, g.kb, s)) != hipSuccess) return launch_failed("gibbs", e);
       }
       {
         // the Gibbs mover reads no prior values; they are formed once for the final particles
         ProfScope ps(ctx, "likelihood", s);
-        if (launch_like_bf(ss, like_bf, s) != hipSuccess || launch_like_stage(ss, like_fp, s) != hipSuccess)
-          return launch_failed("likelihood");
+        if ((e = launch_like_bf(ss, like_bf, s)) != hipSuccess || (e = launch_like_stage(ss, like_fp, s)) != hipSuccess)
+          return launch_failed("likelihood", e);
       }
     }
     hipLaunchKernelGGL(k_status, dim3((ss.ninst + 255) / 256), dim3(256), 0, s, ss, dstat);
-    if (hipGetLastError() != hipSuccess) return launch_failed("status");
+    if ((e = hipGetLastError()) != hipSuccess) return launch_failed("status", e);
     return SMCF_OK;
   }
 
-  int launch_failed(const char* what) {
-    set_error(std::string("stage engine: launch of ") + what + ": " + hipGetErrorString(hipGetLastError()));
+  int launch_failed(const char* what, hipError_t e) {
+    set_error(std::string("stage engine: launch of ") + what + ": " + hipGetErrorString(e));
     return SMCF_EHIP;
   }
 
-  double since_start() const { return std::chrono::duration<double>(Clock::now() - t_start).count(); }
   static double since(const Clock::time_point& t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
   // one stage, its status read back, finished instances harvested
@@ -389,17 +442,12 @@ struct Engine {
   }
 
   void finish(Request& r, int code, const std::string& msg) {
-    if (async) {
-      std::lock_guard<std::mutex> lk(mu);
-      r.rc = code;
-      r.err = msg;
-      r.finished = true;
-      cv_done.notify_all();
-    } else {
-      r.rc = code;
-      r.err = msg;
-      r.finished = true;
-    }
+    std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+    if (async) lk.lock();
+    r.rc = code;
+    r.err = msg;
+    r.finished = true;
+    if (async) cv_done.notify_all();
   }
 
   // set up the next geometries of every member with enough free slots, in one admission on the set-up stream
@@ -483,18 +531,24 @@ struct Engine {
     return true;
   }
 
+  int member_index(const Problem* P) const {
+    for (int i = 0; i < (int)mr.size(); ++i)
+      if (mr[i].P == P) return i;
+    return -1;
+  }
+
   void queue(int member, Request* r) {
     for (int i = 0; i < r->n; ++i) mr[member].pending.push_back(Geo{r, i});
     if (r->n == 0) r->finished = true;
   }
 
-  // serve until every request has finished (and, asynchronously, every client has left)
+  // serve until every request has finished (asynchronously: until every client has left and, for a persistent
+  // engine, the engine is stopped)
   int run() {
-    t_start = Clock::now();
     int r = loop();
     if (r) {
       // the error ends every request still queued or in flight
-      std::string msg = smcf_error_text();
+      std::string msg = smcf_last_error();
       (void)hipStreamSynchronize(ctx->setup_stream);
       (void)hipStreamSynchronize(ctx->stream);
       std::vector<Request*> open;
@@ -510,6 +564,9 @@ struct Engine {
       err = msg;
       for (auto& p : inbox) open.push_back(p.second);
       inbox.clear();
+      // problems that never joined have no client running
+      clients -= (int)add_q.size();
+      add_q.clear();
       for (Request* q : open) {
         q->rc = r;
         q->err = msg;
@@ -527,23 +584,68 @@ struct Engine {
     return SMCF_OK;
   }
 
+  // members that joined or left since the last stage (only between admissions: an admission's lists index members)
+  // (a joining member's client starts at once: its first batch call is routed at the next stage boundary, after
+  // the tables include it; a leaving member's problem is released only after the tables no longer do)
+  int change_members(std::vector<Problem*>& adds, std::vector<Problem*>& leaves) {
+    for (Problem* P : leaves) {
+      int i = member_index(P);
+      if (i >= 0) mr.erase(mr.begin() + i);
+      P->engine = nullptr;
+    }
+    for (Problem* P : adds) {
+      add(P, P->bt.cap);
+      if (on_added) on_added(P);
+    }
+    int r = build();
+    if (r) return r;
+    for (Problem* P : leaves)
+      if (on_removed) on_removed(P);
+    return SMCF_OK;
+  }
+
   int loop() {
     for (;;) {
+      std::vector<Problem*> adds, leaves;
       if (async) {
         std::unique_lock<std::mutex> lk(mu);
         while (!inbox.empty()) {
-          queue(inbox.front().first, inbox.front().second);
+          int i = member_index(inbox.front().first);
+          Request* q = inbox.front().second;
           inbox.pop_front();
+          if (i < 0) {
+            q->rc = SMCF_ESTATE;
+            q->err = "stage engine: batch call of a problem that is not a member";
+            q->finished = true;
+            cv_done.notify_all();
+          } else {
+            queue(i, q);
+          }
         }
-        if (busy_total == 0 && !adm.live && nothing_pending()) {
-          if (clients == 0) return SMCF_OK;
-          cv_engine.wait(lk, [&] { return !inbox.empty() || clients == 0; });
+        if (!adm.live) {
+          adds.swap(add_q);
+          // a member leaves once its client has finished and none of its geometries is left
+          for (auto it = leave_q.begin(); it != leave_q.end();) {
+            int i = member_index(*it);
+            if (i < 0 || (mr[i].busy == 0 && mr[i].pending.empty())) {
+              leaves.push_back(*it);
+              it = leave_q.erase(it);
+            } else {
+              ++it;
+            }
+          }
+        }
+        if (adds.empty() && leaves.empty() && busy_total == 0 && !adm.live && nothing_pending()) {
+          auto over = [&] { return clients == 0 && (!persistent || stopping) && add_q.empty() && leave_q.empty(); };
+          if (over()) return SMCF_OK;
+          cv_engine.wait(lk, [&] { return !inbox.empty() || !add_q.empty() || !leave_q.empty() || over(); });
           continue;
         }
       } else if (busy_total == 0 && !adm.live && nothing_pending()) {
         return SMCF_OK;
       }
       int r;
+      if ((!adds.empty() || !leaves.empty()) && (r = change_members(adds, leaves))) return r;
       if (busy_total > 0 && (r = step())) return r;
       if (adm.live) {
         hipError_t q = busy_total > 0 ? hipEventQuery(adm.ready) : hipEventSynchronize(adm.ready);
@@ -557,8 +659,6 @@ struct Engine {
       if (!adm.live && (r = start_admission())) return r;
     }
   }
-
-  static std::string smcf_error_text() { return std::string(smcf_last_error()); }
 };
 
 int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
@@ -569,8 +669,14 @@ int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, boo
   }
   if (P.engine) return engine_submit_wait(P, N, theta, keys, neglog, nstage);
   Engine E;
-  int r = E.init(P.ctx, {&P}, {std::min(N, P.bt.cap)}, want_moments, false);
+  int r = E.init(P.ctx, want_moments, false);
   if (r) return r;
+  E.add(&P, std::min(N, P.bt.cap));
+  struct Detach {  // the problem leaves this engine however the call ends
+    Problem* p;
+    ~Detach() { p->engine = nullptr; }
+  } detach{&P};
+  if ((r = E.build())) return r;
   P.t_done.assign(N, 0.0);
   P.t_admit.assign(N, 0.0);
   Request q;
@@ -604,11 +710,58 @@ int engine_submit_wait(Problem& P, int N, const double* theta, const uint64_t* k
     set_error(E->err);
     return E->rc;
   }
-  E->inbox.push_back({P.member, &q});
+  E->inbox.push_back({&P, &q});
   E->cv_engine.notify_all();
   E->cv_done.wait(lk, [&] { return q.finished; });
   if (q.rc) set_error(q.err);
   return q.rc;
+}
+
+// Serving: one persistent engine on the server's context; requests' problems join it as members when admitted and
+// leave it when their outer SMC has finished.
+Engine* serve_engine_create(smcf_ctx* ctx, std::function<void(Problem*)> added, std::function<void(Problem*)> removed) {
+  auto* E = new Engine();
+  if (E->init(ctx, false, true)) {
+    delete E;
+    return nullptr;
+  }
+  E->async = true;
+  E->persistent = true;
+  E->on_added = std::move(added);
+  E->on_removed = std::move(removed);
+  return E;
+}
+
+int serve_engine_run(Engine* E) { return E->run(); }
+
+void serve_engine_add(Engine* E, Problem* P) {
+  std::lock_guard<std::mutex> lk(E->mu);
+  E->add_q.push_back(P);
+  ++E->clients;
+  E->cv_engine.notify_all();
+}
+
+void serve_engine_leave(Engine* E, Problem* P) {
+  std::lock_guard<std::mutex> lk(E->mu);
+  E->leave_q.push_back(P);
+  --E->clients;
+  E->cv_engine.notify_all();
+}
+
+void serve_engine_stop(Engine* E) {
+  std::lock_guard<std::mutex> lk(E->mu);
+  E->stopping = true;
+  E->cv_engine.notify_all();
+}
+
+void serve_engine_destroy(Engine* E) { delete E; }
+
+std::mutex& serve_engine_capture_lock(Engine* E) { return E->capture_mu; }
+
+int serve_engine_error(Engine* E, std::string* msg) {
+  std::lock_guard<std::mutex> lk(E->mu);
+  if (msg) *msg = E->err;
+  return E->rc;
 }
 
 }  // namespace smcf
@@ -677,20 +830,24 @@ int smcf_group_calc_likelihood_batch(smcf_group* g, const int* B, const double* 
   }
   if (total == 0) return SMCF_OK;
   SMCF_HIP(hipSetDevice(g->ctx->device));
-  std::vector<Problem*> ps;
-  std::vector<int> W;
+  Engine E;
+  int r = E.init(g->ctx, false, g->graph);
+  if (r) return r;
   for (int i = 0; i < n; ++i) {
     Problem& P = g->members[i]->p;
-    ps.push_back(&P);
-    W.push_back(std::min(B[i], P.bt.cap));
+    E.add(&P, std::min(B[i], P.bt.cap));
   }
-  Engine E;
-  int r = E.init(g->ctx, ps, W, false, g->graph);
-  if (r) return r;
+  struct Detach {
+    smcf_group* g;
+    ~Detach() {
+      for (auto* m : g->members) m->p.engine = nullptr;
+    }
+  } detach{g};
+  if ((r = E.build())) return r;
   std::vector<Request> q(n);
-  auto t0 = std::chrono::steady_clock::now();
+  auto t0 = Clock::now();
   for (int i = 0; i < n; ++i) {
-    Problem& P = *ps[i];
+    Problem& P = g->members[i]->p;
     P.t_done.assign(B[i], 0.0);
     P.t_admit.assign(B[i], 0.0);
     q[i].n = q[i].remaining = B[i];
@@ -714,21 +871,19 @@ int smcf_group_fault_smc_exec(smcf_group* g, const double* const* range, const i
   for (int i = 0; i < n; ++i)
     if (!range[i] || N[i] < 1 || !particles[i] || !likelihood[i]) return SMCF_EINVAL;
   SMCF_HIP(hipSetDevice(g->ctx->device));
-  std::vector<Problem*> ps;
-  std::vector<int> W;
-  for (int i = 0; i < n; ++i) {
-    ps.push_back(&g->members[i]->p);
-    W.push_back(ps.back()->bt.cap);
-  }
   Engine E;
-  E.async = true;
-  int r = E.init(g->ctx, ps, W, false, g->graph);
+  int r = E.init(g->ctx, false, g->graph);
   if (r) return r;
+  E.async = true;
+  for (auto* m : g->members) E.add(&m->p, m->p.bt.cap);
+  struct Detach {
+    smcf_group* g;
+    ~Detach() {
+      for (auto* m : g->members) m->p.engine = nullptr;
+    }
+  } detach{g};
+  if ((r = E.build())) return r;
   E.clients = n;
-  for (int i = 0; i < n; ++i) {
-    ps[i]->engine = &E;
-    ps[i]->member = i;
-  }
   std::vector<int> rcs(n, SMCF_OK);
   std::vector<std::string> errs(n);
   std::vector<std::thread> th;
@@ -747,10 +902,6 @@ int smcf_group_fault_smc_exec(smcf_group* g, const double* const* range, const i
     });
   r = E.run();
   for (auto& t : th) t.join();
-  for (int i = 0; i < n; ++i) {
-    ps[i]->engine = nullptr;
-    ps[i]->member = -1;
-  }
   if (r) return r;
   for (int i = 0; i < n; ++i)
     if (rcs[i]) {

@@ -1,18 +1,19 @@
 // Serving loop: inversion requests (a problem description plus the outer-SMC settings) are admitted against
-// a device-memory budget as they arrive, and run concurrently on one GPU.  No counterpart in the reference,
+// a device-memory budget as they arrive and batched continuously on one GPU.  No counterpart in the reference,
 // whose main.f90 runs one inversion per MPI job.
 //
-// Each admitted request gets its own context (its own HIP streams), its problem uploaded once and kept
-// resident for the whole outer SMC, and a host thread that drives smcf_fault_smc_exec.  Requests that arrive
-// while others run are admitted as soon as the memory the finished ones held is free again (first fit in
-// arrival order), and their kernels overlap on the device with those of the requests already running.
-// Every random number is keyed by the request's seed, so a request's result does not depend on what else
-// runs beside it.
+// The server owns one context and one persistent stage engine.  An admitted request's problem is uploaded once
+// and joins the engine as a member for the whole outer SMC; a host thread per request drives smcf_fault_smc_exec,
+// whose waves are queued to the engine.  Every stage kernel therefore runs once over the instances of all running
+// requests (continuous batching: a request that arrives while others are on the GPU joins the next stage launch
+// once admitted, and its geometries fill its slots as they free up).  When its outer SMC has finished the request
+// leaves the engine, its problem is freed and the next queued requests are admitted in arrival order (first fit
+// under the budget).  Every random number is keyed by the request's seed and an instance's arithmetic does not
+// depend on its neighbours, so a request's result does not depend on what else runs beside it.
 //
-// Admission needs both the device bytes (smcf_problem_bytes plus a fixed per-request overhead for the context,
-// its streams and the runtime's per-stream state) and a free running place: at most max_running requests run at
-// once.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default, at most 32 accepted
-// here) and each request uses two streams, so the default is GPU_MAX_HW_QUEUES / 2, never more than 16.
+// Admission needs the device bytes (smcf_problem_bytes plus a fixed per-request allowance for the engine's tables
+// and staging) and a free running place: at most max_running requests are members at once (each holds a host
+// thread).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,17 +36,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr size_t kRequestOverheadBytes = (size_t)64 << 20;  // context, two streams, runtime state per request
-constexpr int kMaxRunningCap = 16;
-
-int default_max_running() {
-  int q = 4;  // HIP's default number of hardware queues per process
-  if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) {
-    int v = std::atoi(e);
-    if (v > 0) q = std::min(v, 32);
-  }
-  return std::max(1, std::min(kMaxRunningCap, q / 2));
-}
+constexpr size_t kRequestOverheadBytes = (size_t)16 << 20;  // engine tables and admission staging per request
+constexpr int kMaxRunningCap = 256;
+constexpr int kDefaultMaxRunning = 64;
 
 // A request owns copies of everything it was submitted with (the caller may free its arrays at once).
 struct Request {
@@ -69,6 +62,9 @@ struct Request {
   int rc = SMCF_OK;
   std::string err;
   bool done = false;
+  // while admitted
+  smcf_problem* pb = nullptr;
+  std::thread worker;
 };
 
 }  // namespace
@@ -81,35 +77,57 @@ struct smcf_server {
   std::condition_variable cv;
   std::deque<std::shared_ptr<Request>> queue;  // submitted, not yet admitted (arrival order)
   std::map<int64_t, std::shared_ptr<Request>> all;
+  std::map<smcf::Problem*, std::shared_ptr<Request>> live;  // admitted, by problem
   size_t in_use = 0;
   int running = 0;
-  int max_running = 1;
+  int max_running = kDefaultMaxRunning;
   int64_t next_id = 1;
   bool stopping = false;
-  std::thread admitter;
+  bool engine_failed = false;
+  smcf_ctx* ctx = nullptr;
+  smcf::Engine* eng = nullptr;
+  std::thread admitter, engine_thread;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-  void run_request(std::shared_ptr<Request> r) {
-    smcf_ctx* ctx = nullptr;
-    smcf_problem* pb = nullptr;
-    int rc = smcf_ctx_create(device, &ctx);
-    if (rc == SMCF_OK) rc = smcf_problem_create(ctx, &r->desc, r->max_batch, &pb);
-    if (rc == SMCF_OK) {
+  // engine thread: the request's problem is a member now; its outer SMC starts
+  void added(smcf::Problem* P) {
+    std::shared_ptr<Request> r;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      r = live[P];
+    }
+    r->worker = std::thread([this, r] {
+      (void)hipSetDevice(device);
       int nf = (int)r->range.size() / 2;
       r->particles.assign((size_t)nf * r->N, 0.0);
       r->likelihood.assign(r->N, 0.0);
       r->gamma_trace.assign(r->max_stages + 1, 0.0);
-      rc = smcf_fault_smc_exec(pb, r->range.data(), r->N, r->seed, r->max_stages, r->max_batch,
-                               r->particles.data(), r->likelihood.data(), &r->nstage, &r->evidence,
-                               r->gamma_trace.data());
+      int rc = smcf_fault_smc_exec(r->pb, r->range.data(), r->N, r->seed, r->max_stages, r->max_batch,
+                                   r->particles.data(), r->likelihood.data(), &r->nstage, &r->evidence,
+                                   r->gamma_trace.data());
+      r->rc = rc;
+      if (rc) r->err = smcf_last_error();
+      smcf::serve_engine_leave(eng, &r->pb->p);
+    });
+  }
+
+  // engine thread: the request's problem has left the engine; free it and admit the next ones
+  void removed(smcf::Problem* P) {
+    std::shared_ptr<Request> r;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      r = live[P];
+      live.erase(P);
     }
-    std::string err = rc == SMCF_OK ? std::string() : std::string(smcf_last_error());
-    if (pb) smcf_problem_destroy(pb);
-    if (ctx) smcf_ctx_destroy(ctx);
+    finish(r);
+  }
+
+  void finish(std::shared_ptr<Request> r) {
+    if (r->worker.joinable()) r->worker.join();
+    if (r->pb) smcf_problem_destroy(r->pb);
+    r->pb = nullptr;
     std::lock_guard<std::mutex> lk(mu);
-    r->rc = rc;
-    r->err = err;
     r->t_done = now();
     r->done = true;
     in_use -= r->bytes;
@@ -117,23 +135,77 @@ struct smcf_server {
     cv.notify_all();
   }
 
+  void engine_main() {
+    (void)hipSetDevice(device);
+    int rc = smcf::serve_engine_run(eng);
+    if (rc == SMCF_OK) return;
+    // the engine failed: its members' outer SMCs have returned with its error; end them and refuse the rest
+    std::string msg;
+    smcf::serve_engine_error(eng, &msg);
+    std::vector<std::shared_ptr<Request>> left;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      engine_failed = true;
+      for (auto& kv : live) left.push_back(kv.second);
+      live.clear();
+      cv.notify_all();
+    }
+    for (auto& r : left) {
+      if (r->rc == SMCF_OK) {
+        r->rc = rc;
+        r->err = msg;
+      }
+      finish(r);
+    }
+  }
+
   // admits queued requests (first fit in arrival order) whenever memory is released or a request arrives
   void admit_loop() {
+    (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
       bool admitted = false;
-      for (auto it = queue.begin(); it != queue.end();) {
+      for (auto it = queue.begin(); it != queue.end(); ++it) {
         auto r = *it;
+        if (engine_failed) {
+          queue.erase(it);
+          r->rc = SMCF_ESTATE;
+          r->err = "the server's stage engine failed";
+          r->t_done = now();
+          r->done = true;
+          cv.notify_all();
+          admitted = true;
+          break;
+        }
         if (running < max_running && in_use + r->bytes <= budget) {
           in_use += r->bytes;
           ++running;
           r->t_start = now();
-          it = queue.erase(it);
-          // detached: the worker's last act is to decrement `running` under the lock, and destroy waits for 0
-          std::thread(&smcf_server::run_request, this, r).detach();
+          queue.erase(it);
+          lk.unlock();
+          smcf_problem* pb = nullptr;
+          int rc;
+          {
+            std::lock_guard<std::mutex> cl(smcf::serve_engine_capture_lock(eng));
+            rc = smcf_problem_create(ctx, &r->desc, r->max_batch, &pb);
+          }
+          std::string err = rc ? std::string(smcf_last_error()) : std::string();
+          lk.lock();
+          if (rc) {
+            r->rc = rc;
+            r->err = err;
+            r->t_done = now();
+            r->done = true;
+            in_use -= r->bytes;
+            --running;
+            cv.notify_all();
+          } else {
+            r->pb = pb;
+            live[&pb->p] = r;
+            smcf::serve_engine_add(eng, &pb->p);
+          }
           admitted = true;
-        } else {
-          ++it;
+          break;  // the queue may have changed while the problem was created: scan again
         }
       }
       if (stopping && queue.empty()) return;
@@ -146,16 +218,22 @@ extern "C" {
 
 int smcf_server_create(int device, size_t hbm_budget_bytes, smcf_server** out) {
   if (!out || hbm_budget_bytes == 0) return SMCF_EINVAL;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
-    smcf::set_error("no such device");
-    return SMCF_EINVAL;
-  }
+  smcf_ctx* ctx = nullptr;
+  int rc = smcf_ctx_create(device, &ctx);
+  if (rc) return rc;
   auto* s = new smcf_server();
   s->device = device;
   s->budget = hbm_budget_bytes;
-  s->max_running = default_max_running();
   s->t0 = Clock::now();
+  s->ctx = ctx;
+  s->eng = smcf::serve_engine_create(ctx, [s](smcf::Problem* P) { s->added(P); },
+                                     [s](smcf::Problem* P) { s->removed(P); });
+  if (!s->eng) {
+    smcf_ctx_destroy(ctx);
+    delete s;
+    return SMCF_EHIP;
+  }
+  s->engine_thread = std::thread(&smcf_server::engine_main, s);
   s->admitter = std::thread(&smcf_server::admit_loop, s);
   *out = s;
   return SMCF_OK;
@@ -167,6 +245,10 @@ int smcf_server_submit(smcf_server* s, const smcf_problem_desc* d, int max_batch
   auto r = std::make_shared<Request>();
   int np = d->nplane;
   if (np < 1 || !d->nxi_ls || !d->neta_ls || d->nobs < 1) return SMCF_EINVAL;
+  if (d->mover != SMCF_MOVER_GIBBS) {
+    smcf::set_error("the server batches Gibbs-mover requests (HMC inversions run through smcf_fault_smc_exec)");
+    return SMCF_EUNSUPPORTED;
+  }
   r->nxi.assign(d->nxi_ls, d->nxi_ls + np);
   r->neta.assign(d->neta_ls, d->neta_ls + np);
   if (np > 1 && d->xbend) r->xbend.assign(d->xbend, d->xbend + (np - 1));
@@ -252,7 +334,7 @@ int smcf_server_wait(smcf_server* s, int64_t id, double* particles, double* like
 
 int smcf_server_set_max_running(smcf_server* s, int max_running) {
   if (!s || max_running < 1 || max_running > kMaxRunningCap) {
-    smcf::set_error("max_running must be 1 .. 16");
+    smcf::set_error("max_running must be 1 .. " + std::to_string(kMaxRunningCap));
     return SMCF_EINVAL;
   }
   std::lock_guard<std::mutex> lk(s->mu);
@@ -282,6 +364,10 @@ int smcf_server_destroy(smcf_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv.wait(lk, [&] { return s->running == 0; });
   }
+  smcf::serve_engine_stop(s->eng);
+  s->engine_thread.join();
+  smcf::serve_engine_destroy(s->eng);
+  smcf_ctx_destroy(s->ctx);
   delete s;
   return SMCF_OK;
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -228,8 +230,8 @@ struct Problem {
   HmcDev hm;
   HmcHost hh;
   MemberDev* mdev = nullptr;  // this problem as a one-entry member table (device)
-  Engine* engine = nullptr;   // set while the problem is a member of a running group: batch calls go to its engine
-  int member = -1;            // index in that engine
+  Engine* engine = nullptr;   // set while the problem is a member of a running stage engine (group, server): its
+                              // batch calls are queued to that engine
   double max_slip;
   int P, mover, sweeps, precision;
   int hmc_ntau;       // leapfrog steps (50 for SMCF_MOVER_HMC, smc_slip.f90:669)
@@ -441,8 +443,23 @@ int setup_post(Problem& P, int n, const int* sel, hipStream_t s);
 // and variance of every stage's particles.
 int run_stream(Problem& P, int N, const double* theta, const uint64_t* keys, bool want_moments, double* neglog,
                int* nstage);
-// batch call of a problem that is a member of a running group: queued to the group's engine, returns when the
-// engine has finished all N geometries
+// batch call of a problem that is a member of a running engine: queued to it, returns when the engine has finished
+// all N geometries
 int engine_submit_wait(Problem& P, int N, const double* theta, const uint64_t* keys, double* neglog, int* nstage);
+
+// Serving: one persistent engine on the server's context whose members (the admitted requests' problems) join
+// (serve_engine_add, from any thread; `added` runs on the engine's thread once the problem is a member, so its
+// client may start) and leave once their client has finished (serve_engine_leave; `removed` runs on the engine's
+// thread after the problem left).  serve_engine_run serves until serve_engine_stop and the last member has left.
+Engine* serve_engine_create(smcf_ctx* ctx, std::function<void(Problem*)> added, std::function<void(Problem*)> removed);
+int serve_engine_run(Engine* E);
+void serve_engine_add(Engine* E, Problem* P);
+void serve_engine_leave(Engine* E, Problem* P);
+void serve_engine_stop(Engine* E);
+void serve_engine_destroy(Engine* E);
+int serve_engine_error(Engine* E, std::string* msg);
+// held by a thread that calls HIP beside the engine's thread (creating a problem): a stream capture of the stage
+// graph must not see such calls
+std::mutex& serve_engine_capture_lock(Engine* E);
 
 }  // namespace smcf
