@@ -270,8 +270,8 @@ struct Engine {
     std::vector<Cls> cls;
     for (int i = 0; i < n; ++i) {
       Problem& P = *mr[i].P;
-      // 256 particles per workgroup (Gram-form likelihood), 64 (centred split-BF16 likelihood)
-      int chunks = P.bt.QX ? (P.P + 63) / 64 : (P.P + 255) / 256;
+      // 64 particles per likelihood workgroup (Gram form and centred split-BF16 form)
+      int chunks = (P.P + 63) / 64;
       for (int b = 0; b < mr[i].W; ++b) (P.bt.QX ? bf : fp).push_back(make_int4(i, b, chunks, 0));
       if (P.mover != SMCF_MOVER_GIBBS || mr[i].W == 0) continue;
       auto it = std::find_if(cls.begin(), cls.end(),

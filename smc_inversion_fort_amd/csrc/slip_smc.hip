@@ -429,9 +429,11 @@ __device__ __noinline__ double gibbs_draw_fast(double mu, double s, double is, d
 #endif
 constexpr int kLikeMT = SMCF_LIKE_MT;  // 16-row tiles per block of H (64 rows: S is streamed nd / 64 times)
 
-__device__ __forceinline__ void wave_like(int nd, int P, const double* __restrict__ H, const double* __restrict__ S,
-                                          const double* __restrict__ bv, double dsd, double c, int p0, int lane,
-                                          double* __restrict__ Lout) {
+// s^T (H s - 2 b) over the rows of blocks rb0, rb0 + rbstep, ... for the wave's 64 particles p0 .. p0 + 63 (each
+// particle's rows summed in ascending order; part[n] holds this lane's share of particle p0 + 16n + lane % 16)
+__device__ __forceinline__ void wave_like_part(int nd, int P, const double* __restrict__ H,
+                                               const double* __restrict__ S, const double* __restrict__ bv, int p0,
+                                               int lane, int rb0, int rbstep, double (&part)[4]) {
   constexpr int MT = kLikeMT, KB = 16 * MT;
   int fi = lane % 16, fk = lane / 16;
   int pcol[4];
@@ -441,8 +443,7 @@ __device__ __forceinline__ void wave_like(int nd, int P, const double* __restric
 #pragma unroll
   for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)(fk * P + pcol[n]);
   int nd4 = nd & ~3, nblk = (nd + KB - 1) / KB;
-  double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int rb = 0; rb < nblk; ++rb) {
+  for (int rb = rb0; rb < nblk; rb += rbstep) {
     int i0 = rb * KB;
     dbl4 acc[MT][4];
 #pragma unroll
@@ -481,11 +482,20 @@ __device__ __forceinline__ void wave_like(int nd, int P, const double* __restric
   }
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    double v = part[n];
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    int p = p0 + 16 * n + fi;
-    if (fk == 0 && p < P) Lout[p] = (v + dsd) / 2.0 + c;
+    part[n] += __shfl_xor(part[n], 16);
+    part[n] += __shfl_xor(part[n], 32);
+  }
+}
+
+__device__ __forceinline__ void wave_like(int nd, int P, const double* __restrict__ H, const double* __restrict__ S,
+                                          const double* __restrict__ bv, double dsd, double c, int p0, int lane,
+                                          double* __restrict__ Lout) {
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  wave_like_part(nd, P, H, S, bv, p0, lane, 0, 1, part);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    int p = p0 + 16 * n + lane % 16;
+    if (lane < 16 && p < P) Lout[p] = (part[n] + dsd) / 2.0 + c;
   }
 }
 
@@ -512,9 +522,13 @@ __global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
             (pb ? L1 : L0) + (size_t)b * P);
 }
 
-// the same per item (instance, 256-particle chunk) of a stage set (moved particles of the FP64 and FP32 forms)
+// The stage likelihood of the moved particles of the FP64 and FP32 forms: one workgroup per 64 particles of an
+// instance (XCD-aware items), its four wavefronts taking the 64-row blocks of H round-robin for the same particles, so
+// the particles are streamed from memory by one workgroup, its wavefronts sharing them through the caches, instead of
+// once per block; a particle's four partial sums are added in wavefront order.
 __global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
     k_like_items(const int4* __restrict__ items, const MemberDev* __restrict__ mem) {
+  __shared__ double red[4][64];
   int4 it = items[blockIdx.x];
   if (it.x < 0) return;
   const MemberDev& M = mem[it.x];
@@ -522,12 +536,22 @@ __global__ void __launch_bounds__(256, SMCF_LIKE_OCC)
   int b = it.y, nd = M.nd, P = M.P;
   if (!bt.active[b]) return;
   int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  int p0 = it.z * 256 + wave * 64;
+  int p0 = it.z * 64;
   if (p0 >= P) return;
   int pb = bt.par[b];
-  const double* sc = bt.scal + 8 * (size_t)b;
-  wave_like(nd, P, bt.H + (size_t)b * nd * nd, (pb ? bt.S2 : bt.S) + (size_t)b * nd * P, bt.bvec + (size_t)b * nd,
-            bt.dsd[b], sc[3] + sc[4], p0, lane, (pb ? bt.like2 : bt.like) + (size_t)b * P);
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  wave_like_part(nd, P, bt.H + (size_t)b * nd * nd, (pb ? bt.S2 : bt.S) + (size_t)b * nd * P,
+                 bt.bvec + (size_t)b * nd, p0, lane, wave, 4, part);
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+    if (lane < 16) red[wave][16 * n + lane] = part[n];
+  __syncthreads();
+  if (threadIdx.x < 64 && p0 + (int)threadIdx.x < P) {
+    int t = threadIdx.x;
+    const double* sc = bt.scal + 8 * (size_t)b;
+    double v = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    ((pb ? bt.like2 : bt.like) + (size_t)b * P)[p0 + t] = (v + bt.dsd[b]) / 2.0 + (sc[3] + sc[4]);
+  }
 }
 
 hipError_t launch_like_raw(int nd, int P, const double* H, const double* bv, const double* dsd, const double* scal,
@@ -1388,13 +1412,16 @@ __global__ void k_h_to_bf(int nd, const double* __restrict__ Hall, bf16x8* __res
 }
 
 // One workgroup per 64 particles of an instance (XCD-aware items as k_like_mfma): its four wavefronts take the
-// 64-row blocks of H round-robin (wave w: blocks w, w + 4, ...) for the same 64 particles, so the particles' split
-// copy D is streamed from memory by one workgroup, its four wavefronts sharing it through the caches, instead of
-// once per block.  Each wavefront sums its blocks' rows in ascending order; the four partial sums of a particle
-// are added in wavefront order.
+// 64-row blocks of H round-robin (wave w: blocks w, w + 4, ...) for the same 64 particles and step through the
+// contraction together, so each 32-row step of the particles' split copy D is loaded from memory once per
+// workgroup into LDS (double-buffered, one barrier per step) and read from there by the four wavefronts; H's
+// fragments come from memory per wavefront.  The products accumulate in the same order as a wavefront alone would
+// (per step hi.hi, hi.lo, lo.hi), each wavefront sums its blocks' rows in ascending order, and the four partial
+// sums of a particle are added in wavefront order.
 __global__ void __launch_bounds__(256, 2)
     k_like_bf(const int4* __restrict__ items, const MemberDev* __restrict__ mem) {
   constexpr int MT = kLikeBfMT, KB = 16 * MT;
+  __shared__ bf16x8 Dt[2][4][64][2];  // [buffer][row group of the step][particle][hi, lo]
   __shared__ double red[4][64];
   int4 it = items[blockIdx.x];
   if (it.x < 0) return;
@@ -1416,35 +1443,100 @@ __global__ void __launch_bounds__(256, 2)
   int pcol[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) pcol[n] = min(p0 + 16 * n + fi, P - 1);
-  uint32_t bo[4];
+  // this thread's share of a step's D tile: row group wave, particle p0 + lane (clamped; unused past P)
+  const bf16x8* dsrc = DX + ((size_t)wave * P + min(p0 + lane, P - 1)) * 2;
+  const size_t dstep = (size_t)4 * P * 2;
+  uint32_t aoff[MT];
 #pragma unroll
-  for (int n = 0; n < 4; ++n) bo[n] = (uint32_t)((fk * P + pcol[n]) * 2);
-  uint32_t ao[MT];
-#pragma unroll
-  for (int m = 0; m < MT; ++m) ao[m] = (uint32_t)((m * 64 + lane) * 2);
+  for (int m = 0; m < MT; ++m) aoff[m] = (uint32_t)((m * 64 + lane) * 2) * 16u;
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int blk = wave; blk < nblk; blk += 4) {
-    int i0 = blk * KB;
+  const int npass = (nblk + 3) / 4;
+  for (int q = 0; q < npass; ++q) {
+    const int blk = 4 * q + wave;
+    const bool mine = blk < nblk;  // wave-uniform; a wave without a block still loads and waits with the others
+    const char* qa = (const char*)(HX + (size_t)min(blk, nblk - 1) * nst * MT * 128);
     flt4 acc[MT][4];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int n = 0; n < 4; ++n) acc[m][n] = flt4{0.f, 0.f, 0.f, 0.f};
-    mfma_panel_bf3<MT, 2>(HX + (size_t)blk * nst * MT * 128, ao, MT * 128, DX, bo, 8 * (uint32_t)P, nst, acc);
+    bf16x8 dh = dsrc[0], dl = dsrc[1];
+    // H's fragments of the step, requested one step ahead (a wave without a block reads its clamped block's)
+    bf16x8 ah[MT], al[MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+    for (int m = 0; m < MT; ++m) {
+      ah[m] = *(const bf16x8*)(qa + aoff[m]);
+      al[m] = *(const bf16x8*)(qa + aoff[m] + 16u);
+    }
+    __syncthreads();  // the previous pass has finished reading both buffers
+    Dt[0][wave][lane][0] = dh;
+    Dt[0][wave][lane][1] = dl;
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+      const int cur = st & 1;
+      const bool more = st + 1 < nst;
+      bf16x8 nah[MT], nal[MT];
+      if (more) {
+        dh = dsrc[(size_t)(st + 1) * dstep];
+        dl = dsrc[(size_t)(st + 1) * dstep + 1];
+        const char* qn = qa + (size_t)(st + 1) * MT * 128 * 16;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        int row = i0 + 16 * m + 4 * fk + t;
-        if (row < nd) {
-          double mr = mref[row], gr = g[row];
-#pragma unroll
-          for (int n = 0; n < 4; ++n) {
-            double d = S[(size_t)row * P + pcol[n]] - mr;
-            part[n] = fma(d, fma(0.5, (double)acc[m][n][t], gr), part[n]);
-          }
+        for (int m = 0; m < MT; ++m) {
+          nah[m] = *(const bf16x8*)(qn + aoff[m]);
+          nal[m] = *(const bf16x8*)(qn + aoff[m] + 16u);
         }
       }
+      if (mine) {
+        bf16x8 bh[4], bl[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          bh[n] = Dt[cur][fk][16 * n + fi][0];
+          bl[n] = Dt[cur][fk][16 * n + fi][1];
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
+      }
+      if (more) {
+        Dt[cur ^ 1][wave][lane][0] = dh;
+        Dt[cur ^ 1][wave][lane][1] = dl;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          ah[m] = nah[m];
+          al[m] = nal[m];
+        }
+      }
+      __syncthreads();
+    }
+    if (mine) {
+      const int i0 = blk * KB;
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          int row = i0 + 16 * m + 4 * fk + t;
+          if (row < nd) {
+            double mr = mref[row], gr = g[row];
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+              double d = S[(size_t)row * P + pcol[n]] - mr;
+              part[n] = fma(d, fma(0.5, (double)acc[m][n][t], gr), part[n]);
+            }
+          }
+        }
+    }
   }
 #pragma unroll
   for (int n = 0; n < 4; ++n) {

@@ -133,3 +133,38 @@ def test_group_outer_smc_matches_solo():
     for p in probs:
         p.close()
     ctx.close()
+
+
+def test_group_members_vs_oracle():
+    """Three FP64 members with different meshes, station sets and faults in one group call: every geometry's -log Z
+    and stage count equal the CPU oracle's inner SMC of that member (1e-8 relative)."""
+    import oracle as O
+    insts = syn.batch_instances(3, seed=31, patches=(4, 7), nsta=(20, 50), P=(96, 160))
+    ctx = S.Context(0)
+    probs, orcs, ths, keys = [], [], [], []
+    for k, it in enumerate(insts):
+        xmin, xmax, zmin, _ = it["frame"]
+        f = O.Fault([it["nxi"]], [it["neta"]], xmin, xmax, zmin)
+
+        def greens(th, pts, uv, nx, ne, f=f):
+            return f.greens(th, pts, uv, 0, pts.shape[1] // 3)
+        obs = syn.make_observations("c2", it["seed"], greens, nsta=it["nsta"], sigma=it["sigma"], nxi=it["nxi"],
+                                    neta=it["neta"], frame=it["frame"])
+        probs.append(S.Problem(ctx, [it["nxi"]], [it["neta"]], xmin, xmax, zmin, obs, syn.MAX_SLIP, it["P"],
+                               mover=S.MOVER_GIBBS, gibbs_sweeps=3, max_batch=2))
+        orcs.append(O.ProblemHolder(f, obs, syn.MAX_SLIP, it["P"], mover=1, gibbs_sweeps=3))
+        ths.append(syn.sample_thetas(2, 60 + k, frame=it["frame"]))
+        keys.append([O.inner_key(61 + k, 0, 0, j) for j in range(2)])
+    say("case: group of 3 vs oracle, ndim", [p.ndim for p in probs])
+    g = S.Group(ctx, probs)
+    nl, ns = g.fault_calc_likelihood(ths, keys)
+    for k in range(3):
+        for j in range(2):
+            ref, nst = orcs[k].fault_likelihood(ths[k][:, j], keys[k][j])
+            say("member", k, "geometry", j, nl[k][j], ref, ns[k][j], nst)
+            assert ns[k][j] == nst
+            assert nl[k][j] == pytest.approx(ref, rel=1e-8)
+    g.close()
+    for p in probs:
+        p.close()
+    ctx.close()
