@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the serving run's arrival stream; 0 skips it")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
+    ap.add_argument("--serve-sizes", choices=["bench", "c5"], default="bench",
+                    help="request meshes and station counts: bench (4x4..12x8 patches, 50..400 stations) or c5 "
+                         "(4x4..32x16 patches, 50..1000 stations)")
     ap.add_argument("--budget-s", type=float, default=480.0,
                     help="wall-clock budget of the whole run: a leg whose estimate does not fit is skipped")
     ap.add_argument("--batch-n", type=int, default=64, help="inversions of the batching leg (0 skips it)")
@@ -270,10 +273,13 @@ def run_serving(a, S, syn):
     xmin, xmax, zmin, _ = syn.fault_frame()
     gctx = S.Context(0)
     reqs = []
+    # request mesh and station ranges: the bench's default (4x4 .. 12x8 patches, 50 .. 400 stations) or, with
+    # --serve-sizes c5, the serving configuration's own (4x4 .. 32x16 patches, 50 .. 1000 stations)
+    px, pe, ps = ((12, 8, 400) if a.serve_sizes == "bench" else (32, 16, 1000))
     for k, _t in enumerate(arrivals):
-        nxi = int(round(np.exp(rng.uniform(np.log(4), np.log(12)))))
-        neta = int(round(np.exp(rng.uniform(np.log(4), np.log(8)))))
-        nsta = int(round(np.exp(rng.uniform(np.log(50), np.log(400)))))
+        nxi = int(round(np.exp(rng.uniform(np.log(4), np.log(px)))))
+        neta = int(round(np.exp(rng.uniform(np.log(4), np.log(pe)))))
+        nsta = int(round(np.exp(rng.uniform(np.log(50), np.log(ps)))))
         sigma = rng.uniform(0.2, 1.0)
         N = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
 
@@ -313,8 +319,8 @@ def run_serving(a, S, syn):
             "max_latency_s": round(float(lat.max()), 3),
             "requests_per_s": round(len(res) / span, 3), "outer_evals_per_s": round(evals / span, 2),
             "form": "bf16", "budget_gb": 200,
-            "workload": "Poisson arrivals; per request 4x4..12x8 patches, 50..400 stations, 2..10 mm noise, "
-                        "N 64..512 outer particles, 3 outer stages, inner P=512 Gibbs x3"}
+            "workload": "Poisson arrivals; per request 4x4..%dx%d patches, 50..%d stations, 2..10 mm noise, "
+                        "N 64..512 outer particles, 3 outer stages, inner P=512 Gibbs x3" % (px, pe, ps)}
 
 
 def device_greens(S, syn, ctx, frame):
@@ -331,7 +337,7 @@ def device_greens(S, syn, ctx, frame):
     return greens
 
 
-def run_batch(a, S, syn, solo_sample=0):
+def run_batch(a, S, syn, solo_sample=0, extended=False):
     """Batching form: a.batch_n independent inversions drawn from the seed (patch grids 4x4 .. 24x24, 50 .. 800
     stations, faults of any strike with dip 20 .. 80 deg and length 10 .. 60 km, noise 2 .. 10 mm, inner P
     500 .. 2000), split-BF16 form, Gibbs x5, as one group: every member streams a.batch_evals geometries (thetas from
@@ -373,6 +379,31 @@ def run_batch(a, S, syn, solo_sample=0):
                        "split-BF16; %d geometries per inversion from its prior" % (len(probs), a.batch_evals)}
     print("batch: %d evaluations in %.2f s (%.2f/s), mean %.1f inner stages" % (evals, wall, evals / wall,
                                                                                 stages.mean()), flush=True)
+    if extended:
+        # the same call with the stage launched kernel by kernel (identical results)
+        g.set_graph(False)
+        t0 = time.perf_counter()
+        nl2, _ = g.fault_calc_likelihood(ths, keys)
+        w2 = time.perf_counter() - t0
+        g.set_graph(True)
+        same = all(np.array_equal(x, y) for x, y in zip(nl, nl2))
+        out["no_graph"] = {"wall_s": round(w2, 2), "evals_per_s": round(evals / w2, 3), "same_results": bool(same)}
+        print("batch: without the stage graph %.2f s (%.2f/s), results %s" % (w2, evals / w2,
+                                                                             "identical" if same else "DIFFER"),
+              flush=True)
+        # every inversion's outer SMC (fault_smc_exec on the group's threads), N scaled down, one stage
+        Ns = [max(16, it["N"] // 250) for it in insts]
+        ranges = [syn.prior_range(frame=it["frame"]) for it in insts]
+        t0 = time.perf_counter()
+        res = g.fault_smc_exec(ranges, Ns, [a.seed * 1000 + k for k in range(len(insts))], max_stages=1)
+        wo = time.perf_counter() - t0
+        oevals = sum(N * (1 + 2 * r["nstage"]) for N, r in zip(Ns, res))
+        out["outer_smc"] = {"inversions": len(insts), "N_range": [min(Ns), max(Ns)], "max_stages": 1,
+                            "wall_s": round(wo, 2), "evals": oevals, "evals_per_s": round(oevals / wo, 3),
+                            "note": "N_i = max(16, N_i / 250) of the configuration's 1000..8000; evaluations counted as "
+                                    "N initial + 2N per stage"}
+        print("batch: 64 outer SMCs (N %d..%d, one stage) in %.2f s, %.2f evaluations/s" % (min(Ns), max(Ns), wo,
+                                                                                           oevals / wo), flush=True)
     g.close()
     if solo_sample > 0:
         t0 = time.perf_counter()
@@ -457,7 +488,7 @@ def main():
                           "data": "synthetic", "serving": sv}), flush=True)
         return
     if a.workload == "c4":
-        b = run_batch(a, S, syn, solo_sample=4)
+        b = run_batch(a, S, syn, solo_sample=4, extended=True)
         print(json.dumps({"metric": "outer-particle evaluations/s of %d independent inversions as one group"
                                     % a.batch_n, "value": b["evals_per_s"], "unit": "evaluations/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "mixed (split-bf16 matrix + fp32 draws; fp64 "

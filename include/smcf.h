@@ -329,13 +329,15 @@ int smcf_fault_smc_set_outer_stage(smcf_problem* pb, int mode);
  * different meshes, observation sets, particle counts and forms (Gibbs mover), created on the group's context;
  * every stage kernel runs once over the instances of all members (the Gibbs sweeps once per kernel class: form
  * and block rows of the tuning table), and with graphs on (the default) a stage is one HIP graph launch.  Each
- * member's results equal those of the same calls on the member alone.  A problem belongs to one group at a time.
+ * member's results equal those of the same calls on the member alone.  A problem belongs to one group at a time
+ * and is destroyed after its group (smcf_problem_destroy refuses a member: SMCF_ESTATE); group outer SMCs need the
+ * host outer stage (smcf_fault_smc_set_outer_stage 0, the default).
  *   smcf_group_calc_likelihood_batch  smcf_fault_calc_likelihood_batch for every member at once: B[i] geometries
  *                                     (theta[i] ndim_fault x B[i], keys[i]) stream through member i's slots;
  *                                     nstage may be NULL, B[i] may be 0
  *   smcf_group_fault_smc_exec         smcf_fault_smc_exec for every member at once (range[i], N[i], seed[i];
  *                                     particles[i] ndim_fault x N[i], likelihood[i] N[i]; nstage and evidence
- *                                     N-less arrays of one entry per member, may be NULL): each member's outer SMC
+ *                                     one entry per member, may be NULL): each member's outer SMC
  *                                     runs on its own host thread, its waves served by the shared stage loop
  *   smcf_group_set_graph              1: capture each call's stage launches into a HIP graph (default); 0: launch
  *                                     them one by one (identical results) */

@@ -634,6 +634,10 @@ int smcf_set_gibbs_block_rows(int precision, int ndim_lo, int ndim_hi, int rows)
 
 int smcf_problem_destroy(smcf_problem* pb) {
   if (!pb) return SMCF_OK;
+  if (pb->p.in_group || pb->p.engine) {
+    set_error("smcf_problem_destroy: the problem is a member of a group (destroy the group first)");
+    return SMCF_ESTATE;
+  }
   (void)hipSetDevice(pb->p.ctx->device);
   (void)hipStreamSynchronize(pb->p.ctx->stream);
   for (void* p : pb->p.allocs) (void)hipFree(p);

@@ -870,6 +870,12 @@ int smcf_group_fault_smc_exec(smcf_group* g, const double* const* range, const i
   int n = (int)g->members.size();
   for (int i = 0; i < n; ++i)
     if (!range[i] || N[i] < 1 || !particles[i] || !likelihood[i]) return SMCF_EINVAL;
+  for (auto* m : g->members)
+    if (outer_stage_mode(m) != 0) {
+      // the device outer stage would launch on the context's stream from the members' threads, beside the engine
+      set_error("smcf_group_fault_smc_exec: group members run the host outer stage (smcf_fault_smc_set_outer_stage 0)");
+      return SMCF_EUNSUPPORTED;
+    }
   SMCF_HIP(hipSetDevice(g->ctx->device));
   Engine E;
   int r = E.init(g->ctx, false, g->graph);

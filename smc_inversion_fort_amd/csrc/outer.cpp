@@ -539,3 +539,8 @@ int smcf_fault_smc_exec_gsmc(smcf_problem* pb, const double* range, int N, uint6
 }
 
 }  // extern "C"
+
+namespace smcf {
+// the outer-stage mode of a problem (0 host, 1 / 2 device), for callers that run outer SMCs beside a stage engine
+int outer_stage_mode(const smcf_problem* pb) { return outer_mode_of(pb); }
+}  // namespace smcf

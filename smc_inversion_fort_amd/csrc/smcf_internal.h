@@ -431,6 +431,9 @@ struct ProfScope {
 };
 void resolve_profiles(smcf_ctx* ctx);
 
+// the outer-stage mode set by smcf_fault_smc_set_outer_stage (0: host components)
+int outer_stage_mode(const smcf_problem* pb);
+
 // Set-up of n instances of one problem on stream s (slots sel[0 .. n-1], device array; thetas and keys already in
 // their slots), left held (marked done) until the stage engine releases them: setup_pre forms everything up to
 // the Gram quantities, the prior chain runs once over every instance admitted together (launch_init_chain), and
