@@ -55,7 +55,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--evals", type=int, default=64, help="geometry evaluations completed per timed step")
-    ap.add_argument("--slots", type=int, default=1024, help="device slots (inner SMCs in flight)")
+    ap.add_argument("--slots", type=int, default=None,
+                    help="device slots (inner SMCs in flight; default 1024, and 256 for c3, whose 1024 slots would "
+                         "need 278 GB)")
     ap.add_argument("--tail", type=int, default=-1,
                     help="geometries streamed after the last timed one to keep the slots busy (-1: --slots)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
@@ -471,6 +473,8 @@ def main():
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.slots is None:
+        a.slots = 256 if a.workload == "c3" else 1024
     if a.steps < 1 or a.evals < 1 or a.slots < 1:
         print("--steps, --evals and --slots must be >= 1", file=sys.stderr)
         sys.exit(2)
