@@ -205,7 +205,11 @@ def run_form(a, S, syn, obs, precision):
     out = dict(value=K * E / timed_s, ms_per_step=1e3 * timed_s / K, step_s=step_s, stages=ns, neglog=nl,
                stages_timed=ns[timed], eval_lat=(round(float(np.percentile(lat, 50)), 3),
                                                  round(float(np.percentile(lat, 99)), 3)),
-               stream={"geometries": total, "wall_s": round(wall, 2), "evals_per_s": round(total / wall, 3)},
+               # instance_stages_per_s: the inner SMCs' stages (each a resampling, the moves and the likelihood of
+               # P particles) over the stream's wall time, a rate that does not depend on how many stages the
+               # geometries happen to need
+               stream={"geometries": total, "wall_s": round(wall, 2), "evals_per_s": round(total / wall, 3),
+                       "instance_stages_per_s": round(moves / wall, 1)},
                roofline=roof, ndim=ndim, nobs=pb.nobs, P=P, sweeps=sweeps)
     pb.close()
     ctx.close()
