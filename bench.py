@@ -56,10 +56,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--evals", type=int, default=128,
-                    help="geometry evaluations completed per timed step (20 x 128 = 2560 timed completions, about "
-                         "70 s of the FP64 form; the count of completions in such a window fluctuates by about "
-                         "1/sqrt(2560) = 2 %%, against 3 %% at 64 per step)")
+    ap.add_argument("--evals", type=int, default=96,
+                    help="geometry evaluations completed per timed step (20 x 96 = 1920 timed completions, about "
+                         "50 s of the FP64 form; the count of completions in such a window fluctuates by about "
+                         "1/sqrt(1920) = 2.3 %%, against 2.8 %% at 64 per step)")
     ap.add_argument("--slots", type=int, default=None,
                     help="device slots (inner SMCs in flight; default 1024, and 256 for c3, whose 1024 slots would "
                          "need 278 GB)")

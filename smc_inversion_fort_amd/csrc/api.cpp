@@ -356,7 +356,18 @@ static int problem_build(smcf_ctx* ctx, const smcf_problem_desc* d, int max_batc
           if (dr >= 0) band = std::max(band, dr - k);
         }
     }
-    m.chain_lag = std::max(band + 1, (ndof + kChainLanes - 2) / (kChainLanes - 1));
+    // L lanes per component need lag >= band + 1 and (L - 1) lag >= ndof; a chain runs (P - 1) lag + ndof rounds and
+    // a wavefront holds 32 / L chains.  The fewest lanes whose chain is at most a quarter longer than with 32 are
+    // taken (fewer chain wavefronts beside the stage kernels), with the ring within 128 values per lane.
+    auto lag_for = [&](int L) { return std::max(band + 1, (ndof + L - 2) / (L - 1)); };
+    auto rounds_for = [&](int L) { return (double)(P.P - 1) * lag_for(L) + ndof; };
+    m.chain_lanes = kChainLanes;
+    for (int L = 2; L < kChainLanes; L *= 2)
+      if (lag_for(L) < 128 && rounds_for(L) <= 1.25 * rounds_for(kChainLanes)) {
+        m.chain_lanes = L;
+        break;
+      }
+    m.chain_lag = lag_for(m.chain_lanes);
     m.chain_w = 1;
     while (m.chain_w < m.chain_lag + 1) m.chain_w *= 2;
   }
@@ -1086,7 +1097,7 @@ int smcf_slip_gen_init_particles(smcf_problem* pb, const double* theta, uint64_t
   SMCF_HIP(hipMemcpyAsync(P.bt.key, &key, sizeof(uint64_t), hipMemcpyHostToDevice, s));
   int rc = setup_pre(P, 1, nullptr, s);
   if (rc) return rc;
-  CHECK_LAUNCH(launch_init_chain(P.mdev, nullptr, nullptr, 1, P.m.chain_w, s));
+  CHECK_LAUNCH(launch_init_chain(P.mdev, nullptr, nullptr, nullptr, 1, P.m.chain_w, s));
   if ((rc = setup_post(P, 1, nullptr, s))) return rc;
   if (particles && (rc = download_columns(P, P.bt.S, P.P, particles, s))) return rc;
   if (likelihood) SMCF_HIP(hipMemcpyAsync(likelihood, P.bt.like, sizeof(double) * P.P, hipMemcpyDeviceToHost, s));

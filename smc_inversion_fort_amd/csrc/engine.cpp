@@ -121,6 +121,7 @@ struct Admission {
   int half = 0;
   std::vector<AdmitEntry> ent;
   std::vector<int2> list;
+  std::vector<int2> jobs;  // prior-chain wavefronts: (first entry of list, count), one member each
   std::vector<Geo> geo;
   hipEvent_t ready{};
 };
@@ -318,7 +319,7 @@ struct Engine {
       return r;
     for (int h = 0; h < 2; ++h)
       if ((r = grow(b_adm[h], sizeof(AdmitEntry) * std::max(ninst, 1))) ||
-          (r = grow(b_list[h], sizeof(int2) * std::max(ninst, 1))))
+          (r = grow(b_list[h], sizeof(int2) * 2 * std::max(ninst, 1))))  // the list, then the chain jobs
         return r;
     dmem = (MemberDev*)b_mem.p;
     dstat = (double*)b_stat.p;
@@ -480,11 +481,21 @@ struct Engine {
       who.push_back({i, n});
     }
     if (who.empty()) return SMCF_OK;
+    // prior chains: up to 32 / chain_lanes consecutive entries of one member per wavefront
+    adm.jobs.clear();
+    for (size_t k = 0; k < adm.list.size();) {
+      const int mi = adm.list[k].x, per = kChainLanes / mr[mi].P->m.chain_lanes;
+      size_t e = k;
+      while (e < adm.list.size() && adm.list[e].x == mi && (int)(e - k) < per) ++e;
+      adm.jobs.push_back(make_int2((int)k, (int)(e - k)));
+      k = e;
+    }
     adm.half = 1 - adm.half;
-    const int h = adm.half, n = (int)adm.list.size();
+    const int h = adm.half, n = (int)adm.list.size(), nj = (int)adm.jobs.size();
     hipStream_t s2 = ctx->setup_stream;
     SMCF_HIP(hipMemcpyAsync(dadm[h], adm.ent.data(), sizeof(AdmitEntry) * n, hipMemcpyHostToDevice, s2));
     SMCF_HIP(hipMemcpyAsync(dlist[h], adm.list.data(), sizeof(int2) * n, hipMemcpyHostToDevice, s2));
+    SMCF_HIP(hipMemcpyAsync(dlist[h] + n, adm.jobs.data(), sizeof(int2) * nj, hipMemcpyHostToDevice, s2));
     hipLaunchKernelGGL(k_admit, dim3((n + 63) / 64), dim3(64), 0, s2, dmem, dadm[h], n, h);
     SMCF_HIP(hipGetLastError());
     int chain_w = 1, r;
@@ -495,7 +506,7 @@ struct Engine {
     }
     {
       ProfScope ps(ctx, "init_chain", s2);
-      hipError_t e = launch_init_chain(dmem, dlist[h], nullptr, n, chain_w, s2);
+      hipError_t e = launch_init_chain(dmem, dlist[h], nullptr, dlist[h] + n, nj, chain_w, s2);
       if (e != hipSuccess) {
         set_error(std::string("stage engine: launch of init_chain: ") + hipGetErrorString(e));
         return SMCF_EHIP;

@@ -24,19 +24,27 @@ namespace smcf {
 // The chain is pipelined without changing a single operation.  LL couples strike slip (even
 // dimensions) only with strike slip and dip slip only with dip slip, so each component is its
 // own chain.  Within a component, coordinate d of particle k reads particle k's coordinates
-// d' < d and particle k-1's coordinates d < d' <= d + band.  Lane l (of the 32 lanes of a
-// component) runs particles k = l, l + 32, ...; particle k starts at round k * lag with
+// d' < d and particle k-1's coordinates d < d' <= d + band.  Lane l (of the L = chain_lanes lanes
+// of a component) runs particles k = l, l + L, ...; particle k starts at round k * lag with
 // lag > band, so in every round particle k-1 is exactly lag coordinates ahead and has written
 // everything particle k reads.  Each lane keeps its particle's last W coordinates in an LDS
-// ring (W >= lag + 1); 31 * lag >= ndof guarantees that a lane's ring is not reused while the
-// next lane still reads it, and that particle 0 reads zeros from the ring of lane 31.
+// ring (W >= lag + 1); (L - 1) * lag >= ndof guarantees that a lane's ring is not reused while the
+// next lane still reads it, and that particle 0 reads zeros from the ring of lane L - 1.
+//
+// A chain keeps only about ndof / lag particles in flight per component, so one instance leaves most of a
+// wavefront's lanes idle; the 64 lanes hold 32 / L instances of one member instead (lane = q 2L + c L + l for
+// instance q, component c).  The chain wavefronts run beside the stage kernels and hold their registers for the
+// whole chain, so fewer of them leave more room to the sweep workgroups.
 __global__ void __launch_bounds__(64) k_init_chain(const MemberDev* __restrict__ mem, const int2* __restrict__ list,
-                                                   const int* __restrict__ sel) {
+                                                   const int* __restrict__ sel, const int2* __restrict__ jobs) {
   extern __shared__ double ring[];
-  int2 is = list ? list[blockIdx.x] : make_int2(0, sel ? sel[blockIdx.x] : (int)blockIdx.x);
-  const MemberDev& M = mem[is.x];
-  int b = is.y;
-  int lane = threadIdx.x, c = lane / kChainLanes, l = lane % kChainLanes;
+  const int2 job = jobs ? jobs[blockIdx.x] : make_int2((int)blockIdx.x, 1);
+  auto entry = [&](int k) { return list ? list[k] : make_int2(0, sel ? sel[k] : k); };
+  const MemberDev& M = mem[entry(job.x).x];
+  const int L = M.chain_lanes;
+  int lane = threadIdx.x, q = lane / (2 * L), c = (lane / L) % 2, l = lane % L;
+  const bool live = q < job.y;  // lanes of the wavefront's unused instance places run the rounds idle
+  int b = entry(job.x + (live ? q : 0)).y;
   int nd = M.nd, ndof = nd / 2, lag = M.chain_lag, W = M.chain_w, P = M.P;
   double max_slip = M.max_slip;
   uint64_t key = M.bt.key[b];
@@ -45,15 +53,15 @@ __global__ void __launch_bounds__(64) k_init_chain(const MemberDev* __restrict__
   const double* VX = M.bt.llc_val + (size_t)b * nd * 25;
   double* S = M.bt.S + (size_t)b * nd * P;
   double* mine = ring + (size_t)lane * W;
-  const double* prev = ring + (size_t)(c * kChainLanes + (l + kChainLanes - 1) % kChainLanes) * W;
+  const double* prev = ring + (size_t)(q * 2 * L + c * L + (l + L - 1) % L) * W;
   for (int w = 0; w < W; ++w) mine[w] = 0.0;
   __syncthreads();
   long rounds = (long)(P - 1) * lag + ndof;
   for (long t = 0; t < rounds; ++t) {
     long kmax = t / lag;
-    long k = kmax - (((kmax - l) % kChainLanes) + kChainLanes) % kChainLanes;
+    long k = kmax - (((kmax - l) % L) + L) % L;
     long dl = t - k * lag;
-    if (k >= 0 && k < P && dl < ndof) {
+    if (live && k >= 0 && k < P && dl < ndof) {
       int d = (int)dl, i = 2 * d + c;
       const int* ix = IX + (size_t)i * 25;
       const double* vx = VX + (size_t)i * 25;
@@ -105,10 +113,11 @@ __global__ void __launch_bounds__(64) k_init_chain(const MemberDev* __restrict__
   }
 }
 
-hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, int B, int chain_w, hipStream_t s) {
+hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, const int2* jobs, int nw,
+                             int chain_w, hipStream_t s) {
   size_t sh = sizeof(double) * 2 * kChainLanes * (size_t)chain_w;
   if (sh > 64 * 1024) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(k_init_chain, dim3(B), dim3(2 * kChainLanes), sh, s, mem, list, sel);
+  hipLaunchKernelGGL(k_init_chain, dim3(nw), dim3(2 * kChainLanes), sh, s, mem, list, sel, jobs);
   return hipGetLastError();
 }
 

@@ -12,7 +12,8 @@
 
 namespace smcf {
 
-// lanes per slip component in the pipelined prior chain (one 64-lane wave per instance)
+// most lanes per slip component in the pipelined prior chain (a 64-lane wavefront holds 32 / chain_lanes instances
+// of chain_lanes lanes per component)
 constexpr int kChainLanes = 32;
 
 void set_error(const std::string& msg);
@@ -33,6 +34,7 @@ struct MeshDev {
   int fix_xbend;
   int chain_lag;            // pipelined prior chain: rounds between consecutive particles (> LL band per component)
   int chain_w;              // per-lane LDS ring length (power of two >= chain_lag + 1)
+  int chain_lanes;          // lanes per slip component (power of two, (chain_lanes - 1) * chain_lag >= ndof)
   double xmin, xmax, zmin;
   const double* xbend;      // nplane-1 (>=1 allocated)
   const int* plane_nxi;     // nplane
@@ -172,7 +174,7 @@ inline size_t hmc_work_entries(int cap, int P) { return (size_t)cap * 2 * ((size
 struct MemberDev {
   BatchDev bt;
   int nd, nf, P, kb, sweeps;
-  int chain_lag, chain_w;  // pipelined prior chain (MeshDev)
+  int chain_lag, chain_w, chain_lanes;  // pipelined prior chain (MeshDev)
   double max_slip;
 };
 
@@ -282,6 +284,7 @@ inline MemberDev member_of(const Problem& P) {
   d.sweeps = P.sweeps;
   d.chain_lag = P.m.chain_lag;
   d.chain_w = P.m.chain_w;
+  d.chain_lanes = P.m.chain_lanes;
   d.max_slip = P.max_slip;
   return d;
 }
@@ -296,9 +299,12 @@ hipError_t launch_laplacian(const MeshDev& m, BatchDev& bt, int B, hipStream_t s
 // sigmas already in bt.scal are used instead of those of theta
 hipError_t launch_gram(const Problem& P, const MeshDev& m, BatchDev& bt, int B, hipStream_t s,
                        const int* sel = nullptr, bool scal_given = false);
-// prior chain of B instances: (member, slot) = list[k] of the member table mem, or with list == nullptr member 0 and
-// slot sel[k] (slot k when sel == nullptr); chain_w = the largest ring length of the members involved
-hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, int B, int chain_w, hipStream_t s);
+// prior chains: entry k is (member, slot) = list[k] of the member table mem, or with list == nullptr member 0 and slot
+// sel[k] (slot k when sel == nullptr).  Wavefront w runs the entries jobs[w] = (first, count), which belong to one
+// member and number at most 32 / chain_lanes of it; with jobs == nullptr wavefront w runs entry w alone (nw entries).
+// chain_w = the largest ring length of the members involved.
+hipError_t launch_init_chain(const MemberDev* mem, const int2* list, const int* sel, const int2* jobs, int nw,
+                             int chain_w, hipStream_t s);
 // likelihood of the current particles of the given slots of one problem (matrix cores; Gram form; set-up)
 hipError_t launch_like(const MeshDev& m, BatchDev& bt, int B, int use_active, hipStream_t s, const int* sel = nullptr);
 // prior of the current particles
