@@ -8,7 +8,7 @@ to gamma = 1.  At 16x16 patches and 1200 observations (N = 10 000 outer particle
 geometry).  The bench times a stretch of such a stream:
 
   one library call streams  W*Ew + K*E + T  geometries (drawn from the prior) through S slots;
-  warm-up step w  = the next Ew completions (Ew >= 2S/W, so the warm-up spans two fills of every slot: the
+  warm-up step w  = the next Ew completions (Ew >= 2S/W by default, so the warm-up spans two fills of every slot: the
                     first fill's geometries start together, and the slowest of them finish together at the
                     reference's 200-stage limit; a timed window inside that burst overstates the steady rate);
   timed step k    = the next E completions, k = 1..K;
@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--slots", type=int, default=None,
                     help="device slots (inner SMCs in flight; default 1024, and 256 for c3, whose 1024 slots would "
                          "need 278 GB)")
+    ap.add_argument("--warmup-fills", type=float, default=2.0,
+                    help="slot fills the warm-up spans (Ew >= fills * slots / warmup); 1 halves the stream of the "
+                         "32x16-patch HMC workload, whose library call otherwise runs for minutes")
     ap.add_argument("--tail", type=int, default=-1,
                     help="geometries streamed after the last timed one to keep the slots busy (-1: --slots)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
@@ -126,7 +129,7 @@ def keys_for(stream, n):
 
 def stream_plan(a):
     W, K, E = a.warmup, a.steps, a.evals
-    Ew = max(E, -(-2 * a.slots // W)) if W > 0 else 0
+    Ew = max(E, int(np.ceil(a.warmup_fills * a.slots / W))) if W > 0 else 0
     tail = a.slots if a.tail < 0 else a.tail
     return W, K, E, Ew, tail, W * Ew + K * E + tail
 
