@@ -126,18 +126,34 @@ struct Admission {
   hipEvent_t ready{};
 };
 
-// items of the chunked kernels for the given instances (member, slot, chunks): instance g of the list runs on
-// XCD g % 8, its chunks at consecutive positions of that XCD's queue; queues are padded to equal length with empty
-// items
+// items of the chunked kernels for the given instances (member, slot, chunks): each instance in turn goes to the
+// XCD whose queue is shortest (the lowest such XCD; with equal chunk counts XCD g % 8 for instance g), its chunks at
+// consecutive positions of that queue; item 8r + x is entry r of XCD x's queue (workgroup w runs on XCD w % 8), and
+// the queues are padded to equal length with empty items.  The longest queue is at most ceil(T / 8) + c items for T
+// chunks in all and c the most of one instance (xcd_items_bound).
 void xcd_items(const std::vector<int4>& insts, std::vector<int4>& out) {
   std::vector<std::vector<int4>> q(8);
-  for (size_t g = 0; g < insts.size(); ++g)
-    for (int c = 0; c < insts[g].z; ++c) q[g % 8].push_back(make_int4(insts[g].x, insts[g].y, c, 0));
+  for (size_t g = 0; g < insts.size(); ++g) {
+    int x = 0;
+    for (int y = 1; y < 8; ++y)
+      if (q[y].size() < q[x].size()) x = y;
+    for (int c = 0; c < insts[g].z; ++c) q[x].push_back(make_int4(insts[g].x, insts[g].y, c, 0));
+  }
   size_t L = 0;
   for (auto& v : q) L = std::max(L, v.size());
   out.assign(8 * L, make_int4(-1, 0, 0, 0));
   for (int x = 0; x < 8; ++x)
     for (size_t r = 0; r < q[x].size(); ++r) out[8 * r + x] = q[x][r];
+}
+
+// length that xcd_items keeps within for any subset of these instances
+size_t xcd_items_bound(const std::vector<int4>& insts) {
+  size_t T = 0, c = 0;
+  for (auto& e : insts) {
+    T += (size_t)e.z;
+    c = std::max(c, (size_t)e.z);
+  }
+  return insts.empty() ? 0 : 8 * ((T + 7) / 8 + c);
 }
 
 }  // namespace
@@ -167,6 +183,19 @@ struct Engine {
   ItemList like_fp{}, like_bf{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // The chunked kernels' item lists hold only the instances running a geometry, rewritten whenever that set changed
+  // (an instance released or harvested), so the XCD-aware order spreads the running instances evenly over the XCDs:
+  // with the finished and waiting instances left in place the busiest XCD set a launch's time.  The launches cover
+  // each list's full length (the stage graph's grids are fixed); the rest is padding that returns at once.
+  struct ListSrc {
+    size_t off = 0;
+    size_t len = 0;
+    std::vector<int4> inst;   // (member in the launch's table, slot, chunks, 0) of every instance of the list
+    std::vector<int> member;  // the instance's member in mr
+  };
+  std::vector<ListSrc> lists;
+  std::vector<int4> items_h;
+  long alive_version = 0, items_version = -1;
   Admission adm;
   std::vector<double> stat;
   int busy_total = 0;
@@ -284,16 +313,31 @@ struct Engine {
       it->members.push_back(i);
     }
     std::vector<int4> all, tmp;
-    auto add_list = [&](const std::vector<int4>& v, size_t& off, int& cnt) {
+    lists.clear();
+    auto add_list = [&](const std::vector<int4>& v, const std::vector<int>& member, size_t& off, int& cnt) {
+      // every instance's items; the region is as long as any subset's (the running instances') list can be
       xcd_items(v, tmp);
       off = all.size();
-      cnt = (int)tmp.size();
+      size_t len = xcd_items_bound(v);
+      cnt = (int)len;
       all.insert(all.end(), tmp.begin(), tmp.end());
+      all.resize(off + len, make_int4(-1, 0, 0, 0));
+      ListSrc L;
+      L.off = off;
+      L.len = len;
+      L.inst = v;
+      L.member = member;
+      lists.push_back(std::move(L));
+    };
+    auto members_of = [](const std::vector<int4>& v) {
+      std::vector<int> m;
+      for (auto& e : v) m.push_back(e.x);
+      return m;
     };
     size_t off_fp = 0, off_bf = 0;
     int n_fp = 0, n_bf = 0;
-    add_list(fp, off_fp, n_fp);
-    add_list(bf, off_bf, n_bf);
+    add_list(fp, members_of(fp), off_fp, n_fp);
+    add_list(bf, members_of(bf), off_bf, n_bf);
     glaunch.clear();
     std::vector<std::pair<size_t, int>> goff;
     for (auto& c : cls)
@@ -302,14 +346,18 @@ struct Engine {
         gl.precision = c.precision;
         gl.kb = c.kb;
         std::vector<int4> v;
+        std::vector<int> vm;
         for (size_t k = g0; k < std::min(c.members.size(), g0 + kGibbsArgMembers); ++k) {
           int i = c.members[k];
           gl.args.m[k - g0] = gibbs_member_of(*mr[i].P);
           int chunks = (mr[i].P->P + 255) / 256;
-          for (int b = 0; b < mr[i].W; ++b) v.push_back(make_int4((int)(k - g0), b, chunks, 0));
+          for (int b = 0; b < mr[i].W; ++b) {
+            v.push_back(make_int4((int)(k - g0), b, chunks, 0));
+            vm.push_back(i);
+          }
         }
         goff.push_back({0, 0});
-        add_list(v, goff.back().first, goff.back().second);
+        add_list(v, vm, goff.back().first, goff.back().second);
         glaunch.push_back(gl);
       }
     int r;
@@ -333,6 +381,8 @@ struct Engine {
     if (n) SMCF_HIP(hipMemcpyAsync(dmem, md.data(), sizeof(MemberDev) * n, hipMemcpyHostToDevice, s));
     if (ninst) SMCF_HIP(hipMemcpyAsync(dinst, inst.data(), sizeof(int2) * ninst, hipMemcpyHostToDevice, s));
     if (!all.empty()) SMCF_HIP(hipMemcpyAsync(ditems, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice, s));
+    items_h = all;
+    items_version = -1;  // the running instances' lists are written before the next stage
     ss = StageSet{dmem, dinst, ninst, nd_max, P_max, kb_max};
     like_fp = ItemList{ditems + off_fp, n_fp};
     like_bf = ItemList{ditems + off_bf, n_bf};
@@ -413,9 +463,35 @@ struct Engine {
 
   static double since(const Clock::time_point& t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
+  // the item lists of the instances now running a geometry (the host's stream synchronisation at the end of every
+  // stage keeps items_h unchanged until the copy has been read)
+  int upload_items(hipStream_t s) {
+    std::vector<int4> sub, tmp;
+    for (auto& L : lists) {
+      sub.clear();
+      for (size_t k = 0; k < L.inst.size(); ++k)
+        if (mr[L.member[k]].job[L.inst[k].y].r) sub.push_back(L.inst[k]);
+      xcd_items(sub, tmp);
+      if (tmp.size() > L.len) {
+        set_error("stage engine: an item list outgrew its region");
+        return SMCF_ESTATE;
+      }
+      std::copy(tmp.begin(), tmp.end(), items_h.begin() + L.off);
+      std::fill(items_h.begin() + L.off + tmp.size(), items_h.begin() + L.off + L.len, make_int4(-1, 0, 0, 0));
+    }
+    if (!items_h.empty())
+      SMCF_HIP(hipMemcpyAsync(b_items.p, items_h.data(), sizeof(int4) * items_h.size(), hipMemcpyHostToDevice, s));
+    items_version = alive_version;
+    return SMCF_OK;
+  }
+
   // one stage, its status read back, finished instances harvested
   int step() {
     hipStream_t s = ctx->stream;
+    if (items_version != alive_version) {
+      int r = upload_items(s);
+      if (r) return r;
+    }
     if (gexec) {
       SMCF_HIP(hipGraphLaunch(gexec, s));
     } else {
@@ -434,6 +510,7 @@ struct Engine {
         if (r.nstage) r.nstage[g.i] = std::min((int)stat[k + 2] - 1, 201);
         if (r.t_done) r.t_done[g.i] = since(r.t0);
         g.r = nullptr;
+        ++alive_version;
         --m.busy;
         --busy_total;
         m.freed.push_back(b);
@@ -532,6 +609,7 @@ struct Engine {
       ++m.busy;
       ++busy_total;
     }
+    ++alive_version;
     adm.live = false;
     return SMCF_OK;
   }
