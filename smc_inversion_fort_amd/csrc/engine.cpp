@@ -6,7 +6,8 @@
 // observation sets and particle counts differ.  The chunked kernels (Gibbs sweeps, likelihoods) take item lists
 // (member, slot, 256-particle chunk) in XCD-aware order; the Gibbs sweeps launch once per kernel class (form and
 // block rows from the tuning table), so every instance keeps the block rows - and therefore the arithmetic - it has
-// when it runs alone.  Instances that have finished are masked, not compacted.
+// when it runs alone.  The per-instance kernels skip the instances without a running geometry; the item lists are
+// rewritten with the running instances only whenever that set changes, spread evenly over the XCDs.
 //
 // After every stage the host reads the instances' status back in one copy and harvests the finished ones; once
 // enough slots of a member are free its next geometries are set up in them on the set-up stream (each member's
