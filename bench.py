@@ -86,6 +86,9 @@ def parse():
     ap.add_argument("--batch-n", type=int, default=64, help="inversions of the batching leg (0 skips it)")
     ap.add_argument("--batch-slots", type=int, default=16, help="device slots per inversion in the batching leg")
     ap.add_argument("--batch-evals", type=int, default=32, help="geometries per inversion in the batching leg")
+    ap.add_argument("--no-batch-outer", action="store_true",
+                    help="c4: leave out the inversions' outer SMCs on the group's threads (the group call, its "
+                         "kernel-by-kernel repeat and the solo sample still run)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: the headline (16x16 patches, Gibbs x5, FP64 and mixed forms); c3: 32x16 patches, "
                          "3000 observations, HMC with 10 leapfrog steps tuned to 0.65 acceptance, FP64; c4: the "
@@ -405,6 +408,7 @@ def run_batch(a, S, syn, solo_sample=0, extended=False):
         print("batch: without the stage graph %.2f s (%.2f/s), results %s" % (w2, evals / w2,
                                                                              "identical" if same else "DIFFER"),
               flush=True)
+    if extended and not a.no_batch_outer:
         # every inversion's outer SMC (fault_smc_exec on the group's threads), N scaled down, one stage
         Ns = [max(16, it["N"] // 250) for it in insts]
         ranges = [syn.prior_range(frame=it["frame"]) for it in insts]
