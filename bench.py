@@ -100,7 +100,10 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="fill and copy 1 GiB on the device first (known-size kernels that calibrate rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE against the library's kernels in the same run)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.slots is None:
+        a.slots = 256 if a.workload == "c3" else 1024
+    return a
 
 
 def elapsed():
@@ -489,8 +492,6 @@ def main():
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if a.slots is None:
-        a.slots = 256 if a.workload == "c3" else 1024
     if a.steps < 1 or a.evals < 1 or a.slots < 1:
         print("--steps, --evals and --slots must be >= 1", file=sys.stderr)
         sys.exit(2)
