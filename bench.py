@@ -397,8 +397,17 @@ def run_batch(a, S, syn, solo_sample=0, extended=False):
            "workload": "%d inversions drawn from the seed: patch grids 4x4..24x24, 50..800 GNSS stations, strike "
                        "0..360, dip 20..80, length 10..60 km, 2..10 mm noise, inner P 500..2000, Gibbs x5, "
                        "split-BF16; %d geometries per inversion from its prior" % (len(probs), a.batch_evals)}
-    print("batch: %d evaluations in %.2f s (%.2f/s), mean %.1f inner stages" % (evals, wall, evals / wall,
-                                                                                stages.mean()), flush=True)
+    # latency per instance: each inversion's last result, and each geometry's set-up launch to result, from the
+    # start of the group call
+    done = np.array([float(p.batch_times(a.batch_evals).max()) for p in probs])
+    lat = np.concatenate([p.batch_times(a.batch_evals) - p.batch_admit_times(a.batch_evals) for p in probs])
+    out["inversion_done_s"] = {"p50": round(float(np.percentile(done, 50)), 3),
+                               "p99": round(float(np.percentile(done, 99)), 3), "max": round(float(done.max()), 3)}
+    out["eval_latency_s"] = {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)}
+    print("batch: %d evaluations in %.2f s (%.2f/s), mean %.1f inner stages; inversions done p50 %.2f s, p99 %.2f s"
+          % (evals, wall, evals / wall, stages.mean(), out["inversion_done_s"]["p50"], out["inversion_done_s"]["p99"]),
+          flush=True)
     if extended:
         # the same call with the stage launched kernel by kernel (identical results)
         g.set_graph(False)
