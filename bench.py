@@ -313,7 +313,7 @@ def run_serving(a, S, syn):
         obs = syn.make_observations("c2", 1000 + k, greens, nsta=nsta, sigma=sigma, nxi=nxi, neta=neta)
         desc = S.problem_desc([nxi], [neta], xmin, xmax, zmin, obs, syn.MAX_SLIP, 512, mover=S.MOVER_GIBBS,
                               gibbs_sweeps=3, precision=S.PRECISION_MIXED_BF16)
-        reqs.append(dict(desc=desc, N=N, nsta=nsta))
+        reqs.append(dict(desc=desc, N=N, nsta=nsta, obs=obs, nxi=nxi, neta=neta))
     gctx.close()
     srv = S.Server(0, hbm_budget_bytes=200 << 30)
     rr = syn.prior_range()
@@ -333,7 +333,31 @@ def run_serving(a, S, syn):
     evals = sum(r["N"] * (1 + 2 * max(x["nstage"] - 1, 0)) for r, x in zip(reqs, res))
     print("serving: p50 %.3f s  p99 %.3f s  over %d requests" % (np.percentile(lat, 50), np.percentile(lat, 99),
                                                                    len(lat)), flush=True)
+    # posterior error of the served (split-BF16) results against the same requests in the FP64 form, for the two
+    # requests with the fewest outer particles: the same seed drives both outer SMCs, so the differences are those
+    # of the arithmetic form carried through the outer stages, where a likelihood that differs in its fifth digit
+    # can change which parents the resampling picks (the negative log evidence, in nats; the posterior mean of each
+    # geometry parameter in units of its FP64 posterior standard deviation)
+    errs = []
+    ectx = S.Context(0)
+    for k in sorted(range(len(reqs)), key=lambda j: reqs[j]["N"])[:2]:
+        r = reqs[k]
+        p64 = S.Problem(ectx, [r["nxi"]], [r["neta"]], xmin, xmax, zmin, r["obs"], syn.MAX_SLIP, 512,
+                        mover=S.MOVER_GIBBS, gibbs_sweeps=3, max_batch=r["N"], precision=S.PRECISION_FP64)
+        f = p64.fault_smc_exec(rr, r["N"], 7000 + k, max_stages=3)
+        nds = p64.ndim
+        p64.close()
+        x64, xbf = f["particles"], res[k]["particles"]
+        sd = np.maximum(x64.std(axis=1), 1e-12)
+        errs.append({"request": k, "N": r["N"], "ndim_slip": nds,
+                     "neglog_evidence_fp64": float("%.6g" % f["evidence"]),
+                     "neglog_evidence_diff": float("%.3g" % abs(res[k]["evidence"] - f["evidence"])),
+                     "mean_diff_in_sd_max": float("%.3g" % float(np.max(np.abs(xbf.mean(axis=1) - x64.mean(axis=1))
+                                                                          / sd)))})
+    ectx.close()
+    print("serving: vs the FP64 form", errs, flush=True)
     return {"requests": len(res), "seconds": a.serve_seconds, "arrival_rate_per_s": a.serve_rate,
+            "error_vs_fp64": errs,
             "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
             "max_latency_s": round(float(lat.max()), 3),
