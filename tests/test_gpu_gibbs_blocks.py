@@ -67,3 +67,31 @@ def test_gibbs_blocks_vs_oracle(ctx, mesh, P, sweeps, nsta):
         np.testing.assert_allclose(d["mean"], o["mean"], rtol=1e-8, atol=1e-10)
     finally:
         pb.close()
+
+
+def test_batch_path_16x16_vs_oracle(ctx):
+    """The batched entry point (fault_calc_likelihood: the stage engine with its item lists, packed prior chains and
+    per-stage graph) at the benchmark's mesh and sweep count: each geometry's stage count and -log Z against the
+    oracle's fault likelihood (its own Green's matrix and Laplacian), 4 geometries through 4 slots."""
+    say("case: batched fault likelihood 16x16 patches, P=128, 5 sweeps, 4 geometries vs oracle")
+    xmin, xmax, zmin, _ = syn.fault_frame()
+
+    def greens(th, pts, uv, nx, ne):
+        return O.Fault([nx], [ne], xmin, xmax, zmin).greens(th, pts, uv, 0, pts.shape[1] // 3)
+
+    obs = syn.make_observations("c2", 4, greens, nsta=30, sigma=0.5, nxi=16, neta=16)
+    P = 128
+    pb = S.Problem(ctx, [16], [16], xmin, xmax, zmin, obs, syn.MAX_SLIP, P, mover=S.MOVER_GIBBS, gibbs_sweeps=5,
+                   max_batch=4)
+    try:
+        po = O.ProblemHolder(O.Fault([16], [16], xmin, xmax, zmin), obs, syn.MAX_SLIP, P, mover=1, gibbs_sweeps=5)
+        th = syn.sample_thetas(4, 77)
+        keys = [O.inner_key(31, 0, 0, i) for i in range(4)]
+        nl, ns = pb.fault_calc_likelihood(th, keys)
+        for k in range(4):
+            ref, nst = po.fault_likelihood(th[:, k], keys[k])
+            say("  geometry %d: stages %d / %d, -log Z %.12g / %.12g" % (k, ns[k], nst, nl[k], ref))
+            assert ns[k] == nst
+            assert nl[k] == pytest.approx(ref, rel=1e-8)
+    finally:
+        pb.close()
