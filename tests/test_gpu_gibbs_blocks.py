@@ -6,8 +6,9 @@ k_gibbs processes coordinates in blocks (a matrix-core product over every coordi
 16-coordinate tile the left-looking corrections and draws), so a slip vector of one partial block, as on the
 4x3-patch mesh of the other parity checks, leaves the hand-over between blocks and tiles unexercised.  Here:
 8x8 patches (ndim 112), 12x11 (ndim 242: the last tile holds 2 coordinates) and the 16x16 mesh of the benchmark
-(ndim 480, 5 sweeps per stage).  Same stages, gammas to 1e-9, -log Z to 1e-8 relative, particles to 1e-7, as the
-small-mesh check."""
+(ndim 480, 5 sweeps per stage), there also with P = 600: a workgroup sweeps 256 particles of an instance, so
+three workgroups, the last one partly filled.  Same stages, gammas to 1e-9, -log Z to 1e-8 relative, particles
+to 1e-7, as the small-mesh check."""
 import sys
 
 import numpy as np
@@ -19,7 +20,7 @@ from smc_inversion_fort_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((8, 8), 128, 3, 12), ((12, 11), 128, 3, 20), ((16, 16), 128, 5, 30)]
+CASES = [((8, 8), 128, 3, 12), ((12, 11), 128, 3, 20), ((16, 16), 128, 5, 30), ((16, 16), 600, 5, 30)]
 
 
 def say(*a):
@@ -34,7 +35,7 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("mesh,P,sweeps,nsta", CASES, ids=["%dx%d" % c[0] for c in CASES])
+@pytest.mark.parametrize("mesh,P,sweeps,nsta", CASES, ids=["%dx%d-P%d" % (c[0] + (c[1],)) for c in CASES])
 def test_gibbs_blocks_vs_oracle(ctx, mesh, P, sweeps, nsta):
     nxi, neta = mesh
     say("case: FP64 Gibbs inner SMC %dx%d patches, P=%d, %d sweeps, %d stations vs oracle" % (nxi, neta, P, sweeps,
