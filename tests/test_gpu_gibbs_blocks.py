@@ -1,4 +1,4 @@
-"""The FP64 inner SMC (slip_smc_exec, smc_slip.f90:1445) with the truncated-MVN Gibbs mover in place of the
+"""The FP64 inner SMC (slip_smc_exec, smc_slip.f90:1445-1663) with the truncated-MVN Gibbs mover in place of the
 reference's HMC moves (smc_slip.f90:591-730), against the oracle on meshes whose slip vector spans several
 coordinate blocks of the sweep kernel.
 
